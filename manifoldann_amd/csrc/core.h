@@ -1,0 +1,56 @@
+// Device-resident operands and the search pipelines shared by the C ABI.
+#pragma once
+
+#include "runtime.h"
+#include "util.h"
+
+struct mann_dataset {
+  int device = 0;
+  int64_t d = 0, n = 0, d16 = 0, kp = 0, n_pad = 0;
+  int normalized = 0;            // bf16 copy holds unit-normalised rows (cosine)
+  float* x32 = nullptr;          // n_pad x d16 FP32 rows (exact operand)
+  mann::bf16_t* xb = nullptr;    // n_pad x kp bf16 rows (MFMA operand)
+  float* xn = nullptr;           // |x~|^2, +inf on padding
+  float* err = nullptr;          // |x - x~| per row (rounded up)
+  float* nrm = nullptr;          // |x~| per row (rounded up)
+  float* xn2 = nullptr;          // A16 |x|^2 per row
+  float ex = 0.0f, rx = 0.0f;    // max err, max nrm
+  // stride sample for threshold estimation
+  int m = 0, m_pad = 0;
+  mann::bf16_t* sb = nullptr;
+  float* sxn = nullptr;
+  ~mann_dataset();
+};
+
+namespace mann {
+
+// Query operands (scratch of the calling thread, or views of a dataset).
+struct QueryOps {
+  int64_t nq = 0;
+  const float* q32 = nullptr;
+  const bf16_t* qb = nullptr;
+  const float* qn = nullptr;
+  const float* qe = nullptr;
+  const float* qr = nullptr;
+  const float* qn2 = nullptr;
+};
+
+// Upload (if host) and convert a query matrix; scratch of this thread.
+QueryOps prepare_queries(const float* q, int64_t d, int64_t nq, int64_t ldq, int normalize,
+                         int d16, int kp, const char* tag);
+QueryOps dataset_as_queries(const mann_dataset& ds);
+
+mann_dataset* create_dataset(const float* data, int64_t d, int64_t n, int64_t ld, int metric);
+
+struct SearchStats {
+  long long reranked = 0, overflow = 0, pass2 = 0, exact = 0;
+};
+SearchStats& last_stats();
+
+// Exact k nearest rows of `ds` for each query (bf16 MFMA first pass, certified
+// FP32 re-rank, refinement pass, exact fallback).  out_* are device pointers,
+// nq x k.  exclude_self drops the row whose id equals the query index.
+void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int exclude_self,
+               int64_t* out_ids, float* out_dist, int* out_count);
+
+}  // namespace mann

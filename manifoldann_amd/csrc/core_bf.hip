@@ -1,0 +1,371 @@
+// Runtime plumbing, dataset/query preparation and the exact-kNN pipeline
+// (BruteForceIndex query and build_knn_graph).
+#include <algorithm>
+#include <cstring>
+
+#include "core.h"
+
+namespace mann {
+
+// ------------------------------------------------------------------ runtime --
+
+ThreadCtx& ctx() {
+  thread_local ThreadCtx c;
+  int dev = 0;
+  MANN_HIP(hipGetDevice(&dev));
+  if (c.device != dev || c.stream == nullptr) {
+    // buffers of another device are dropped (not freed: they belong to it)
+    c.bufs.clear();
+    c.device = dev;
+    MANN_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  }
+  return c;
+}
+
+Timer& timer() {
+  static Timer t;
+  return t;
+}
+
+int timer_begin(const char* name, hipStream_t st) {
+  Timer& t = timer();
+  if (!t.enabled) return -1;
+  std::lock_guard<std::mutex> g(t.mu);
+  Timer::Pending p;
+  p.name = name;
+  MANN_HIP(hipEventCreate(&p.a));
+  MANN_HIP(hipEventCreate(&p.b));
+  MANN_HIP(hipEventRecord(p.a, st));
+  t.pending.push_back(p);
+  return (int)t.pending.size() - 1;
+}
+
+void timer_end(int token, hipStream_t st) {
+  if (token < 0) return;
+  Timer& t = timer();
+  std::lock_guard<std::mutex> g(t.mu);
+  if (token < (int)t.pending.size()) (void)hipEventRecord(t.pending[token].b, st);
+}
+
+void timer_flush() {
+  Timer& t = timer();
+  std::lock_guard<std::mutex> g(t.mu);
+  for (auto& p : t.pending) {
+    (void)hipEventSynchronize(p.b);
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      auto& e = t.totals[p.name];
+      e.first += 1;
+      e.second += ms;
+    }
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  t.pending.clear();
+}
+
+bool is_device_ptr(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t attr;
+  hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+SearchStats& last_stats() {
+  thread_local SearchStats s;
+  return s;
+}
+
+// ------------------------------------------------------------------ dataset --
+
+}  // namespace mann
+
+mann_dataset::~mann_dataset() {
+  (void)hipSetDevice(device);
+  void* ps[] = {x32, xb, xn, err, nrm, xn2, sb, sxn};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+}
+
+namespace mann {
+
+static constexpr int SAMPLE_ROWS = 4096;
+
+mann_dataset* create_dataset(const float* data, int64_t d, int64_t n, int64_t ld, int metric) {
+  MANN_REQUIRE(d > 0, ERR_ARGUMENT, "Dataset must have at least one dimension");
+  MANN_REQUIRE(n > 0, ERR_ARGUMENT, "Dataset must contain at least one point");
+  MANN_REQUIRE(ld >= d, ERR_ARGUMENT, "leading dimension smaller than d");
+  MANN_REQUIRE(n < (int64_t)1 << 31, ERR_UNSUPPORTED, "more than 2^31 points");
+  ThreadCtx& c = ctx();
+  hipStream_t st = c.stream;
+  auto* ds = new mann_dataset();
+  ds->device = c.device;
+  ds->d = d;
+  ds->n = n;
+  ds->d16 = round_up(d, 16);
+  ds->kp = round_up(d, 64);
+  ds->n_pad = round_up(n, 256) + 256;
+  ds->normalized = metric == COSINE;
+  try {
+    const float* src = data;
+    float* staged = nullptr;
+    if (!is_device_ptr(data)) {
+      MANN_HIP(hipMalloc(&staged, sizeof(float) * (size_t)(ld * (n - 1) + d)));
+      MANN_HIP(hipMemcpy(staged, data, sizeof(float) * (size_t)(ld * (n - 1) + d),
+                         hipMemcpyHostToDevice));
+      src = staged;
+    }
+    MANN_HIP(hipMalloc(&ds->x32, sizeof(float) * ds->n_pad * ds->d16));
+    MANN_HIP(hipMalloc(&ds->xb, sizeof(bf16_t) * ds->n_pad * ds->kp));
+    MANN_HIP(hipMalloc(&ds->xn, sizeof(float) * ds->n_pad));
+    MANN_HIP(hipMalloc(&ds->err, sizeof(float) * ds->n_pad));
+    MANN_HIP(hipMalloc(&ds->nrm, sizeof(float) * ds->n_pad));
+    MANN_HIP(hipMalloc(&ds->xn2, sizeof(float) * ds->n_pad));
+    unsigned* mx = c.buf("ds_max").as<unsigned>(2);
+    fill_i32((int*)mx, 2, 0, st);
+    launch_prep_rows(src, ld, n, ds->n_pad, (int)d, (int)ds->d16, (int)ds->kp, ds->normalized,
+                     ds->x32, ds->xb, ds->xn, ds->err, ds->nrm, mx, mx + 1, st);
+    launch_a16_norms(ds->x32, n, (int)ds->d16, ds->xn2, st);
+    // stride sample: rows floor(i * n / m)
+    ds->m = (int)std::min<int64_t>(n, SAMPLE_ROWS);
+    ds->m_pad = (int)round_up(ds->m, 256) + 256;
+    std::vector<int> sidx(ds->m);
+    for (int i = 0; i < ds->m; ++i) sidx[i] = (int)((int64_t)i * n / ds->m);
+    int* didx = c.buf("ds_sidx").as<int>(ds->m);
+    MANN_HIP(hipMemcpyAsync(didx, sidx.data(), sizeof(int) * ds->m, hipMemcpyHostToDevice, st));
+    MANN_HIP(hipMalloc(&ds->sb, sizeof(bf16_t) * (size_t)ds->m_pad * ds->kp));
+    MANN_HIP(hipMalloc(&ds->sxn, sizeof(float) * (size_t)ds->m_pad));
+    gather_rows_bf16(ds->xb, ds->xn, didx, ds->m, ds->m_pad, (int)ds->kp, ds->sb, ds->sxn, st);
+    unsigned hm[2];
+    MANN_HIP(hipMemcpyAsync(hm, mx, sizeof(hm), hipMemcpyDeviceToHost, st));
+    MANN_HIP(hipStreamSynchronize(st));
+    MANN_HIP(hipGetLastError());
+    if (staged) (void)hipFree(staged);
+    memcpy(&ds->ex, &hm[0], 4);
+    memcpy(&ds->rx, &hm[1], 4);
+  } catch (...) {
+    delete ds;
+    throw;
+  }
+  return ds;
+}
+
+QueryOps prepare_queries(const float* q, int64_t d, int64_t nq, int64_t ldq, int normalize,
+                         int d16, int kp, const char* tag) {
+  ThreadCtx& c = ctx();
+  hipStream_t st = c.stream;
+  std::string t(tag);
+  QueryOps Q;
+  Q.nq = nq;
+  if (nq == 0) return Q;
+  int64_t nq_pad = round_up(nq, 256);
+  const float* src = q;
+  if (!is_device_ptr(q)) {
+    float* stg = c.buf(t + "_stage").as<float>((size_t)(ldq * (nq - 1) + d));
+    MANN_HIP(hipMemcpyAsync(stg, q, sizeof(float) * (size_t)(ldq * (nq - 1) + d),
+                            hipMemcpyHostToDevice, st));
+    src = stg;
+  }
+  float* q32 = c.buf(t + "_q32").as<float>(nq_pad * d16);
+  bf16_t* qb = c.buf(t + "_qb").as<bf16_t>(nq_pad * kp);
+  float* qn = c.buf(t + "_qn").as<float>(nq_pad);
+  float* qe = c.buf(t + "_qe").as<float>(nq_pad);
+  float* qr = c.buf(t + "_qr").as<float>(nq_pad);
+  float* qn2 = c.buf(t + "_qn2").as<float>(nq_pad);
+  {
+    ScopedTime tm("prep_queries", st);
+    launch_prep_rows(src, ldq, nq, nq_pad, (int)d, d16, kp, normalize, q32, qb, qn, qe, qr,
+                     nullptr, nullptr, st);
+    launch_a16_norms(q32, nq, d16, qn2, st);
+  }
+  Q.q32 = q32;
+  Q.qb = qb;
+  Q.qn = qn;
+  Q.qe = qe;
+  Q.qr = qr;
+  Q.qn2 = qn2;
+  return Q;
+}
+
+QueryOps dataset_as_queries(const mann_dataset& ds) {
+  QueryOps Q;
+  Q.nq = ds.n;
+  Q.q32 = ds.x32;
+  Q.qb = ds.xb;
+  Q.qn = ds.xn;
+  Q.qe = ds.err;
+  Q.qr = ds.nrm;
+  Q.qn2 = ds.xn2;
+  return Q;
+}
+
+// ------------------------------------------------------------ exact search --
+
+void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int exclude_self,
+               int64_t* out_ids, float* out_dist, int* out_count) {
+  ThreadCtx& c = ctx();
+  hipStream_t st = c.stream;
+  SearchStats& S = last_stats();
+  S = SearchStats();
+  const int nq = (int)Q.nq;
+  if (nq == 0 || k <= 0) return;
+  const int n = (int)ds.n;
+  const int k_req = k + (exclude_self ? 1 : 0);
+  const int k_eff = std::min(k_req, n);
+  const int cap = select_cap_max();
+  MANN_REQUIRE(k_req <= cap / 2, ERR_UNSUPPORTED, "k too large for the GPU selection buffer");
+  const int BM = scan_tile_rows(), BN = scan_tile_queries();
+
+  unsigned* cand = c.buf("bf_cand").as<unsigned>((size_t)nq * cap * 2);
+  int* cnt = c.buf("bf_cnt").as<int>(nq);
+  float* tau = c.buf("bf_tau").as<float>(nq);
+  float* tau2 = c.buf("bf_tau2").as<float>(nq);
+  int* flags = c.buf("bf_flags").as<int>(8);   // [0] pass1 count, [1] pass2 count, [2..] stats
+  int* list1 = c.buf("bf_list1").as<int>(nq);
+  int* list2 = c.buf("bf_list2").as<int>(nq);
+  fill_i32(flags, 8, 0, st);
+  fill_i32(cnt, nq, 0, st);
+
+  // ---- emission threshold per query
+  if (n <= cap) {
+    fill_f32(tau, nq, INFINITY, st);
+  } else {
+    float* dense = c.buf("bf_dense").as<float>((size_t)nq * ds.m);
+    ScanArgs a{};
+    a.xb = ds.sb;
+    a.xn = ds.sxn;
+    a.qb = Q.qb;
+    a.kp = (int)ds.kp;
+    a.d = (int)ds.d;
+    a.nq = nq;
+    a.parts = 1;
+    a.rows_per_part = (int)round_up(ds.m, BM);
+    a.n_rows = ds.m;
+    a.dense = dense;
+    a.ldd = ds.m;
+    {
+      ScopedTime tm("scan_dense", st);
+      launch_scan(a, SCAN_DENSE, (int)ceil_div(nq, BN), false, st);
+    }
+    TauArgs ta{dense, (int64_t)ds.m, ds.m, std::min(ds.m, k_req), tau, nq};
+    ScopedTime tm("tau_sample", st);
+    launch_tau_from_sample(ta, st);
+  }
+
+  // ---- filtered first pass over all rows
+  const int qtiles = (int)ceil_div(nq, BN);
+  const int row_tiles = (int)ceil_div(n, BM);
+  int parts = std::max(1, std::min(row_tiles, (int)ceil_div(2048, qtiles)));
+  int rpp = (int)round_up(ceil_div(n, parts), BM);
+  parts = (int)ceil_div(n, rpp);
+  ScanArgs a{};
+  a.xb = ds.xb;
+  a.xn = ds.xn;
+  a.qb = Q.qb;
+  a.kp = (int)ds.kp;
+  a.d = (int)ds.d;
+  a.nq = nq;
+  a.parts = parts;
+  a.rows_per_part = rpp;
+  a.n_rows = n;
+  a.tau = tau;
+  a.cand = cand;
+  a.cnt = cnt;
+  a.cap = cap;
+  {
+    ScopedTime tm("scan_filter", st);
+    launch_scan(a, SCAN_FILTER, qtiles * parts, false, st);
+  }
+
+  SelectArgs s{};
+  s.nq = nq;
+  s.cand = cand;
+  s.cnt = cnt;
+  s.cap = cap;
+  s.tau = tau;
+  s.x32 = ds.x32;
+  s.q32 = Q.q32;
+  s.xn2 = ds.xn2;
+  s.qn2 = Q.qn2;
+  s.d = (int)ds.d;
+  s.d16 = (int)ds.d16;
+  s.metric = metric;
+  s.qn = Q.qn;
+  s.qe = Q.qe;
+  s.qr = Q.qr;
+  s.ex = ds.ex;
+  s.rx = ds.rx;
+  s.k = k;
+  s.k_req = k_req;
+  s.exclude_self = exclude_self;
+  s.k_eff = k_eff;
+  s.shortlist = std::min(cap, std::max(2 * k_req, 32));
+  s.out_ids = out_ids;
+  s.out_dist = out_dist;
+  s.out_count = out_count;
+  s.flag_count = flags;
+  s.flag_list = list1;
+  s.tau_new = tau2;
+  s.stats = flags + 2;
+  {
+    ScopedTime tm("select_rerank", st);
+    launch_select_rerank(s, nq, st);
+  }
+
+  // ---- second pass for queries that failed certification (device-side count)
+  reset_listed(cnt, list1, flags, nq, st);
+  a.qidx = list1;
+  a.nq_dev = flags;
+  a.tau = tau2;
+  {
+    ScopedTime tm("scan_filter_pass2", st);
+    launch_scan(a, SCAN_FILTER, qtiles * parts, false, st);
+  }
+  s.qlist = list1;
+  s.nq_dev = flags;
+  s.tau = tau2;
+  s.flag_count = flags + 1;
+  s.flag_list = list2;
+  s.tau_new = tau;  // unused afterwards
+  {
+    ScopedTime tm("select_rerank_pass2", st);
+    launch_select_rerank(s, nq, st);
+  }
+  // ---- exact fallback for what is still uncertified
+  ExactArgs e{};
+  e.qlist = list2;
+  e.nq_dev = flags + 1;
+  e.x32 = ds.x32;
+  e.q32 = Q.q32;
+  e.xn2 = ds.xn2;
+  e.qn2 = Q.qn2;
+  e.d = (int)ds.d;
+  e.d16 = (int)ds.d16;
+  e.metric = metric;
+  e.n_rows = n;
+  e.k = k;
+  e.k_req = k_req;
+  e.exclude_self = exclude_self;
+  e.out_ids = out_ids;
+  e.out_dist = out_dist;
+  e.out_count = out_count;
+  MANN_REQUIRE(k_req <= exact_kmax(), ERR_UNSUPPORTED, "k too large for the exact fallback");
+  {
+    ScopedTime tm("exact_fallback", st);
+    launch_exact_topk(e, nq, st);
+  }
+  int hf[8];
+  MANN_HIP(hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, st));
+  MANN_HIP(hipStreamSynchronize(st));
+  MANN_HIP(hipGetLastError());
+  S.pass2 = hf[0];
+  S.exact = hf[1];
+  S.reranked = hf[2];
+  S.overflow = hf[3];
+}
+
+}  // namespace mann

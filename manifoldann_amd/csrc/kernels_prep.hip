@@ -1,0 +1,97 @@
+// Row preparation: FP32 padded copy, BF16 copy for the MFMA first pass, FP64
+// norms / quantisation-error norms (the bounds that certify the re-rank), and
+// A16-order FP32 squared norms (the norms-form arithmetic of the oracle).
+#include "launch.h"
+
+namespace mann {
+
+// One wavefront per row.  src rows have stride ld (Julia column stride).
+//  x32 : rows of d16 floats (zero padded)        -- exact re-rank operand
+//  xb  : rows of kp bf16 (zero padded)           -- MFMA operand
+//  xn  : |bf16(x)|^2 (FP64 sum rounded to FP32); +inf on padding rows so a
+//        padding row can never pass a threshold
+//  err : |x - bf16(x)| rounded up; nrm : |bf16(x)| rounded up
+// normalize != 0 (cosine): the bf16 copy holds x/|x| (|x| in FP64) and the
+// bounds refer to the normalised vector.
+__global__ __launch_bounds__(256) void prep_rows_kernel(const float* __restrict__ src, int64_t ld,
+                                                        int64_t n, int64_t n_pad, int d, int d16,
+                                                        int kp, int normalize,
+                                                        float* __restrict__ x32,
+                                                        bf16_t* __restrict__ xb,
+                                                        float* __restrict__ xn,
+                                                        float* __restrict__ err,
+                                                        float* __restrict__ nrm,
+                                                        unsigned* __restrict__ max_err,
+                                                        unsigned* __restrict__ max_nrm) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n_pad) return;
+  if (row >= n) {
+    for (int c = lane; c < d16; c += 64) x32[row * d16 + c] = 0.0f;
+    for (int c = lane; c < kp; c += 64) xb[row * kp + c] = (bf16_t)0.0f;
+    if (lane == 0) xn[row] = INFINITY;
+    return;
+  }
+  const float* s = src + row * ld;
+  double scale = 1.0;
+  if (normalize) {
+    double ss = 0.0;
+    for (int c = lane; c < d; c += 64) ss += (double)s[c] * (double)s[c];
+    for (int o = 32; o >= 1; o >>= 1) ss += __shfl_xor(ss, o);
+    scale = ss > 0.0 ? 1.0 / sqrt(ss) : 0.0;
+  }
+  double nb = 0.0, ee = 0.0;
+  for (int c = lane; c < kp; c += 64) {
+    float v = c < d ? s[c] : 0.0f;
+    if (c < d16) x32[row * d16 + c] = v;
+    double t = (double)v * scale;
+    bf16_t b = (bf16_t)(float)t;
+    xb[row * kp + c] = b;
+    double bd = (double)(float)b;
+    nb += bd * bd;
+    ee += (t - bd) * (t - bd);
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    nb += __shfl_xor(nb, o);
+    ee += __shfl_xor(ee, o);
+  }
+  if (lane == 0) {
+    xn[row] = (float)nb;
+    float e = (float)(sqrt(ee) * (1.0 + 1e-6)) + 1e-30f;
+    float r = (float)(sqrt(nb) * (1.0 + 1e-6)) + 1e-30f;
+    if (err) err[row] = e;
+    if (nrm) nrm[row] = r;
+    if (max_err) atomicMax(max_err, __float_as_uint(e));
+    if (max_nrm) atomicMax(max_nrm, __float_as_uint(r));
+  }
+}
+
+// A16-order FP32 squared norm of each row of x32 (16 lanes per row).
+__global__ __launch_bounds__(256) void a16_norms_kernel(const float* __restrict__ x32, int64_t n,
+                                                        int d16, float* __restrict__ out) {
+  const int j = threadIdx.x & 15;
+  const int64_t row = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+  float acc = 0.0f;
+  if (row < n) {
+    const float* x = x32 + row * d16;
+    for (int t = 0; t < d16; t += 16) acc = __builtin_fmaf(x[t + j], x[t + j], acc);
+  }
+  acc = group16_tree(acc);
+  if (row < n && j == 0) out[row] = acc;
+}
+
+void launch_prep_rows(const float* src, int64_t ld, int64_t n, int64_t n_pad, int d, int d16,
+                      int kp, int normalize, float* x32, bf16_t* xb, float* xn, float* err,
+                      float* nrm, unsigned* max_err, unsigned* max_nrm, hipStream_t st) {
+  if (n_pad <= 0) return;
+  hipLaunchKernelGGL(prep_rows_kernel, dim3((unsigned)ceil_div(n_pad, 4)), dim3(256), 0, st, src,
+                     ld, n, n_pad, d, d16, kp, normalize, x32, xb, xn, err, nrm, max_err, max_nrm);
+}
+
+void launch_a16_norms(const float* x32, int64_t n, int d16, float* out, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(a16_norms_kernel, dim3((unsigned)ceil_div(n, 16)), dim3(256), 0, st, x32, n,
+                     d16, out);
+}
+
+}  // namespace mann

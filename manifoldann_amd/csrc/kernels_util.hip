@@ -1,0 +1,70 @@
+// Small device utilities (fills, gathers, id conversion) used by the host layer.
+#include "launch.h"
+#include "util.h"
+
+namespace mann {
+
+__global__ void fill_i32_kernel(int* p, int64_t n, int v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void fill_f32_kernel(float* p, int64_t n, float v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+// cnt[list[i]] = 0 for i < *count
+__global__ void reset_listed_kernel(int* cnt, const int* list, const int* count, int max_n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < max_n && i < *count) cnt[list[i]] = 0;
+}
+// gather rows of a bf16 matrix and their norms: dst[i] = src[idx[i]], rows [m, m_pad) zero / +inf
+__global__ void gather_rows_bf16_kernel(const bf16_t* src, const float* srcn, const int* idx,
+                                        int m, int m_pad, int kp, bf16_t* dst, float* dstn) {
+  int row = blockIdx.x;
+  if (row >= m_pad) return;
+  const bf16_t* s = row < m ? src + (int64_t)idx[row] * kp : nullptr;
+  for (int c = threadIdx.x; c < kp; c += blockDim.x)
+    dst[(int64_t)row * kp + c] = s ? s[c] : (bf16_t)0.0f;
+  if (threadIdx.x == 0) dstn[row] = row < m ? srcn[idx[row]] : INFINITY;
+}
+__global__ void gather_rows_f32_kernel(const float* src, const int* idx, int m, int w, float* dst) {
+  int row = blockIdx.x;
+  if (row >= m) return;
+  const float* s = src + (int64_t)idx[row] * w;
+  for (int c = threadIdx.x; c < w; c += blockDim.x) dst[(int64_t)row * w + c] = s[c];
+}
+__global__ void gather_f32_kernel(const float* src, const int* idx, int m, float* dst) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) dst[i] = src[idx[i]];
+}
+
+void fill_i32(int* p, int64_t n, int v, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, p, n, v);
+}
+void fill_f32(float* p, int64_t n, float v, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_f32_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, p, n, v);
+}
+void reset_listed(int* cnt, const int* list, const int* count, int max_n, hipStream_t st) {
+  if (max_n <= 0) return;
+  hipLaunchKernelGGL(reset_listed_kernel, dim3((unsigned)ceil_div(max_n, 256)), dim3(256), 0, st,
+                     cnt, list, count, max_n);
+}
+void gather_rows_bf16(const bf16_t* src, const float* srcn, const int* idx, int m, int m_pad,
+                      int kp, bf16_t* dst, float* dstn, hipStream_t st) {
+  if (m_pad <= 0) return;
+  hipLaunchKernelGGL(gather_rows_bf16_kernel, dim3(m_pad), dim3(64), 0, st, src, srcn, idx, m,
+                     m_pad, kp, dst, dstn);
+}
+void gather_rows_f32(const float* src, const int* idx, int m, int w, float* dst, hipStream_t st) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(gather_rows_f32_kernel, dim3(m), dim3(64), 0, st, src, idx, m, w, dst);
+}
+void gather_f32(const float* src, const int* idx, int m, float* dst, hipStream_t st) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(gather_f32_kernel, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, st, src,
+                     idx, m, dst);
+}
+
+}  // namespace mann

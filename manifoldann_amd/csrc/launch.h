@@ -1,0 +1,133 @@
+// Kernel launch interface shared by the kernel files and the C-ABI layer.
+#pragma once
+
+#include "common.h"
+
+namespace mann {
+
+enum ScanMode : int { SCAN_FILTER = 0, SCAN_DENSE = 1, SCAN_TOP1 = 2 };
+
+struct ScanItem {
+  int qoff;       // offset into the query index list (or first query id)
+  int nq;         // queries in this item (<= tile width)
+  int row_begin;  // base rows [row_begin, row_end) of the (reordered) base
+  int row_end;
+};
+
+struct ScanArgs {
+  const bf16_t* xb;   // base rows, kp bf16 each (padded rows: zeros)
+  const float* xn;    // |x~|^2 per base row, +inf on padding
+  const bf16_t* qb;   // query rows, kp bf16 each
+  int kp;             // padded K (multiple of 64)
+  int d;              // true dimension
+  const int* qidx;    // query id list (nullptr: identity)
+  // grid mode
+  int nq;             // number of queries (or ...)
+  const int* nq_dev;  // ... read from device memory when non-null
+  int parts;          // row partitions per query tile
+  int rows_per_part;  // multiple of the tile height
+  int n_rows;         // rows scanned in grid mode: [0, n_rows)
+  // persistent mode
+  const ScanItem* items;
+  const int* n_items_dev;
+  int* item_counter;
+  // FILTER / TOP1 outputs
+  const float* tau;   // per query threshold on a'
+  unsigned* cand;     // [nq_total][cap] pairs (float bits of a', row)
+  int* cnt;           // per query emitted count (may exceed cap: overflow)
+  int cap;
+  // DENSE output
+  float* dense;
+  int64_t ldd;
+  // TOP1 certification inputs
+  const float* t1_qn;
+  const float* t1_qe;
+  const float* t1_qr;
+  float t1_ex, t1_rx;
+};
+
+int scan_tile_rows();
+int scan_tile_queries();
+void launch_scan(const ScanArgs& a, int mode, int grid, bool persistent, hipStream_t st);
+
+void launch_prep_rows(const float* src, int64_t ld, int64_t n, int64_t n_pad, int d, int d16,
+                      int kp, int normalize, float* x32, bf16_t* xb, float* xn, float* err,
+                      float* nrm, unsigned* max_err, unsigned* max_nrm, hipStream_t st);
+void launch_a16_norms(const float* x32, int64_t n, int d16, float* out, hipStream_t st);
+
+// ---- selection / exact re-rank -------------------------------------------------
+
+struct TauArgs {
+  const float* dense;  // [nq][m]
+  int64_t ldd;
+  int m;               // sample size
+  int rank;            // 1-based rank of the threshold value
+  float* tau;          // out
+  int nq;
+};
+void launch_tau_from_sample(const TauArgs& a, hipStream_t st);
+
+struct SelectArgs {
+  // queries processed: blockIdx -> qlist[blockIdx] (or identity), count on device
+  const int* qlist;
+  const int* nq_dev;
+  int nq;
+  // candidates
+  const unsigned* cand;
+  const int* cnt;
+  int cap;
+  const float* tau;
+  // exact operands
+  const float* x32;    // base rows (reordered space for IVF), d16 floats each
+  const float* q32;    // query rows, d16 floats
+  const float* xn2;    // A16 |x|^2 per base row (cosine / norms form)
+  const float* qn2;    // A16 |q|^2 per query
+  int d, d16;
+  int metric;          // EUCLIDEAN / SQEUCLIDEAN / COSINE / NORMS_FORM
+  const int* perm;     // base row -> original point id (nullptr: identity)
+  // certification inputs
+  const float* qn;     // |q~|^2
+  const float* qe;     // |q - q~|
+  const float* qr;     // |q~|
+  float ex, rx;        // dataset bounds
+  // request
+  int k;               // neighbours returned per query
+  int k_req;           // neighbours selected before self removal (k or k+1)
+  int exclude_self;    // drop id == query id (kNN graph)
+  int k_eff;           // min(k_req, rows available)
+  int shortlist;       // first re-rank batch (oversampling, >= k_req)
+  // outputs
+  int64_t* out_ids;    // [nq][k] 1-based, 0 = empty
+  float* out_dist;     // [nq][k]
+  int* out_count;      // optional: results per query (after self removal)
+  // refinement
+  int* flag_count;     // queries that failed certification
+  int* flag_list;
+  float* tau_new;      // refined threshold for a second pass
+  int* stats;          // [0] reranked candidates total, [1] overflow count
+};
+enum ExtraMetric : int { NORMS_FORM = 3 };
+void launch_select_rerank(const SelectArgs& a, int grid, hipStream_t st);
+
+struct ExactArgs {
+  const int* qlist;
+  const int* nq_dev;
+  const float* x32;
+  const float* q32;
+  const float* xn2;
+  const float* qn2;
+  int d, d16, metric;
+  const int* perm;
+  // row ranges per query: ranges [range_off[q], range_off[q+1]) of (begin, end)
+  // pairs in `ranges`; nullptr -> the single range [0, n_rows)
+  const int* range_off;
+  const int2* ranges;
+  int n_rows;
+  int k, k_req, exclude_self;
+  int64_t* out_ids;
+  float* out_dist;
+  int* out_count;
+};
+void launch_exact_topk(const ExactArgs& a, int grid, hipStream_t st);
+
+}  // namespace mann

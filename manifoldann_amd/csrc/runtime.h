@@ -1,0 +1,76 @@
+// Host runtime: per-thread streams and workspaces, device buffers, kernel timing.
+#pragma once
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "launch.h"
+
+namespace mann {
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void* get(size_t need) {
+    if (need > bytes) {
+      if (p) (void)hipFree(p);
+      p = nullptr;
+      bytes = 0;
+      size_t want = need + need / 8 + 256;
+      MANN_HIP(hipMalloc(&p, want));
+      bytes = want;
+    }
+    return p;
+  }
+  template <class T>
+  T* as(size_t count) { return (T*)get(count * sizeof(T)); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+// Per host thread: one stream per device and named scratch buffers.  Every
+// exported call runs on the calling thread's stream, so calls from different
+// threads never share scratch memory (the re-entrancy contract of
+// ann_index.jl:60-70).
+struct ThreadCtx {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  std::map<std::string, DevBuf> bufs;
+  DevBuf& buf(const std::string& name) { return bufs[name]; }
+};
+ThreadCtx& ctx();
+
+// Kernel timing (HIP events on the launching stream), enabled by
+// mann_profile_enable().  Records are resolved at the end of each API call.
+struct Timer {
+  bool enabled = false;
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::map<std::string, std::pair<long long, double>> totals;  // count, ms
+  std::mutex mu;
+};
+Timer& timer();
+// Bracket one launch: begin() returns a token passed to end().
+int timer_begin(const char* name, hipStream_t st);
+void timer_end(int token, hipStream_t st);
+void timer_flush();
+
+bool is_device_ptr(const void* p);
+
+struct ScopedTime {
+  int tok;
+  hipStream_t st;
+  ScopedTime(const char* name, hipStream_t s) : tok(timer_begin(name, s)), st(s) {}
+  ~ScopedTime() { timer_end(tok, st); }
+};
+
+}  // namespace mann

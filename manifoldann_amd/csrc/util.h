@@ -1,0 +1,19 @@
+// Declarations of the small device utilities in kernels_util.hip.
+#pragma once
+
+#include "common.h"
+
+namespace mann {
+
+void fill_i32(int* p, int64_t n, int v, hipStream_t st);
+void fill_f32(float* p, int64_t n, float v, hipStream_t st);
+void reset_listed(int* cnt, const int* list, const int* count, int max_n, hipStream_t st);
+void gather_rows_bf16(const bf16_t* src, const float* srcn, const int* idx, int m, int m_pad,
+                      int kp, bf16_t* dst, float* dstn, hipStream_t st);
+void gather_rows_f32(const float* src, const int* idx, int m, int w, float* dst, hipStream_t st);
+void gather_f32(const float* src, const int* idx, int m, float* dst, hipStream_t st);
+
+int select_cap_max();
+int exact_kmax();
+
+}  // namespace mann

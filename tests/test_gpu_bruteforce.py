@@ -1,0 +1,153 @@
+"""GPU parity of the exact-kNN path (BruteForceIndex batch query, kNN graph)
+against the CPU oracle: ids and distances bit-identical."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import manifoldann_amd as M
+from manifoldann_amd import datasets
+from oracle import oracle as O
+
+
+def _banner(msg):
+    print(f"\n[gpu] {msg}", flush=True)
+
+
+def test_known_answer_bruteforce():
+    # bruteforce_tests.jl:22-35
+    _banner("known answer")
+    data = np.array([[0.0, 1.0, 2.0], [0.0, 0.0, 0.0]], np.float32)
+    index = M.build_index(M.BruteForceIndex, data)
+    nb = M.query(index, data, np.array([0.25, 0.0], np.float32), 2)
+    assert M.neighbor_ids(nb) == [1, 2]
+    with pytest.raises(M.ArgumentError):
+        M.query(index, data[:, :2], np.array([0.25, 0.0], np.float32), 2)
+    with pytest.raises(M.DimensionMismatch):
+        M.query(index, data, np.zeros(3, np.float32), 1)
+
+
+def test_config1_gaussian_bit_exact():
+    # 10,000 x 128 N(0,1), 1,000 queries, k=10, L2
+    _banner("config 10000x128 k=10")
+    X = datasets.gaussian(10_000, 128, seed=1)
+    Q = datasets.gaussian(1_000, 128, seed=2)
+    index = M.build_index(M.BruteForceIndex, X.T)
+    ds = M.Dataset(X.T)
+    ids, dist = M.query_arrays(index, ds, Q.T, 10)
+    st = M.last_stats()
+    oi, od = O.bf_knn(X, Q, 10)
+    assert (ids == oi).all(), np.argwhere(ids != oi)[:5]
+    assert (dist.view(np.uint32) == od.view(np.uint32)).all()
+    # distance error against FP64, stated: max relative
+    _, d64 = O.fp64_knn(X, Q[:100], 10)
+    rel = np.abs(dist[:100].astype(np.float64) - d64) / d64
+    print(f"stats={st} max rel err vs fp64={rel.max():.3e}", flush=True)
+    assert rel.max() < 1e-5
+
+
+@pytest.mark.parametrize("metric", [M.SqEuclidean(), M.CosineDist()])
+def test_other_metrics_bit_exact(metric):
+    _banner(f"metric {type(metric).__name__}")
+    rng = np.random.default_rng(7)
+    X = rng.standard_normal((3000, 37)).astype(np.float32)
+    Q = rng.standard_normal((129, 37)).astype(np.float32)
+    index = M.build_index(M.BruteForceIndex, X.T, distance=metric)
+    ids, dist = M.query_arrays(index, X.T, Q.T, 9)
+    oi, od = O.bf_knn(X, Q, 9, metric.code)
+    assert (ids == oi).all()
+    assert (dist.view(np.uint32) == od.view(np.uint32)).all()
+
+
+def test_small_n_padding_and_k_clamp():
+    _banner("small n")
+    rng = np.random.default_rng(8)
+    X = rng.standard_normal((5, 16)).astype(np.float32)
+    Q = rng.standard_normal((3, 16)).astype(np.float32)
+    index = M.build_index(M.BruteForceIndex, X.T)
+    ids, dist = M.query_arrays(index, X.T, Q.T, 8)
+    oi, od = O.bf_knn(X, Q, 8)
+    assert (ids == oi).all() and (ids[:, 5:] == 0).all()
+    assert np.isinf(dist[:, 5:]).all()
+
+
+def test_empty_and_k_zero():
+    _banner("empty batch")
+    X = np.ones((4, 10), np.float32)
+    index = M.build_index(M.BruteForceIndex, X)
+    ids, dist = M.query_arrays(index, X, np.zeros((4, 0), np.float32), 3)
+    assert ids.shape == (0, 3)
+    with pytest.raises(M.DimensionMismatch):
+        M.query_arrays(index, X, np.zeros((5, 0), np.float32), 3)
+    assert M.query(index, X, np.zeros(4, np.float32), 0) == []
+
+
+def test_ties_and_fallback_paths():
+    # every point identical: all distances tie, so the sampled threshold admits
+    # nothing, the second pass overflows and the exact fallback must return the
+    # lowest ids (Julia partialsortperm tie rule).
+    _banner("ties / fallback")
+    X = np.ones((5000, 24), np.float32)
+    X[4000] = 2.0
+    Q = np.ones((4, 24), np.float32)
+    index = M.build_index(M.BruteForceIndex, X.T)
+    ids, dist = M.query_arrays(index, X.T, Q.T, 6)
+    st = M.last_stats()
+    assert ids.tolist() == [[1, 2, 3, 4, 5, 6]] * 4
+    assert (dist == 0).all()
+    assert st[3] == 4, st  # all four went to the exact fallback
+
+
+def test_device_queries_torch():
+    torch = pytest.importorskip("torch")
+    _banner("torch device tensors")
+    X = datasets.gaussian(2000, 64, seed=3)
+    Q = datasets.gaussian(300, 64, seed=4)
+    index = M.build_index(M.BruteForceIndex, X.T)
+    Xd = torch.from_numpy(X).cuda()
+    Qd = torch.from_numpy(Q).cuda()
+    ds = M.Dataset(Xd.t())
+    ids, dist = M.query_arrays(index, ds, Qd.t(), 5)
+    assert ids.is_cuda
+    oi, od = O.bf_knn(X, Q, 5)
+    assert (ids.cpu().numpy() == oi).all()
+    assert (dist.cpu().numpy().view(np.uint32) == od.view(np.uint32)).all()
+
+
+def test_knn_graph_known_answer():
+    # graph_tests.jl:4-48
+    _banner("graph known answer")
+    data = np.arange(1, 13, dtype=np.float32).reshape(4, 3).T  # reshape(1:12, 3, 4)
+    index = M.build_index(M.BruteForceIndex, data)
+    g = M.build_knn_graph(index, data, k=2)
+    assert g.neighbors == [[2, 3], [1, 3], [2, 4], [3, 2]]
+    assert g.k == 2 and g.include_self is False
+    gs = M.build_knn_graph(index, data, k=3, include_self=True)
+    for i, nb in enumerate(gs.neighbors, 1):
+        assert len(nb) == 3 and i in nb
+    gu = M.build_knn_graph(index, data, k=2, directed=False)
+    assert gu.neighbors == [[2, 3], [1, 3, 4], [1, 2, 4], [2, 3]]
+    assert gu.k == 3 and gu.metadata["original_k"] == 2 and gu.metadata["directed"] is False
+    for bad in (dict(k=0), dict(k=4), dict(k=2, metadata=["oops"])):
+        with pytest.raises(M.ArgumentError):
+            M.build_knn_graph(index, data, **bad)
+
+
+def test_knn_graph_config2_subset_bit_exact():
+    # 60,000 x 784 clipped 10-cluster mixture, k=20, self-join; rows checked
+    # against the oracle on a sample of points.
+    _banner("graph 60000x784 k=20")
+    X = datasets.clipped_mixture(60_000, 784, seed=2)
+    index = M.build_index(M.BruteForceIndex, X.T)
+    ds = M.Dataset(X.T)
+    adj, dist = M.build_knn_graph(index, ds, k=20, return_arrays=True)
+    st = M.last_stats()
+    print(f"stats={st}", flush=True)
+    rows = np.random.default_rng(0).choice(60_000, 96, replace=False)
+    oi, od = O.bf_knn(X, X[rows], 21)
+    for t, r in enumerate(rows):
+        keep = [(i, v) for i, v in zip(oi[t], od[t]) if i != r + 1][:20]
+        assert adj[r].tolist() == [i for i, _ in keep], r
+        assert dist[r].tolist() == [float(v) for _, v in keep]
+    assert (adj != (np.arange(60_000)[:, None] + 1)).all()   # no self loops
+    assert ((adj >= 1) & (adj <= 60_000)).all()
