@@ -78,6 +78,76 @@ int mann_knn_graph(mann_dataset_t data, int64_t k, int include_self, int metric,
 int mann_knn_graph_symmetrize(const int64_t* adj, int64_t n, int64_t k, int64_t* out_offsets,
                               int64_t* out_ids, int64_t* out_total, int64_t* out_max_degree);
 
+/* assign_clusters!(pairwise_euclidean!(X, C)) -- the k-means E-step
+ * (src/transforms/kmeans/distance.jl:68-107, 287-310): nearest centroid by the
+ * FP32 norms form sqrt(max((|c|^2 + |x|^2) - 2 c.x, 0)), ties to the lower
+ * centroid.  centroids is a d x k matrix; out_assign (n) is 1-based;
+ * out_mindist (optional) receives the distance to the assigned centroid. */
+int mann_kmeans_assign(mann_dataset_t X, const float* centroids, int64_t k, int64_t* out_assign,
+                       float* out_mindist);
+
+/* init_kmeans_plus_plus (src/transforms/kmeans/init.jl:56-128) with draws from
+ * the library's splitmix64 stream of `seed` (Julia's RNG streams cannot be
+ * reproduced): u_0 picks floor(u_0 n); round i picks the first point whose
+ * blocked cumulative D^2 reaches u_i * total.  out_idx (optional) 1-based. */
+int mann_kmeanspp(mann_dataset_t X, int64_t k, uint64_t seed, float* out_centroids,
+                  int64_t* out_idx);
+
+/* fit!(::KMeansTransform, X; rng) -> lloyd! (src/transforms/KMeansTransform.jl:
+ * 106-162, src/transforms/kmeans/lloyd.jl:36-134): init 0 = the given
+ * init_centroids (d x k), 1 = kmeans++ (seed), 2 = random distinct points
+ * (seed).  M-step sums in FP64 over a fixed order; empty clusters take the
+ * farthest point (lloyd.jl:81-104); stops when the max centroid movement < tol.
+ * Errors: MANN_ERR_ASSERTION for k <= 0, k > n, max_iters <= 0, tol <= 0. */
+int mann_kmeans_fit(mann_dataset_t X, int64_t k, int64_t max_iters, double tol, int init,
+                    uint64_t seed, const float* init_centroids, float* out_centroids,
+                    int64_t* out_assign, int64_t* out_iters);
+
+/* transform(::KMeansTransform, x) for a batch (src/transforms/KMeansTransform.jl:
+ * 181-190 -> src/transforms/kmeans/distance.jl:213-242): out_dist[q][c] =
+ * sqrt(max((|c|^2 + |q|^2) - 2 c.q, 0)) in FP32 (A16 order); centroids d x nlist. */
+int mann_route_distances(const float* centroids, int64_t nlist, int64_t d, const float* queries,
+                         int64_t nq, int64_t ldq, float* out_dist);
+
+/* build_index(MultiLevelIndex, X, TransformedConfig(KMeansTransform, TopKRouting,
+ * TerminalConfig(BruteForceIndex))) after the k-means fit
+ * (src/indices/multilevel/builder.jl:61-218, partition_by_transform
+ * src/transforms/utils.jl:75-174): buckets = max(assign); each bucket's points
+ * in ascending id; the base is copied list-contiguously.  assign is 1-based. */
+int mann_ivf_build(mann_dataset_t X, const float* centroids, int64_t nlist,
+                   const int64_t* assign, mann_ivf_t* out);
+int mann_ivf_destroy(mann_ivf_t ivf);
+/* number of buckets and (optional) the size of each */
+int mann_ivf_info(mann_ivf_t ivf, int64_t* n_buckets, int64_t* list_sizes);
+/* the point ids of the lists, concatenated in bucket order (1-based) */
+int mann_ivf_list_ids(mann_ivf_t ivf, int64_t* out_ids);
+/* routing distances of the index's centroids (as mann_route_distances) */
+int mann_ivf_route(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq, int64_t ldq,
+                   float* out_dist);
+/* select_indices(TopKRouting(nprobe), KMeansAssignment(route_dist[q]), buckets)
+ * followed by _resolve_child_positions (src/indices/multilevel/routing.jl:94-118,
+ * query.jl:108-121): 1-based buckets in probe order, empty buckets dropped,
+ * 0-padded to nprobe. */
+int mann_ivf_probe(mann_ivf_t ivf, const float* route_dist, int64_t nq, int64_t nprobe,
+                   int64_t* out_buckets);
+/* query(::MultiLevelIndex, data, Q, k) -- batched (src/indices/multilevel/
+ * query.jl:279-512 with SimpleMerge, merge.jl:79-124): exact top-k over the
+ * probed lists, ties by (distance, bucket, id) as the batched per-child merge
+ * produces.  route_dist (optional, nq x nlist, smaller = probe first) replaces
+ * the centroid distances (a learned router passes its negated scores). */
+int mann_ivf_search(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq, int64_t ldq,
+                    int64_t k, int64_t nprobe, int metric, const float* route_dist,
+                    int64_t* out_ids, float* out_dist);
+
+/* Tuning knobs of the search kernels (process-wide): "scan_cfg" (first-pass
+ * tile: 0 = 128 rows x 128 queries, 1 = 256 x 128, 2 = 128 x 256),
+ * "target_groups" (workgroups the row partitioning aims for), "sample_rank"
+ * (emission threshold = that multiple of k in the row sample), "shortlist_mult"
+ * (first exact re-rank batch = that multiple of k).  No reference counterpart:
+ * the reference has no tunable kernels. */
+int mann_tune_set(const char* key, int64_t value);
+int mann_tune_get(const char* key, int64_t* value);
+
 /* Kernel timing of the calling process (HIP events on the launching stream). */
 int mann_profile_enable(int on);
 int mann_profile_reset(void);

@@ -28,6 +28,21 @@ SIGNATURES = [
     ("mann_bf_knn", ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, ctypes.c_int, _vp, _vp]),
     ("mann_knn_graph", ctypes.c_int, [_vp, _i64, ctypes.c_int, ctypes.c_int, _vp, _vp]),
     ("mann_knn_graph_symmetrize", ctypes.c_int, [_vp, _i64, _i64, _vp, _vp, _i64p, _i64p]),
+    ("mann_kmeans_assign", ctypes.c_int, [_vp, _vp, _i64, _vp, _vp]),
+    ("mann_kmeanspp", ctypes.c_int, [_vp, _i64, ctypes.c_uint64, _vp, _vp]),
+    ("mann_kmeans_fit", ctypes.c_int, [_vp, _i64, _i64, ctypes.c_double, ctypes.c_int,
+                                       ctypes.c_uint64, _vp, _vp, _vp, _i64p]),
+    ("mann_ivf_build", ctypes.c_int, [_vp, _vp, _i64, _vp, ctypes.POINTER(_vp)]),
+    ("mann_ivf_destroy", ctypes.c_int, [_vp]),
+    ("mann_ivf_info", ctypes.c_int, [_vp, _i64p, _vp]),
+    ("mann_ivf_list_ids", ctypes.c_int, [_vp, _vp]),
+    ("mann_ivf_route", ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _vp]),
+    ("mann_ivf_probe", ctypes.c_int, [_vp, _vp, _i64, _i64, _vp]),
+    ("mann_ivf_search", ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, _i64, ctypes.c_int, _vp,
+                                       _vp, _vp]),
+    ("mann_route_distances", ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _i64, _vp]),
+    ("mann_tune_set", ctypes.c_int, [ctypes.c_char_p, _i64]),
+    ("mann_tune_get", ctypes.c_int, [ctypes.c_char_p, _i64p]),
     ("mann_profile_enable", ctypes.c_int, [ctypes.c_int]),
     ("mann_profile_reset", ctypes.c_int, []),
     ("mann_profile_query", ctypes.c_int, [ctypes.c_char_p, _i64p, ctypes.POINTER(ctypes.c_double)]),
@@ -65,9 +80,21 @@ class ArgumentError(ValueError):
 
 
 def lib():
-    """Load libmann_hip.so (built by manifoldann_amd.build); raise if absent."""
+    """Load libmann_hip.so (built by manifoldann_amd.build); raise if absent.
+
+    torch ships its own HIP runtime; when torch is already imported its runtime
+    is initialised first, so both runtimes see the device.
+    """
     global _lib
     if _lib is None:
+        import sys
+        if "torch" in sys.modules:
+            t = sys.modules["torch"]
+            try:
+                if t.cuda.is_available():
+                    t.cuda.init()
+            except Exception:
+                pass
         if not os.path.exists(LIB_PATH):
             raise MannError(f"{LIB_PATH} is missing: run `python -m manifoldann_amd.build` "
                             "(there is no CPU fallback)")

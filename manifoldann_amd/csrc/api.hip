@@ -7,6 +7,7 @@
 
 #include "../../include/mann.h"
 #include "core.h"
+#include "ivf.h"
 
 namespace mann {
 
@@ -196,6 +197,323 @@ int mann_knn_graph_symmetrize(const int64_t* adj, int64_t n, int64_t k, int64_t*
     }
     if (out_total) *out_total = w;
     if (out_max_degree) *out_max_degree = maxd;
+  });
+}
+
+// Upload (if needed) a k x d centroid matrix into a k x d16 device buffer.
+static float* stage_centroids(const float* C, int64_t k, const mann_dataset& ds, const char* tag) {
+  ThreadCtx& c = ctx();
+  hipStream_t st = c.stream;
+  float* dst = c.buf(tag).as<float>((size_t)k * ds.d16);
+  const float* src = C;
+  if (!is_device_ptr(C)) {
+    float* stg = c.buf(std::string(tag) + "_stage").as<float>((size_t)k * ds.d);
+    MANN_HIP(hipMemcpyAsync(stg, C, sizeof(float) * k * ds.d, hipMemcpyHostToDevice, st));
+    src = stg;
+  }
+  copy_rows(src, ds.d, dst, ds.d16, (int)ds.d, k, st);
+  return dst;
+}
+
+// Write k x d16 device centroids into the caller's k x d matrix.
+static void emit_centroids(const float* C, int64_t k, const mann_dataset& ds, float* out) {
+  ThreadCtx& c = ctx();
+  hipStream_t st = c.stream;
+  OutBuf<float> ob(out, (size_t)(k * ds.d), "out_centroids");
+  copy_rows(C, ds.d16, ob.dev, ds.d, (int)ds.d, k, st);
+  ob.finish();
+}
+
+int mann_kmeans_assign(mann_dataset_t X, const float* centroids, int64_t k, int64_t* out_assign,
+                       float* out_mindist) {
+  return guarded([&] {
+    MANN_REQUIRE(X && centroids && out_assign, ERR_NULL, "null pointer");
+    MANN_REQUIRE(k > 0, ERR_ASSERTION, "Number of clusters must be positive");
+    MANN_REQUIRE(!X->normalized, ERR_ARGUMENT, "k-means runs on a Euclidean dataset");
+    hipStream_t st = ctx().stream;
+    const int64_t n = X->n;
+    float* C = stage_centroids(centroids, k, *X, "ka_c");
+    int* assign = ctx().buf("ka_assign").as<int>(n);
+    float* md = ctx().buf("ka_md").as<float>(n);
+    long long ovf = 0;
+    kmeans_estep(*X, C, (int)k, assign, md, false, &ovf);
+    last_stats() = SearchStats();
+    last_stats().overflow = ovf;
+    OutBuf<int64_t> oa(out_assign, (size_t)n, "out_assign");
+    ids_to_i64(assign, n, oa.dev, st);
+    oa.finish();
+    if (out_mindist) {
+      OutBuf<float> om(out_mindist, (size_t)n, "out_mindist");
+      MANN_HIP(hipMemcpyAsync(om.dev, md, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+      om.finish();
+    }
+    MANN_HIP(hipStreamSynchronize(st));
+    timer_flush();
+  });
+}
+
+int mann_kmeanspp(mann_dataset_t X, int64_t k, uint64_t seed, float* out_centroids,
+                  int64_t* out_idx) {
+  return guarded([&] {
+    MANN_REQUIRE(X && out_centroids, ERR_NULL, "null pointer");
+    MANN_REQUIRE(k > 0, ERR_ASSERTION, "Number of clusters must be positive");
+    MANN_REQUIRE(k <= X->n, ERR_ASSERTION,
+                 "Cannot initialize " + std::to_string(k) + " clusters with only " +
+                     std::to_string(X->n) + " points");
+    hipStream_t st = ctx().stream;
+    float* C = ctx().buf("pp_c").as<float>((size_t)k * X->d16);
+    int* picked = ctx().buf("pp_picked").as<int>(k);
+    kmeans_pp(*X, (int)k, seed, C, picked);
+    emit_centroids(C, k, *X, out_centroids);
+    if (out_idx) {
+      OutBuf<int64_t> oi(out_idx, (size_t)k, "out_idx");
+      ids_to_i64(picked, k, oi.dev, st);
+      oi.finish();
+    }
+    MANN_HIP(hipStreamSynchronize(st));
+    timer_flush();
+  });
+}
+
+int mann_kmeans_fit(mann_dataset_t X, int64_t k, int64_t max_iters, double tol, int init,
+                    uint64_t seed, const float* init_centroids, float* out_centroids,
+                    int64_t* out_assign, int64_t* out_iters) {
+  return guarded([&] {
+    MANN_REQUIRE(X && out_centroids, ERR_NULL, "null pointer");
+    // KMeansTransform constructor assertions (KMeansTransform.jl:66-69) and fit! (:110)
+    MANN_REQUIRE(k > 0, ERR_ASSERTION, "Number of clusters must be positive");
+    MANN_REQUIRE(max_iters > 0, ERR_ASSERTION, "max_iters must be positive");
+    MANN_REQUIRE(tol > 0, ERR_ASSERTION, "tol must be positive");
+    MANN_REQUIRE(k <= X->n, ERR_ASSERTION,
+                 "Cannot fit " + std::to_string(k) + " clusters with only " +
+                     std::to_string(X->n) + " points");
+    MANN_REQUIRE(init >= 0 && init <= 2, ERR_ARGUMENT,
+                 "Unknown initialization strategy");
+    MANN_REQUIRE(!X->normalized, ERR_ARGUMENT, "k-means runs on a Euclidean dataset");
+    ThreadCtx& c = ctx();
+    hipStream_t st = c.stream;
+    const int64_t n = X->n;
+    float* C = c.buf("kf_c").as<float>((size_t)k * X->d16);
+    if (init == 0) {
+      MANN_REQUIRE(init_centroids != nullptr, ERR_NULL, "null initial centroids");
+      float* src = stage_centroids(init_centroids, k, *X, "kf_init");
+      MANN_HIP(hipMemcpyAsync(C, src, sizeof(float) * k * X->d16, hipMemcpyDeviceToDevice, st));
+    } else if (init == 1) {
+      int* picked = c.buf("pp_picked").as<int>(k);
+      kmeans_pp(*X, (int)k, seed, C, picked);
+    } else {
+      std::vector<int> idx = random_indices(n, (int)k, seed);
+      int* didx = c.buf("kf_ridx").as<int>(k);
+      MANN_HIP(hipMemcpyAsync(didx, idx.data(), sizeof(int) * k, hipMemcpyHostToDevice, st));
+      gather_rows_f32(X->x32, didx, (int)k, (int)X->d16, C, st);
+    }
+    int* assign = c.buf("kf_assign").as<int>(n);
+    float* md = c.buf("kf_md").as<float>(n);
+    long long ovf = 0;
+    int64_t iters = kmeans_lloyd(*X, C, (int)k, max_iters, tol, assign, md, &ovf);
+    last_stats() = SearchStats();
+    last_stats().overflow = ovf;
+    emit_centroids(C, k, *X, out_centroids);
+    if (out_assign) {
+      OutBuf<int64_t> oa(out_assign, (size_t)n, "out_assign");
+      ids_to_i64(assign, n, oa.dev, st);
+      oa.finish();
+    }
+    if (out_iters) *out_iters = iters;
+    MANN_HIP(hipStreamSynchronize(st));
+    timer_flush();
+  });
+}
+
+int mann_ivf_build(mann_dataset_t X, const float* centroids, int64_t nlist,
+                   const int64_t* assign, mann_ivf_t* out) {
+  return guarded([&] {
+    MANN_REQUIRE(X && centroids && assign && out, ERR_NULL, "null pointer");
+    MANN_REQUIRE(nlist > 0, ERR_ASSERTION, "Number of clusters must be positive");
+    std::vector<int64_t> ha(X->n);
+    if (is_device_ptr(assign)) {
+      MANN_HIP(hipMemcpy(ha.data(), assign, sizeof(int64_t) * X->n, hipMemcpyDeviceToHost));
+    } else {
+      memcpy(ha.data(), assign, sizeof(int64_t) * X->n);
+    }
+    float* C = stage_centroids(centroids, nlist, *X, "ivb_c");
+    *out = ivf_build(*X, C, (int)nlist, ha.data());
+  });
+}
+
+int mann_ivf_destroy(mann_ivf_t ivf) {
+  return guarded([&] { delete ivf; });
+}
+
+int mann_ivf_info(mann_ivf_t ivf, int64_t* n_buckets, int64_t* list_sizes) {
+  return guarded([&] {
+    MANN_REQUIRE(ivf != nullptr, ERR_NULL, "null index");
+    if (n_buckets) *n_buckets = ivf->nb;
+    if (list_sizes)
+      for (int b = 0; b < ivf->nb; ++b) list_sizes[b] = ivf->h_off[b + 1] - ivf->h_off[b];
+  });
+}
+
+int mann_ivf_list_ids(mann_ivf_t ivf, int64_t* out_ids) {
+  return guarded([&] {
+    MANN_REQUIRE(ivf && out_ids, ERR_NULL, "null pointer");
+    std::vector<int> p(ivf->n);
+    MANN_HIP(hipMemcpy(p.data(), ivf->perm, sizeof(int) * ivf->n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < ivf->n; ++i) out_ids[i] = (int64_t)p[i] + 1;
+  });
+}
+
+static QueryOps ivf_queries(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq,
+                            int64_t ldq) {
+  MANN_REQUIRE(d == ivf->d, ERR_DIMENSION,
+               "Expected queries with " + std::to_string(ivf->d) + " rows, got " +
+                   std::to_string(d));
+  MANN_REQUIRE(nq >= 0, ERR_ARGUMENT, "negative query count");
+  MANN_REQUIRE(nq == 0 || (queries && ldq >= d), ERR_ARGUMENT, "bad query matrix");
+  return prepare_queries(queries, d, nq, ldq, ivf->normalized, (int)ivf->d16, (int)ivf->kp,
+                         "ivq");
+}
+
+int mann_ivf_route(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq, int64_t ldq,
+                   float* out_dist) {
+  return guarded([&] {
+    MANN_REQUIRE(ivf && out_dist, ERR_NULL, "null pointer");
+    QueryOps Q = ivf_queries(ivf, queries, d, nq, ldq);
+    if (nq == 0) return;
+    OutBuf<float> od(out_dist, (size_t)(nq * ivf->nlist), "out_route");
+    ivf_route(*ivf, Q, od.dev);
+    od.finish();
+    MANN_HIP(hipStreamSynchronize(ctx().stream));
+    timer_flush();
+  });
+}
+
+int mann_ivf_probe(mann_ivf_t ivf, const float* route_dist, int64_t nq, int64_t nprobe,
+                   int64_t* out_buckets) {
+  return guarded([&] {
+    MANN_REQUIRE(ivf && route_dist && out_buckets, ERR_NULL, "null pointer");
+    MANN_REQUIRE(nprobe > 0, ERR_ARGUMENT, "k must be positive");
+    if (nq == 0) return;
+    ThreadCtx& c = ctx();
+    hipStream_t st = c.stream;
+    const float* R = route_dist;
+    if (!is_device_ptr(route_dist)) {
+      float* rb = c.buf("pr_R").as<float>((size_t)nq * ivf->nlist);
+      MANN_HIP(hipMemcpyAsync(rb, route_dist, sizeof(float) * nq * ivf->nlist,
+                              hipMemcpyHostToDevice, st));
+      R = rb;
+    }
+    int* probes = c.buf("pr_probes").as<int>((size_t)nq * nprobe);
+    int* pcount = c.buf("pr_pcount").as<int>(nq);
+    int* prows = c.buf("pr_prows").as<int>(nq);
+    ivf_probe(*ivf, R, (int)nq, (int)nprobe, probes, pcount, prows);
+    std::vector<int> hp((size_t)nq * nprobe);
+    MANN_HIP(hipMemcpyAsync(hp.data(), probes, sizeof(int) * hp.size(), hipMemcpyDeviceToHost, st));
+    MANN_HIP(hipStreamSynchronize(st));
+    std::vector<int64_t> tmp(hp.size());
+    for (size_t i = 0; i < hp.size(); ++i) tmp[i] = hp[i] + 1;  // -1 -> 0 (empty slot)
+    if (is_device_ptr(out_buckets))
+      MANN_HIP(hipMemcpy(out_buckets, tmp.data(), sizeof(int64_t) * tmp.size(),
+                         hipMemcpyHostToDevice));
+    else
+      memcpy(out_buckets, tmp.data(), sizeof(int64_t) * tmp.size());
+  });
+}
+
+int mann_ivf_search(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq, int64_t ldq,
+                    int64_t k, int64_t nprobe, int metric, const float* route_dist,
+                    int64_t* out_ids, float* out_dist) {
+  return guarded([&] {
+    MANN_REQUIRE(ivf != nullptr, ERR_NULL, "null index");
+    check_metric(metric);
+    MANN_REQUIRE((metric == COSINE) == (ivf->normalized != 0), ERR_ARGUMENT,
+                 "index was built for another metric");
+    MANN_REQUIRE(nprobe > 0, ERR_ARGUMENT, "k must be positive");
+    QueryOps Q = ivf_queries(ivf, queries, d, nq, ldq);
+    if (nq == 0 || k <= 0) {
+      last_stats() = SearchStats();
+      return;
+    }
+    MANN_REQUIRE(out_ids && out_dist, ERR_NULL, "null pointer");
+    ThreadCtx& c = ctx();
+    const float* R = route_dist;
+    if (R && !is_device_ptr(R)) {
+      float* rb = c.buf("sr_R").as<float>((size_t)nq * ivf->nlist);
+      MANN_HIP(hipMemcpyAsync(rb, route_dist, sizeof(float) * nq * ivf->nlist,
+                              hipMemcpyHostToDevice, c.stream));
+      R = rb;
+    }
+    OutBuf<int64_t> oi(out_ids, (size_t)(nq * k), "out_ids");
+    OutBuf<float> od(out_dist, (size_t)(nq * k), "out_dist");
+    ivf_search(*ivf, Q, (int)k, (int)nprobe, metric, R, oi.dev, od.dev);
+    oi.finish();
+    od.finish();
+    timer_flush();
+  });
+}
+
+int mann_route_distances(const float* centroids, int64_t nlist, int64_t d, const float* queries,
+                         int64_t nq, int64_t ldq, float* out_dist) {
+  return guarded([&] {
+    MANN_REQUIRE(centroids && out_dist, ERR_NULL, "null pointer");
+    MANN_REQUIRE(nlist > 0 && d > 0, ERR_ARGUMENT, "empty centroid matrix");
+    if (nq == 0) return;
+    MANN_REQUIRE(queries && ldq >= d, ERR_ARGUMENT, "bad query matrix");
+    ThreadCtx& c = ctx();
+    hipStream_t st = c.stream;
+    const int64_t d16 = round_up(d, 16), kp = round_up(d, 64);
+    QueryOps Q = prepare_queries(queries, d, nq, ldq, 0, (int)d16, (int)kp, "rdq");
+    float* C = c.buf("rd_c").as<float>((size_t)nlist * d16);
+    const float* src = centroids;
+    if (!is_device_ptr(centroids)) {
+      float* stg = c.buf("rd_c_stage").as<float>((size_t)nlist * d);
+      MANN_HIP(hipMemcpyAsync(stg, centroids, sizeof(float) * nlist * d, hipMemcpyHostToDevice,
+                              st));
+      src = stg;
+    }
+    copy_rows(src, d, C, d16, (int)d, nlist, st);
+    float* cn2 = c.buf("rd_cn2").as<float>(nlist);
+    launch_a16_norms(C, nlist, (int)d16, cn2, st);
+    OutBuf<float> od(out_dist, (size_t)(nq * nlist), "out_route");
+    launch_route_dist(Q.q32, Q.qn2, (int)nq, C, cn2, (int)nlist, (int)d16, od.dev, st);
+    od.finish();
+    MANN_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int mann_tune_set(const char* key, int64_t value) {
+  return guarded([&] {
+    MANN_REQUIRE(key != nullptr, ERR_NULL, "null key");
+    std::string k(key);
+    Tuning& t = tuning();
+    if (k == "scan_cfg") {
+      MANN_REQUIRE(value >= 0 && value <= 2, ERR_ARGUMENT, "scan_cfg must be 0, 1 or 2");
+      t.scan_cfg = (int)value;
+    } else if (k == "target_groups") {
+      MANN_REQUIRE(value >= 1, ERR_ARGUMENT, "target_groups must be positive");
+      t.target_groups = (int)value;
+    } else if (k == "sample_rank") {
+      MANN_REQUIRE(value >= 1 && value <= 64, ERR_ARGUMENT, "sample_rank must be in 1..64");
+      t.sample_rank = (int)value;
+    } else if (k == "shortlist_mult") {
+      MANN_REQUIRE(value >= 1 && value <= 16, ERR_ARGUMENT, "shortlist_mult must be in 1..16");
+      t.shortlist_mult = (int)value;
+    } else {
+      throw Error{ERR_ARGUMENT, "unknown tuning key " + k};
+    }
+  });
+}
+
+int mann_tune_get(const char* key, int64_t* value) {
+  return guarded([&] {
+    MANN_REQUIRE(key && value, ERR_NULL, "null pointer");
+    std::string k(key);
+    const Tuning& t = tuning();
+    if (k == "scan_cfg") *value = t.scan_cfg;
+    else if (k == "target_groups") *value = t.target_groups;
+    else if (k == "sample_rank") *value = t.sample_rank;
+    else if (k == "shortlist_mult") *value = t.shortlist_mult;
+    else throw Error{ERR_ARGUMENT, "unknown tuning key " + k};
   });
 }
 

@@ -1,0 +1,68 @@
+// Exact FP32 metrics (A16 order) and certification bounds shared by the
+// selection kernels.
+#pragma once
+
+#include "launch.h"
+
+namespace mann {
+
+// Smallest a' threshold T such that every row with a' > T has an FP32 exact
+// metric strictly above dk.  Ingredients: |a' + |q~|^2 - |q~ - x~|^2| <= G
+// (FP32 accumulation of bf16 products), |x - x~| <= ex and |q - q~| <= qe
+// (bf16 rounding, triangle inequality), relative error rho of the FP32
+// difference form, absolute error of the cosine form, error H of the norms
+// form.  metric: EUCLIDEAN / SQEUCLIDEAN / COSINE / NORMS_FORM.
+__device__ inline double exclusion_threshold(double dk, int metric, double qn, double qe,
+                                             double qr, double ex, double rx, int d) {
+  const double u = 1.0 / 16777216.0;
+  double R = qr + rx;
+  double G = (2.0 * d + 8.0) * u * R * R;
+  double rho = (d + 8.0) * u;
+  double L;
+  if (metric == EUCLIDEAN) {
+    L = dk / (1.0 - rho) + qe + ex;
+  } else if (metric == SQEUCLIDEAN) {
+    L = sqrt(dk / (1.0 - rho)) + qe + ex;
+  } else if (metric == COSINE) {
+    double delta = (2.0 * d + 16.0) * u;
+    L = sqrt(2.0 * (dk + delta)) + qe + ex;
+  } else {
+    double R32 = R + qe + ex;
+    double H = (d + 8.0) * u * R32 * R32;
+    double f = dk / (1.0 - 8.0 * u);
+    L = sqrt(f * f + H) + qe + ex;
+  }
+  return (L * L - qn + G) * (1.0 + 1e-9) + 1e-30;
+}
+
+// Exact FP32 metric of base row r against query q (16-lane group, A16 order).
+__device__ __forceinline__ float exact_metric(const float* __restrict__ x32,
+                                              const float* __restrict__ q32,
+                                              const float* __restrict__ xn2,
+                                              const float* __restrict__ qn2, int d16, int metric,
+                                              int64_t r, int64_t q, int j) {
+  const float* x = x32 + r * d16;
+  const float* y = q32 + q * d16;
+  float acc = 0.0f;
+  if (metric == EUCLIDEAN || metric == SQEUCLIDEAN) {
+    for (int t = 0; t < d16; t += 16) {
+      float df = x[t + j] - y[t + j];
+      acc = __builtin_fmaf(df, df, acc);
+    }
+  } else {
+    for (int t = 0; t < d16; t += 16) acc = __builtin_fmaf(x[t + j], y[t + j], acc);
+  }
+  acc = group16_tree(acc);
+  if (metric == EUCLIDEAN) return __builtin_sqrtf(acc);
+  if (metric == SQEUCLIDEAN) return acc;
+  if (metric == COSINE) {
+    float v = 1.0f - acc / (__builtin_sqrtf(xn2[r]) * __builtin_sqrtf(qn2[q]));
+    return v > 0.0f ? v : 0.0f;
+  }
+  // norms form (kmeans/distance.jl:99-101): sqrt(max((|c|^2 + |x|^2) - 2 c.x, 0))
+  float tt = xn2[r] + qn2[q];
+  float sq = tt - 2.0f * acc;
+  return __builtin_sqrtf(sq > 0.0f ? sq : 0.0f);
+}
+
+}  // namespace mann

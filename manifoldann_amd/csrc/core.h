@@ -1,6 +1,8 @@
 // Device-resident operands and the search pipelines shared by the C ABI.
 #pragma once
 
+#include <vector>
+
 #include "runtime.h"
 #include "util.h"
 
@@ -20,6 +22,32 @@ struct mann_dataset {
   mann::bf16_t* sb = nullptr;
   float* sxn = nullptr;
   ~mann_dataset();
+};
+
+// IVF index handle: list-contiguous copy of the base plus routing operands.
+struct mann_ivf {
+  int device = 0;
+  int64_t d = 0, n = 0, d16 = 0, kp = 0, n_pad = 0;
+  int nlist = 0;   // routing distances per query (centroids)
+  int nb = 0;      // buckets = max assignment (partition_by_transform)
+  int normalized = 0;
+  float* c32 = nullptr;
+  float* cn2 = nullptr;
+  int* list_off = nullptr;  // nb + 1
+  int* perm = nullptr;      // reordered row -> point id (0-based)
+  int* lorder = nullptr;    // lists by size, largest first
+  std::vector<int> h_off;
+  float* x32 = nullptr;
+  mann::bf16_t* xb = nullptr;
+  float* xn = nullptr;
+  float* xn2 = nullptr;
+  float ex = 0.0f, rx = 0.0f;
+  ~mann_ivf() {
+    (void)hipSetDevice(device);
+    void* ps[] = {c32, cn2, list_off, perm, lorder, x32, xb, xn, xn2};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+  }
 };
 
 namespace mann {
@@ -52,5 +80,21 @@ SearchStats& last_stats();
 // nq x k.  exclude_self drops the row whose id equals the query index.
 void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int exclude_self,
                int64_t* out_ids, float* out_dist, int* out_count);
+
+// k-means (core_kmeans.hip).  C is k x d16 device FP32; assign 0-based.
+void kmeans_estep(const mann_dataset& ds, const float* C, int k, int* assign, float* mindist,
+                  bool primed, long long* overflow_out);
+int64_t kmeans_lloyd(const mann_dataset& ds, float* C, int k, int64_t max_iters, double tol,
+                     int* assign, float* mindist, long long* overflow);
+void kmeans_pp(const mann_dataset& ds, int k, uint64_t seed, float* C, int* picked);
+std::vector<int> random_indices(int64_t n, int k, uint64_t seed);
+
+// IVF (core_ivf.hip)
+mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int64_t* assign_host);
+void ivf_route(const mann_ivf& iv, const QueryOps& Q, float* R);
+void ivf_probe(const mann_ivf& iv, const float* R, int nq, int nprobe, int* probes, int* pcount,
+               int* prows);
+void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int metric,
+                const float* route_in, int64_t* out_ids, float* out_dist);
 
 }  // namespace mann

@@ -75,6 +75,11 @@ bool is_device_ptr(const void* p) {
   return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
 }
 
+Tuning& tuning() {
+  static Tuning t;
+  return t;
+}
+
 SearchStats& last_stats() {
   thread_local SearchStats s;
   return s;
@@ -216,22 +221,29 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   const int n = (int)ds.n;
   const int k_req = k + (exclude_self ? 1 : 0);
   const int k_eff = std::min(k_req, n);
-  const int cap = select_cap_max();
-  MANN_REQUIRE(k_req <= cap / 2, ERR_UNSUPPORTED, "k too large for the GPU selection buffer");
+  const int cap_total = select_cap_max();
+  MANN_REQUIRE(k_req <= cap_total / 2, ERR_UNSUPPORTED, "k too large for the GPU selection buffer");
   const int BM = scan_tile_rows(), BN = scan_tile_queries();
+  // row partitions: enough workgroups to fill the chip, at most 32 regions
+  const int qtiles = (int)ceil_div(nq, BN);
+  const int row_tiles = (int)ceil_div(n, BM);
+  int parts =
+      std::max(1, std::min(std::min(row_tiles, 32), (int)ceil_div(tuning().target_groups, qtiles)));
+  int rpp = (int)round_up(ceil_div(n, parts), BM);
+  parts = (int)ceil_div(n, rpp);
+  const int cap = cap_total / parts;
 
-  unsigned* cand = c.buf("bf_cand").as<unsigned>((size_t)nq * cap * 2);
-  int* cnt = c.buf("bf_cnt").as<int>(nq);
+  unsigned* cand = c.buf("bf_cand").as<unsigned>((size_t)nq * parts * cap * 2);
+  int* cnt = c.buf("bf_cnt").as<int>((size_t)nq * parts);
   float* tau = c.buf("bf_tau").as<float>(nq);
   float* tau2 = c.buf("bf_tau2").as<float>(nq);
   int* flags = c.buf("bf_flags").as<int>(8);   // [0] pass1 count, [1] pass2 count, [2..] stats
   int* list1 = c.buf("bf_list1").as<int>(nq);
   int* list2 = c.buf("bf_list2").as<int>(nq);
   fill_i32(flags, 8, 0, st);
-  fill_i32(cnt, nq, 0, st);
 
-  // ---- emission threshold per query
-  if (n <= cap) {
+  // ---- emission threshold per query (every row fits its region: emit all)
+  if (rpp <= cap) {
     fill_f32(tau, nq, INFINITY, st);
   } else {
     float* dense = c.buf("bf_dense").as<float>((size_t)nq * ds.m);
@@ -249,19 +261,14 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
     a.ldd = ds.m;
     {
       ScopedTime tm("scan_dense", st);
-      launch_scan(a, SCAN_DENSE, (int)ceil_div(nq, BN), false, st);
+      launch_scan(a, SCAN_DENSE, (int)ceil_div(nq, BN), SCAN_KIND_GRID, st);
     }
-    TauArgs ta{dense, (int64_t)ds.m, ds.m, std::min(ds.m, k_req), tau, nq};
+    TauArgs ta{dense, (int64_t)ds.m, ds.m, std::min(ds.m, tuning().sample_rank * k_req), tau, nq};
     ScopedTime tm("tau_sample", st);
     launch_tau_from_sample(ta, st);
   }
 
   // ---- filtered first pass over all rows
-  const int qtiles = (int)ceil_div(nq, BN);
-  const int row_tiles = (int)ceil_div(n, BM);
-  int parts = std::max(1, std::min(row_tiles, (int)ceil_div(2048, qtiles)));
-  int rpp = (int)round_up(ceil_div(n, parts), BM);
-  parts = (int)ceil_div(n, rpp);
   ScanArgs a{};
   a.xb = ds.xb;
   a.xn = ds.xn;
@@ -278,7 +285,7 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   a.cap = cap;
   {
     ScopedTime tm("scan_filter", st);
-    launch_scan(a, SCAN_FILTER, qtiles * parts, false, st);
+    launch_scan(a, SCAN_FILTER, qtiles * parts, SCAN_KIND_GRID, st);
   }
 
   SelectArgs s{};
@@ -286,6 +293,7 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   s.cand = cand;
   s.cnt = cnt;
   s.cap = cap;
+  s.parts = parts;
   s.tau = tau;
   s.x32 = ds.x32;
   s.q32 = Q.q32;
@@ -303,7 +311,7 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   s.k_req = k_req;
   s.exclude_self = exclude_self;
   s.k_eff = k_eff;
-  s.shortlist = std::min(cap, std::max(2 * k_req, 32));
+  s.shortlist = std::min(cap_total, std::max(tuning().shortlist_mult * k_req, 32));
   s.out_ids = out_ids;
   s.out_dist = out_dist;
   s.out_count = out_count;
@@ -317,13 +325,12 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   }
 
   // ---- second pass for queries that failed certification (device-side count)
-  reset_listed(cnt, list1, flags, nq, st);
   a.qidx = list1;
   a.nq_dev = flags;
   a.tau = tau2;
   {
     ScopedTime tm("scan_filter_pass2", st);
-    launch_scan(a, SCAN_FILTER, qtiles * parts, false, st);
+    launch_scan(a, SCAN_FILTER, qtiles * parts, SCAN_KIND_REFINE, st);
   }
   s.qlist = list1;
   s.nq_dev = flags;

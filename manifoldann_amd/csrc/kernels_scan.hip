@@ -6,19 +6,27 @@
 // a' + |q~|^2 is the squared distance of the bf16 vectors) with operands staged
 // through double-buffered LDS, then runs the epilogue of its mode on the
 // accumulators in registers:
-//   FILTER : emit (a', row) to the query's candidate buffer when a' < tau[q]
+//   FILTER : emit (a', row) when a' < tau[q] into the query's private region
+//            of this item (no global atomics: counters live in LDS)
 //   DENSE  : store every a' (threshold estimation on a row sample)
 //   TOP1   : FILTER whose per-lane threshold tightens to the certification band
 //            of the smallest a' seen so far (k-means assignment)
+//   GATHER : FILTER for work items that share queries (IVF lists): emissions go
+//            to an LDS buffer per query and reach the query's global buffer
+//            with one atomic per (item, query) at the end of the item
 // Only candidates cross HBM; the BM x BN block never leaves registers.
+//
+// Layout: 4 waves as 2 (rows) x 2 (queries); a wave owns 32*WM rows x 32*WN
+// queries, i.e. WM x WN MFMA blocks.  MFMA 32x32 output: lane l holds query
+// column l%32 and rows 8*(e/4) + 4*(l/32) + e%4 of register e.
 #include "launch.h"
 
 namespace mann {
 
 namespace {
 
-constexpr int BK = 64;    // K chunk staged per LDS buffer
-constexpr int LPAD = 8;   // bf16 padding per LDS row (kills ds_read_b128 bank conflicts)
+constexpr int BK = 32;    // K chunk staged per LDS buffer
+constexpr int LPAD = 8;   // bf16 padding per LDS row (conflict-free ds_read_b128)
 
 // Exclusion threshold (a' domain) for the smallest norms-form distance, given
 // the first-pass value v of an emitted candidate: any centroid whose a' is at
@@ -40,21 +48,28 @@ __device__ float top1_threshold(float v, float qn, float qe, float qr, float ex,
   return (float)t * (1.0f + 1e-6f);
 }
 
-template <int WM, int WN, int MODE>
-struct ScanTile {
+template <int WM, int WN>
+struct Tile {
   static constexpr int BM = 64 * WM;
   static constexpr int BN = 64 * WN;
-  static constexpr int A_CHUNKS = BM * (BK / 8) / 256;   // 16B chunks per thread
+  static constexpr int A_CHUNKS = BM * (BK / 8) / 256;  // 16B chunks per thread
   static constexpr int B_CHUNKS = BN * (BK / 8) / 256;
+  static constexpr int LROW = BK + LPAD;
+  static constexpr size_t LDS_BYTES = (size_t)2 * (BM + BN) * LROW * sizeof(bf16_t) + BN * 4;
+  // GATHER mode: per query column a small LDS candidate buffer
+  static constexpr int LCAP = BN > 128 ? 16 : 32;
+  static constexpr size_t LDS_GATHER = LDS_BYTES + (size_t)BN * LCAP * 8;
 };
 
 template <int WM, int WN, int MODE>
 __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, int row_begin,
-                                          int row_end, bf16_t* lds) {
-  using T = ScanTile<WM, WN, MODE>;
-  constexpr int BM = T::BM, BN = T::BN, LROW = BK + LPAD;
-  bf16_t* As = lds;                       // [2][BM][LROW]
-  bf16_t* Bs = lds + 2 * BM * LROW;       // [2][BN][LROW]
+                                          int row_end, int region, unsigned char* smem) {
+  using T = Tile<WM, WN>;
+  constexpr int BM = T::BM, BN = T::BN, LROW = T::LROW;
+  bf16_t* As = (bf16_t*)smem;                   // [2][BM][LROW]
+  bf16_t* Bs = As + 2 * BM * LROW;              // [2][BN][LROW]
+  int* lcnt = (int*)(Bs + 2 * BN * LROW);       // [BN] emitted count per query column
+  uint2* lbuf = (uint2*)(lcnt + BN);            // [BN][LCAP] (GATHER mode)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave & 1, wc = wave >> 1;
   const int kp = a.kp, nk = kp / BK;
@@ -62,23 +77,21 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
   if (ntiles <= 0 || nqi <= 0) return;
   const int S = ntiles * nk;
 
-  // query rows of the B tile (global query ids)
   const int* qidx = a.qidx;
   int brow[T::B_CHUNKS];
 #pragma unroll
   for (int p = 0; p < T::B_CHUNKS; ++p) {
-    int c = (tid + 256 * p) >> 3;
-    int gq = c < nqi ? (qidx ? qidx[qoff + c] : qoff + c) : (qidx ? qidx[qoff] : qoff);
-    brow[p] = gq;
+    int c = (tid + 256 * p) >> 2;
+    brow[p] = c < nqi ? (qidx ? qidx[qoff + c] : qoff + c) : (qidx ? qidx[qoff] : qoff);
   }
-  // per-lane query columns and thresholds
-  int gqc[WN];
+  int gqc[WN], qcl[WN];
   float tau_l[WN];
 #pragma unroll
   for (int bn = 0; bn < WN; ++bn) {
     int c = wc * 32 * WN + bn * 32 + (lane & 31);
     int gq = c < nqi ? (qidx ? qidx[qoff + c] : qoff + c) : -1;
     gqc[bn] = gq;
+    qcl[bn] = c;
     tau_l[bn] = (MODE != SCAN_DENSE && gq >= 0) ? a.tau[gq] : -INFINITY;
   }
 
@@ -89,13 +102,13 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
 #pragma unroll
     for (int p = 0; p < T::A_CHUNKS; ++p) {
       int c = tid + 256 * p;
-      int r = c >> 3, cc = (c & 7) * 8;
+      int r = c >> 2, cc = (c & 3) * 8;
       ra[p] = *(const intx4*)(a.xb + (int64_t)(r0 + r) * kp + kc * BK + cc);
     }
 #pragma unroll
     for (int p = 0; p < T::B_CHUNKS; ++p) {
       int c = tid + 256 * p;
-      int cc = (c & 7) * 8;
+      int cc = (c & 3) * 8;
       rb[p] = *(const intx4*)(a.qb + (int64_t)brow[p] * kp + kc * BK + cc);
     }
   };
@@ -103,13 +116,13 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
 #pragma unroll
     for (int p = 0; p < T::A_CHUNKS; ++p) {
       int c = tid + 256 * p;
-      int r = c >> 3, cc = (c & 7) * 8;
+      int r = c >> 2, cc = (c & 3) * 8;
       *(intx4*)(As + (buf * BM + r) * LROW + cc) = ra[p];
     }
 #pragma unroll
     for (int p = 0; p < T::B_CHUNKS; ++p) {
       int c = tid + 256 * p;
-      int r = c >> 3, cc = (c & 7) * 8;
+      int r = c >> 2, cc = (c & 3) * 8;
       *(intx4*)(Bs + (buf * BN + r) * LROW + cc) = rb[p];
     }
   };
@@ -122,7 +135,8 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
 
-  __syncthreads();  // LDS may still be read by a previous item of this workgroup
+  __syncthreads();  // LDS may still be in use by a previous item of this workgroup
+  for (int c = tid; c < BN; c += 256) lcnt[c] = 0;
   gload(0);
   lstore(0);
   __syncthreads();
@@ -152,72 +166,119 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
     __syncthreads();
 
     const int t = s / nk;
-    if (s - t * nk == nk - 1) {
-      // ---------------- epilogue for row tile t ----------------
-      const int r0 = row_begin + t * BM;
+    if (s - t * nk != nk - 1) continue;
+    // ---------------- epilogue for row tile t ----------------
+    const int r0 = row_begin + t * BM;
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+      unsigned m[WM];
+      int total = 0;
 #pragma unroll
       for (int i = 0; i < WM; ++i) {
         const int rb0 = r0 + wr * 32 * WM + i * 32 + 4 * (lane >> 5);
-        float nxv[16];
+        m[i] = 0;
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
+        for (int e = 0; e < 16; ++e) {
+          const int row = rb0 + 8 * (e >> 2) + (e & 3);
+          const float nx = row < row_end ? a.xn[row] : INFINITY;
+          const float v = __builtin_fmaf(-2.0f, acc[i][j][e], nx);
+          acc[i][j][e] = v;
+          m[i] |= (v < tau_l[j] ? 1u : 0u) << e;
+        }
+        total += __builtin_popcount(m[i]);
+      }
+      if (MODE == SCAN_DENSE) {
+        if (gqc[j] >= 0) {
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            int row = rb0 + 8 * g + e;
-            nxv[4 * g + e] = row < row_end ? a.xn[row] : INFINITY;
+          for (int i = 0; i < WM; ++i) {
+            const int rb0 = r0 + wr * 32 * WM + i * 32 + 4 * (lane >> 5);
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+              const int row = rb0 + 8 * (e >> 2) + (e & 3);
+              if (row < row_end) a.dense[(int64_t)gqc[j] * a.ldd + (row - row_begin)] = acc[i][j][e];
+            }
           }
         }
+      } else if (total > 0) {
+        const int gq = gqc[j];
+        int pos = atomicAdd(&lcnt[qcl[j]], total);
+        unsigned* reg = a.cand + ((int64_t)gq * a.parts + region) * (int64_t)a.cap * 2;
 #pragma unroll
-        for (int j = 0; j < WN; ++j) {
-#pragma unroll
-          for (int e = 0; e < 16; ++e) {
+        for (int i = 0; i < WM; ++i) {
+          const int rb0 = r0 + wr * 32 * WM + i * 32 + 4 * (lane >> 5);
+          unsigned mm = m[i];
+          while (mm) {
+            const int e = __builtin_ctz(mm);
+            mm &= mm - 1;
             const int row = rb0 + 8 * (e >> 2) + (e & 3);
-            const float v = __builtin_fmaf(-2.0f, acc[i][j][e], nxv[e]);
-            if (MODE == SCAN_DENSE) {
-              if (gqc[j] >= 0 && row < row_end)
-                a.dense[(int64_t)gqc[j] * a.ldd + (row - row_begin)] = v;
-            } else if (v < tau_l[j]) {
-              const int gq = gqc[j];
-              int pos = atomicAdd(a.cnt + gq, 1);
-              if (pos < a.cap) {
-                uint2 c;
-                c.x = __float_as_uint(v);
-                c.y = (unsigned)row;
-                *(uint2*)(a.cand + ((int64_t)gq * a.cap + pos) * 2) = c;
+            float v = 0.0f;
+#pragma unroll
+            for (int ee = 0; ee < 16; ++ee) v = ee == e ? acc[i][j][ee] : v;
+            uint2 cv;
+            cv.x = __float_as_uint(v);
+            cv.y = (unsigned)row;
+            if (MODE == SCAN_GATHER) {
+              if (pos < T::LCAP) {
+                lbuf[qcl[j] * T::LCAP + pos] = cv;
+              } else {  // spill straight to the query's global buffer
+                int gp = atomicAdd(a.cnt + gq, 1);
+                if (gp < a.cap) *(uint2*)(a.cand + ((int64_t)gq * a.cap + gp) * 2) = cv;
               }
-              if (MODE == SCAN_TOP1) {
-                float th = top1_threshold(v, a.t1_qn[gq], a.t1_qe[gq], a.t1_qr[gq], a.t1_ex,
-                                          a.t1_rx, a.d);
-                tau_l[j] = fminf(tau_l[j], th);
-              }
+            } else if (pos < a.cap) {
+              *(uint2*)(reg + 2 * pos) = cv;
             }
-            acc[i][j][e] = 0.0f;
+            ++pos;
+            if (MODE == SCAN_TOP1) {
+              float th = top1_threshold(v, a.t1_qn[gq], a.t1_qe[gq], a.t1_qr[gq], a.t1_ex,
+                                        a.t1_rx, a.d);
+              tau_l[j] = fminf(tau_l[j], th);
+            }
           }
         }
       }
+#pragma unroll
+      for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
+    }
+  }
+  if (MODE == SCAN_GATHER) {
+    __syncthreads();
+    for (int c = tid; c < nqi; c += 256) {
+      const int n = min(lcnt[c], T::LCAP);
+      if (n == 0) continue;
+      const int gq = qidx ? qidx[qoff + c] : qoff + c;
+      const int base = atomicAdd(a.cnt + gq, n);
+      unsigned* dst = a.cand + (int64_t)gq * a.cap * 2;
+      for (int i = 0; i < n && base + i < a.cap; ++i)
+        *(uint2*)(dst + 2 * (base + i)) = lbuf[c * T::LCAP + i];
+    }
+  } else if (MODE != SCAN_DENSE) {
+    __syncthreads();
+    for (int c = tid; c < nqi; c += 256) {
+      int gq = qidx ? qidx[qoff + c] : qoff + c;
+      a.cnt[(int64_t)gq * a.parts + region] = lcnt[c];
     }
   }
 }
 
-template <int WM, int WN, int MODE>
+template <int WM, int WN, int MODE, int REFINE>
 __global__ __launch_bounds__(256) void scan_grid_kernel(ScanArgs a) {
-  using T = ScanTile<WM, WN, MODE>;
+  using T = Tile<WM, WN>;
   extern __shared__ __align__(16) unsigned char smem[];
-  bf16_t* lds = (bf16_t*)smem;
-  int nq = a.nq_dev ? *a.nq_dev : a.nq;
+  int nq = REFINE ? *a.nq_dev : a.nq;
   int qtile = blockIdx.x / a.parts, part = blockIdx.x - qtile * a.parts;
   int qoff = qtile * T::BN;
   if (qoff >= nq) return;
   int nqi = min(T::BN, nq - qoff);
   int rb = part * a.rows_per_part;
   int re = min(a.n_rows, rb + a.rows_per_part);
-  scan_item<WM, WN, MODE>(a, qoff, nqi, rb, re, lds);
+  scan_item<WM, WN, MODE>(a, qoff, nqi, rb, re, part, smem);
 }
 
 template <int WM, int WN, int MODE>
 __global__ __launch_bounds__(256) void scan_items_kernel(ScanArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  bf16_t* lds = (bf16_t*)smem;
   __shared__ int s_item;
   const int n_items = *a.n_items_dev;
   for (;;) {
@@ -227,40 +288,61 @@ __global__ __launch_bounds__(256) void scan_items_kernel(ScanArgs a) {
     __syncthreads();
     if (it >= n_items) break;
     const ScanItem w = a.items[it];
-    scan_item<WM, WN, MODE>(a, w.qoff, w.nq, w.row_begin, w.row_end, lds);
+    scan_item<WM, WN, MODE>(a, w.qoff, w.nq, w.row_begin, w.row_end, w.region, smem);
   }
 }
 
+template <class K>
+void set_lds(K kern, size_t lds) {
+  MANN_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
+}
+
 template <int WM, int WN, int MODE>
-void launch_t(const ScanArgs& a, int grid, bool persistent, hipStream_t st) {
-  using T = ScanTile<WM, WN, MODE>;
-  size_t lds = (size_t)2 * (T::BM + T::BN) * (BK + LPAD) * sizeof(bf16_t);
+void launch_t(const ScanArgs& a, int grid, int kind, hipStream_t st) {
+  using T = Tile<WM, WN>;
+  const size_t lds = MODE == SCAN_GATHER ? T::LDS_GATHER : T::LDS_BYTES;
   static bool attr_set = false;  // dynamic LDS above 64 KiB must be opted into
   if (!attr_set) {
-    MANN_HIP(hipFuncSetAttribute((const void*)scan_items_kernel<WM, WN, MODE>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    MANN_HIP(hipFuncSetAttribute((const void*)scan_grid_kernel<WM, WN, MODE>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    set_lds(scan_items_kernel<WM, WN, MODE>, lds);
+    set_lds(scan_grid_kernel<WM, WN, MODE, 0>, lds);
+    set_lds(scan_grid_kernel<WM, WN, MODE, 1>, lds);
     attr_set = true;
   }
-  if (persistent)
+  if (kind == SCAN_KIND_ITEMS)
     hipLaunchKernelGGL((scan_items_kernel<WM, WN, MODE>), dim3(grid), dim3(256), lds, st, a);
+  else if (kind == SCAN_KIND_REFINE)
+    hipLaunchKernelGGL((scan_grid_kernel<WM, WN, MODE, 1>), dim3(grid), dim3(256), lds, st, a);
   else
-    hipLaunchKernelGGL((scan_grid_kernel<WM, WN, MODE>), dim3(grid), dim3(256), lds, st, a);
+    hipLaunchKernelGGL((scan_grid_kernel<WM, WN, MODE, 0>), dim3(grid), dim3(256), lds, st, a);
 }
 
 }  // namespace
 
-int scan_tile_rows() { return ScanTile<2, 2, 0>::BM; }
-int scan_tile_queries() { return ScanTile<2, 2, 0>::BN; }
+// Tile configurations (wave tile = 32*WM rows x 32*WN queries).
+static const int kCfgWM[3] = {2, 4, 2};
+static const int kCfgWN[3] = {2, 2, 4};
 
-void launch_scan(const ScanArgs& a, int mode, int grid, bool persistent, hipStream_t st) {
-  if (grid <= 0) return;
+int scan_tile_rows() { return 64 * kCfgWM[tuning().scan_cfg]; }
+int scan_tile_queries() { return 64 * kCfgWN[tuning().scan_cfg]; }
+
+template <int WM, int WN>
+static void launch_cfg(const ScanArgs& a, int mode, int grid, int kind, hipStream_t st) {
   switch (mode) {
-    case SCAN_FILTER: launch_t<2, 2, SCAN_FILTER>(a, grid, persistent, st); break;
-    case SCAN_DENSE: launch_t<2, 2, SCAN_DENSE>(a, grid, persistent, st); break;
-    case SCAN_TOP1: launch_t<2, 2, SCAN_TOP1>(a, grid, persistent, st); break;
+    case SCAN_FILTER: launch_t<WM, WN, SCAN_FILTER>(a, grid, kind, st); break;
+    case SCAN_DENSE: launch_t<WM, WN, SCAN_DENSE>(a, grid, kind, st); break;
+    case SCAN_TOP1: launch_t<WM, WN, SCAN_TOP1>(a, grid, kind, st); break;
+    case SCAN_GATHER: launch_t<WM, WN, SCAN_GATHER>(a, grid, kind, st); break;
     default: break;
+  }
+}
+
+void launch_scan(const ScanArgs& a, int mode, int grid, int kind, hipStream_t st) {
+  if (grid <= 0) return;
+  switch (tuning().scan_cfg) {
+    case 1: launch_cfg<4, 2>(a, mode, grid, kind, st); break;
+    case 2: launch_cfg<2, 4>(a, mode, grid, kind, st); break;
+    default: launch_cfg<2, 2>(a, mode, grid, kind, st); break;
   }
 }
 

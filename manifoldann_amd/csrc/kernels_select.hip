@@ -7,7 +7,7 @@
 // (distance, row) with the row in the scanned layout; for the IVF layout
 // (lists contiguous, ascending ids inside a list) that is (dist, bucket, id),
 // the batched reference merge order.
-#include "launch.h"
+#include "bounds.h"
 
 namespace mann {
 
@@ -40,63 +40,6 @@ __device__ __forceinline__ int next_pow2(int x) {
   int p = 1;
   while (p < x) p <<= 1;
   return p;
-}
-
-// Exact FP32 metric of base row r against query q (16-lane group, A16 order).
-__device__ __forceinline__ float exact_metric(const float* __restrict__ x32,
-                                              const float* __restrict__ q32,
-                                              const float* __restrict__ xn2,
-                                              const float* __restrict__ qn2, int d16, int metric,
-                                              int64_t r, int64_t q, int j) {
-  const float* x = x32 + r * d16;
-  const float* y = q32 + q * d16;
-  float acc = 0.0f;
-  if (metric == EUCLIDEAN || metric == SQEUCLIDEAN) {
-    for (int t = 0; t < d16; t += 16) {
-      float df = x[t + j] - y[t + j];
-      acc = __builtin_fmaf(df, df, acc);
-    }
-  } else {
-    for (int t = 0; t < d16; t += 16) acc = __builtin_fmaf(x[t + j], y[t + j], acc);
-  }
-  acc = group16_tree(acc);
-  if (metric == EUCLIDEAN) return __builtin_sqrtf(acc);
-  if (metric == SQEUCLIDEAN) return acc;
-  if (metric == COSINE) {
-    float v = 1.0f - acc / (__builtin_sqrtf(xn2[r]) * __builtin_sqrtf(qn2[q]));
-    return v > 0.0f ? v : 0.0f;
-  }
-  // norms form (kmeans/distance.jl:99-101): sqrt(max((|c|^2 + |x|^2) - 2 c.x, 0))
-  float tt = xn2[r] + qn2[q];
-  float sq = tt - 2.0f * acc;
-  return __builtin_sqrtf(sq > 0.0f ? sq : 0.0f);
-}
-
-// Smallest a' threshold T such that every row with a' > T has an FP32 exact
-// metric strictly above dk (see the derivation in the scan kernel header and
-// the design notes): |a' + |q~|^2 - |q~ - x~|^2| <= G, |x - x~| <= ex,
-// |q - q~| <= qe, FP32 relative error of the exact form <= rho.
-__device__ double exclusion_threshold(double dk, int metric, double qn, double qe, double qr,
-                                      double ex, double rx, int d) {
-  const double u = 1.0 / 16777216.0;
-  double R = qr + rx;
-  double G = (2.0 * d + 8.0) * u * R * R;
-  double rho = (d + 8.0) * u;
-  double L;
-  if (metric == EUCLIDEAN) {
-    L = dk / (1.0 - rho) + qe + ex;
-  } else if (metric == SQEUCLIDEAN) {
-    L = sqrt(dk / (1.0 - rho)) + qe + ex;
-  } else if (metric == COSINE) {
-    double delta = (2.0 * d + 16.0) * u;
-    L = sqrt(2.0 * (dk + delta)) + qe + ex;
-  } else {
-    double R32 = R + qe + ex;
-    double H = (d + 8.0) * u * R32 * R32;
-    double f = dk / (1.0 - 8.0 * u);
-    L = sqrt(f * f + H) + qe + ex;
-  }
-  return (L * L - qn + G) * (1.0 + 1e-9) + 1e-30;
 }
 
 __global__ __launch_bounds__(256) void tau_from_sample_kernel(TauArgs a) {
@@ -159,15 +102,26 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(SelectArgs a) {
   __shared__ double s_T;
   const int tid = threadIdx.x, g = tid >> 4, j = tid & 15;
 
-  const int craw = a.cnt[q];
-  const int c = craw < a.cap ? craw : a.cap;
-  const bool overflow = craw > a.cap;
+  int c = 0;
+  bool overflow = false;
+  for (int p = 0; p < a.parts; ++p) {
+    const int cp = a.cnt[(int64_t)q * a.parts + p];
+    overflow |= cp > a.cap;
+    c += cp < a.cap ? cp : a.cap;
+  }
   const int P = next_pow2(c > 0 ? c : 1);
-  const unsigned* cq = a.cand + (int64_t)q * a.cap * 2;
   for (int i = tid; i < P; i += 256) {
     uint64_t key = ~0ull;
     if (i < c) {
-      uint2 e = *(const uint2*)(cq + 2 * i);
+      int p = 0, base = 0;
+      for (;; ++p) {
+        int cp = a.cnt[(int64_t)q * a.parts + p];
+        cp = cp < a.cap ? cp : a.cap;
+        if (i < base + cp) break;
+        base += cp;
+      }
+      const unsigned* cq = a.cand + ((int64_t)q * a.parts + p) * (int64_t)a.cap * 2;
+      uint2 e = *(const uint2*)(cq + 2 * (i - base));
       key = pack_key(__uint_as_float(e.x), e.y);
     }
     keys[i] = key;
@@ -176,6 +130,7 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(SelectArgs a) {
   bitonic_sort(keys, P);
 
   const double qn = a.qn[q], qe = a.qe[q], qr = a.qr[q];
+  const int k_eff = a.k_eff_q ? a.k_eff_q[q] : a.k_eff;
   int done = 0;
   int target = c < a.shortlist ? c : a.shortlist;
   double T = INFINITY;
@@ -192,8 +147,8 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(SelectArgs a) {
     bitonic_sort(ex, P2);
     if (tid == 0) {
       double Tn = INFINITY;
-      if (done >= a.k_eff && a.k_eff > 0) {
-        float dk = ord2f((uint32_t)(ex[a.k_eff - 1] >> 32));
+      if (done >= k_eff && k_eff > 0) {
+        float dk = ord2f((uint32_t)(ex[k_eff - 1] >> 32));
         Tn = exclusion_threshold(dk, a.metric, qn, qe, qr, a.ex, a.rx, a.d);
       }
       // rows with a' <= Tn must be re-ranked: count them in the sorted keys
@@ -220,12 +175,12 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(SelectArgs a) {
     bool certified;
     if (overflow) certified = false;
     else if (tq == INFINITY) certified = true;
-    else certified = (done >= a.k_eff) && ((double)tq > T);
+    else certified = (done >= k_eff) && ((double)tq > T);
     if (!certified) {
       int f = atomicAdd(a.flag_count, 1);
       a.flag_list[f] = q;
       float tn = INFINITY;
-      if (done >= a.k_eff && T < 3.0e38) tn = nextafterf((float)T, INFINITY);
+      if (done >= k_eff && T < 3.0e38) tn = nextafterf((float)T, INFINITY);
       if (!(tn > tq) && !overflow) tn = INFINITY;  // no progress possible: emit everything
       a.tau_new[q] = tn;
     }
@@ -262,11 +217,11 @@ __global__ __launch_bounds__(256) void exact_topk_kernel(ExactArgs a) {
   const int kr = a.k_req < EX_KMAX ? a.k_req : EX_KMAX;
   for (int i = tid; i < EX_BATCH + EX_KMAX; i += 256) m[i] = ~0ull;
   __syncthreads();
-  int nr = a.range_off ? a.range_off[q + 1] - a.range_off[q] : 1;
+  int nr = a.range_cnt ? a.range_cnt[q] : 1;
   for (int ri = 0; ri < nr; ++ri) {
     int rb = 0, re = a.n_rows;
-    if (a.range_off) {
-      int2 rg = a.ranges[a.range_off[q] + ri];
+    if (a.range_cnt) {
+      int2 rg = a.ranges[(int64_t)q * a.range_stride + ri];
       rb = rg.x;
       re = rg.y;
     }

@@ -38,6 +38,30 @@ __global__ void gather_f32_kernel(const float* src, const int* idx, int m, float
   if (i < m) dst[i] = src[idx[i]];
 }
 
+__global__ void ids_to_i64_kernel(const int* src, int64_t n, int64_t* dst) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (int64_t)src[i] + 1;
+}
+// copy rows between strides (rows of w floats)
+__global__ void copy_rows_kernel(const float* src, int64_t lds, float* dst, int64_t ldd, int w,
+                                 int64_t rows) {
+  int64_t r = blockIdx.x;
+  if (r >= rows) return;
+  for (int c = threadIdx.x; c < ldd; c += blockDim.x) dst[r * ldd + c] = c < w ? src[r * lds + c] : 0.0f;
+}
+
+void ids_to_i64(const int* src, int64_t n, int64_t* dst, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(ids_to_i64_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, src, n,
+                     dst);
+}
+void copy_rows(const float* src, int64_t lds, float* dst, int64_t ldd, int w, int64_t rows,
+               hipStream_t st) {
+  if (rows <= 0) return;
+  hipLaunchKernelGGL(copy_rows_kernel, dim3((unsigned)rows), dim3(64), 0, st, src, lds, dst, ldd, w,
+                     rows);
+}
+
 void fill_i32(int* p, int64_t n, int v, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, p, n, v);

@@ -5,13 +5,15 @@
 
 namespace mann {
 
-enum ScanMode : int { SCAN_FILTER = 0, SCAN_DENSE = 1, SCAN_TOP1 = 2 };
+enum ScanMode : int { SCAN_FILTER = 0, SCAN_DENSE = 1, SCAN_TOP1 = 2, SCAN_GATHER = 3 };
+enum ScanKind : int { SCAN_KIND_GRID = 0, SCAN_KIND_REFINE = 1, SCAN_KIND_ITEMS = 2 };
 
 struct ScanItem {
   int qoff;       // offset into the query index list (or first query id)
   int nq;         // queries in this item (<= tile width)
   int row_begin;  // base rows [row_begin, row_end) of the (reordered) base
   int row_end;
+  int region;     // candidate region of each query written by this item
 };
 
 struct ScanArgs {
@@ -31,11 +33,11 @@ struct ScanArgs {
   const ScanItem* items;
   const int* n_items_dev;
   int* item_counter;
-  // FILTER / TOP1 outputs
+  // FILTER / TOP1 outputs: query q, region r owns cand[(q * parts + r) * cap ..]
   const float* tau;   // per query threshold on a'
-  unsigned* cand;     // [nq_total][cap] pairs (float bits of a', row)
-  int* cnt;           // per query emitted count (may exceed cap: overflow)
-  int cap;
+  unsigned* cand;     // pairs (float bits of a', row)
+  int* cnt;           // [q * parts + r] emitted count (may exceed cap: overflow)
+  int cap;            // capacity of one region
   // DENSE output
   float* dense;
   int64_t ldd;
@@ -46,9 +48,18 @@ struct ScanArgs {
   float t1_ex, t1_rx;
 };
 
+// Run-time tuning knobs (set through mann_tune_set; chosen by the autotuner).
+struct Tuning {
+  int scan_cfg = 0;          // scan tile: 0 = 128x128, 1 = 256x128, 2 = 128x256 (rows x queries)
+  int target_groups = 2048;  // workgroups the row partitioning aims for
+  int sample_rank = 1;       // threshold rank = sample_rank * k_req in the row sample
+  int shortlist_mult = 2;    // first re-rank batch = shortlist_mult * k_req (>= 32)
+};
+Tuning& tuning();
+
 int scan_tile_rows();
 int scan_tile_queries();
-void launch_scan(const ScanArgs& a, int mode, int grid, bool persistent, hipStream_t st);
+void launch_scan(const ScanArgs& a, int mode, int grid, int kind, hipStream_t st);
 
 void launch_prep_rows(const float* src, int64_t ld, int64_t n, int64_t n_pad, int d, int d16,
                       int kp, int normalize, float* x32, bf16_t* xb, float* xn, float* err,
@@ -75,7 +86,8 @@ struct SelectArgs {
   // candidates
   const unsigned* cand;
   const int* cnt;
-  int cap;
+  int cap;             // capacity of one region
+  int parts;           // regions per query
   const float* tau;
   // exact operands
   const float* x32;    // base rows (reordered space for IVF), d16 floats each
@@ -95,6 +107,7 @@ struct SelectArgs {
   int k_req;           // neighbours selected before self removal (k or k+1)
   int exclude_self;    // drop id == query id (kNN graph)
   int k_eff;           // min(k_req, rows available)
+  const int* k_eff_q;  // per-query k_eff (IVF: rows in the probed lists), or null
   int shortlist;       // first re-rank batch (oversampling, >= k_req)
   // outputs
   int64_t* out_ids;    // [nq][k] 1-based, 0 = empty
@@ -118,10 +131,11 @@ struct ExactArgs {
   const float* qn2;
   int d, d16, metric;
   const int* perm;
-  // row ranges per query: ranges [range_off[q], range_off[q+1]) of (begin, end)
-  // pairs in `ranges`; nullptr -> the single range [0, n_rows)
-  const int* range_off;
+  // row ranges of query q: ranges[q * range_stride + i], i < range_cnt[q];
+  // range_cnt == nullptr -> the single range [0, n_rows)
+  const int* range_cnt;
   const int2* ranges;
+  int range_stride;
   int n_rows;
   int k, k_req, exclude_self;
   int64_t* out_ids;

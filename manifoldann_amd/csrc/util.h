@@ -13,6 +13,11 @@ void gather_rows_bf16(const bf16_t* src, const float* srcn, const int* idx, int 
 void gather_rows_f32(const float* src, const int* idx, int m, int w, float* dst, hipStream_t st);
 void gather_f32(const float* src, const int* idx, int m, float* dst, hipStream_t st);
 
+void ids_to_i64(const int* src, int64_t n, int64_t* dst, hipStream_t st);
+// dst[r][c] = src[r][c] for c < w, 0 for w <= c < ldd
+void copy_rows(const float* src, int64_t lds, float* dst, int64_t ldd, int w, int64_t rows,
+               hipStream_t st);
+
 int select_cap_max();
 int exact_kmax();
 

@@ -1,0 +1,429 @@
+"""Host-side mirror of ManifoldANN.jl's k-means transform and IVF
+(MultiLevelIndex{KMeansTransform, TopKRouting, BruteForceIndex}) over the HIP
+library.
+
+Reference files: transforms/KMeansTransform.jl, transforms/kmeans/*.jl,
+transforms/utils.jl, indices/multilevel/{config,builder,routing,query,merge}.jl.
+Randomness (kmeans++ draws, :random init, subsampling) comes from the
+library's splitmix64 stream of an integer seed: Julia RNG streams cannot be
+reproduced outside Julia.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Any, Optional
+
+import numpy as np
+
+from ._lib import ArgumentError, DimensionMismatch, check, lib, shutting_down
+from .index import (BruteForceIndex, Dataset, Euclidean, Neighbor, _data_shape, _dataset_for,
+                    _is_torch, _to_neighbors, as_matrix, default_distance)
+
+KMEANS_DEFAULT_MAX_ITERATIONS = 100   # utils/constants.jl:127
+KMEANS_DEFAULT_TOLERANCE = 1e-6       # utils/constants.jl:135
+IVF_DEFAULT_NPROBE = 5                # utils/constants.jl:154
+
+
+def _seed_of(rng) -> int:
+    if rng is None:
+        return 0x5EED
+    if isinstance(rng, (int, np.integer)):
+        return int(rng) & (2**64 - 1)
+    if isinstance(rng, np.random.Generator):
+        return int(rng.integers(0, 2**63))
+    raise ArgumentError("rng must be an integer seed or a numpy Generator")
+
+
+def random_indices(n: int, k: int, seed: int) -> np.ndarray:
+    """First k entries of a Fisher-Yates permutation driven by splitmix64
+    (0-based) -- the library's :random init and subsample draw."""
+    p = np.arange(n, dtype=np.int64)
+    s = seed & (2**64 - 1)
+    M = 2**64 - 1
+    for i in range(n - 1, 0, -1):
+        s = (s + 0x9E3779B97F4A7C15) & M
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z = z ^ (z >> 31)
+        u = (z >> 11) * (1.0 / 9007199254740992.0)
+        j = min(int(u * (i + 1)), i)
+        p[i], p[j] = p[j], p[i]
+    return p[:k].copy()
+
+
+# ---------------------------------------------------------------------------
+# transforms (transforms/Transform.jl, KMeansTransform.jl)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class KMeansAssignment:
+    """KMeansTransform.jl:85-87: distances to every centroid."""
+    distances: np.ndarray
+
+
+@dataclass
+class TransformResult:
+    """Transform.jl:36-39."""
+    data: Any
+    assignment: Any
+
+
+class KMeansTransform:
+    """KMeansTransform.jl:41-79 (assertions as in the reference constructor)."""
+
+    def __init__(self, k: int, distance=default_distance, init: str = "kmeans_plus_plus",
+                 max_iters: int = KMEANS_DEFAULT_MAX_ITERATIONS,
+                 tol: float = KMEANS_DEFAULT_TOLERANCE, subsample_size: Optional[int] = None):
+        assert k > 0, "Number of clusters must be positive"
+        assert init in ("random", "kmeans_plus_plus"), "init must be :random or :kmeans_plus_plus"
+        assert max_iters > 0, "max_iters must be positive"
+        assert tol > 0, "tol must be positive"
+        if subsample_size is not None and subsample_size <= 0:
+            raise ArgumentError("subsample_size must be positive or nothing")
+        if not isinstance(distance, Euclidean) or type(distance) is not Euclidean:
+            raise ArgumentError("the GPU k-means supports the Euclidean distance")
+        self.k = k
+        self.distance = distance
+        self.init = init
+        self.max_iters = max_iters
+        self.tol = tol
+        self.subsample_size = subsample_size
+        self.centroids = None           # d x k float32 (Julia layout)
+        self.centroid_norms = None
+        self.pending_assignments = None
+        self.n_iters = None
+        self.init_centroids = None      # optional d x k injected initial centroids
+
+
+def _centroids_rows(C_dk):
+    """d x k (Julia) -> k x d row-major float32 for the C ABI."""
+    return np.ascontiguousarray(np.asarray(C_dk, np.float32).T)
+
+
+def fit(t: KMeansTransform, X, rng=None):
+    """fit!(t, X; rng) (KMeansTransform.jl:106-162)."""
+    d, n = _data_shape(X)
+    assert t.k <= n, f"Cannot fit {t.k} clusters with only {n} points"
+    seed = _seed_of(rng)
+    use_sub = t.subsample_size is not None and t.subsample_size < n
+    if use_sub:
+        sample_n = max(t.subsample_size, t.k)
+        idx = random_indices(n, sample_n, seed)
+        Xa = X.data if isinstance(X, Dataset) else X
+        if isinstance(X, Dataset):
+            raise ArgumentError("subsample_size needs the data matrix, not a prepared Dataset")
+        Xt = Xa[:, idx] if not _is_torch(Xa) else Xa[:, idx.tolist()]
+        ds_train = Dataset(Xt)
+        seed = (seed * 0x9E3779B97F4A7C15 + 1) & (2**64 - 1)
+    else:
+        ds_train = _dataset_for(X, Euclidean())
+    init_code = {"kmeans_plus_plus": 1, "random": 2}[t.init]
+    init_ptr = None
+    if t.init_centroids is not None:
+        init_rows = _centroids_rows(t.init_centroids)
+        init_code, init_ptr = 0, init_rows.ctypes.data
+    out_c = np.zeros((t.k, d), np.float32)
+    n_train = ds_train.n
+    assign = np.zeros(n_train, np.int64)
+    iters = ctypes.c_int64()
+    check(lib().mann_kmeans_fit(ds_train.handle, t.k, t.max_iters, t.tol, init_code, seed, init_ptr,
+                                out_c.ctypes.data, assign.ctypes.data, ctypes.byref(iters)))
+    if use_sub:
+        ds_full = _dataset_for(X, Euclidean())
+        assign = np.zeros(n, np.int64)
+        check(lib().mann_kmeans_assign(ds_full.handle, out_c.ctypes.data, t.k, assign.ctypes.data,
+                                       None))
+    t.centroids = np.asfortranarray(out_c.T)
+    t.centroid_norms = (out_c.astype(np.float64) ** 2).sum(1).astype(np.float32)
+    t.pending_assignments = assign
+    t.n_iters = int(iters.value)
+    return t
+
+
+def kmeans_assign(X, centroids_dk):
+    """assign_clusters!(pairwise_euclidean!) -> (assign 1-based, mindist)."""
+    ds = _dataset_for(X, Euclidean())
+    C = _centroids_rows(centroids_dk)
+    k = C.shape[0]
+    a = np.zeros(ds.n, np.int64)
+    md = np.zeros(ds.n, np.float32)
+    check(lib().mann_kmeans_assign(ds.handle, C.ctypes.data, k, a.ctypes.data, md.ctypes.data))
+    return a, md
+
+
+def kmeans_plus_plus(X, k, seed):
+    """init_kmeans_plus_plus with the splitmix64 stream -> (d x k, 1-based idx)."""
+    ds = _dataset_for(X, Euclidean())
+    out = np.zeros((k, ds.d), np.float32)
+    idx = np.zeros(k, np.int64)
+    check(lib().mann_kmeanspp(ds.handle, k, seed, out.ctypes.data, idx.ctypes.data))
+    return np.asfortranarray(out.T), idx
+
+
+def take_pending_assignments(t: KMeansTransform):
+    a = t.pending_assignments
+    t.pending_assignments = None
+    return a
+
+
+def preserves_data(t) -> bool:
+    return True
+
+
+# ---------------------------------------------------------------------------
+# routing / merging (indices/multilevel/routing.jl, merge.jl)
+# ---------------------------------------------------------------------------
+
+class TopKRouting:
+    """routing.jl:36-44."""
+
+    def __init__(self, k: int):
+        if k <= 0:
+            raise ArgumentError("k must be positive")
+        self.k = k
+
+
+class ExhaustiveRouting:
+    pass
+
+
+def select_indices(strategy, assignment, indices):
+    """routing.jl:94-127 (1-based positions)."""
+    if isinstance(strategy, ExhaustiveRouting):
+        return list(range(1, len(indices) + 1))
+    n_children = len(indices)
+    if n_children <= 0:
+        return []
+    dist = np.asarray(assignment.distances, np.float32)
+    avail = min(n_children, dist.shape[0])
+    if avail <= 0:
+        raise ArgumentError("Transform produced no routing distances for TopKRouting")
+    kk = min(strategy.k, avail)
+    order = np.lexsort((np.arange(avail), dist[:avail]))[:kk]
+    return [int(i) + 1 for i in order]
+
+
+class SimpleMerge:
+    pass
+
+
+class DisjointMerge:
+    pass
+
+
+def merge_results(strategy, result_lists, k):
+    """merge.jl:79-169 on lists of Neighbor."""
+    allv = [nb for lst in result_lists for nb in lst]
+    if not allv:
+        return []
+    if isinstance(strategy, SimpleMerge):
+        srt = sorted(allv, key=lambda nb: nb.dist)   # Python's sort is stable
+        out, seen = [], set()
+        for nb in srt:
+            if nb.id in seen:
+                continue
+            seen.add(nb.id)
+            out.append(nb)
+            if len(out) >= min(k, len(allv)):
+                break
+        return out
+    # DisjointMerge: bounded max-heap with strict '<' eviction, then (dist, id)
+    cap = min(k, len(allv))
+    heap = []
+    for nb in allv:
+        if len(heap) < cap:
+            heap.append(nb)
+            x = len(heap) - 1
+            while x > 0:
+                p = (x - 1) // 2
+                if heap[x].dist > heap[p].dist:
+                    heap[x], heap[p] = heap[p], heap[x]
+                    x = p
+                else:
+                    break
+        elif nb.dist < heap[0].dist:
+            heap[0] = nb
+            x = 0
+            while True:
+                l, r, g = 2 * x + 1, 2 * x + 2, x
+                if l < len(heap) and heap[l].dist > heap[g].dist:
+                    g = l
+                if r < len(heap) and heap[r].dist > heap[g].dist:
+                    g = r
+                if g == x:
+                    break
+                heap[x], heap[g] = heap[g], heap[x]
+                x = g
+    return sorted(heap, key=lambda nb: (nb.dist, nb.id))
+
+
+# ---------------------------------------------------------------------------
+# MultiLevelIndex (indices/multilevel/*)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class TerminalConfig:
+    index_type: Any
+    params: tuple = ()
+
+
+@dataclass
+class TransformedConfig:
+    transform: Any
+    routing: Any
+    child_config: Any
+
+
+class MultiLevelIndex:
+    """One KMeans level over BruteForce leaves, held on the GPU."""
+
+    def __init__(self, transform, routing, handle, d, n, nlist, merge_strategy, distance):
+        self.transform = transform
+        self.routing = routing
+        self.handle = handle
+        self.d, self.n, self.nlist = d, n, nlist
+        self.merge_strategy = merge_strategy
+        self.distance = distance
+        nb = ctypes.c_int64()
+        check(lib().mann_ivf_info(handle, ctypes.byref(nb), None))
+        sizes = np.zeros(nb.value, np.int64)
+        check(lib().mann_ivf_info(handle, ctypes.byref(nb), sizes.ctypes.data))
+        self.n_buckets = int(nb.value)
+        self.list_sizes = sizes
+        # bucket_lookup (builder.jl:133-185): bucket -> child position, 0 if empty
+        self.bucket_lookup = np.zeros(self.n_buckets, np.int64)
+        pos = 0
+        for b in range(self.n_buckets):
+            if sizes[b] > 0:
+                pos += 1
+                self.bucket_lookup[b] = pos
+        self.root = self
+
+    @property
+    def id_mappings(self):
+        """id_mappings[child] = ascending global ids (transforms/utils.jl:160-171)."""
+        ids = np.zeros(self.n, np.int64)
+        check(lib().mann_ivf_list_ids(self.handle, ids.ctypes.data))
+        off = np.concatenate([[0], np.cumsum(self.list_sizes)])
+        return [ids[off[b]:off[b + 1]] for b in range(self.n_buckets) if self.list_sizes[b] > 0]
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and not shutting_down():
+            try:
+                lib().mann_ivf_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+
+def build_index(index_type, X, config=None, merge_strategy=None, distance=default_distance,
+                rng=None):
+    """build_index(MultiLevelIndex, X, config; merge_strategy, distance, rng)
+    (builder.jl:61-76) for a KMeans transform over BruteForce terminals."""
+    if index_type is not MultiLevelIndex:
+        raise ArgumentError(f"unsupported index type {index_type}")
+    if not isinstance(config, TransformedConfig) or not isinstance(config.transform, KMeansTransform):
+        raise ArgumentError("the GPU MultiLevelIndex needs a KMeansTransform root")
+    if not (isinstance(config.child_config, TerminalConfig)
+            and config.child_config.index_type is BruteForceIndex):
+        raise ArgumentError("the GPU MultiLevelIndex needs BruteForce terminals")
+    import copy
+    t = copy.deepcopy(config.transform)     # builder.jl:103
+    fit(t, X, rng=rng)
+    assign = take_pending_assignments(t)
+    ds = _dataset_for(X, Euclidean())
+    C = _centroids_rows(t.centroids)
+    h = ctypes.c_void_p()
+    check(lib().mann_ivf_build(ds.handle, C.ctypes.data, t.k, assign.ctypes.data, ctypes.byref(h)))
+    return MultiLevelIndex(t, config.routing, h, ds.d, ds.n, t.k,
+                           merge_strategy or SimpleMerge(), distance)
+
+
+def build_index_from_assignment(X, centroids_dk, assign, routing, merge_strategy=None,
+                                distance=default_distance):
+    """The IVF layout from an existing k-means fit (centroids d x k, 1-based assignment)."""
+    t = KMeansTransform(k=np.asarray(centroids_dk).shape[1])
+    t.centroids = np.asfortranarray(np.asarray(centroids_dk, np.float32))
+    ds = _dataset_for(X, Euclidean())
+    C = _centroids_rows(t.centroids)
+    a = np.ascontiguousarray(assign, np.int64)
+    h = ctypes.c_void_p()
+    check(lib().mann_ivf_build(ds.handle, C.ctypes.data, t.k, a.ctypes.data, ctypes.byref(h)))
+    return MultiLevelIndex(t, routing, h, ds.d, ds.n, t.k, merge_strategy or SimpleMerge(),
+                           distance)
+
+
+def transform(t, x):
+    """transform(::KMeansTransform, x) (KMeansTransform.jl:181-190): the FP32
+    norms-form distances to every centroid, computed on the GPU."""
+    if t.centroids is None:
+        raise ArgumentError("KMeansTransform must be fitted before transforming")
+    qm = as_matrix(x)
+    C = _centroids_rows(t.centroids)
+    if qm.d != C.shape[1]:
+        raise DimensionMismatch(f"Expected query length {C.shape[1]}")
+    out = np.zeros((qm.n, C.shape[0]), np.float32)
+    check(lib().mann_route_distances(C.ctypes.data, C.shape[0], qm.d, qm.ptr, qm.n, qm.ld,
+                                     out.ctypes.data))
+    if np.ndim(x) == 1:
+        return TransformResult(x, KMeansAssignment(out[0]))
+    return [TransformResult(None, KMeansAssignment(r)) for r in out]
+
+
+def route_distances(index: MultiLevelIndex, Q):
+    """Routing distances (nq, nlist) of the index's KMeans transform."""
+    qm = as_matrix(Q)
+    if qm.d != index.d:
+        raise DimensionMismatch(f"Expected queries with {index.d} rows, got {qm.d}")
+    out = np.zeros((qm.n, index.nlist), np.float32)
+    check(lib().mann_ivf_route(index.handle, qm.ptr, qm.d, qm.n, qm.ld, out.ctypes.data))
+    return out
+
+
+def probe_lists(index: MultiLevelIndex, route):
+    """TopKRouting + empty-bucket removal per query -> (nq, nprobe) 1-based buckets, 0 pad."""
+    route = np.ascontiguousarray(route, np.float32)
+    nq = route.shape[0]
+    nprobe = index.routing.k
+    out = np.zeros((nq, nprobe), np.int64)
+    check(lib().mann_ivf_probe(index.handle, route.ctypes.data, nq, nprobe, out.ctypes.data))
+    return out
+
+
+def query_arrays(index: MultiLevelIndex, data, Q, k, route=None, nprobe=None):
+    """Batched query (query.jl:279-300) -> (ids (nq, k), dist (nq, k))."""
+    qm = as_matrix(Q)
+    d, _ = _data_shape(data)
+    if qm.d != d:
+        raise DimensionMismatch(f"Expected queries with {d} rows, got {qm.d}")
+    nq = qm.n
+    k = max(int(k), 0)
+    npb = nprobe if nprobe is not None else index.routing.k
+    if qm.device:
+        import torch
+        ids = torch.zeros((nq, k), dtype=torch.int64, device=qm.keep.device)
+        dist = torch.full((nq, k), float("inf"), dtype=torch.float32, device=qm.keep.device)
+        pi, pd = ids.data_ptr(), dist.data_ptr()
+    else:
+        ids = np.zeros((nq, k), np.int64)
+        dist = np.full((nq, k), np.inf, np.float32)
+        pi, pd = ids.ctypes.data, dist.ctypes.data
+    if nq == 0 or k == 0:
+        return ids, dist
+    rp = None
+    if route is not None:
+        rp = route.data_ptr() if _is_torch(route) else np.ascontiguousarray(route, np.float32)
+        rp = rp if isinstance(rp, int) else rp.ctypes.data
+    check(lib().mann_ivf_search(index.handle, qm.ptr, qm.d, nq, qm.ld, k, npb,
+                                index.distance.code, rp, pi, pd))
+    return ids, dist
+
+
+def query(index: MultiLevelIndex, data, q, k):
+    single = (np.ndim(q) == 1) if not _is_torch(q) else q.dim() == 1
+    ids, dist = query_arrays(index, data, q, k)
+    res = _to_neighbors(ids, dist)
+    return res[0] if single else res
