@@ -1,14 +1,19 @@
 """Benchmark of the MI355X search core (one JSON line on stdout).
 
-Default workload (N=1): the all-points kNN graph self-join on the 60,000 x 784
-clipped 10-cluster mixture (k=20, L2) -- the exact-kNN configuration whose
-throughput is reported as TFLOP/s = 2 n^2 d / t.  One step is one full
-build_knn_graph over data already resident in HBM (device dataset prepared
-once, outside the timed region, like the reference harness's prepare_*).
+Headline workload (`value`): the all-points kNN graph self-join on the
+60,000 x 784 clipped 10-cluster mixture (k=20, L2, exact ids), reported as
+TFLOP/s = 2 n^2 d / t.  One step is one full build_knn_graph over data already
+resident in HBM (the device dataset is prepared once, outside the timed
+region, like the reference harness's prepare_*).
 
-Also reported: the dominant kernel's roofline (HIP-event timing of the
-bf16-MFMA scan kernel inside this run) and the CPU oracle timed on this
-host's cores on a bounded query sample.
+Also reported (key "ivf"): the IVF workload -- 1,000,000 x 256 Zipf mixture,
+nlist 1,024, 20 Lloyd iterations, 10,000 queries, k=10 -- as QPS at
+recall@10 >= 0.95 for centroid TopKRouting and for the learned probe router
+(QPS = queries / wall time of one batched search call, routing included,
+median of 5 after a warm-up, as benchmarking/benchmark.py:799-828).
+
+The dominant kernel's roofline uses HIP-event timing of its launches in this
+run; the CPU oracle is timed on this host's cores on a bounded sample.
 """
 from __future__ import annotations
 
@@ -21,6 +26,206 @@ import time
 
 METRIC = "QPS at recall@10≥0.95, learned vs centroid IVF routing; exact kNN TFLOP/s"
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def kernel_avg_ms(L, check, names):
+    out = {}
+    for name in names:
+        c = ctypes.c_int64()
+        tot = ctypes.c_double()
+        check(L.mann_profile_query(name.encode(), ctypes.byref(c), ctypes.byref(tot)))
+        if c.value:
+            out[name] = round(tot.value / c.value, 4)
+    return out
+
+
+def bench_knn_graph(args, M, datasets, L, check, np, torch):
+    n, d, k = args.n, args.d, args.k
+    log(f"kNN graph: generating {n}x{d} clipped mixture")
+    X = datasets.clipped_mixture(n, d, seed=2)
+    Xd = torch.from_numpy(X).cuda()
+    ds = M.Dataset(Xd.t())
+    ids = torch.zeros((n, k), dtype=torch.int64, device="cuda")
+    dist = torch.zeros((n, k), dtype=torch.float32, device="cuda")
+
+    def step():
+        check(L.mann_knn_graph(ds.handle, k, 0, M.EUCLIDEAN, ids.data_ptr(), dist.data_ptr()))
+
+    for i in range(args.warmup):
+        step()
+    log(f"kNN graph: {args.warmup} warmup steps done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / args.steps
+    stats = M.last_stats()
+    flop = 2.0 * n * n * d
+    log(f"kNN graph: {1e3 * t:.3f} ms/step")
+    check(L.mann_profile_enable(1))
+    check(L.mann_profile_reset())
+    step()
+    kernels = kernel_avg_ms(L, check, ["scan_filter", "scan_dense", "tau_sample", "select_rerank",
+                                       "scan_filter_pass2", "select_rerank_pass2",
+                                       "exact_fallback"])
+    check(L.mann_profile_enable(0))
+    scan_ms = kernels.get("scan_filter", float("nan"))
+    achieved = flop / (scan_ms * 1e-3) / 1e12
+    roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": MFMA_BF16_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4),
+            "traffic": None,
+            "kernel": "scan_grid_kernel FILTER (bf16 MFMA distances + threshold emit)",
+            "algorithmic_flop_per_launch": flop, "avg_launch_ms": scan_ms}
+    cpu = None
+    if not args.skip_cpu:
+        from oracle import oracle as O
+        cores = host_cores()
+        nsq = 256
+        rows = np.random.default_rng(1).choice(n, nsq, replace=False)
+        O.bf_knn(X, X[rows[:8]], k + 1, nthreads=cores)
+        t1 = time.perf_counter()
+        O.bf_knn(X, X[rows], k + 1, nthreads=cores)
+        tc = time.perf_counter() - t1
+        cpu = {"value": round(2.0 * nsq * n * d / tc / 1e12, 6), "unit": "TFLOP/s",
+               "cores": cores, "kind": "port",
+               "sample": f"{nsq} self-join queries (k=21) against all {n} points in {tc:.2f} s "
+                         f"(oracle per-pair A16 FP32 scan + partial sort, OpenMP)"}
+    res = {"ms_per_step": 1e3 * t, "value": flop / t / 1e12, "kernels_ms": kernels,
+           "stats": {"reranked": stats[0], "overflow": stats[1], "second_pass": stats[2],
+                     "exact_fallback": stats[3]}}
+    del ds, Xd
+    torch.cuda.empty_cache()
+    return res, roof, cpu
+
+
+def qps_of(fn, nq, reps, torch):
+    times = []
+    fn()
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    times.sort()
+    return nq / times[len(times) // 2]
+
+
+def bench_ivf(args, M, datasets, L, check, np, torch):
+    from manifoldann_amd import ivf as I
+    from manifoldann_amd import router as RT
+    n, d, nlist, nq = args.ivf_n, 256, args.nlist, args.ivf_nq
+    t_all = time.perf_counter()
+    log(f"IVF: generating {n}x{d} Zipf mixture")
+    X = datasets.zipf_anisotropic_mixture(n, d, components=1000, seed=3)
+    Xd = torch.from_numpy(X).cuda()
+    ds = M.Dataset(Xd.t())
+    t0 = time.perf_counter()
+    t = I.KMeansTransform(k=nlist, max_iters=args.iters, tol=1e-12)
+    I.fit(t, ds, rng=7)
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t0
+    log(f"IVF: k-means fit {fit_s:.2f} s ({t.n_iters} iterations)")
+    index = I.build_index_from_assignment(ds, t.centroids, t.pending_assignments,
+                                          I.TopKRouting(1))
+    Q = datasets.mixture_queries(nq, d, components=1000, seed=3, query_seed=33)
+    Qd = torch.from_numpy(Q).cuda()
+    bf = M.BruteForceIndex(d, n)
+    gt, _ = M.query_arrays(bf, ds, Qd.t(), 10)
+    gt = gt.cpu().numpy()
+    ids = torch.zeros((nq, 10), dtype=torch.int64, device="cuda")
+    dist = torch.zeros((nq, 10), dtype=torch.float32, device="cuda")
+
+    def recall(found):
+        f = found.cpu().numpy()
+        return float(np.mean([len(set(a.tolist()) & set(b.tolist())) / 10.0
+                              for a, b in zip(f, gt)]))
+
+    sweep_c = []
+    for npb in (1, 2, 4, 8, 16, 32, 64):
+        def run(npb=npb):
+            check(L.mann_ivf_search(index.handle, Qd.data_ptr(), d, nq, d, 10, npb, 0, None,
+                                    ids.data_ptr(), dist.data_ptr()))
+        qps = qps_of(run, nq, args.reps, torch)
+        rc = recall(ids)
+        sweep_c.append({"nprobe": npb, "recall": round(rc, 4), "qps": round(qps, 1)})
+        log(f"IVF centroid nprobe={npb}: recall {rc:.4f} qps {qps:.0f}")
+
+    # learned router: labels from the exact kNN of training queries
+    t1 = time.perf_counter()
+    Tq = datasets.mixture_queries(args.train_q, d, components=1000, seed=3, query_seed=77)
+    Tqd = torch.from_numpy(Tq).cuda()
+    labels = RT.neighbour_list_labels(index, ds, Tqd.t(), k=10)
+    r = RT.LearnedProbeRouter(d, nlist, hidden=512, seed=11)
+    loss = r.train(Tqd.t(), labels, epochs=args.epochs, batch=1024, lr=2e-3, seed=5)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t1
+    log(f"IVF: router labels+training {train_s:.2f} s, final loss {loss:.3f}")
+    sweep_l = []
+    for npb in (1, 2, 4, 8, 16, 32, 64):
+        def run(npb=npb):
+            check(L.mann_ivf_search_routed(index.handle, r.handle, Qd.data_ptr(), d, nq, d, 10,
+                                           npb, 0, ids.data_ptr(), dist.data_ptr()))
+        qps = qps_of(run, nq, args.reps, torch)
+        rc = recall(ids)
+        sweep_l.append({"nprobe": npb, "recall": round(rc, 4), "qps": round(qps, 1)})
+        log(f"IVF learned nprobe={npb}: recall {rc:.4f} qps {qps:.0f}")
+
+    def at95(sw):
+        ok = [s for s in sw if s["recall"] >= 0.95]
+        return max(ok, key=lambda s: s["qps"]) if ok else None
+
+    bc, bl = at95(sweep_c), at95(sweep_l)
+    # kernel timing of the list scan at the centroid operating point
+    kern = {}
+    if bc:
+        check(L.mann_profile_enable(1))
+        check(L.mann_profile_reset())
+        check(L.mann_ivf_search(index.handle, Qd.data_ptr(), d, nq, d, 10, bc["nprobe"], 0, None,
+                                ids.data_ptr(), dist.data_ptr()))
+        kern = kernel_avg_ms(L, check, ["prep_queries", "ivf_route", "ivf_probe", "ivf_tau",
+                                        "ivf_plan", "ivf_scan", "ivf_select",
+                                        "ivf_exact_fallback"])
+        check(L.mann_profile_enable(0))
+    cpu = None
+    if not args.skip_cpu and bc:
+        from oracle import oracle as O
+        cores = host_cores()
+        m = 200
+        off = np.concatenate([[0], np.cumsum(index.list_sizes)])
+        lids = np.zeros(n, np.int64)
+        check(L.mann_ivf_list_ids(index.handle, lids.ctypes.data))
+        Cc = np.ascontiguousarray(t.centroids.T)
+        R = O.route_distances(Q[:m], Cc, nthreads=cores)
+        t2 = time.perf_counter()
+        R = O.route_distances(Q[:m], Cc, nthreads=cores)
+        O.ivf_search(X, off, lids, index.n_buckets, Q[:m], R, 10, bc["nprobe"], nthreads=cores)
+        tc = time.perf_counter() - t2
+        cpu = {"value": round(m / tc, 1), "unit": "QPS", "cores": cores, "kind": "port",
+               "sample": f"{m} queries at nprobe={bc['nprobe']} (oracle routing + per-list "
+                         f"scans + merge, OpenMP) in {tc:.2f} s"}
+    out = {"workload": f"IVF n={n} d={d} nlist={nlist} iters={t.n_iters} nq={nq} k=10",
+           "kmeans_fit_s": round(fit_s, 3), "router_train_s": round(train_s, 3),
+           "router_final_loss": round(loss, 4), "train_queries": args.train_q,
+           "centroid": sweep_c, "learned": sweep_l,
+           "centroid_at_0.95": bc, "learned_at_0.95": bl,
+           "learned_over_centroid_qps": (round(bl["qps"] / bc["qps"], 3) if bc and bl else None),
+           "kernels_ms": kern, "cpu_baseline": cpu,
+           "total_s": round(time.perf_counter() - t_all, 1)}
+    return out
 
 
 def main():
@@ -29,9 +234,17 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-ivf", action="store_true")
     ap.add_argument("--n", type=int, default=60_000)
     ap.add_argument("--d", type=int, default=784)
     ap.add_argument("--k", type=int, default=20)
+    ap.add_argument("--ivf-n", type=int, default=1_000_000)
+    ap.add_argument("--nlist", type=int, default=1024)
+    ap.add_argument("--iters", type=int, default=20)
+    ap.add_argument("--ivf-nq", type=int, default=10_000)
+    ap.add_argument("--train-q", type=int, default=200_000)
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--reps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -42,96 +255,34 @@ def main():
     import torch
 
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    torch.cuda.init()
     import manifoldann_amd as M
     from manifoldann_amd import datasets
     from manifoldann_amd._lib import check, lib
-
-    n, d, k = args.n, args.d, args.k
-    print(f"[bench] generating {n}x{d} clipped mixture", file=sys.stderr, flush=True)
-    X = datasets.clipped_mixture(n, d, seed=2)
-    Xd = torch.from_numpy(X).cuda()
-    ds = M.Dataset(Xd.t())
-    ids = torch.zeros((n, k), dtype=torch.int64, device="cuda")
-    dist = torch.zeros((n, k), dtype=torch.float32, device="cuda")
     L = lib()
 
-    def step():
-        check(L.mann_knn_graph(ds.handle, k, 0, M.EUCLIDEAN, ids.data_ptr(), dist.data_ptr()))
-
-    for i in range(args.warmup):
-        step()
-        print(f"[bench] warmup {i + 1}/{args.warmup}", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    t = time.perf_counter() - t0
-    stats = M.last_stats()
-    ms = 1e3 * t / args.steps
-    flop = 2.0 * n * n * d
-    value = flop / (t / args.steps) / 1e12
-    print(f"[bench] timed {args.steps} steps: {ms:.3f} ms/step", file=sys.stderr, flush=True)
-
-    # dominant kernel duration from HIP events on its own stream (untimed step)
-    check(L.mann_profile_enable(1))
-    check(L.mann_profile_reset())
-    step()
-    kernels = {}
-    for name in ("scan_filter", "scan_dense", "tau_sample", "select_rerank", "scan_filter_pass2",
-                 "select_rerank_pass2", "exact_fallback"):
-        c = ctypes.c_int64()
-        tot = ctypes.c_double()
-        check(L.mann_profile_query(name.encode(), ctypes.byref(c), ctypes.byref(tot)))
-        if c.value:
-            kernels[name] = round(tot.value / c.value, 4)
-    check(L.mann_profile_enable(0))
-    scan_ms = kernels.get("scan_filter", float("nan"))
-    achieved = flop / (scan_ms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": MFMA_BF16_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4),
-                "traffic": None, "kernel": "scan_filter (bf16 MFMA distance + threshold emit)",
-                "algorithmic_flop_per_launch": flop}
-
-    cpu = None
-    if not args.skip_cpu:
-        from oracle import oracle as O
-        cores = os.cpu_count() or 1
-        try:
-            cores = len(os.sched_getaffinity(0))
-        except AttributeError:
-            pass
-        nsq = 256
-        rows = np.random.default_rng(1).choice(n, nsq, replace=False)
-        O.bf_knn(X, X[rows[:8]], k + 1, nthreads=cores)  # warm
-        t1 = time.perf_counter()
-        O.bf_knn(X, X[rows], k + 1, nthreads=cores)
-        tc = time.perf_counter() - t1
-        cpu = {"value": round(2.0 * nsq * n * d / tc / 1e12, 6), "unit": "TFLOP/s",
-               "cores": cores, "kind": "port",
-               "sample": f"{nsq} self-join queries (k=21) against all {n} points, "
-                         f"{tc:.2f} s, oracle per-pair A16 FP32 scan + partial sort"}
-
+    res, roof, cpu = bench_knn_graph(args, M, datasets, L, check, np, torch)
+    ivf = None if args.skip_ivf else bench_ivf(args, M, datasets, L, check, np, torch)
     out = {
         "metric": METRIC,
-        "value": round(value, 3),
+        "value": round(res["value"], 3),
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 4),
+        "ms_per_step": round(res["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16 first pass (fp32 accumulate) + fp32 exact re-rank",
-        "data": "synthetic (clipped 10-cluster Gaussian mixture, sigma 0.15, Philox seed 2)",
-        "config": {"workload": f"kNN graph self-join n={n} d={d} k={k} L2 (exact ids)",
-                   "n": n, "d": d, "k": k},
-        "roofline": roofline,
+        "dtype": "bf16 (fp32 accumulate) first pass + fp32 exact re-rank",
+        "data": "synthetic (Philox-seeded Gaussian mixtures; no datasets downloaded)",
+        "config": {"workload": f"kNN graph self-join n={args.n} d={args.d} k={args.k} L2, "
+                               f"exact ids", "n": args.n, "d": args.d, "k": args.k},
+        "roofline": roof,
         "cpu_baseline": cpu,
-        "kernels_ms": kernels,
-        "search_stats": {"reranked": stats[0], "overflow": stats[1], "second_pass": stats[2],
-                         "exact_fallback": stats[3]},
+        "kernels_ms": res["kernels_ms"],
+        "search_stats": res["stats"],
+        "ivf": ivf,
     }
     print(json.dumps(out))
 

@@ -139,6 +139,34 @@ int mann_ivf_search(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq,
                     int64_t k, int64_t nprobe, int metric, const float* route_dist,
                     int64_t* out_ids, float* out_dist);
 
+/* Learned probe router -- no counterpart in the reference: an MLP
+ * d -> hidden (ReLU) -> nlist whose scores rank the IVF lists of a query
+ * (higher = probe first).  It plugs into the reference's AbstractTransform seam
+ * (src/transforms/Transform.jl:36-83) as a transform returning
+ * KMeansAssignment(-scores), with TopKRouting unchanged
+ * (src/indices/multilevel/routing.jl:94-118).  Parameters are FP32 (W1 hidden x d,
+ * b1, W2 nlist x hidden, b2, row-major, concatenated); GEMMs run on bf16
+ * operands with FP32 accumulation.  labels: n x L matrix of 1-based list ids
+ * (0 = none); the target of a row is uniform over its labels. */
+int mann_router_create(int64_t d, int64_t hidden, int64_t nlist, uint64_t seed,
+                       mann_router_t* out);
+int mann_router_destroy(mann_router_t r);
+int mann_router_params(mann_router_t r, float* out, int64_t* count);
+int mann_router_set_params(mann_router_t r, const float* params);
+/* gradient of the mean softmax cross-entropy over one batch (no update) */
+int mann_router_grad(mann_router_t r, const float* X, int64_t d, int64_t n, int64_t ldx,
+                     const int64_t* labels, int64_t L, float* out_grad, double* out_loss);
+/* mini-batch Adam over shuffled rows; out_loss = mean loss of the last epoch */
+int mann_router_train(mann_router_t r, const float* X, int64_t d, int64_t n, int64_t ldx,
+                      const int64_t* labels, int64_t L, int64_t epochs, int64_t batch, double lr,
+                      uint64_t seed, double* out_loss);
+int mann_router_scores(mann_router_t r, const float* Q, int64_t d, int64_t nq, int64_t ldq,
+                       float* out_scores);
+/* mann_ivf_search with the router's negated scores as routing distances */
+int mann_ivf_search_routed(mann_ivf_t ivf, mann_router_t r, const float* queries, int64_t d,
+                           int64_t nq, int64_t ldq, int64_t k, int64_t nprobe, int metric,
+                           int64_t* out_ids, float* out_dist);
+
 /* Tuning knobs of the search kernels (process-wide): "scan_cfg" (first-pass
  * tile: 0 = 128 rows x 128 queries, 1 = 256 x 128, 2 = 128 x 256),
  * "target_groups" (workgroups the row partitioning aims for), "sample_rank"

@@ -41,6 +41,18 @@ SIGNATURES = [
     ("mann_ivf_search", ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, _i64, ctypes.c_int, _vp,
                                        _vp, _vp]),
     ("mann_route_distances", ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _i64, _vp]),
+    ("mann_router_create", ctypes.c_int, [_i64, _i64, _i64, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    ("mann_router_destroy", ctypes.c_int, [_vp]),
+    ("mann_router_params", ctypes.c_int, [_vp, _vp, _i64p]),
+    ("mann_router_set_params", ctypes.c_int, [_vp, _vp]),
+    ("mann_router_grad", ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _vp, _i64, _vp,
+                                        ctypes.POINTER(ctypes.c_double)]),
+    ("mann_router_train", ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,
+                                         ctypes.c_double, ctypes.c_uint64,
+                                         ctypes.POINTER(ctypes.c_double)]),
+    ("mann_router_scores", ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _vp]),
+    ("mann_ivf_search_routed", ctypes.c_int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64,
+                                              ctypes.c_int, _vp, _vp]),
     ("mann_tune_set", ctypes.c_int, [ctypes.c_char_p, _i64]),
     ("mann_tune_get", ctypes.c_int, [ctypes.c_char_p, _i64p]),
     ("mann_profile_enable", ctypes.c_int, [ctypes.c_int]),

@@ -8,6 +8,7 @@
 #include "../../include/mann.h"
 #include "core.h"
 #include "ivf.h"
+#include "mlp.h"
 
 namespace mann {
 
@@ -478,6 +479,149 @@ int mann_route_distances(const float* centroids, int64_t nlist, int64_t d, const
     launch_route_dist(Q.q32, Q.qn2, (int)nq, C, cn2, (int)nlist, (int)d16, od.dev, st);
     od.finish();
     MANN_HIP(hipStreamSynchronize(st));
+  });
+}
+
+// Dense device copy (row stride d) of an n x d row matrix with stride ld.
+static const float* stage_rows(const float* X, int64_t d, int64_t n, int64_t ld, const char* tag,
+                               int64_t* ld_out) {
+  ThreadCtx& c = ctx();
+  const float* src = X;
+  if (!is_device_ptr(X)) {
+    float* stg = c.buf(std::string(tag) + "_stage").as<float>((size_t)(ld * (n - 1) + d));
+    MANN_HIP(hipMemcpyAsync(stg, X, sizeof(float) * (ld * (n - 1) + d), hipMemcpyHostToDevice,
+                            c.stream));
+    src = stg;
+  }
+  float* dst = c.buf(tag).as<float>((size_t)(n * d));
+  copy_rows(src, ld, dst, d, (int)d, n, c.stream);
+  *ld_out = d;
+  return dst;
+}
+
+static const int* stage_labels(const int64_t* labels, int64_t count, const char* tag) {
+  ThreadCtx& c = ctx();
+  const int64_t* src = labels;
+  if (!is_device_ptr(labels)) {
+    int64_t* stg = c.buf(std::string(tag) + "_stage").as<int64_t>(count);
+    MANN_HIP(hipMemcpyAsync(stg, labels, sizeof(int64_t) * count, hipMemcpyHostToDevice,
+                            c.stream));
+    src = stg;
+  }
+  int* dst = c.buf(tag).as<int>(count);
+  labels_to_i32(src, count, dst, c.stream);
+  return dst;
+}
+
+int mann_router_create(int64_t d, int64_t hidden, int64_t nlist, uint64_t seed,
+                       mann_router_t* out) {
+  return guarded([&] {
+    MANN_REQUIRE(out != nullptr, ERR_NULL, "null output handle");
+    MANN_REQUIRE(d > 0 && hidden > 0 && nlist > 0, ERR_ARGUMENT, "router sizes must be positive");
+    *out = router_create((int)d, (int)hidden, (int)nlist, seed);
+  });
+}
+
+int mann_router_destroy(mann_router_t r) {
+  return guarded([&] { delete r; });
+}
+
+int mann_router_params(mann_router_t r, float* out, int64_t* count) {
+  return guarded([&] {
+    MANN_REQUIRE(r != nullptr, ERR_NULL, "null router");
+    if (count) *count = r->nparam;
+    if (out) MANN_HIP(hipMemcpy(out, r->w, sizeof(float) * r->nparam, hipMemcpyDefault));
+  });
+}
+
+int mann_router_set_params(mann_router_t r, const float* params) {
+  return guarded([&] {
+    MANN_REQUIRE(r && params, ERR_NULL, "null pointer");
+    MANN_HIP(hipMemcpy(r->w, params, sizeof(float) * r->nparam, hipMemcpyDefault));
+  });
+}
+
+int mann_router_grad(mann_router_t r, const float* X, int64_t d, int64_t n, int64_t ldx,
+                     const int64_t* labels, int64_t L, float* out_grad, double* out_loss) {
+  return guarded([&] {
+    MANN_REQUIRE(r && X && labels && out_grad, ERR_NULL, "null pointer");
+    MANN_REQUIRE(d == r->d, ERR_DIMENSION, "Expected inputs with " + std::to_string(r->d) + " rows");
+    MANN_REQUIRE(n > 0 && L > 0 && ldx >= d, ERR_ARGUMENT, "bad batch");
+    ThreadCtx& c = ctx();
+    int64_t ld = 0;
+    const float* Xd = stage_rows(X, d, n, ldx, "rg_X", &ld);
+    const int* lb = stage_labels(labels, n * L, "rg_L");
+    float* g = c.buf("rg_grad").as<float>(r->nparam);
+    float* loss = c.buf("rg_loss").as<float>(1);
+    router_grad(*r, Xd, ld, (int)n, lb, (int)L, g, loss);
+    float hl = 0;
+    MANN_HIP(hipMemcpyAsync(out_grad, g, sizeof(float) * r->nparam, hipMemcpyDefault, c.stream));
+    MANN_HIP(hipMemcpyAsync(&hl, loss, sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    MANN_HIP(hipStreamSynchronize(c.stream));
+    if (out_loss) *out_loss = hl;
+  });
+}
+
+int mann_router_train(mann_router_t r, const float* X, int64_t d, int64_t n, int64_t ldx,
+                      const int64_t* labels, int64_t L, int64_t epochs, int64_t batch, double lr,
+                      uint64_t seed, double* out_loss) {
+  return guarded([&] {
+    MANN_REQUIRE(r && X && labels, ERR_NULL, "null pointer");
+    MANN_REQUIRE(d == r->d, ERR_DIMENSION, "Expected inputs with " + std::to_string(r->d) + " rows");
+    MANN_REQUIRE(n > 0 && L > 0 && ldx >= d && epochs > 0 && batch > 0 && batch <= n,
+                 ERR_ARGUMENT, "bad training arguments");
+    int64_t ld = 0;
+    const float* Xd = stage_rows(X, d, n, ldx, "rt_X", &ld);
+    const int* lb = stage_labels(labels, n * L, "rt_L");
+    double loss = router_train(*r, Xd, ld, n, lb, (int)L, (int)epochs, (int)batch, (float)lr, seed);
+    if (out_loss) *out_loss = loss;
+    timer_flush();
+  });
+}
+
+int mann_router_scores(mann_router_t r, const float* Q, int64_t d, int64_t nq, int64_t ldq,
+                       float* out_scores) {
+  return guarded([&] {
+    MANN_REQUIRE(r && Q && out_scores, ERR_NULL, "null pointer");
+    MANN_REQUIRE(d == r->d, ERR_DIMENSION, "Expected queries with " + std::to_string(r->d) + " rows");
+    if (nq == 0) return;
+    int64_t ld = 0;
+    const float* Qd = stage_rows(Q, d, nq, ldq, "rs_Q", &ld);
+    OutBuf<float> os(out_scores, (size_t)(nq * r->c), "out_scores");
+    router_scores(*r, Qd, ld, nq, os.dev);
+    os.finish();
+    MANN_HIP(hipStreamSynchronize(ctx().stream));
+    timer_flush();
+  });
+}
+
+int mann_ivf_search_routed(mann_ivf_t ivf, mann_router_t r, const float* queries, int64_t d,
+                           int64_t nq, int64_t ldq, int64_t k, int64_t nprobe, int metric,
+                           int64_t* out_ids, float* out_dist) {
+  return guarded([&] {
+    MANN_REQUIRE(ivf && r, ERR_NULL, "null handle");
+    MANN_REQUIRE(r->c == ivf->nlist && r->d == ivf->d, ERR_DIMENSION,
+                 "router does not match the index");
+    check_metric(metric);
+    MANN_REQUIRE((metric == COSINE) == (ivf->normalized != 0), ERR_ARGUMENT,
+                 "index was built for another metric");
+    MANN_REQUIRE(nprobe > 0, ERR_ARGUMENT, "k must be positive");
+    QueryOps Q = ivf_queries(ivf, queries, d, nq, ldq);
+    if (nq == 0 || k <= 0) {
+      last_stats() = SearchStats();
+      return;
+    }
+    MANN_REQUIRE(out_ids && out_dist, ERR_NULL, "null pointer");
+    ThreadCtx& c = ctx();
+    float* S = c.buf("sr_scores").as<float>((size_t)nq * r->c);
+    router_scores(*r, Q.q32, ivf->d16, nq, S);
+    launch_neg(S, S, nq * r->c, c.stream);
+    OutBuf<int64_t> oi(out_ids, (size_t)(nq * k), "out_ids");
+    OutBuf<float> od(out_dist, (size_t)(nq * k), "out_dist");
+    ivf_search(*ivf, Q, (int)k, (int)nprobe, metric, S, oi.dev, od.dev);
+    oi.finish();
+    od.finish();
+    timer_flush();
   });
 }
 

@@ -50,6 +50,27 @@ struct mann_ivf {
   }
 };
 
+// Learned probe router handle: FP32 master parameters + Adam moments.
+struct mann_router {
+  int device = 0;
+  int d = 0, h = 0, c = 0;
+  int step = 0;
+  float* w = nullptr;   // all parameters: W1 (h x d), b1 (h), W2 (c x h), b2 (c)
+  float* m1 = nullptr;  // Adam moments
+  float* m2 = nullptr;
+  int64_t nparam = 0;
+  float* W1() const { return w; }
+  float* b1() const { return w + (int64_t)h * d; }
+  float* W2() const { return b1() + h; }
+  float* b2() const { return W2() + (int64_t)c * h; }
+  ~mann_router() {
+    (void)hipSetDevice(device);
+    if (w) (void)hipFree(w);
+    if (m1) (void)hipFree(m1);
+    if (m2) (void)hipFree(m2);
+  }
+};
+
 namespace mann {
 
 // Query operands (scratch of the calling thread, or views of a dataset).
@@ -96,5 +117,13 @@ void ivf_probe(const mann_ivf& iv, const float* R, int nq, int nprobe, int* prob
                int* prows);
 void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int metric,
                 const float* route_in, int64_t* out_ids, float* out_dist);
+
+// learned router (core_router.hip)
+mann_router* router_create(int d, int h, int c, uint64_t seed);
+void router_scores(const mann_router& r, const float* X, int64_t ldx, int64_t n, float* out);
+void router_grad(const mann_router& r, const float* X, int64_t ldx, int B, const int* labels,
+                 int L, float* grad, float* loss);
+double router_train(mann_router& r, const float* X, int64_t ldx, int64_t n, const int* labels,
+                    int L, int epochs, int batch, float lr, uint64_t seed);
 
 }  // namespace mann

@@ -1,0 +1,182 @@
+// Learned probe router: BF16 MFMA GEMM with fused epilogues, softmax
+// cross-entropy against soft list labels, column sums and Adam.
+//
+// gemm: C[m][n] = sum_k A(m,k) * B(n,k) (+ bias[n]) (ReLU) with arbitrary
+// element strides for A and B, so one kernel serves the forward pass (X W^T),
+// the weight gradients (G^T H) and the input gradients (G W).  Operands are
+// FP32 in memory and rounded to bf16 (RNE) when staged into LDS; accumulation
+// is FP32 in the MFMA.
+#include "mlp.h"
+
+namespace mann {
+
+namespace {
+
+constexpr int GT = 64;    // tile (m and n)
+constexpr int GK = 32;    // k chunk
+constexpr int GPAD = 8;
+
+__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
+  __shared__ bf16_t As[GT][GK + GPAD];
+  __shared__ bf16_t Bs[GT][GK + GPAD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;  // 2 x 2 waves of 32 x 32
+  const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+  floatx16 acc;
+  for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+  for (int k0 = 0; k0 < g.K; k0 += GK) {
+    // stage: 64 x 32 elements per operand, 8 per thread
+    for (int i = tid; i < GT * GK; i += 256) {
+      const int r = i / GK, kk = i - r * GK;
+      const int m = m0 + r, n = n0 + r, k = k0 + kk;
+      float av = (m < g.M && k < g.K) ? g.A[(int64_t)m * g.sam + (int64_t)k * g.sak] : 0.0f;
+      float bv = (n < g.N && k < g.K) ? g.B[(int64_t)n * g.sbn + (int64_t)k * g.sbk] : 0.0f;
+      As[r][kk] = (bf16_t)av;
+      Bs[r][kk] = (bf16_t)bv;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < GK / 16; ++ks) {
+      const int koff = ks * 16 + (lane >> 5) * 8;
+      bf16x8 fa = *(const bf16x8*)&As[wm * 32 + (lane & 31)][koff];
+      bf16x8 fb = *(const bf16x8*)&Bs[wn * 32 + (lane & 31)][koff];
+      // output rows follow the A operand, columns the B operand
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, fb, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  const int n = n0 + wn * 32 + (lane & 31);
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const int m = m0 + wm * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+    if (m < g.M && n < g.N) {
+      float v = acc[e];
+      if (g.bias) v += g.bias[n];
+      if (g.relu) v = v > 0.0f ? v : 0.0f;
+      if (g.accumulate) v += g.C[(int64_t)m * g.ldc + n];
+      g.C[(int64_t)m * g.ldc + n] = v;
+    }
+  }
+}
+
+// One workgroup per row: softmax over the logits, cross-entropy against the
+// label distribution (counts / L of the row's list labels), gradient
+// (softmax - target) / batch written in place of the logits.
+__global__ __launch_bounds__(256) void softmax_xent_kernel(float* __restrict__ Z, int ncls, int batch,
+                                                           const int* __restrict__ labels, int L,
+                                                           float* __restrict__ loss) {
+  __shared__ float red[256];
+  __shared__ float s_max, s_sum;
+  const int b = blockIdx.x;
+  float* z = Z + (int64_t)b * ncls;
+  float mx = -INFINITY;
+  for (int c = threadIdx.x; c < ncls; c += 256) mx = fmaxf(mx, z[c]);
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) s_max = red[0];
+  __syncthreads();
+  float sum = 0.0f;
+  for (int c = threadIdx.x; c < ncls; c += 256) sum += __expf(z[c] - s_max);
+  red[threadIdx.x] = sum;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) s_sum = red[0];
+  __syncthreads();
+  const float lse = s_max + __logf(s_sum);
+  const int* lb = labels + (int64_t)b * L;
+  int nl = 0;
+  for (int t = 0; t < L; ++t) nl += lb[t] >= 0;
+  const float w = nl > 0 ? 1.0f / (float)nl : 0.0f;
+  float lsum = 0.0f;
+  for (int c = threadIdx.x; c < ncls; c += 256) {
+    float tc = 0.0f;
+    for (int t = 0; t < L; ++t) tc += lb[t] == c ? w : 0.0f;
+    const float p = __expf(z[c] - lse);
+    lsum += tc > 0.0f ? -tc * (z[c] - lse) : 0.0f;
+    z[c] = (p - tc) / (float)batch;
+  }
+  red[threadIdx.x] = lsum;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss, red[0] / (float)batch);
+}
+
+// out[n] = sum_m G[m][n]
+__global__ void colsum_kernel(const float* __restrict__ G, int M, int N, float* __restrict__ out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  float s = 0.0f;
+  for (int m = 0; m < M; ++m) s += G[(int64_t)m * N + n];
+  out[n] = s;
+}
+
+// dH *= (H > 0)
+__global__ void relu_mask_kernel(float* __restrict__ dH, const float* __restrict__ H, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && !(H[i] > 0.0f)) dH[i] = 0.0f;
+}
+
+__global__ void adam_kernel(float* __restrict__ w, const float* __restrict__ g,
+                            float* __restrict__ m1, float* __restrict__ m2, int64_t n, float lr,
+                            float b1, float b2, float eps, float c1, float c2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float a = b1 * m1[i] + (1.0f - b1) * gi;
+  const float b = b2 * m2[i] + (1.0f - b2) * gi * gi;
+  m1[i] = a;
+  m2[i] = b;
+  w[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
+__global__ void neg_kernel(const float* __restrict__ s, float* __restrict__ d, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = -s[i];
+}
+
+}  // namespace
+
+void launch_gemm(const GemmArgs& g, hipStream_t st) {
+  if (g.M <= 0 || g.N <= 0) return;
+  dim3 grid((unsigned)ceil_div(g.N, GT), (unsigned)ceil_div(g.M, GT));
+  hipLaunchKernelGGL(gemm_kernel, grid, dim3(256), 0, st, g);
+}
+
+void launch_softmax_xent(float* Z, int ncls, int batch, const int* labels, int L, float* loss,
+                         hipStream_t st) {
+  if (batch <= 0) return;
+  hipLaunchKernelGGL(softmax_xent_kernel, dim3(batch), dim3(256), 0, st, Z, ncls, batch, labels, L,
+                     loss);
+}
+
+void launch_colsum(const float* G, int M, int N, float* out, hipStream_t st) {
+  hipLaunchKernelGGL(colsum_kernel, dim3((unsigned)ceil_div(N, 256)), dim3(256), 0, st, G, M, N,
+                     out);
+}
+
+void launch_relu_mask(float* dH, const float* H, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(relu_mask_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, dH, H, n);
+}
+
+void launch_adam(float* w, const float* g, float* m1, float* m2, int64_t n, float lr, float b1,
+                 float b2, float eps, int step, hipStream_t st) {
+  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  hipLaunchKernelGGL(adam_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, w, g, m1, m2,
+                     n, lr, b1, b2, eps, c1, c2);
+}
+
+void launch_neg(const float* s, float* d, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(neg_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, s, d, n);
+}
+
+}  // namespace mann

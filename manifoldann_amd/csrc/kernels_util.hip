@@ -33,6 +33,16 @@ __global__ void gather_rows_f32_kernel(const float* src, const int* idx, int m, 
   const float* s = src + (int64_t)idx[row] * w;
   for (int c = threadIdx.x; c < w; c += blockDim.x) dst[(int64_t)row * w + c] = s[c];
 }
+__global__ void gather_rows_i32_kernel(const int* src, const int* idx, int m, int w, int* dst) {
+  int row = blockIdx.x;
+  if (row >= m) return;
+  const int* s = src + (int64_t)idx[row] * w;
+  for (int c = threadIdx.x; c < w; c += blockDim.x) dst[(int64_t)row * w + c] = s[c];
+}
+__global__ void labels_to_i32_kernel(const int64_t* src, int64_t n, int* dst) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (int)(src[i] - 1);
+}
 __global__ void gather_f32_kernel(const float* src, const int* idx, int m, float* dst) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m) dst[i] = src[idx[i]];
@@ -84,6 +94,15 @@ void gather_rows_bf16(const bf16_t* src, const float* srcn, const int* idx, int 
 void gather_rows_f32(const float* src, const int* idx, int m, int w, float* dst, hipStream_t st) {
   if (m <= 0) return;
   hipLaunchKernelGGL(gather_rows_f32_kernel, dim3(m), dim3(64), 0, st, src, idx, m, w, dst);
+}
+void gather_rows_i32(const int* src, const int* idx, int m, int w, int* dst, hipStream_t st) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(gather_rows_i32_kernel, dim3(m), dim3(64), 0, st, src, idx, m, w, dst);
+}
+void labels_to_i32(const int64_t* src, int64_t n, int* dst, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(labels_to_i32_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, src,
+                     n, dst);
 }
 void gather_f32(const float* src, const int* idx, int m, float* dst, hipStream_t st) {
   if (m <= 0) return;

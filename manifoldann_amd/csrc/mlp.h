@@ -1,0 +1,28 @@
+// Learned probe router kernels (BF16 MFMA GEMM, softmax cross-entropy, Adam).
+#pragma once
+
+#include "launch.h"
+
+namespace mann {
+
+struct GemmArgs {
+  const float* A;       // A(m, k) = A[m * sam + k * sak]
+  const float* B;       // B(n, k) = B[n * sbn + k * sbk]
+  float* C;             // C[m * ldc + n]
+  int64_t sam, sak, sbn, sbk, ldc;
+  int M, N, K;
+  const float* bias;    // optional, per n
+  int relu;             // clamp at 0 after the bias
+  int accumulate;       // C += result
+};
+
+void launch_gemm(const GemmArgs& g, hipStream_t st);
+void launch_softmax_xent(float* Z, int ncls, int batch, const int* labels, int L, float* loss,
+                         hipStream_t st);
+void launch_colsum(const float* G, int M, int N, float* out, hipStream_t st);
+void launch_relu_mask(float* dH, const float* H, int64_t n, hipStream_t st);
+void launch_adam(float* w, const float* g, float* m1, float* m2, int64_t n, float lr, float b1,
+                 float b2, float eps, int step, hipStream_t st);
+void launch_neg(const float* s, float* d, int64_t n, hipStream_t st);
+
+}  // namespace mann

@@ -12,6 +12,9 @@ void gather_rows_bf16(const bf16_t* src, const float* srcn, const int* idx, int 
                       int kp, bf16_t* dst, float* dstn, hipStream_t st);
 void gather_rows_f32(const float* src, const int* idx, int m, int w, float* dst, hipStream_t st);
 void gather_f32(const float* src, const int* idx, int m, float* dst, hipStream_t st);
+void gather_rows_i32(const int* src, const int* idx, int m, int w, int* dst, hipStream_t st);
+// 1-based int64 labels (0 = none) -> 0-based int32 (-1 = none)
+void labels_to_i32(const int64_t* src, int64_t n, int* dst, hipStream_t st);
 
 void ids_to_i64(const int* src, int64_t n, int64_t* dst, hipStream_t st);
 // dst[r][c] = src[r][c] for c < w, 0 for w <= c < ldd

@@ -1,0 +1,133 @@
+"""Learned probe router over the HIP library.
+
+No counterpart exists in the reference: the router is a transform that plugs
+into its AbstractTransform seam (transforms/Transform.jl:36-83).  Its
+`transform(q)` returns KMeansAssignment(-scores), so the unchanged TopKRouting
+(indices/multilevel/routing.jl:94-118) probes the highest-scoring lists first.
+Training labels are the lists holding each training query's exact top-k,
+computed by the exact GPU kNN over the same base.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._lib import ArgumentError, check, lib, shutting_down
+from .index import Dataset, _is_torch, as_matrix
+from .ivf import KMeansAssignment, TransformResult, preserves_data  # noqa: F401
+
+
+class LearnedProbeRouter:
+    """MLP d -> hidden (ReLU) -> nlist with FP32 master weights (bf16 GEMMs)."""
+
+    def __init__(self, d: int, nlist: int, hidden: int = 512, seed: int = 0):
+        self.d, self.nlist, self.hidden = d, nlist, hidden
+        h = ctypes.c_void_p()
+        check(lib().mann_router_create(d, hidden, nlist, seed, ctypes.byref(h)))
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and not shutting_down():
+            try:
+                lib().mann_router_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+    @property
+    def n_params(self) -> int:
+        c = ctypes.c_int64()
+        check(lib().mann_router_params(self.handle, None, ctypes.byref(c)))
+        return int(c.value)
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.n_params, np.float32)
+        check(lib().mann_router_params(self.handle, out.ctypes.data, None))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.n_params:
+            raise ArgumentError("parameter vector has the wrong length")
+        check(lib().mann_router_set_params(self.handle, p.ctypes.data))
+
+    def grad(self, X, labels):
+        """(loss, gradient) of the mean cross-entropy on one batch; X is d x n."""
+        m = as_matrix(X)
+        lab = np.ascontiguousarray(labels, np.int64)
+        g = np.zeros(self.n_params, np.float32)
+        loss = ctypes.c_double()
+        check(lib().mann_router_grad(self.handle, m.ptr, m.d, m.n, m.ld, lab.ctypes.data,
+                                     lab.shape[1], g.ctypes.data, ctypes.byref(loss)))
+        return loss.value, g
+
+    def train(self, X, labels, epochs=5, batch=1024, lr=2e-3, seed=1):
+        m = as_matrix(X)
+        if _is_torch(labels):
+            lp, L = labels.data_ptr(), labels.shape[1]
+        else:
+            lab = np.ascontiguousarray(labels, np.int64)
+            lp, L = lab.ctypes.data, lab.shape[1]
+        loss = ctypes.c_double()
+        check(lib().mann_router_train(self.handle, m.ptr, m.d, m.n, m.ld, lp, L, epochs, batch,
+                                      lr, seed, ctypes.byref(loss)))
+        return loss.value
+
+    def scores(self, Q):
+        m = as_matrix(Q)
+        out = np.zeros((m.n, self.nlist), np.float32)
+        check(lib().mann_router_scores(self.handle, m.ptr, m.d, m.n, m.ld, out.ctypes.data))
+        return out
+
+
+class LearnedProbeTransform:
+    """Transform seam: wraps the fitted KMeansTransform (which still defines the
+    partition) and replaces its routing distances by the router's -scores."""
+
+    def __init__(self, kmeans, router: LearnedProbeRouter):
+        self.kmeans = kmeans
+        self.router = router
+
+    def transform(self, x):
+        s = self.router.scores(np.asarray(x, np.float32).reshape(-1, 1))
+        return TransformResult(x, KMeansAssignment(-s[0]))
+
+    def take_pending_assignments(self):
+        a = self.kmeans.pending_assignments
+        self.kmeans.pending_assignments = None
+        return a
+
+
+def neighbour_list_labels(index, data, train_queries, k=10):
+    """Labels of each training query: the (1-based) lists of its exact top-k."""
+    from .index import BruteForceIndex, query_arrays
+    ds = data if isinstance(data, Dataset) else Dataset(data)
+    bf = BruteForceIndex(ds.d, ds.n)
+    ids, _ = query_arrays(bf, ds, train_queries, k)
+    ids = ids.cpu().numpy() if _is_torch(ids) else ids
+    bucket_of = np.zeros(index.n + 1, np.int64)
+    all_ids = np.zeros(index.n, np.int64)
+    check(lib().mann_ivf_list_ids(index.handle, all_ids.ctypes.data))
+    off = np.concatenate([[0], np.cumsum(index.list_sizes)])
+    for b in range(index.n_buckets):
+        bucket_of[all_ids[off[b]:off[b + 1]]] = b + 1
+    return bucket_of[ids]
+
+
+def query_arrays_routed(index, router: LearnedProbeRouter, Q, k, nprobe):
+    """IVF batched query with the router's routing (TopKRouting over -scores)."""
+    m = as_matrix(Q)
+    if m.device:
+        import torch
+        ids = torch.zeros((m.n, k), dtype=torch.int64, device=m.keep.device)
+        dist = torch.zeros((m.n, k), dtype=torch.float32, device=m.keep.device)
+        pi, pd = ids.data_ptr(), dist.data_ptr()
+    else:
+        ids = np.zeros((m.n, k), np.int64)
+        dist = np.zeros((m.n, k), np.float32)
+        pi, pd = ids.ctypes.data, dist.ctypes.data
+    check(lib().mann_ivf_search_routed(index.handle, router.handle, m.ptr, m.d, m.n, m.ld, k,
+                                       nprobe, index.distance.code, pi, pd))
+    return ids, dist
