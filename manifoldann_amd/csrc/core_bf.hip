@@ -222,7 +222,8 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   const int k_req = k + (exclude_self ? 1 : 0);
   const int k_eff = std::min(k_req, n);
   const int cap_total = select_cap_max();
-  MANN_REQUIRE(k_req <= cap_total / 2, ERR_UNSUPPORTED, "k too large for the GPU selection buffer");
+  MANN_REQUIRE(k_req <= select_exmax() / 2, ERR_UNSUPPORTED,
+               "k too large for the GPU selection buffer (max 256)");
   const int BM = scan_tile_rows(), BN = scan_tile_queries();
   // row partitions: enough workgroups to fill the chip, at most 32 regions
   const int qtiles = (int)ceil_div(nq, BN);
@@ -311,7 +312,7 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   s.k_req = k_req;
   s.exclude_self = exclude_self;
   s.k_eff = k_eff;
-  s.shortlist = std::min(cap_total, std::max(tuning().shortlist_mult * k_req, 32));
+  s.shortlist = std::min(select_exmax() / 2, std::max(tuning().shortlist_mult * k_req, 32));
   s.out_ids = out_ids;
   s.out_dist = out_dist;
   s.out_count = out_count;

@@ -116,7 +116,8 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
   const int nq = (int)Q.nq;
   if (nq == 0 || k <= 0) return;
   const int cap = select_cap_max();
-  MANN_REQUIRE(k <= cap / 2, ERR_UNSUPPORTED, "k too large for the GPU selection buffer");
+  MANN_REQUIRE(k <= select_exmax() / 2, ERR_UNSUPPORTED,
+               "k too large for the GPU selection buffer (max 256)");
   const int BN = scan_tile_queries();
   const float* R = route_in;
   if (!R) {
@@ -237,7 +238,7 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
   s.k_req = k;
   s.k_eff = k;
   s.k_eff_q = keff;
-  s.shortlist = std::min(cap, std::max(tuning().shortlist_mult * k, 32));
+  s.shortlist = std::min(select_exmax() / 2, std::max(tuning().shortlist_mult * k, 32));
   s.out_ids = out_ids;
   s.out_dist = out_dist;
   s.flag_count = ctr + 2;

@@ -92,120 +92,328 @@ __global__ __launch_bounds__(256) void tau_from_sample_kernel(TauArgs a) {
   if (threadIdx.x == 0) a.tau[q] = ord2f(prefix);
 }
 
+// ---- wave-per-query selection ------------------------------------------------
+
+constexpr int KPL = SEL_CAP / 64;   // candidate keys held per lane
+constexpr int EXMAX = 512;          // exactly re-ranked candidates per query
+constexpr int EXR = EXMAX / 64;     // per lane, in the register sort
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Bitonic sort of N (64..EXMAX, power of two) keys: element e lives in lane
+// e % 64, register e / 64.
+template <int R>
+__device__ __forceinline__ void wave_sort(uint64_t (&v)[R], int N, int lane) {
+  for (int size = 2; size <= N; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride >= 64) {
+        const int sr = stride >> 6;
+#pragma unroll
+        for (int jj = 0; jj < R; ++jj) {
+          if (jj * 64 >= N || (jj & sr)) continue;
+          const int e = jj * 64 + lane;
+          const bool asc = (e & size) == 0;
+          uint64_t lo = v[jj], hi = v[jj | sr];
+          if ((lo > hi) == asc) {
+            v[jj] = hi;
+            v[jj | sr] = lo;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int jj = 0; jj < R; ++jj) {
+          if (jj * 64 >= N) continue;
+          const int e = jj * 64 + lane;
+          const uint64_t o = __shfl_xor(v[jj], stride);
+          const bool asc = (e & size) == 0;
+          const bool low = (lane & stride) == 0;
+          const uint64_t mn = v[jj] < o ? v[jj] : o, mx = v[jj] < o ? o : v[jj];
+          v[jj] = (low == asc) ? mn : mx;
+        }
+      }
+    }
+  }
+}
+
+// Exact metric of up to 4 rows at once for one 16-lane group (A16 order each).
+__device__ __forceinline__ void exact_metric4(const SelectArgs& a, const uint32_t* rows, int nr,
+                                              int64_t q, int j, float* out) {
+  const float* y = a.q32 + q * a.d16;
+  float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  const float* x[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) x[u] = a.x32 + (int64_t)rows[u < nr ? u : 0] * a.d16;
+  const bool diff = a.metric == EUCLIDEAN || a.metric == SQEUCLIDEAN;
+  for (int t = 0; t < a.d16; t += 16) {
+    const float yv = y[t + j];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float xv = x[u][t + j];
+      if (diff) {
+        const float df = xv - yv;
+        acc[u] = __builtin_fmaf(df, df, acc[u]);
+      } else {
+        acc[u] = __builtin_fmaf(xv, yv, acc[u]);
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    float v = group16_tree(acc[u]);
+    const int64_t r = rows[u < nr ? u : 0];
+    if (a.metric == EUCLIDEAN) v = __builtin_sqrtf(v);
+    else if (a.metric == COSINE) {
+      float cv = 1.0f - v / (__builtin_sqrtf(a.xn2[r]) * __builtin_sqrtf(a.qn2[q]));
+      v = cv > 0.0f ? cv : 0.0f;
+    } else if (a.metric == NORMS_FORM) {
+      float tt = a.xn2[r] + a.qn2[q];
+      float sq = tt - 2.0f * v;
+      v = __builtin_sqrtf(sq > 0.0f ? sq : 0.0f);
+    }
+    out[u] = v;
+  }
+}
+
+// Re-rank ex[from, to) (candidate keys -> exact keys) with the wave's 4 groups.
+__device__ __forceinline__ void rerank_range(const SelectArgs& a, uint64_t* ex, int from, int to,
+                                             int64_t q, int lane) {
+  const int g = lane >> 4, j = lane & 15;
+  for (int i0 = from + g * 4; i0 < to; i0 += 16) {
+    const int nr = min(4, to - i0);
+    uint32_t rows[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) rows[u] = u < nr ? (uint32_t)(ex[i0 + u] & 0xffffffffu) : 0u;
+    float dv[4];
+    exact_metric4(a, rows, nr, q, j, dv);
+    if (j == 0)
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (u < nr) ex[i0 + u] = pack_key(dv[u], rows[u]);
+  }
+  wave_sync();
+}
+
 __global__ __launch_bounds__(256) void select_rerank_kernel(SelectArgs a) {
+  __shared__ unsigned hist_all[4][256];
+  __shared__ uint64_t ex_all[4][EXMAX];
+  __shared__ int pre_all[4][40];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nq = a.nq_dev ? *a.nq_dev : a.nq;
-  if ((int)blockIdx.x >= nq) return;
-  const int q = a.qlist ? a.qlist[blockIdx.x] : (int)blockIdx.x;
-  __shared__ uint64_t keys[SEL_CAP];
-  __shared__ uint64_t ex[SEL_CAP];
-  __shared__ int s_m1;
-  __shared__ double s_T;
-  const int tid = threadIdx.x, g = tid >> 4, j = tid & 15;
+  const int slot = blockIdx.x * 4 + w;
+  if (slot >= nq) return;
+  const int q = a.qlist ? a.qlist[slot] : slot;
+  unsigned* hist = hist_all[w];
+  uint64_t* ex = ex_all[w];
+  int* pre = pre_all[w];
 
-  int c = 0;
-  bool overflow = false;
-  for (int p = 0; p < a.parts; ++p) {
-    const int cp = a.cnt[(int64_t)q * a.parts + p];
-    overflow |= cp > a.cap;
-    c += cp < a.cap ? cp : a.cap;
+  // ---- candidates of all regions, KPL keys per lane
+  int cp = 0;
+  bool ovf = false;
+  if (lane < a.parts) {
+    const int v = a.cnt[(int64_t)q * a.parts + lane];
+    ovf = v > a.cap;
+    cp = v < a.cap ? v : a.cap;
   }
-  const int P = next_pow2(c > 0 ? c : 1);
-  for (int i = tid; i < P; i += 256) {
-    uint64_t key = ~0ull;
+  bool overflow = __ballot(ovf) != 0;
+  int inc = cp;
+  for (int o = 1; o < 64; o <<= 1) {
+    int t = __shfl_up(inc, o);
+    if (lane >= o) inc += t;
+  }
+  if (lane < a.parts) pre[lane] = inc - cp;  // exclusive prefix
+  const int c = __shfl(inc, 63);
+  wave_sync();
+  uint64_t key[KPL];
+#pragma unroll
+  for (int t = 0; t < KPL; ++t) {
+    const int i = t * 64 + lane;
+    uint64_t kv = ~0ull;
     if (i < c) {
-      int p = 0, base = 0;
-      for (;; ++p) {
-        int cp = a.cnt[(int64_t)q * a.parts + p];
-        cp = cp < a.cap ? cp : a.cap;
-        if (i < base + cp) break;
-        base += cp;
-      }
+      int p = a.parts - 1;
+      while (p > 0 && pre[p] > i) --p;
       const unsigned* cq = a.cand + ((int64_t)q * a.parts + p) * (int64_t)a.cap * 2;
-      uint2 e = *(const uint2*)(cq + 2 * (i - base));
-      key = pack_key(__uint_as_float(e.x), e.y);
+      const uint2 e = *(const uint2*)(cq + 2 * (i - pre[p]));
+      kv = pack_key(__uint_as_float(e.x), e.y);
     }
-    keys[i] = key;
+    key[t] = kv;
   }
-  __syncthreads();
-  bitonic_sort(keys, P);
 
-  const double qn = a.qn[q], qe = a.qe[q], qr = a.qr[q];
+  // ---- shortlist cutoff: the s-th smallest a' (radix select on its order key)
+  const int s = c < a.shortlist ? c : a.shortlist;
+  uint32_t cut = 0;
+  if (s > 0) {
+    uint32_t prefix = 0, mask = 0;
+    unsigned want = (unsigned)s;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b) hist[4 * lane + b] = 0;
+      wave_sync();
+#pragma unroll
+      for (int t = 0; t < KPL; ++t) {
+        const uint32_t o = (uint32_t)(key[t] >> 32);
+        if (t * 64 + lane < c && (o & mask) == prefix) atomicAdd(&hist[(o >> shift) & 255u], 1u);
+      }
+      wave_sync();
+      const unsigned h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2],
+                     h3 = hist[4 * lane + 3];
+      const unsigned tot = h0 + h1 + h2 + h3;
+      unsigned cum = tot;
+      for (int o = 1; o < 64; o <<= 1) {
+        unsigned t = __shfl_up(cum, o);
+        if (lane >= o) cum += t;
+      }
+      const unsigned excl = cum - tot;
+      const bool here = excl < want && cum >= want;
+      const unsigned long long hm = __ballot(here);
+      const int src = __ffsll((long long)hm) - 1;
+      unsigned bin = 0, nw = 0;
+      if (here) {
+        unsigned cc = excl;
+        if (cc + h0 >= want) { bin = 4 * lane; nw = want - cc; }
+        else if (cc + h0 + h1 >= want) { bin = 4 * lane + 1; nw = want - cc - h0; }
+        else if (cc + h0 + h1 + h2 >= want) { bin = 4 * lane + 2; nw = want - cc - h0 - h1; }
+        else { bin = 4 * lane + 3; nw = want - cc - h0 - h1 - h2; }
+      }
+      bin = __shfl(bin, src);
+      want = __shfl(nw, src);
+      prefix |= bin << shift;
+      mask |= 255u << shift;
+      wave_sync();
+    }
+    cut = prefix;
+  }
+
+  // ---- S1 = candidates with a' <= cutoff (ties included), compacted into ex
+  int n1 = 0;
+  bool exovf = false;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int t = 0; t < KPL; ++t) {
+    const bool pred = s > 0 && t * 64 + lane < c && (uint32_t)(key[t] >> 32) <= cut;
+    const unsigned long long m = __ballot(pred);
+    const int pos = n1 + __popcll(m & lt);
+    if (pred && pos < EXMAX) ex[pos] = key[t];
+    n1 += __popcll(m);
+  }
+  if (n1 > EXMAX) {
+    exovf = true;
+    n1 = EXMAX;
+  }
+  wave_sync();
+  rerank_range(a, ex, 0, n1, q, lane);
+
+  // ---- exact order, D_k, exclusion threshold, second batch
   const int k_eff = a.k_eff_q ? a.k_eff_q[q] : a.k_eff;
-  int done = 0;
-  int target = c < a.shortlist ? c : a.shortlist;
-  double T = INFINITY;
-  for (int round = 0; round < 4; ++round) {
-    for (int i = done + g; i < target; i += 16) {
-      uint32_t r = (uint32_t)(keys[i] & 0xffffffffu);
-      float dv = exact_metric(a.x32, a.q32, a.xn2, a.qn2, a.d16, a.metric, r, q, j);
-      if (j == 0) ex[i] = pack_key(dv, r);
+  uint64_t v[EXR];
+  auto sort_ex = [&](int n) {
+    int N = 64;
+    while (N < n) N <<= 1;
+#pragma unroll
+    for (int jj = 0; jj < EXR; ++jj) {
+      const int e = jj * 64 + lane;
+      v[jj] = e < n ? ex[e] : ~0ull;
     }
-    done = target;
-    int P2 = next_pow2(done > 0 ? done : 1);
-    for (int i = done + tid; i < P2; i += 256) ex[i] = ~0ull;
-    __syncthreads();
-    bitonic_sort(ex, P2);
-    if (tid == 0) {
-      double Tn = INFINITY;
-      if (done >= k_eff && k_eff > 0) {
-        float dk = ord2f((uint32_t)(ex[k_eff - 1] >> 32));
-        Tn = exclusion_threshold(dk, a.metric, qn, qe, qr, a.ex, a.rx, a.d);
-      }
-      // rows with a' <= Tn must be re-ranked: count them in the sorted keys
-      int lo = 0, hi = c;
-      while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        float av = ord2f((uint32_t)(keys[mid] >> 32));
-        if ((double)av <= Tn) lo = mid + 1;
-        else hi = mid;
-      }
-      s_m1 = lo;
-      s_T = Tn;
+    wave_sort(v, N, lane);
+#pragma unroll
+    for (int jj = 0; jj < EXR; ++jj) {
+      const int e = jj * 64 + lane;
+      if (e < n) ex[e] = v[jj];
     }
-    __syncthreads();
-    T = s_T;
-    const int m1 = s_m1;
-    __syncthreads();
-    if (m1 <= done) break;
-    target = m1;
+    wave_sync();
+  };
+  auto threshold = [&](int n) -> double {
+    double T = INFINITY;
+    if (n >= k_eff && k_eff > 0) {
+      const uint64_t kk = ex[k_eff - 1];
+      T = exclusion_threshold(ord2f((uint32_t)(kk >> 32)), a.metric, a.qn[q], a.qe[q], a.qr[q],
+                              a.ex, a.rx, a.d);
+    }
+    return T;
+  };
+  sort_ex(n1);
+  double T = threshold(n1);
+  int n = n1;
+  {
+    // S2: cutoff < a' and a' <= T
+#pragma unroll
+    for (int t = 0; t < KPL; ++t) {
+      const uint32_t o = (uint32_t)(key[t] >> 32);
+      const bool pred = t * 64 + lane < c && (s == 0 || o > cut) &&
+                        (double)ord2f(o) <= T;
+      const unsigned long long m = __ballot(pred);
+      const int pos = n + __popcll(m & lt);
+      if (pred && pos < EXMAX) ex[pos] = key[t];
+      n += __popcll(m);
+    }
+    if (n > EXMAX) {
+      exovf = true;
+      n = EXMAX;
+    }
+    wave_sync();
+    if (n > n1) {
+      rerank_range(a, ex, n1, n, q, lane);
+      sort_ex(n);
+      T = threshold(n);
+    }
   }
 
-  if (tid == 0) {
-    const float tq = a.tau[q];
-    bool certified;
-    if (overflow) certified = false;
-    else if (tq == INFINITY) certified = true;
-    else certified = (done >= k_eff) && ((double)tq > T);
+  // ---- certification and outputs
+  const float tq = a.tau[q];
+  const bool bad = overflow || exovf;
+  bool certified;
+  if (bad) certified = false;
+  else if (tq == INFINITY) certified = true;
+  else certified = (n >= k_eff) && ((double)tq > T);
+  if (lane == 0) {
     if (!certified) {
-      int f = atomicAdd(a.flag_count, 1);
+      const int f = atomicAdd(a.flag_count, 1);
       a.flag_list[f] = q;
       float tn = INFINITY;
-      if (done >= k_eff && T < 3.0e38) tn = nextafterf((float)T, INFINITY);
+      if (n >= k_eff && T < 3.0e38) tn = nextafterf((float)T, INFINITY);
       if (!(tn > tq) && !overflow) tn = INFINITY;  // no progress possible: emit everything
       a.tau_new[q] = tn;
     }
     if (a.stats) {
-      atomicAdd(a.stats, done);
-      if (overflow) atomicAdd(a.stats + 1, 1);
-    }
-    int o = 0;
-    int64_t* oi = a.out_ids + (int64_t)q * a.k;
-    float* od = a.out_dist + (int64_t)q * a.k;
-    int lim = done < a.k_req ? done : a.k_req;
-    for (int i = 0; i < lim && o < a.k; ++i) {
-      uint32_t r = (uint32_t)(ex[i] & 0xffffffffu);
-      int64_t id = a.perm ? (int64_t)a.perm[r] : (int64_t)r;
-      if (a.exclude_self && id == q) continue;
-      oi[o] = id + 1;
-      od[o] = ord2f((uint32_t)(ex[i] >> 32));
-      ++o;
-    }
-    if (a.out_count) a.out_count[q] = o;
-    for (int i = o; i < a.k; ++i) {
-      oi[i] = 0;
-      od[i] = INFINITY;
+      atomicAdd(a.stats, n);
+      if (bad) atomicAdd(a.stats + 1, 1);
     }
   }
+  const int lim = n < a.k_req ? n : a.k_req;
+  int64_t* oi = a.out_ids + (int64_t)q * a.k;
+  float* od = a.out_dist + (int64_t)q * a.k;
+  // position of the query itself among the first lim (kNN graph self removal)
+  int selfpos = -1;
+  for (int e0 = 0; e0 < lim; e0 += 64) {
+    const int e = e0 + lane;
+    bool is_self = false;
+    if (a.exclude_self && e < lim) {
+      const uint32_t r = (uint32_t)(ex[e] & 0xffffffffu);
+      const int64_t id = a.perm ? (int64_t)a.perm[r] : (int64_t)r;
+      is_self = id == q;
+    }
+    const unsigned long long m = __ballot(is_self);
+    if (m && selfpos < 0) selfpos = e0 + __ffsll((long long)m) - 1;
+  }
+  const int count = (selfpos >= 0 ? lim - 1 : lim) < a.k ? (selfpos >= 0 ? lim - 1 : lim) : a.k;
+  for (int e = lane; e < lim; e += 64) {
+    if (e == selfpos) continue;
+    const int o = (selfpos >= 0 && e > selfpos) ? e - 1 : e;
+    if (o >= a.k) continue;
+    const uint64_t kk = ex[e];
+    const uint32_t r = (uint32_t)(kk & 0xffffffffu);
+    const int64_t id = a.perm ? (int64_t)a.perm[r] : (int64_t)r;
+    oi[o] = id + 1;
+    od[o] = ord2f((uint32_t)(kk >> 32));
+  }
+  for (int o = count + lane; o < a.k; o += 64) {
+    oi[o] = 0;
+    od[o] = INFINITY;
+  }
+  if (a.out_count && lane == 0) a.out_count[q] = count;
 }
 
 __global__ __launch_bounds__(256) void exact_topk_kernel(ExactArgs a) {
@@ -271,6 +479,7 @@ __global__ __launch_bounds__(256) void exact_topk_kernel(ExactArgs a) {
 }  // namespace
 
 int select_cap_max() { return SEL_CAP; }
+int select_exmax() { return EXMAX; }
 int exact_kmax() { return EX_KMAX; }
 
 void launch_tau_from_sample(const TauArgs& a, hipStream_t st) {
@@ -278,9 +487,10 @@ void launch_tau_from_sample(const TauArgs& a, hipStream_t st) {
   hipLaunchKernelGGL(tau_from_sample_kernel, dim3(a.nq), dim3(256), 0, st, a);
 }
 
-void launch_select_rerank(const SelectArgs& a, int grid, hipStream_t st) {
-  if (grid <= 0) return;
-  hipLaunchKernelGGL(select_rerank_kernel, dim3(grid), dim3(256), 0, st, a);
+void launch_select_rerank(const SelectArgs& a, int nq_max, hipStream_t st) {
+  if (nq_max <= 0) return;
+  hipLaunchKernelGGL(select_rerank_kernel, dim3((unsigned)ceil_div(nq_max, 4)), dim3(256), 0, st,
+                     a);
 }
 
 void launch_exact_topk(const ExactArgs& a, int grid, hipStream_t st) {

@@ -120,7 +120,8 @@ struct SelectArgs {
   int* stats;          // [0] reranked candidates total, [1] overflow count
 };
 enum ExtraMetric : int { NORMS_FORM = 3 };
-void launch_select_rerank(const SelectArgs& a, int grid, hipStream_t st);
+// one wavefront per query; nq_max bounds the device-side query count
+void launch_select_rerank(const SelectArgs& a, int nq_max, hipStream_t st);
 
 struct ExactArgs {
   const int* qlist;

@@ -22,6 +22,7 @@ void copy_rows(const float* src, int64_t lds, float* dst, int64_t ldd, int w, in
                hipStream_t st);
 
 int select_cap_max();
+int select_exmax();   // exactly re-ranked candidates per query
 int exact_kmax();
 
 }  // namespace mann
