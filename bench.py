@@ -170,7 +170,13 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
     Tqd = torch.from_numpy(Tq).cuda()
     labels = RT.neighbour_list_labels(index, ds, Tqd.t(), k=10)
     r = RT.LearnedProbeRouter(d, nlist, hidden=512, seed=11)
-    loss = r.train(Tqd.t(), labels, epochs=args.epochs, batch=1024, lr=2e-3, seed=5)
+    gamma = None
+    if not args.no_prior:
+        m = min(16384, args.train_q)
+        gamma = r.fit_prior(index, Tqd[:m].t(), labels[:m])
+        r.zero_output_layer()
+        log(f"IVF: centroid prior scale {gamma:.4g}")
+    loss = r.train(Tqd.t(), labels, epochs=args.epochs, batch=1024, lr=args.lr, seed=5)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t1
     log(f"IVF: router labels+training {train_s:.2f} s, final loss {loss:.3f}")
@@ -191,14 +197,16 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
     bc, bl = at95(sweep_c), at95(sweep_l)
     # kernel timing of the list scan at the centroid operating point
     kern = {}
+    ivf_stats = None
     if bc:
         check(L.mann_profile_enable(1))
         check(L.mann_profile_reset())
         check(L.mann_ivf_search(index.handle, Qd.data_ptr(), d, nq, d, 10, bc["nprobe"], 0, None,
                                 ids.data_ptr(), dist.data_ptr()))
-        kern = kernel_avg_ms(L, check, ["prep_queries", "ivf_route", "ivf_probe", "ivf_tau",
-                                        "ivf_plan", "ivf_scan", "ivf_select",
+        kern = kernel_avg_ms(L, check, ["prep_queries", "ivf_route", "ivf_probe", "ivf_plan",
+                                        "ivf_residual", "ivf_tau", "ivf_scan", "ivf_select",
                                         "ivf_exact_fallback"])
+        ivf_stats = M.last_stats()
         check(L.mann_profile_enable(0))
     cpu = None
     if not args.skip_cpu and bc:
@@ -220,10 +228,11 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
     out = {"workload": f"IVF n={n} d={d} nlist={nlist} iters={t.n_iters} nq={nq} k=10",
            "kmeans_fit_s": round(fit_s, 3), "router_train_s": round(train_s, 3),
            "router_final_loss": round(loss, 4), "train_queries": args.train_q,
+           "router_prior_scale": gamma, "router_epochs": args.epochs,
            "centroid": sweep_c, "learned": sweep_l,
            "centroid_at_0.95": bc, "learned_at_0.95": bl,
            "learned_over_centroid_qps": (round(bl["qps"] / bc["qps"], 3) if bc and bl else None),
-           "kernels_ms": kern, "cpu_baseline": cpu,
+           "kernels_ms": kern, "search_stats": ivf_stats, "cpu_baseline": cpu,
            "total_s": round(time.perf_counter() - t_all, 1)}
     return out
 
@@ -244,11 +253,15 @@ def main():
     ap.add_argument("--ivf-nq", type=int, default=10_000)
     ap.add_argument("--train-q", type=int, default=200_000)
     ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--no-prior", action="store_true",
+                    help="train the router without the centroid prior")
     ap.add_argument("--reps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the sharded design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: the search path runs on one GPU; --gpus {args.gpus} is not "
+              "supported (multi-GPU sharding is a design, not an implementation)",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -153,6 +153,11 @@ int mann_router_create(int64_t d, int64_t hidden, int64_t nlist, uint64_t seed,
 int mann_router_destroy(mann_router_t r);
 int mann_router_params(mann_router_t r, float* out, int64_t* count);
 int mann_router_set_params(mann_router_t r, const float* params);
+/* Centroid prior (gamma != 0): the logits gain gamma * (2 q.c_l - |c_l|^2), i.e.
+ * -gamma |q - c_l|^2 up to a per-query constant, with c_l the index's centroids
+ * (copied).  The MLP then learns a correction of the centroid routing.
+ * gamma == 0 removes the prior (ivf may be null). */
+int mann_router_set_prior(mann_router_t r, mann_ivf_t ivf, double gamma);
 /* gradient of the mean softmax cross-entropy over one batch (no update) */
 int mann_router_grad(mann_router_t r, const float* X, int64_t d, int64_t n, int64_t ldx,
                      const int64_t* labels, int64_t L, float* out_grad, double* out_loss);

@@ -45,6 +45,7 @@ SIGNATURES = [
     ("mann_router_destroy", ctypes.c_int, [_vp]),
     ("mann_router_params", ctypes.c_int, [_vp, _vp, _i64p]),
     ("mann_router_set_params", ctypes.c_int, [_vp, _vp]),
+    ("mann_router_set_prior", ctypes.c_int, [_vp, _vp, ctypes.c_double]),
     ("mann_router_grad", ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _vp, _i64, _vp,
                                         ctypes.POINTER(ctypes.c_double)]),
     ("mann_router_train", ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64,

@@ -122,7 +122,7 @@ int mann_bf_knn(mann_dataset_t base, const float* queries, int64_t d, int64_t nq
                  "dataset was prepared for another metric");
     MANN_REQUIRE(nq < (1 << 30), ERR_UNSUPPORTED, "too many queries in one call");
     QueryOps Q = prepare_queries(queries, d, nq, ldq, base->normalized, (int)base->d16,
-                                 (int)base->kp, "bfq");
+                                 (int)base->kp, "bfq", base->center);
     OutBuf<int64_t> oi(out_ids, (size_t)(nq * k), "out_ids");
     OutBuf<float> od(out_dist, (size_t)(nq * k), "out_dist");
     bf_search(*base, Q, (int)k, metric, 0, oi.dev, od.dev, nullptr);
@@ -541,6 +541,19 @@ int mann_router_set_params(mann_router_t r, const float* params) {
   });
 }
 
+int mann_router_set_prior(mann_router_t r, mann_ivf_t ivf, double gamma) {
+  return guarded([&] {
+    MANN_REQUIRE(r != nullptr, ERR_NULL, "null router");
+    if (gamma != 0.0) {
+      MANN_REQUIRE(ivf != nullptr, ERR_NULL, "null index");
+      MANN_REQUIRE(r->c == ivf->nlist && r->d == ivf->d, ERR_DIMENSION,
+                   "router does not match the index");
+    }
+    MANN_HIP(hipSetDevice(r->device));
+    router_set_prior(*r, ivf, (float)gamma);
+  });
+}
+
 int mann_router_grad(mann_router_t r, const float* X, int64_t d, int64_t n, int64_t ldx,
                      const int64_t* labels, int64_t L, float* out_grad, double* out_loss) {
   return guarded([&] {
@@ -631,8 +644,11 @@ int mann_tune_set(const char* key, int64_t value) {
     std::string k(key);
     Tuning& t = tuning();
     if (k == "scan_cfg") {
-      MANN_REQUIRE(value >= 0 && value <= 2, ERR_ARGUMENT, "scan_cfg must be 0, 1 or 2");
+      MANN_REQUIRE(value >= 0 && value < scan_num_cfg(), ERR_ARGUMENT, "scan_cfg out of range");
       t.scan_cfg = (int)value;
+    } else if (k == "xcd_map") {
+      MANN_REQUIRE(value >= 0 && value <= 2, ERR_ARGUMENT, "xcd_map must be 0, 1 or 2");
+      t.xcd_map = (int)value;
     } else if (k == "target_groups") {
       MANN_REQUIRE(value >= 1, ERR_ARGUMENT, "target_groups must be positive");
       t.target_groups = (int)value;
@@ -654,6 +670,7 @@ int mann_tune_get(const char* key, int64_t* value) {
     std::string k(key);
     const Tuning& t = tuning();
     if (k == "scan_cfg") *value = t.scan_cfg;
+    else if (k == "xcd_map") *value = t.xcd_map;
     else if (k == "target_groups") *value = t.target_groups;
     else if (k == "sample_rank") *value = t.sample_rank;
     else if (k == "shortlist_mult") *value = t.shortlist_mult;

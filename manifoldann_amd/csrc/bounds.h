@@ -6,14 +6,15 @@
 
 namespace mann {
 
-// Smallest a' threshold T such that every row with a' > T has an FP32 exact
-// metric strictly above dk.  Ingredients: |a' + |q~|^2 - |q~ - x~|^2| <= G
+// Smallest first-pass threshold T (squared-distance domain) such that every row
+// whose first-pass value S^ exceeds T has an FP32 exact metric strictly above dk.
+// Ingredients: |S^ - |q~ - x~|^2| <= G
 // (FP32 accumulation of bf16 products), |x - x~| <= ex and |q - q~| <= qe
 // (bf16 rounding, triangle inequality), relative error rho of the FP32
 // difference form, absolute error of the cosine form, error H of the norms
 // form.  metric: EUCLIDEAN / SQEUCLIDEAN / COSINE / NORMS_FORM.
-__device__ inline double exclusion_threshold(double dk, int metric, double qn, double qe,
-                                             double qr, double ex, double rx, int d) {
+__device__ inline double exclusion_threshold(double dk, int metric, double qe, double qr,
+                                             double ex, double rx, int d) {
   const double u = 1.0 / 16777216.0;
   double R = qr + rx;
   double G = (2.0 * d + 8.0) * u * R * R;
@@ -32,7 +33,22 @@ __device__ inline double exclusion_threshold(double dk, int metric, double qn, d
     double f = dk / (1.0 - 8.0 * u);
     L = sqrt(f * f + H) + qe + ex;
   }
-  return (L * L - qn + G) * (1.0 + 1e-9) + 1e-30;
+  return (L * L + G) * (1.0 + 1e-9) + 1e-30;
+}
+
+// Final step of the exact FP32 metric from the A16 sum acc (the sum of squared
+// differences for the Euclidean metrics, the dot product otherwise).
+__device__ __forceinline__ float finish_metric(float acc, int metric, float xn2r, float qn2q) {
+  if (metric == EUCLIDEAN) return __builtin_sqrtf(acc);
+  if (metric == SQEUCLIDEAN) return acc;
+  if (metric == COSINE) {
+    float v = 1.0f - acc / (__builtin_sqrtf(xn2r) * __builtin_sqrtf(qn2q));
+    return v > 0.0f ? v : 0.0f;
+  }
+  // norms form (kmeans/distance.jl:99-101): sqrt(max((|c|^2 + |x|^2) - 2 c.x, 0))
+  float tt = xn2r + qn2q;
+  float sq = tt - 2.0f * acc;
+  return __builtin_sqrtf(sq > 0.0f ? sq : 0.0f);
 }
 
 // Exact FP32 metric of base row r against query q (16-lane group, A16 order).
@@ -53,16 +69,8 @@ __device__ __forceinline__ float exact_metric(const float* __restrict__ x32,
     for (int t = 0; t < d16; t += 16) acc = __builtin_fmaf(x[t + j], y[t + j], acc);
   }
   acc = group16_tree(acc);
-  if (metric == EUCLIDEAN) return __builtin_sqrtf(acc);
-  if (metric == SQEUCLIDEAN) return acc;
-  if (metric == COSINE) {
-    float v = 1.0f - acc / (__builtin_sqrtf(xn2[r]) * __builtin_sqrtf(qn2[q]));
-    return v > 0.0f ? v : 0.0f;
-  }
-  // norms form (kmeans/distance.jl:99-101): sqrt(max((|c|^2 + |x|^2) - 2 c.x, 0))
-  float tt = xn2[r] + qn2[q];
-  float sq = tt - 2.0f * acc;
-  return __builtin_sqrtf(sq > 0.0f ? sq : 0.0f);
+  const bool diff = metric == EUCLIDEAN || metric == SQEUCLIDEAN;
+  return finish_metric(acc, metric, diff ? 0.0f : xn2[r], diff ? 0.0f : qn2[q]);
 }
 
 }  // namespace mann

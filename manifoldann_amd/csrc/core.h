@@ -16,6 +16,7 @@ struct mann_dataset {
   float* err = nullptr;          // |x - x~| per row (rounded up)
   float* nrm = nullptr;          // |x~| per row (rounded up)
   float* xn2 = nullptr;          // A16 |x|^2 per row
+  float* center = nullptr;       // d floats subtracted before the bf16 rounding (or null)
   float ex = 0.0f, rx = 0.0f;    // max err, max nrm
   // stride sample for threshold estimation
   int m = 0, m_pad = 0;
@@ -59,6 +60,11 @@ struct mann_router {
   float* m1 = nullptr;  // Adam moments
   float* m2 = nullptr;
   int64_t nparam = 0;
+  // optional centroid prior: logits += gamma * (2 x.c - |c|^2) (c x pld floats)
+  float* pc = nullptr;
+  float* pcn = nullptr;
+  int64_t pld = 0;
+  float gamma = 0.0f;
   float* W1() const { return w; }
   float* b1() const { return w + (int64_t)h * d; }
   float* W2() const { return b1() + h; }
@@ -68,6 +74,8 @@ struct mann_router {
     if (w) (void)hipFree(w);
     if (m1) (void)hipFree(m1);
     if (m2) (void)hipFree(m2);
+    if (pc) (void)hipFree(pc);
+    if (pcn) (void)hipFree(pcn);
   }
 };
 
@@ -86,7 +94,7 @@ struct QueryOps {
 
 // Upload (if host) and convert a query matrix; scratch of this thread.
 QueryOps prepare_queries(const float* q, int64_t d, int64_t nq, int64_t ldq, int normalize,
-                         int d16, int kp, const char* tag);
+                         int d16, int kp, const char* tag, const float* center = nullptr);
 QueryOps dataset_as_queries(const mann_dataset& ds);
 
 mann_dataset* create_dataset(const float* data, int64_t d, int64_t n, int64_t ld, int metric);
@@ -99,6 +107,11 @@ SearchStats& last_stats();
 // Exact k nearest rows of `ds` for each query (bf16 MFMA first pass, certified
 // FP32 re-rank, refinement pass, exact fallback).  out_* are device pointers,
 // nq x k.  exclude_self drops the row whose id equals the query index.
+// Exact fallback for the nflag queries listed in e.qlist (count also in
+// *e.nq_dev): segmented exact top-k over their row ranges; max_segments bounds
+// the segments of one query (rows / exact_seg_rows() + ranges).
+void run_exact_fallback(ExactArgs e, int nflag, int max_segments);
+
 void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int exclude_self,
                int64_t* out_ids, float* out_dist, int* out_count);
 
@@ -120,6 +133,8 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
 
 // learned router (core_router.hip)
 mann_router* router_create(int d, int h, int c, uint64_t seed);
+// attach (gamma != 0) or drop (gamma == 0) the centroid prior of an IVF index
+void router_set_prior(mann_router& r, const mann_ivf* iv, float gamma);
 void router_scores(const mann_router& r, const float* X, int64_t ldx, int64_t n, float* out);
 void router_grad(const mann_router& r, const float* X, int64_t ldx, int B, const int* labels,
                  int L, float* grad, float* loss);

@@ -91,7 +91,7 @@ SearchStats& last_stats() {
 
 mann_dataset::~mann_dataset() {
   (void)hipSetDevice(device);
-  void* ps[] = {x32, xb, xn, err, nrm, xn2, sb, sxn};
+  void* ps[] = {x32, xb, xn, err, nrm, xn2, sb, sxn, center};
   for (void* p : ps)
     if (p) (void)hipFree(p);
 }
@@ -132,8 +132,15 @@ mann_dataset* create_dataset(const float* data, int64_t d, int64_t n, int64_t ld
     MANN_HIP(hipMalloc(&ds->xn2, sizeof(float) * ds->n_pad));
     unsigned* mx = c.buf("ds_max").as<unsigned>(2);
     fill_i32((int*)mx, 2, 0, st);
+    if (!ds->normalized) {
+      // first pass on centred vectors: |x - mean| is smaller than |x|, so the
+      // bf16 rounding bounds (and the certified band) shrink
+      MANN_HIP(hipMalloc(&ds->center, sizeof(float) * d));
+      double* tmp = c.buf("ds_colsum").as<double>(d);
+      launch_mean_rows(src, ld, n, (int)d, tmp, ds->center, st);
+    }
     launch_prep_rows(src, ld, n, ds->n_pad, (int)d, (int)ds->d16, (int)ds->kp, ds->normalized,
-                     ds->x32, ds->xb, ds->xn, ds->err, ds->nrm, mx, mx + 1, st);
+                     ds->x32, ds->xb, ds->xn, ds->err, ds->nrm, mx, mx + 1, st, ds->center);
     launch_a16_norms(ds->x32, n, (int)ds->d16, ds->xn2, st);
     // stride sample: rows floor(i * n / m)
     ds->m = (int)std::min<int64_t>(n, SAMPLE_ROWS);
@@ -160,7 +167,7 @@ mann_dataset* create_dataset(const float* data, int64_t d, int64_t n, int64_t ld
 }
 
 QueryOps prepare_queries(const float* q, int64_t d, int64_t nq, int64_t ldq, int normalize,
-                         int d16, int kp, const char* tag) {
+                         int d16, int kp, const char* tag, const float* center) {
   ThreadCtx& c = ctx();
   hipStream_t st = c.stream;
   std::string t(tag);
@@ -184,7 +191,7 @@ QueryOps prepare_queries(const float* q, int64_t d, int64_t nq, int64_t ldq, int
   {
     ScopedTime tm("prep_queries", st);
     launch_prep_rows(src, ldq, nq, nq_pad, (int)d, d16, kp, normalize, q32, qb, qn, qe, qr,
-                     nullptr, nullptr, st);
+                     nullptr, nullptr, st, center);
     launch_a16_norms(q32, nq, d16, qn2, st);
   }
   Q.q32 = q32;
@@ -209,6 +216,19 @@ QueryOps dataset_as_queries(const mann_dataset& ds) {
 }
 
 // ------------------------------------------------------------ exact search --
+
+void run_exact_fallback(ExactArgs e, int nflag, int max_segments) {
+  if (nflag <= 0) return;
+  ThreadCtx& c = ctx();
+  const int kr = std::min(e.k_req, exact_kmax());
+  const int CH = 2048;
+  e.nseg = max_segments;
+  e.partial = c.buf("ex_partial").as<uint64_t>((size_t)std::min(nflag, CH) * max_segments * kr);
+  for (int base = 0; base < nflag; base += CH) {
+    e.slot_base = base;
+    launch_exact_topk(e, std::min(CH, nflag - base), c.stream);
+  }
+}
 
 void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int exclude_self,
                int64_t* out_ids, float* out_dist, int* out_count) {
@@ -252,6 +272,7 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
     a.xb = ds.sb;
     a.xn = ds.sxn;
     a.qb = Q.qb;
+    a.qnorm = Q.qn;
     a.kp = (int)ds.kp;
     a.d = (int)ds.d;
     a.nq = nq;
@@ -274,6 +295,9 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   a.xb = ds.xb;
   a.xn = ds.xn;
   a.qb = Q.qb;
+  a.qnorm = Q.qn;
+  a.xcd_map = tuning().xcd_map;
+  a.qtiles = qtiles;
   a.kp = (int)ds.kp;
   a.d = (int)ds.d;
   a.nq = nq;
@@ -303,7 +327,6 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   s.d = (int)ds.d;
   s.d16 = (int)ds.d16;
   s.metric = metric;
-  s.qn = Q.qn;
   s.qe = Q.qe;
   s.qr = Q.qr;
   s.ex = ds.ex;
@@ -362,12 +385,13 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   e.out_dist = out_dist;
   e.out_count = out_count;
   MANN_REQUIRE(k_req <= exact_kmax(), ERR_UNSUPPORTED, "k too large for the exact fallback");
-  {
-    ScopedTime tm("exact_fallback", st);
-    launch_exact_topk(e, nq, st);
-  }
   int hf[8];
   MANN_HIP(hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, st));
+  MANN_HIP(hipStreamSynchronize(st));
+  {
+    ScopedTime tm("exact_fallback", st);
+    run_exact_fallback(e, hf[1], (int)ceil_div(n, exact_seg_rows()));
+  }
   MANN_HIP(hipStreamSynchronize(st));
   MANN_HIP(hipGetLastError());
   S.pass2 = hf[0];

@@ -3,6 +3,12 @@
 // batched query (route -> invert -> one persistent scan over all (list, query
 // tile, row chunk) items -> certified exact re-rank in (dist, bucket, id)
 // order, which is the reference's batched SimpleMerge order).
+//
+// Residual encoding: the bf16 operands of list l hold x - c_l and q - c_l
+// (c_l its centroid).  Distances are translation invariant, so the first pass
+// approximates the same |q - x|; the residuals are far shorter than the
+// vectors, which narrows the certification band of every bound by the same
+// factor.  The exact FP32 re-rank always reads the original rows.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -38,8 +44,6 @@ mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int
     iv->nlist = nlist;
     iv->nb = nb;
     iv->normalized = ds.normalized;
-    iv->ex = ds.ex;
-    iv->rx = ds.rx;
     // stable counting sort of the assignment (transforms/utils.jl:142-174)
     std::vector<int> a32(n);
     for (int64_t j = 0; j < n; ++j) a32[j] = (int)assign_host[j] - 1;
@@ -64,22 +68,35 @@ mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int
     });
     MANN_HIP(hipMalloc(&iv->lorder, sizeof(int) * nb));
     MANN_HIP(hipMemcpyAsync(iv->lorder, order.data(), sizeof(int) * nb, hipMemcpyHostToDevice, st));
-    // list-contiguous copy of the base
+    // routing operands: centroids and their A16 norms (KMeansTransform.jl:157-158).
+    // Buckets beyond the centroids (an external assignment) get a zero centre.
+    const int nc = std::max(nb, nlist);
+    MANN_HIP(hipMalloc(&iv->c32, sizeof(float) * (size_t)nc * iv->d16));
+    MANN_HIP(hipMalloc(&iv->cn2, sizeof(float) * (size_t)nlist));
+    MANN_HIP(hipMemsetAsync(iv->c32, 0, sizeof(float) * (size_t)nc * iv->d16, st));
+    MANN_HIP(hipMemcpyAsync(iv->c32, C, sizeof(float) * (size_t)nlist * iv->d16,
+                            hipMemcpyDeviceToDevice, st));
+    launch_a16_norms(iv->c32, nlist, (int)iv->d16, iv->cn2, st);
+    // list-contiguous copy of the base; bf16 residuals against the list centre
     MANN_HIP(hipMalloc(&iv->x32, sizeof(float) * iv->n_pad * iv->d16));
     MANN_HIP(hipMalloc(&iv->xb, sizeof(bf16_t) * iv->n_pad * iv->kp));
     MANN_HIP(hipMalloc(&iv->xn, sizeof(float) * iv->n_pad));
     MANN_HIP(hipMalloc(&iv->xn2, sizeof(float) * iv->n_pad));
     MANN_HIP(hipMemsetAsync(iv->x32, 0, sizeof(float) * iv->n_pad * iv->d16, st));
     gather_rows_f32(ds.x32, iv->perm, (int)n, (int)iv->d16, iv->x32, st);
-    gather_rows_bf16(ds.xb, ds.xn, iv->perm, (int)n, (int)iv->n_pad, (int)iv->kp, iv->xb, iv->xn,
-                     st);
     gather_f32(ds.xn2, iv->perm, (int)n, iv->xn2, st);
-    // routing operands: centroids and their A16 norms (KMeansTransform.jl:157-158)
-    MANN_HIP(hipMalloc(&iv->c32, sizeof(float) * (size_t)nlist * iv->d16));
-    MANN_HIP(hipMalloc(&iv->cn2, sizeof(float) * (size_t)nlist));
-    MANN_HIP(hipMemcpyAsync(iv->c32, C, sizeof(float) * (size_t)nlist * iv->d16,
-                            hipMemcpyDeviceToDevice, st));
-    launch_a16_norms(iv->c32, nlist, (int)iv->d16, iv->cn2, st);
+    int* rowb = c.buf("ivb_rowb").as<int>(n);
+    gather_rows_i32(da, iv->perm, (int)n, 1, rowb, st);
+    unsigned* mx = c.buf("ivb_max").as<unsigned>(2);
+    MANN_HIP(hipMemsetAsync(mx, 0, sizeof(unsigned) * 2, st));
+    launch_prep_rows(iv->x32, iv->d16, n, iv->n_pad, (int)iv->d, (int)iv->d16, (int)iv->kp,
+                     iv->normalized, iv->x32, iv->xb, iv->xn, nullptr, nullptr, mx, mx + 1, st,
+                     iv->c32, rowb, iv->d16);
+    unsigned hm[2];
+    MANN_HIP(hipMemcpyAsync(hm, mx, sizeof(hm), hipMemcpyDeviceToHost, st));
+    MANN_HIP(hipStreamSynchronize(st));
+    std::memcpy(&iv->ex, &hm[0], 4);
+    std::memcpy(&iv->rx, &hm[1], 4);
     MANN_HIP(hipStreamSynchronize(st));
     MANN_HIP(hipGetLastError());
   } catch (...) {
@@ -130,6 +147,50 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
   int* prows = c.buf("iv_prows").as<int>(nq);
   ivf_probe(iv, R, nq, nprobe, probes, pcount, prows);
 
+  // invert probe lists into work items
+  const int nb = iv.nb;
+  int* qcount = c.buf("iv_qcount").as<int>(nb);
+  int* qfill = c.buf("iv_qfill").as<int>(nb);
+  int* qoff = c.buf("iv_qoff").as<int>(nb);
+  int* ioff = c.buf("iv_ioff").as<int>(nb);
+  const int64_t max_entries = (int64_t)nq * nprobe;
+  int* qlist = c.buf("iv_qlist").as<int>((size_t)max_entries);
+  int* qlist_list = c.buf("iv_qlist_list").as<int>((size_t)max_entries);
+  int* slot_of = c.buf("iv_slot_of").as<int>((size_t)max_entries);
+  int64_t max_items = 0;
+  for (int b = 0; b < nb; ++b) {
+    int64_t sz = iv.h_off[b + 1] - iv.h_off[b];
+    max_items += ceil_div(sz, ROWS_ITEM) * (int64_t)ceil_div(nq, BN);
+  }
+  max_items = std::min<int64_t>(max_items, (int64_t)nq * nprobe * ceil_div(iv.n, ROWS_ITEM));
+  ScanItem* items = c.buf("iv_items").as<ScanItem>((size_t)std::max<int64_t>(max_items, 1));
+  // [0] n_items, [1] item counter, [2] flagged after pass 1, [3] flagged after
+  // pass 2, [4] entries, [5] refinement items, [6] refinement item counter
+  int* ctr = c.buf("iv_ctr").as<int>(8);
+  int* stats = c.buf("iv_stats").as<int>(4);
+  fill_i32(qcount, nb, 0, st);
+  fill_i32(qfill, nb, 0, st);
+  fill_i32(ctr, 8, 0, st);
+  fill_i32(stats, 4, 0, st);
+  {
+    ScopedTime tm("ivf_plan", st);
+    launch_ivf_plan(probes, pcount, nq, nprobe, nb, iv.list_off, iv.lorder, BN, ROWS_ITEM, qcount,
+                    qfill, qoff, ioff, qlist, qlist_list, slot_of, items, ctr, ctr + 4, st);
+  }
+  // residual query rows per (list, query) entry and per-query bounds
+  bf16_t* qres = c.buf("iv_qres").as<bf16_t>((size_t)max_entries * iv.kp);
+  float* rnorm = c.buf("iv_rnorm").as<float>((size_t)max_entries);
+  unsigned* qem = c.buf("iv_qem").as<unsigned>((size_t)nq * 2);
+  fill_i32((int*)qem, (int64_t)nq * 2, 0, st);
+  {
+    ScopedTime tm("ivf_residual", st);
+    launch_residual_queries(Q.q32, iv.c32, (int)iv.d, (int)iv.d16, (int)iv.kp, iv.normalized,
+                            qlist, qlist_list,
+                            ctr + 4, max_entries, qres, rnorm, qem, qem + nq, st);
+  }
+  const float* qe = (const float*)qem;
+  const float* qr = (const float*)(qem + nq);
+
   // emission thresholds (certified by an exact row sample)
   float* tau = c.buf("iv_tau").as<float>(nq);
   int* keff = c.buf("iv_keff").as<int>(nq);
@@ -149,9 +210,8 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
   ta.xn2 = iv.xn2;
   ta.q32 = Q.q32;
   ta.qn2 = Q.qn2;
-  ta.qn = Q.qn;
-  ta.qe = Q.qe;
-  ta.qr = Q.qr;
+  ta.qe = qe;
+  ta.qr = qr;
   ta.ex = iv.ex;
   ta.rx = iv.rx;
   ta.tau = tau;
@@ -161,39 +221,15 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
     launch_ivf_tau(ta, st);
   }
 
-  // invert probe lists into work items
-  const int nb = iv.nb;
-  int* qcount = c.buf("iv_qcount").as<int>(nb);
-  int* qfill = c.buf("iv_qfill").as<int>(nb);
-  int* qoff = c.buf("iv_qoff").as<int>(nb);
-  int* ioff = c.buf("iv_ioff").as<int>(nb);
-  int* qlist = c.buf("iv_qlist").as<int>((size_t)nq * nprobe);
-  int64_t max_items = 0;
-  for (int b = 0; b < nb; ++b) {
-    int64_t sz = iv.h_off[b + 1] - iv.h_off[b];
-    max_items += ceil_div(sz, ROWS_ITEM) * (int64_t)ceil_div(nq, BN);
-  }
-  max_items = std::min<int64_t>(max_items, (int64_t)nq * nprobe * ceil_div(iv.n, ROWS_ITEM));
-  ScanItem* items = c.buf("iv_items").as<ScanItem>((size_t)std::max<int64_t>(max_items, 1));
-  int* ctr = c.buf("iv_ctr").as<int>(4);  // [0] n_items, [1] item counter, [2] flags, [3] stats
-  int* stats = c.buf("iv_stats").as<int>(4);
-  fill_i32(qcount, nb, 0, st);
-  fill_i32(qfill, nb, 0, st);
-  fill_i32(ctr, 4, 0, st);
-  fill_i32(stats, 4, 0, st);
-  {
-    ScopedTime tm("ivf_plan", st);
-    launch_ivf_plan(probes, pcount, nq, nprobe, nb, iv.list_off, iv.lorder, BN, ROWS_ITEM, qcount,
-                    qfill, qoff, ioff, qlist, items, ctr, st);
-  }
-
   unsigned* cand = c.buf("iv_cand").as<unsigned>((size_t)nq * cap * 2);
   int* cnt = c.buf("iv_cnt").as<int>(nq);
   fill_i32(cnt, nq, 0, st);
   ScanArgs a{};
   a.xb = iv.xb;
   a.xn = iv.xn;
-  a.qb = Q.qb;
+  a.qb = qres;
+  a.col_norm = rnorm;
+  a.b_by_slot = 1;
   a.kp = (int)iv.kp;
   a.d = (int)iv.d;
   a.qidx = qlist;
@@ -229,9 +265,8 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
   s.d16 = (int)iv.d16;
   s.metric = metric;
   s.perm = iv.perm;
-  s.qn = Q.qn;
-  s.qe = Q.qe;
-  s.qr = Q.qr;
+  s.qe = qe;
+  s.qr = qr;
   s.ex = iv.ex;
   s.rx = iv.rx;
   s.k = k;
@@ -249,37 +284,72 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
     ScopedTime tm("ivf_select", st);
     launch_select_rerank(s, nq, st);
   }
-  // overflowed queries: exact scan of their probed lists
-  int2* ranges = c.buf("iv_ranges").as<int2>((size_t)nq * nprobe);
-  launch_probe_ranges(probes, pcount, nq, nprobe, iv.list_off, ranges, st);
-  ExactArgs e{};
-  e.qlist = flist;
-  e.nq_dev = ctr + 2;
-  e.x32 = iv.x32;
-  e.q32 = Q.q32;
-  e.xn2 = iv.xn2;
-  e.qn2 = Q.qn2;
-  e.d = (int)iv.d;
-  e.d16 = (int)iv.d16;
-  e.metric = metric;
-  e.perm = iv.perm;
-  e.range_cnt = pcount;
-  e.ranges = ranges;
-  e.range_stride = nprobe;
-  e.k = k;
-  e.k_req = k;
-  e.out_ids = out_ids;
-  e.out_dist = out_dist;
-  {
-    ScopedTime tm("ivf_exact_fallback", st);
-    launch_exact_topk(e, nq, st);
-  }
-  int h[4], hs[4];
+  int h[8], hs[4];
   MANN_HIP(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
+  MANN_HIP(hipStreamSynchronize(st));
+  const int nflag = h[2];
+  int nflag2 = 0;
+  if (nflag > 0) {
+    // refinement: the flagged queries re-scan their lists with the threshold
+    // of the candidates they already hold (single-query items)
+    const int64_t rmax = (int64_t)nflag * (ceil_div(iv.n, ROWS_ITEM) + nprobe);
+    ScanItem* ritems = c.buf("iv_ritems").as<ScanItem>((size_t)rmax);
+    int* flist2 = c.buf("iv_flist2").as<int>(nq);
+    {
+      ScopedTime tm("ivf_refine_scan", st);
+      launch_refine_items(flist, ctr + 2, nflag, nprobe, probes, pcount, slot_of, iv.list_off,
+                          ROWS_ITEM, ritems, ctr + 5, st);
+      reset_listed(cnt, flist, ctr + 2, nflag, st);
+      a.items = ritems;
+      a.n_items_dev = ctr + 5;
+      a.item_counter = ctr + 6;
+      a.tau = tau2;
+      launch_scan(a, SCAN_GATHER, 2 * dev_cus, SCAN_KIND_ITEMS, st);
+    }
+    s.qlist = flist;
+    s.nq_dev = ctr + 2;
+    s.tau = tau2;
+    s.flag_count = ctr + 3;
+    s.flag_list = flist2;
+    s.tau_new = tau;  // not read afterwards
+    {
+      ScopedTime tm("ivf_select_pass2", st);
+      launch_select_rerank(s, nflag, st);
+    }
+    MANN_HIP(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
+    MANN_HIP(hipStreamSynchronize(st));
+    nflag2 = h[3];
+    if (nflag2 > 0) {
+      // still uncertified: exact scan of their probed lists
+      int2* ranges = c.buf("iv_ranges").as<int2>((size_t)nq * nprobe);
+      launch_probe_ranges(probes, pcount, nq, nprobe, iv.list_off, ranges, st);
+      ExactArgs e{};
+      e.qlist = flist2;
+      e.nq_dev = ctr + 3;
+      e.x32 = iv.x32;
+      e.q32 = Q.q32;
+      e.xn2 = iv.xn2;
+      e.qn2 = Q.qn2;
+      e.d = (int)iv.d;
+      e.d16 = (int)iv.d16;
+      e.metric = metric;
+      e.perm = iv.perm;
+      e.range_cnt = pcount;
+      e.ranges = ranges;
+      e.range_stride = nprobe;
+      e.k = k;
+      e.k_req = k;
+      e.out_ids = out_ids;
+      e.out_dist = out_dist;
+      ScopedTime tm("ivf_exact_fallback", st);
+      run_exact_fallback(e, nflag2, (int)ceil_div(iv.n, exact_seg_rows()) + nprobe);
+    }
+  }
   MANN_HIP(hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, st));
   MANN_HIP(hipStreamSynchronize(st));
   MANN_HIP(hipGetLastError());
-  S.exact = h[2];
+  S.pass2 = nflag;
+  S.exact = nflag2;
   S.reranked = hs[0];
   S.overflow = hs[1];
 }

@@ -47,7 +47,7 @@ CentroidOps prepare_centroids(const float* C, int k, const mann_dataset& ds, con
   unsigned* mx = c.buf(t + "_mx").as<unsigned>(2);
   fill_i32((int*)mx, 2, 0, st);
   launch_prep_rows(C, ds.d16, k, P.k_pad, (int)ds.d, (int)ds.d16, (int)ds.kp, 0, P.c32, P.cb, P.cn,
-                   ce, cr, mx, mx + 1, st);
+                   ce, cr, mx, mx + 1, st, ds.center);
   launch_a16_norms(P.c32, k, (int)ds.d16, P.cn2, st);
   unsigned h[2];
   MANN_HIP(hipMemcpyAsync(h, mx, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -88,7 +88,7 @@ void kmeans_estep(const mann_dataset& ds, const float* C, int k, int* assign, fl
   t.overflow = ovf;
   if (primed) {
     ScopedTime tm("kmeans_prime", st);
-    launch_prime_threshold(t, ds.xn, ds.err, ds.nrm, P.ex, P.rx, (int)ds.d, tau, st);
+    launch_prime_threshold(t, ds.err, ds.nrm, P.ex, P.rx, (int)ds.d, tau, st);
   } else {
     fill_f32(tau, n, INFINITY, st);
   }
@@ -106,7 +106,7 @@ void kmeans_estep(const mann_dataset& ds, const float* C, int k, int* assign, fl
   a.cand = cand;
   a.cnt = cnt;
   a.cap = TOP1_CAP;
-  a.t1_qn = ds.xn;
+  a.qnorm = ds.xn;
   a.t1_qe = ds.err;
   a.t1_qr = ds.nrm;
   a.t1_ex = P.ex;

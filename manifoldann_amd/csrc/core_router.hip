@@ -49,7 +49,25 @@ mann_router* router_create(int d, int h, int c, uint64_t seed) {
   return r;
 }
 
-// Z = W2 relu(W1 x + b1) + b2 for B rows of X (row stride ldx); H kept for backward.
+void router_set_prior(mann_router& r, const mann_ivf* iv, float gamma) {
+  if (r.pc) (void)hipFree(r.pc);
+  if (r.pcn) (void)hipFree(r.pcn);
+  r.pc = r.pcn = nullptr;
+  r.gamma = 0.0f;
+  if (gamma == 0.0f || !iv) return;
+  MANN_REQUIRE(r.d <= prior_dmax(), ERR_UNSUPPORTED, "router prior supports d <= 1024");
+  MANN_HIP(hipMalloc(&r.pc, sizeof(float) * (size_t)r.c * iv->d16));
+  MANN_HIP(hipMalloc(&r.pcn, sizeof(float) * (size_t)r.c));
+  MANN_HIP(hipMemcpy(r.pc, iv->c32, sizeof(float) * (size_t)r.c * iv->d16,
+                     hipMemcpyDeviceToDevice));
+  MANN_HIP(hipMemcpy(r.pcn, iv->cn2, sizeof(float) * (size_t)r.c, hipMemcpyDeviceToDevice));
+  r.pld = iv->d16;
+  r.gamma = gamma;
+}
+
+// Z = W2 relu(W1 x + b1) + b2 (+ the centroid prior) for B rows of X (row
+// stride ldx); H kept for backward.  The prior has no parameters, so the
+// backward pass is unchanged.
 static void router_forward(const mann_router& r, const float* X, int64_t ldx, int B, float* H,
                            float* Z, hipStream_t st) {
   GemmArgs g{};
@@ -81,6 +99,8 @@ static void router_forward(const mann_router& r, const float* X, int64_t ldx, in
   g2.K = r.h;
   g2.bias = r.b2();
   launch_gemm(g2, st);
+  if (r.gamma != 0.0f && r.pc)
+    launch_prior(X, ldx, B, r.d, r.pc, r.pld, r.pcn, r.c, r.gamma, Z, r.c, st);
 }
 
 // Gradients of the mean cross-entropy into grad (same layout as the parameters).

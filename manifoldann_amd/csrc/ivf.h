@@ -16,8 +16,7 @@ struct IvfTauArgs {
   const float* xn2;
   const float* q32;
   const float* qn2;
-  const float* qn;          // certification inputs of the queries
-  const float* qe;
+  const float* qe;          // certification inputs of the queries
   const float* qr;
   float ex, rx;             // dataset bounds
   float* tau;               // out: emission threshold
@@ -32,8 +31,19 @@ void launch_probe_select(const float* R, int nq, int nlist, int avail, int nprob
                          hipStream_t st);
 void launch_ivf_plan(const int* probes, const int* pcount, int nq, int nprobe, int nb,
                      const int* list_off, const int* lorder, int bn, int rows_item, int* qcount,
-                     int* qfill, int* qoff, int* ioff, int* qlist, ScanItem* items, int* n_items,
+                     int* qfill, int* qoff, int* ioff, int* qlist, int* qlist_list,
+                     int* slot_of, ScanItem* items, int* n_items, int* n_entries,
                      hipStream_t st);
+// single-query items over the probed lists of the flagged queries
+void launch_refine_items(const int* flist, const int* nflag, int nflag_host, int nprobe,
+                         const int* probes, const int* pcount, const int* slot_of,
+                         const int* list_off, int rows_item, ScanItem* items, int* n_items,
+                         hipStream_t st);
+// bf16 residual rows q - c_list for the entries of the inverted probe lists
+void launch_residual_queries(const float* q32, const float* c32, int d, int d16, int kp,
+                             int normalize, const int* qlist, const int* qlist_list, const int* n_dev,
+                             int64_t max_entries, bf16_t* qres, float* rnorm, unsigned* qe_max,
+                             unsigned* qr_max, hipStream_t st);
 void launch_ivf_tau(const IvfTauArgs& a, hipStream_t st);
 void launch_probe_ranges(const int* probes, const int* pcount, int nq, int nprobe,
                          const int* list_off, int2* ranges, hipStream_t st);

@@ -10,26 +10,37 @@ namespace mann {
 namespace {
 
 // R[q][c] = norms form to centroid c (kmeans/distance.jl:213-242), A16 order.
-// Block = one query x 16 centroids (one 16-lane group each).
+// One block per query: 16 groups of 16 lanes walk the centroids.
 __global__ __launch_bounds__(256) void route_dist_kernel(const float* __restrict__ q32,
                                                          const float* __restrict__ qn2, int nq,
                                                          const float* __restrict__ c32,
                                                          const float* __restrict__ cn2, int nlist,
                                                          int d16, float* __restrict__ R) {
-  const int cblocks = (nlist + 15) / 16;
-  const int q = blockIdx.x / cblocks;
-  const int c = (blockIdx.x - q * cblocks) * 16 + (threadIdx.x >> 4);
-  const int j = threadIdx.x & 15;
-  if (q >= nq || c >= nlist) return;
-  const float* x = c32 + (int64_t)c * d16;
+  const int q = blockIdx.x;
+  if (q >= nq) return;
+  const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
   const float* y = q32 + (int64_t)q * d16;
-  float acc = 0.0f;
-  for (int t = 0; t < d16; t += 16) acc = __builtin_fmaf(x[t + j], y[t + j], acc);
-  acc = group16_tree(acc);
-  if (j == 0) {
-    float tt = cn2[c] + qn2[q];
-    float sq = tt - 2.0f * acc;
-    R[(int64_t)q * nlist + c] = __builtin_sqrtf(sq > 0.0f ? sq : 0.0f);
+  const float qn = qn2[q];
+  for (int c0 = 4 * g; c0 < nlist; c0 += 64) {
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    const float* x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) x[u] = c32 + (int64_t)min(c0 + u, nlist - 1) * d16;
+    for (int t = 0; t < d16; t += 16) {
+      const float yv = y[t + j];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[u] = __builtin_fmaf(x[u][t + j], yv, acc[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float dot = group16_tree(acc[u]);
+      const int c = c0 + u;
+      if (j == 0 && c < nlist) {
+        float tt = cn2[c] + qn;
+        float sq = tt - 2.0f * dot;
+        R[(int64_t)q * nlist + c] = __builtin_sqrtf(sq > 0.0f ? sq : 0.0f);
+      }
+    }
   }
 }
 
@@ -103,7 +114,8 @@ __global__ __launch_bounds__(1024) void plan_kernel(const int* __restrict__ qcou
                                                     const int* __restrict__ lorder, int bn,
                                                     int rows_item, int* __restrict__ qoff,
                                                     int* __restrict__ ioff,
-                                                    int* __restrict__ n_items) {
+                                                    int* __restrict__ n_items,
+                                                    int* __restrict__ n_entries) {
   __shared__ int part[1024];
   const int tid = threadIdx.x;
   const int per = (nb + 1023) / 1024;
@@ -119,6 +131,7 @@ __global__ __launch_bounds__(1024) void plan_kernel(const int* __restrict__ qcou
       part[i] = run;
       run += v;
     }
+    *n_entries = run;
   }
   __syncthreads();
   int run = part[tid];
@@ -160,7 +173,8 @@ __global__ __launch_bounds__(1024) void plan_kernel(const int* __restrict__ qcou
 
 __global__ void list_fill_kernel(const int* __restrict__ probes, const int* __restrict__ pcount,
                                  int nq, int nprobe, const int* __restrict__ qoff,
-                                 int* __restrict__ qfill, int* __restrict__ qlist) {
+                                 int* __restrict__ qfill, int* __restrict__ qlist,
+                                 int* __restrict__ qlist_list, int* __restrict__ slot_of) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)nq * nprobe) return;
   const int q = (int)(i / nprobe), r = (int)(i - (int64_t)q * nprobe);
@@ -168,6 +182,81 @@ __global__ void list_fill_kernel(const int* __restrict__ probes, const int* __re
   const int b = probes[i];
   const int pos = atomicAdd(&qfill[b], 1);
   qlist[qoff[b] + pos] = q;
+  qlist_list[qoff[b] + pos] = b;
+  slot_of[i] = qoff[b] + pos;
+}
+
+// Refinement items: for each flagged query, its probed lists in ROWS_ITEM
+// chunks as single-query items (slot = the query's entry in the list).
+__global__ void refine_items_kernel(const int* __restrict__ flist, const int* __restrict__ nflag,
+                                    int nprobe, const int* __restrict__ probes,
+                                    const int* __restrict__ pcount,
+                                    const int* __restrict__ slot_of,
+                                    const int* __restrict__ list_off, int rows_item,
+                                    ScanItem* __restrict__ items, int* __restrict__ n_items) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = (int)(i / nprobe), r = (int)(i - (int64_t)f * nprobe);
+  if (f >= *nflag) return;
+  const int q = flist[f];
+  if (r >= pcount[q]) return;
+  const int b = probes[(int64_t)q * nprobe + r];
+  const int slot = slot_of[(int64_t)q * nprobe + r];
+  const int lo = list_off[b], hi = list_off[b + 1];
+  const int ns = (hi - lo + rows_item - 1) / rows_item;
+  const int base = atomicAdd(n_items, ns);
+  for (int s = 0; s < ns; ++s) {
+    ScanItem it;
+    it.qoff = slot;
+    it.nq = 1;
+    it.row_begin = lo + s * rows_item;
+    it.row_end = min(hi, it.row_begin + rows_item);
+    it.region = 0;
+    items[base + s] = it;
+  }
+}
+
+// Residual query rows for every (list, query) entry: bf16(q - c_list), its
+// squared norm (FP64 sum), and per query the maxima of |r - bf16(r)| and
+// |bf16(r)| over its entries (the bounds of the certification).
+__global__ __launch_bounds__(256) void residual_queries_kernel(
+    const float* __restrict__ q32, const float* __restrict__ c32, int d, int d16, int kp,
+    int normalize,
+    const int* __restrict__ qlist, const int* __restrict__ qlist_list, const int* __restrict__ n_dev,
+    bf16_t* __restrict__ qres, float* __restrict__ rnorm, unsigned* __restrict__ qe_max,
+    unsigned* __restrict__ qr_max) {
+  const int lane = threadIdx.x & 63;
+  const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (e >= *n_dev) return;
+  const int q = qlist[e], b = qlist_list[e];
+  const float* y = q32 + (int64_t)q * d16;
+  const float* cen = c32 + (int64_t)b * d16;
+  double scale = 1.0;
+  if (normalize) {  // cosine: the residual of the unit vector (as the base rows)
+    double ss = 0.0;
+    for (int c = lane; c < d; c += 64) ss += (double)y[c] * (double)y[c];
+    for (int o = 32; o >= 1; o >>= 1) ss += __shfl_xor(ss, o);
+    scale = ss > 0.0 ? 1.0 / sqrt(ss) : 0.0;
+  }
+  double nb = 0.0, ee = 0.0;
+  for (int c = lane; c < kp; c += 64) {
+    const double t = c < d ? (double)y[c] * scale - (double)cen[c] : 0.0;
+    const bf16_t v = (bf16_t)(float)t;
+    qres[e * kp + c] = v;
+    const double bd = (double)(float)v;
+    nb += bd * bd;
+    ee += (t - bd) * (t - bd);
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    nb += __shfl_xor(nb, o);
+    ee += __shfl_xor(ee, o);
+  }
+  if (lane == 0) {
+    rnorm[e] = (float)nb;
+    const float er = (float)(sqrt(ee) * (1.0 + 1e-6)) + 1e-30f;
+    const float nr = (float)(sqrt(nb) * (1.0 + 1e-6)) + 1e-30f;
+    atomicMax(qe_max + q, __float_as_uint(er));
+    atomicMax(qr_max + q, __float_as_uint(nr));
+  }
 }
 
 __global__ void items_fill_kernel(const int* __restrict__ qcount, int nb,
@@ -193,51 +282,128 @@ __global__ void items_fill_kernel(const int* __restrict__ qcount, int nb,
     }
 }
 
-constexpr int TAU_M = 2048;
+constexpr int TAU_M = 1024;
 
 // Emission threshold of each query from the exact distances of the first
 // TAU_M rows of its probed lists (probe order): the k-th smallest exact value
 // is an upper bound of the final k-th, so T* of it certifies the emitted set.
 __global__ __launch_bounds__(256) void ivf_tau_kernel(IvfTauArgs a) {
-  __shared__ uint64_t keys[TAU_M];
+  constexpr int MAXL = 64;  // probed lists the sample may span
+  __shared__ float vals[TAU_M];
+  __shared__ int cum[MAXL + 1];
+  __shared__ int s_nl, s_m;
+  __shared__ unsigned hist[256];
+  __shared__ unsigned s_sel, s_want;
   const int q = blockIdx.x;
   if (q >= a.nq) return;
   const int tid = threadIdx.x, g = tid >> 4, j = tid & 15;
   const int np = a.pcount[q];
   const int total = a.prows[q];
-  const int m = total < TAU_M ? total : TAU_M;
+  const int m0 = total < TAU_M ? total : TAU_M;
   const int k_eff = a.k < total ? a.k : total;
   if (tid == 0) a.k_eff_q[q] = k_eff;
-  if (m <= 0 || k_eff <= 0 || total <= a.emit_all_below) {
+  if (m0 <= 0 || k_eff <= 0 || total <= a.emit_all_below) {
     if (tid == 0) a.tau[q] = INFINITY;
     return;
   }
-  for (int i = g; i < TAU_M; i += 16) {
-    uint64_t key = ~0ull;
-    if (i < m) {
-      // locate sample position i in the probe-order concatenation
-      int r = 0, base = 0, row = -1;
-      for (; r < np; ++r) {
-        const int b = a.probes[(int64_t)q * a.nprobe + r];
-        const int sz = a.list_off[b + 1] - a.list_off[b];
-        if (i < base + sz) {
-          row = a.list_off[b] + (i - base);
-          break;
-        }
-        base += sz;
-      }
-      float dv = exact_metric(a.x32, a.q32, a.xn2, a.qn2, a.d16, a.metric, row, q, j);
-      key = pack_key(dv, (uint32_t)row);
+  if (tid == 0) {  // prefix sums of the list sizes until the sample is covered
+    int run = 0, r = 0;
+    for (; r < np && r < MAXL && run < m0; ++r) {
+      cum[r] = run;
+      const int b = a.probes[(int64_t)q * a.nprobe + r];
+      run += a.list_off[b + 1] - a.list_off[b];
     }
-    if (j == 0) keys[i] = key;
+    cum[r] = run;
+    s_nl = r;
+    s_m = run < m0 ? run : m0;
   }
   __syncthreads();
-  int P = 1;
-  while (P < m) P <<= 1;
-  bitonic_sort_u64(keys, P);
+  const int nl = s_nl, m = s_m;
+  if (m < k_eff) {  // sample too small to bound the k-th distance: emit everything
+    if (tid == 0) a.tau[q] = INFINITY;
+    return;
+  }
+  auto row_of = [&](int i) {
+    int r = 0;
+    while (r + 1 < nl && cum[r + 1] <= i) ++r;
+    const int b = a.probes[(int64_t)q * a.nprobe + r];
+    return a.list_off[b] + (i - cum[r]);
+  };
+  const float* y = a.q32 + (int64_t)q * a.d16;
+  const bool diff = a.metric == EUCLIDEAN || a.metric == SQEUCLIDEAN;
+  const float qn = diff ? 0.0f : a.qn2[q];
+  for (int i0 = 4 * g; i0 < m; i0 += 64) {
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    int row[4];
+    const float* x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      row[u] = row_of(min(i0 + u, m - 1));
+      x[u] = a.x32 + (int64_t)row[u] * a.d16;
+    }
+    if (diff) {
+      for (int t = 0; t < a.d16; t += 16) {
+        const float yv = y[t + j];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const float df = x[u][t + j] - yv;
+          acc[u] = __builtin_fmaf(df, df, acc[u]);
+        }
+      }
+    } else {
+      for (int t = 0; t < a.d16; t += 16) {
+        const float yv = y[t + j];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] = __builtin_fmaf(x[u][t + j], yv, acc[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float s = group16_tree(acc[u]);
+      const float v = finish_metric(s, a.metric, diff ? 0.0f : a.xn2[row[u]], qn);
+      if (j == 0 && i0 + u < m) vals[i0 + u] = v;
+    }
+  }
+  __syncthreads();
+  // k_eff-th smallest value: radix select on the float order keys
+  uint32_t prefix = 0, mask = 0;
+  if (tid == 0) s_want = (unsigned)k_eff;
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    hist[tid] = 0;
+    __syncthreads();
+    for (int i = tid; i < m; i += 256) {
+      const uint32_t o = f2ord(vals[i]);
+      if ((o & mask) == prefix) atomicAdd(&hist[(o >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (tid < 64) {
+      const unsigned h0 = hist[4 * tid], h1 = hist[4 * tid + 1], h2 = hist[4 * tid + 2],
+                     h3 = hist[4 * tid + 3];
+      const unsigned tot = h0 + h1 + h2 + h3;
+      unsigned inc = tot;
+      for (int o = 1; o < 64; o <<= 1) {
+        const unsigned t = __shfl_up(inc, o);
+        if (tid >= o) inc += t;
+      }
+      const unsigned excl = inc - tot, want = s_want;
+      if (excl < want && inc >= want) {
+        unsigned cc = excl, b;
+        if (cc + h0 >= want) b = 4 * tid;
+        else if (cc + h0 + h1 >= want) { b = 4 * tid + 1; cc += h0; }
+        else if (cc + h0 + h1 + h2 >= want) { b = 4 * tid + 2; cc += h0 + h1; }
+        else { b = 4 * tid + 3; cc += h0 + h1 + h2; }
+        s_sel = b;
+        s_want = want - cc;
+      }
+    }
+    __syncthreads();
+    prefix |= s_sel << shift;
+    mask |= 255u << shift;
+    __syncthreads();
+  }
   if (tid == 0) {
-    float dk = ord2f((uint32_t)(keys[k_eff - 1] >> 32));
-    double T = exclusion_threshold(dk, a.metric, a.qn[q], a.qe[q], a.qr[q], a.ex, a.rx, a.d);
+    const float dk = ord2f(prefix);
+    const double T = exclusion_threshold(dk, a.metric, a.qe[q], a.qr[q], a.ex, a.rx, a.d);
     a.tau[q] = T < 3.0e38 ? nextafterf((float)T * (1.0f + 1e-6f), INFINITY) : INFINITY;
   }
 }
@@ -264,9 +430,8 @@ int route_max() { return ROUTE_MAX; }
 void launch_route_dist(const float* q32, const float* qn2, int nq, const float* c32,
                        const float* cn2, int nlist, int d16, float* R, hipStream_t st) {
   if (nq <= 0) return;
-  const int cblocks = (nlist + 15) / 16;
-  hipLaunchKernelGGL(route_dist_kernel, dim3((unsigned)((int64_t)nq * cblocks)), dim3(256), 0, st,
-                     q32, qn2, nq, c32, cn2, nlist, d16, R);
+  hipLaunchKernelGGL(route_dist_kernel, dim3(nq), dim3(256), 0, st, q32, qn2, nq, c32, cn2, nlist,
+                     d16, R);
 }
 
 void launch_probe_select(const float* R, int nq, int nlist, int avail, int nprobe,
@@ -279,18 +444,41 @@ void launch_probe_select(const float* R, int nq, int nlist, int avail, int nprob
 
 void launch_ivf_plan(const int* probes, const int* pcount, int nq, int nprobe, int nb,
                      const int* list_off, const int* lorder, int bn, int rows_item, int* qcount,
-                     int* qfill, int* qoff, int* ioff, int* qlist, ScanItem* items, int* n_items,
+                     int* qfill, int* qoff, int* ioff, int* qlist, int* qlist_list,
+                     int* slot_of, ScanItem* items, int* n_items, int* n_entries,
                      hipStream_t st) {
   const int64_t tot = (int64_t)nq * nprobe;
   const unsigned g = (unsigned)ceil_div(tot, 256);
   hipLaunchKernelGGL(list_count_kernel, dim3(g), dim3(256), 0, st, probes, pcount, nq, nprobe,
                      qcount);
   hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(1024), 0, st, qcount, nb, list_off, lorder, bn,
-                     rows_item, qoff, ioff, n_items);
+                     rows_item, qoff, ioff, n_items, n_entries);
   hipLaunchKernelGGL(list_fill_kernel, dim3(g), dim3(256), 0, st, probes, pcount, nq, nprobe, qoff,
-                     qfill, qlist);
+                     qfill, qlist, qlist_list, slot_of);
   hipLaunchKernelGGL(items_fill_kernel, dim3((unsigned)ceil_div(nb, 256)), dim3(256), 0, st, qcount,
                      nb, list_off, lorder, qoff, ioff, bn, rows_item, items);
+}
+
+void launch_residual_queries(const float* q32, const float* c32, int d, int d16, int kp,
+                             int normalize, const int* qlist, const int* qlist_list, const int* n_dev,
+                             int64_t max_entries, bf16_t* qres, float* rnorm, unsigned* qe_max,
+                             unsigned* qr_max, hipStream_t st) {
+  if (max_entries <= 0) return;
+  hipLaunchKernelGGL(residual_queries_kernel, dim3((unsigned)ceil_div(max_entries, 4)), dim3(256),
+                     0, st, q32, c32, d, d16, kp, normalize, qlist, qlist_list, n_dev, qres, rnorm,
+                     qe_max,
+                     qr_max);
+}
+
+void launch_refine_items(const int* flist, const int* nflag, int nflag_host, int nprobe,
+                         const int* probes, const int* pcount, const int* slot_of,
+                         const int* list_off, int rows_item, ScanItem* items, int* n_items,
+                         hipStream_t st) {
+  const int64_t tot = (int64_t)nflag_host * nprobe;
+  if (tot <= 0) return;
+  hipLaunchKernelGGL(refine_items_kernel, dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0, st,
+                     flist, nflag, nprobe, probes, pcount, slot_of, list_off, rows_item, items,
+                     n_items);
 }
 
 void launch_ivf_tau(const IvfTauArgs& a, hipStream_t st) {

@@ -65,8 +65,8 @@ __global__ __launch_bounds__(256) void top1_select_kernel(Top1Args a) {
 // Emission threshold of the next E-step: the exclusion threshold of the exact
 // distance to the centroid the point held before (an upper bound of the new
 // minimum), so the scan emits only centroids that can still win.
-__global__ __launch_bounds__(256) void prime_threshold_kernel(Top1Args a, const float* qn,
-                                                              const float* qe, const float* qr,
+__global__ __launch_bounds__(256) void prime_threshold_kernel(Top1Args a, const float* qe,
+                                                              const float* qr,
                                                               float ex, float rx, int d,
                                                               float* tau) {
   const int j = threadIdx.x & 15;
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void prime_threshold_kernel(Top1Args a, const 
     double H = (d + 8.0) * u * R32 * R32;
     double f = (double)v / (1.0 - 8.0 * u);
     double L = sqrt(f * f + H) + qe[p] + ex;
-    double t = (L * L - qn[p] + G) * (1.0 + 1e-9) + 1e-30;
+    double t = (L * L + G) * (1.0 + 1e-9) + 1e-30;
     tau[p] = nextafterf((float)t * (1.0f + 1e-6f), INFINITY);
   }
 }
@@ -337,11 +337,11 @@ void launch_top1_select(const Top1Args& a, hipStream_t st) {
   hipLaunchKernelGGL(top1_select_kernel, dim3((unsigned)ceil_div(a.n, 16)), dim3(256), 0, st, a);
 }
 
-void launch_prime_threshold(const Top1Args& a, const float* qn, const float* qe, const float* qr,
-                            float ex, float rx, int d, float* tau, hipStream_t st) {
+void launch_prime_threshold(const Top1Args& a, const float* qe, const float* qr, float ex, float rx,
+                            int d, float* tau, hipStream_t st) {
   if (a.n <= 0) return;
   hipLaunchKernelGGL(prime_threshold_kernel, dim3((unsigned)ceil_div(a.n, 16)), dim3(256), 0, st,
-                     a, qn, qe, qr, ex, rx, d, tau);
+                     a, qe, qr, ex, rx, d, tau);
 }
 
 int sort_block() { return SORT_BLOCK; }

@@ -175,6 +175,51 @@ void launch_adam(float* w, const float* g, float* m1, float* m2, int64_t n, floa
                      n, lr, b1, b2, eps, c1, c2);
 }
 
+// Centroid prior of the router: Z[b][c] += gamma * (2 x_b.c_c - |c_c|^2), i.e.
+// -gamma |x_b - c_c|^2 up to a per-row constant (softmax and ranking ignore
+// it).  FP32; a block holds 8 rows in LDS, each thread walks one centroid.
+constexpr int PRIOR_ROWS = 8;
+constexpr int PRIOR_DMAX = 1024;
+__global__ __launch_bounds__(256) void prior_kernel(const float* __restrict__ X, int64_t ldx, int B,
+                                                    int d, const float* __restrict__ C,
+                                                    int64_t ldc, const float* __restrict__ cn2,
+                                                    int nlist, float gamma, float* __restrict__ Z,
+                                                    int64_t ldz) {
+  __shared__ float xs[PRIOR_ROWS][PRIOR_DMAX];
+  const int b0 = blockIdx.x * PRIOR_ROWS;
+  for (int i = threadIdx.x; i < PRIOR_ROWS * d; i += 256) {
+    const int r = i / d, t = i - r * d;
+    xs[r][t] = b0 + r < B ? X[(int64_t)(b0 + r) * ldx + t] : 0.0f;
+  }
+  __syncthreads();
+  for (int c = blockIdx.y * 256 + threadIdx.x; c < nlist; c += 256 * gridDim.y) {
+    float acc[PRIOR_ROWS];
+#pragma unroll
+    for (int r = 0; r < PRIOR_ROWS; ++r) acc[r] = 0.0f;
+    const float* cr = C + (int64_t)c * ldc;
+    for (int t = 0; t < d; ++t) {
+      const float cv = cr[t];
+#pragma unroll
+      for (int r = 0; r < PRIOR_ROWS; ++r) acc[r] = __builtin_fmaf(xs[r][t], cv, acc[r]);
+    }
+    const float cc = cn2[c];
+#pragma unroll
+    for (int r = 0; r < PRIOR_ROWS; ++r)
+      if (b0 + r < B) Z[(int64_t)(b0 + r) * ldz + c] += gamma * (2.0f * acc[r] - cc);
+  }
+}
+
+int prior_dmax() { return PRIOR_DMAX; }
+
+void launch_prior(const float* X, int64_t ldx, int B, int d, const float* C, int64_t ldc,
+                  const float* cn2, int nlist, float gamma, float* Z, int64_t ldz,
+                  hipStream_t st) {
+  if (B <= 0 || nlist <= 0) return;
+  dim3 grid((unsigned)ceil_div(B, PRIOR_ROWS), (unsigned)ceil_div(nlist, 256));
+  hipLaunchKernelGGL(prior_kernel, grid, dim3(256), 0, st, X, ldx, B, d, C, ldc, cn2, nlist, gamma,
+                     Z, ldz);
+}
+
 void launch_neg(const float* s, float* d, int64_t n, hipStream_t st) {
   hipLaunchKernelGGL(neg_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, s, d, n);
 }

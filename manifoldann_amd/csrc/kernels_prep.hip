@@ -13,9 +13,15 @@ namespace mann {
 //  err : |x - bf16(x)| rounded up; nrm : |bf16(x)| rounded up
 // normalize != 0 (cosine): the bf16 copy holds x/|x| (|x| in FP64) and the
 // bounds refer to the normalised vector.
+// center (optional): the bf16 copy holds x - center (center_row[row] selects a
+// row of `center` per point, e.g. its IVF list centroid); distances are
+// translation invariant, and the smaller norms tighten every bound.
 __global__ __launch_bounds__(256) void prep_rows_kernel(const float* __restrict__ src, int64_t ld,
                                                         int64_t n, int64_t n_pad, int d, int d16,
                                                         int kp, int normalize,
+                                                        const float* __restrict__ center,
+                                                        const int* __restrict__ center_row,
+                                                        int64_t center_ld,
                                                         float* __restrict__ x32,
                                                         bf16_t* __restrict__ xb,
                                                         float* __restrict__ xn,
@@ -33,6 +39,8 @@ __global__ __launch_bounds__(256) void prep_rows_kernel(const float* __restrict_
     return;
   }
   const float* s = src + row * ld;
+  const float* cen = center ? (center_row ? center + (int64_t)center_row[row] * center_ld : center)
+                            : nullptr;
   double scale = 1.0;
   if (normalize) {
     double ss = 0.0;
@@ -45,6 +53,7 @@ __global__ __launch_bounds__(256) void prep_rows_kernel(const float* __restrict_
     float v = c < d ? s[c] : 0.0f;
     if (c < d16) x32[row * d16 + c] = v;
     double t = (double)v * scale;
+    if (cen && c < d) t -= (double)cen[c];
     bf16_t b = (bf16_t)(float)t;
     xb[row * kp + c] = b;
     double bd = (double)(float)b;
@@ -82,10 +91,38 @@ __global__ __launch_bounds__(256) void a16_norms_kernel(const float* __restrict_
 
 void launch_prep_rows(const float* src, int64_t ld, int64_t n, int64_t n_pad, int d, int d16,
                       int kp, int normalize, float* x32, bf16_t* xb, float* xn, float* err,
-                      float* nrm, unsigned* max_err, unsigned* max_nrm, hipStream_t st) {
+                      float* nrm, unsigned* max_err, unsigned* max_nrm, hipStream_t st,
+                      const float* center, const int* center_row, int64_t center_ld) {
   if (n_pad <= 0) return;
   hipLaunchKernelGGL(prep_rows_kernel, dim3((unsigned)ceil_div(n_pad, 4)), dim3(256), 0, st, src,
-                     ld, n, n_pad, d, d16, kp, normalize, x32, xb, xn, err, nrm, max_err, max_nrm);
+                     ld, n, n_pad, d, d16, kp, normalize, center, center_row, center_ld, x32, xb,
+                     xn, err, nrm, max_err, max_nrm);
+}
+
+// column sums in FP64 (one block per 1024 rows, atomics across blocks)
+__global__ __launch_bounds__(256) void colsum64_kernel(const float* __restrict__ x, int64_t ld,
+                                                       int64_t n, int d, double* __restrict__ out) {
+  const int64_t r0 = (int64_t)blockIdx.x * 1024;
+  const int64_t r1 = r0 + 1024 < n ? r0 + 1024 : n;
+  for (int c = threadIdx.x; c < d; c += 256) {
+    double s = 0.0;
+    for (int64_t r = r0; r < r1; ++r) s += (double)x[r * ld + c];
+    atomicAdd(out + c, s);
+  }
+}
+__global__ void mean_from_sum_kernel(const double* __restrict__ sum, int64_t n, int d,
+                                     float* __restrict__ mean) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < d) mean[c] = (float)(sum[c] / (double)n);
+}
+
+void launch_mean_rows(const float* x, int64_t ld, int64_t n, int d, double* tmp, float* mean,
+                      hipStream_t st) {
+  MANN_HIP(hipMemsetAsync(tmp, 0, sizeof(double) * d, st));
+  hipLaunchKernelGGL(colsum64_kernel, dim3((unsigned)ceil_div(n, 1024)), dim3(256), 0, st, x, ld, n,
+                     d, tmp);
+  hipLaunchKernelGGL(mean_from_sum_kernel, dim3((unsigned)ceil_div(d, 256)), dim3(256), 0, st, tmp,
+                     n, d, mean);
 }
 
 void launch_a16_norms(const float* x32, int64_t n, int d16, float* out, hipStream_t st) {

@@ -28,27 +28,27 @@ namespace {
 constexpr int BK = 32;    // K chunk staged per LDS buffer
 constexpr int LPAD = 8;   // bf16 padding per LDS row (conflict-free ds_read_b128)
 
-// Exclusion threshold (a' domain) for the smallest norms-form distance, given
-// the first-pass value v of an emitted candidate: any centroid whose a' is at
-// least the result has an FP32 norms-form distance strictly above the
-// candidate's.  Derivation: |a'+|q~|^2 - |q~-x~|^2| <= G, triangle inequality
-// for the bf16 rounding (qe + ex), FP32 norms-form error H.
-__device__ float top1_threshold(float v, float qn, float qe, float qr, float ex, float rx, int d) {
+// Exclusion threshold (squared-distance domain) for the smallest norms-form
+// distance, given the first-pass value v of an emitted candidate: any centroid
+// whose first-pass value is at least the result has an FP32 norms-form distance
+// strictly above the candidate's.  Derivation: |S^ - |q~ - x~|^2| <= G, triangle
+// inequality for the bf16 rounding (qe + ex), FP32 norms-form error H.
+__device__ float top1_threshold(float v, float qe, float qr, float ex, float rx, int d) {
   const double u = 1.0 / 16777216.0;
   double R = (double)qr + rx;
   double G = (2.0 * d + 8.0) * u * R * R;
   double R32 = R + qe + ex;
   double H = (d + 8.0) * u * R32 * R32;
-  double s = (double)v + qn + G;
+  double s = (double)v + G;
   double dub = sqrt(s > 0.0 ? s : 0.0) + qe + ex;       // upper bound of |q - x|
   double fub = sqrt(dub * dub + H) * (1.0 + 8.0 * u);   // upper bound of FP32 F(v)
   double f2 = fub / (1.0 - 8.0 * u);
   double L = sqrt(f2 * f2 + H) + qe + ex;
-  double t = (L * L - qn + G) * (1.0 + 1e-9) + 1e-30;
+  double t = (L * L + G) * (1.0 + 1e-9) + 1e-30;
   return (float)t * (1.0f + 1e-6f);
 }
 
-template <int WM, int WN>
+template <int WM, int WN, int PF>
 struct Tile {
   static constexpr int BM = 64 * WM;
   static constexpr int BN = 64 * WN;
@@ -61,10 +61,13 @@ struct Tile {
   static constexpr size_t LDS_GATHER = LDS_BYTES + (size_t)BN * LCAP * 8;
 };
 
-template <int WM, int WN, int MODE>
+// One work item.  Values are in the squared-distance domain:
+//   S^ = |x~|^2 - 2 q~.x~ + |q~|^2   (fma, then the query's norm added)
+// Global loads run PF K-chunks ahead through registers; LDS is double-buffered.
+template <int WM, int WN, int PF, int MODE>
 __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, int row_begin,
                                           int row_end, int region, unsigned char* smem) {
-  using T = Tile<WM, WN>;
+  using T = Tile<WM, WN, PF>;
   constexpr int BM = T::BM, BN = T::BN, LROW = T::LROW;
   bf16_t* As = (bf16_t*)smem;                   // [2][BM][LROW]
   bf16_t* Bs = As + 2 * BM * LROW;              // [2][BN][LROW]
@@ -82,10 +85,13 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
 #pragma unroll
   for (int p = 0; p < T::B_CHUNKS; ++p) {
     int c = (tid + 256 * p) >> 2;
-    brow[p] = c < nqi ? (qidx ? qidx[qoff + c] : qoff + c) : (qidx ? qidx[qoff] : qoff);
+    if (a.b_by_slot)
+      brow[p] = qoff + (c < nqi ? c : 0);
+    else
+      brow[p] = c < nqi ? (qidx ? qidx[qoff + c] : qoff + c) : (qidx ? qidx[qoff] : qoff);
   }
   int gqc[WN], qcl[WN];
-  float tau_l[WN];
+  float tau_l[WN], qo[WN];
 #pragma unroll
   for (int bn = 0; bn < WN; ++bn) {
     int c = wc * 32 * WN + bn * 32 + (lane & 31);
@@ -93,37 +99,40 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
     gqc[bn] = gq;
     qcl[bn] = c;
     tau_l[bn] = (MODE != SCAN_DENSE && gq >= 0) ? a.tau[gq] : -INFINITY;
+    float o = 0.0f;
+    if (gq >= 0) o = a.col_norm ? a.col_norm[qoff + c] : (a.qnorm ? a.qnorm[gq] : 0.0f);
+    qo[bn] = o;
   }
 
-  intx4 ra[T::A_CHUNKS], rb[T::B_CHUNKS];
-  auto gload = [&](int s) {
+  intx4 ra[PF][T::A_CHUNKS], rb[PF][T::B_CHUNKS];
+  auto gload = [&](int R, int s) {
     int t = s / nk, kc = s - t * nk;
     int r0 = row_begin + t * BM;
 #pragma unroll
     for (int p = 0; p < T::A_CHUNKS; ++p) {
       int c = tid + 256 * p;
       int r = c >> 2, cc = (c & 3) * 8;
-      ra[p] = *(const intx4*)(a.xb + (int64_t)(r0 + r) * kp + kc * BK + cc);
+      ra[R][p] = *(const intx4*)(a.xb + (int64_t)(r0 + r) * kp + kc * BK + cc);
     }
 #pragma unroll
     for (int p = 0; p < T::B_CHUNKS; ++p) {
       int c = tid + 256 * p;
       int cc = (c & 3) * 8;
-      rb[p] = *(const intx4*)(a.qb + (int64_t)brow[p] * kp + kc * BK + cc);
+      rb[R][p] = *(const intx4*)(a.qb + (int64_t)brow[p] * kp + kc * BK + cc);
     }
   };
-  auto lstore = [&](int buf) {
+  auto lstore = [&](int R, int buf) {
 #pragma unroll
     for (int p = 0; p < T::A_CHUNKS; ++p) {
       int c = tid + 256 * p;
       int r = c >> 2, cc = (c & 3) * 8;
-      *(intx4*)(As + (buf * BM + r) * LROW + cc) = ra[p];
+      *(intx4*)(As + (buf * BM + r) * LROW + cc) = ra[R][p];
     }
 #pragma unroll
     for (int p = 0; p < T::B_CHUNKS; ++p) {
       int c = tid + 256 * p;
       int r = c >> 2, cc = (c & 3) * 8;
-      *(intx4*)(Bs + (buf * BN + r) * LROW + cc) = rb[p];
+      *(intx4*)(Bs + (buf * BN + r) * LROW + cc) = rb[R][p];
     }
   };
 
@@ -135,15 +144,83 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
 
-  __syncthreads();  // LDS may still be in use by a previous item of this workgroup
-  for (int c = tid; c < BN; c += 256) lcnt[c] = 0;
-  gload(0);
-  lstore(0);
-  __syncthreads();
+  // Epilogue of one BM-row tile: per MFMA block, S^ from the accumulators
+  // (not written back), threshold mask, then the rare emissions.
+  auto epilogue = [&](int t) {
+    const int r0 = row_begin + t * BM;
+#pragma unroll
+    for (int i = 0; i < WM; ++i) {
+      const int rb0 = r0 + wr * 32 * WM + i * 32 + 4 * (lane >> 5);
+      float nx[16];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int row = rb0 + 8 * (e >> 2) + (e & 3);
+        nx[e] = row < row_end ? a.xn[row] : INFINITY;
+      }
+#pragma unroll
+      for (int j = 0; j < WN; ++j) {
+        if (MODE == SCAN_DENSE) {
+          if (gqc[j] >= 0) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+              const int row = rb0 + 8 * (e >> 2) + (e & 3);
+              const float v = __builtin_fmaf(-2.0f, acc[i][j][e], nx[e]) + qo[j];
+              if (row < row_end) a.dense[(int64_t)gqc[j] * a.ldd + (row - row_begin)] = v;
+            }
+          }
+          continue;
+        }
+        unsigned m = 0;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          const float v = __builtin_fmaf(-2.0f, acc[i][j][e], nx[e]) + qo[j];
+          m |= (v < tau_l[j] ? 1u : 0u) << e;
+        }
+        if (m == 0) continue;
+        const int gq = gqc[j];
+        int pos = atomicAdd(&lcnt[qcl[j]], __builtin_popcount(m));
+        unsigned* reg = a.cand + ((int64_t)gq * a.parts + region) * (int64_t)a.cap * 2;
+        while (m) {
+          const int e = __builtin_ctz(m);
+          m &= m - 1;
+          const int row = rb0 + 8 * (e >> 2) + (e & 3);
+          float av = 0.0f, xv = 0.0f;
+#pragma unroll
+          for (int ee = 0; ee < 16; ++ee) {
+            av = ee == e ? acc[i][j][ee] : av;
+            xv = ee == e ? nx[ee] : xv;
+          }
+          const float v = __builtin_fmaf(-2.0f, av, xv) + qo[j];
+          uint2 cv;
+          cv.x = __float_as_uint(v);
+          cv.y = (unsigned)row;
+          if (MODE == SCAN_GATHER) {
+            if (pos < T::LCAP) {
+              lbuf[qcl[j] * T::LCAP + pos] = cv;
+            } else {  // spill straight to the query's global buffer
+              int gp = atomicAdd(a.cnt + gq, 1);
+              if (gp < a.cap) *(uint2*)(a.cand + ((int64_t)gq * a.cap + gp) * 2) = cv;
+            }
+          } else if (pos < a.cap) {
+            *(uint2*)(reg + 2 * pos) = cv;
+          }
+          ++pos;
+          if (MODE == SCAN_TOP1) {
+            float th = top1_threshold(v, a.t1_qe[gq], a.t1_qr[gq], a.t1_ex, a.t1_rx, a.d);
+            tau_l[j] = fminf(tau_l[j], th);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+      for (int j = 0; j < WN; ++j)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
+  };
 
-  for (int s = 0; s < S; ++s) {
-    const int buf = s & 1;
-    if (s + 1 < S) gload(s + 1);
+  auto compute = [&](int buf) {
     const bf16_t* Ab = As + buf * BM * LROW;
     const bf16_t* Bb = Bs + buf * BN * LROW;
 #pragma unroll
@@ -162,86 +239,30 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
         for (int j = 0; j < WN; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
     }
-    if (s + 1 < S) lstore(buf ^ 1);
-    __syncthreads();
+  };
 
+  // step s uses register set R = s % PF
+  auto step = [&](int R, int s) {
+    if (s + PF < S) gload(R, s + PF);
+    compute(s & 1);
+    if (s + 1 < S) lstore((R + 1) % PF, (s + 1) & 1);
+    __syncthreads();
     const int t = s / nk;
-    if (s - t * nk != nk - 1) continue;
-    // ---------------- epilogue for row tile t ----------------
-    const int r0 = row_begin + t * BM;
+    if (s - t * nk == nk - 1) epilogue(t);
+  };
+
+  __syncthreads();  // LDS may still be in use by a previous item of this workgroup
+  for (int c = tid; c < BN; c += 256) lcnt[c] = 0;
 #pragma unroll
-    for (int j = 0; j < WN; ++j) {
-      unsigned m[WM];
-      int total = 0;
-#pragma unroll
-      for (int i = 0; i < WM; ++i) {
-        const int rb0 = r0 + wr * 32 * WM + i * 32 + 4 * (lane >> 5);
-        m[i] = 0;
-#pragma unroll
-        for (int e = 0; e < 16; ++e) {
-          const int row = rb0 + 8 * (e >> 2) + (e & 3);
-          const float nx = row < row_end ? a.xn[row] : INFINITY;
-          const float v = __builtin_fmaf(-2.0f, acc[i][j][e], nx);
-          acc[i][j][e] = v;
-          m[i] |= (v < tau_l[j] ? 1u : 0u) << e;
-        }
-        total += __builtin_popcount(m[i]);
-      }
-      if (MODE == SCAN_DENSE) {
-        if (gqc[j] >= 0) {
-#pragma unroll
-          for (int i = 0; i < WM; ++i) {
-            const int rb0 = r0 + wr * 32 * WM + i * 32 + 4 * (lane >> 5);
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-              const int row = rb0 + 8 * (e >> 2) + (e & 3);
-              if (row < row_end) a.dense[(int64_t)gqc[j] * a.ldd + (row - row_begin)] = acc[i][j][e];
-            }
-          }
-        }
-      } else if (total > 0) {
-        const int gq = gqc[j];
-        int pos = atomicAdd(&lcnt[qcl[j]], total);
-        unsigned* reg = a.cand + ((int64_t)gq * a.parts + region) * (int64_t)a.cap * 2;
-#pragma unroll
-        for (int i = 0; i < WM; ++i) {
-          const int rb0 = r0 + wr * 32 * WM + i * 32 + 4 * (lane >> 5);
-          unsigned mm = m[i];
-          while (mm) {
-            const int e = __builtin_ctz(mm);
-            mm &= mm - 1;
-            const int row = rb0 + 8 * (e >> 2) + (e & 3);
-            float v = 0.0f;
-#pragma unroll
-            for (int ee = 0; ee < 16; ++ee) v = ee == e ? acc[i][j][ee] : v;
-            uint2 cv;
-            cv.x = __float_as_uint(v);
-            cv.y = (unsigned)row;
-            if (MODE == SCAN_GATHER) {
-              if (pos < T::LCAP) {
-                lbuf[qcl[j] * T::LCAP + pos] = cv;
-              } else {  // spill straight to the query's global buffer
-                int gp = atomicAdd(a.cnt + gq, 1);
-                if (gp < a.cap) *(uint2*)(a.cand + ((int64_t)gq * a.cap + gp) * 2) = cv;
-              }
-            } else if (pos < a.cap) {
-              *(uint2*)(reg + 2 * pos) = cv;
-            }
-            ++pos;
-            if (MODE == SCAN_TOP1) {
-              float th = top1_threshold(v, a.t1_qn[gq], a.t1_qe[gq], a.t1_qr[gq], a.t1_ex,
-                                        a.t1_rx, a.d);
-              tau_l[j] = fminf(tau_l[j], th);
-            }
-          }
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < WM; ++i)
-#pragma unroll
-        for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
-    }
+  for (int r = 0; r < PF; ++r)
+    if (r < S) gload(r, r);
+  lstore(0, 0);
+  __syncthreads();
+  for (int s = 0; s < S; s += PF) {
+    step(0, s);
+    if (PF > 1 && s + 1 < S) step(1 % PF, s + 1);
   }
+
   if (MODE == SCAN_GATHER) {
     __syncthreads();
     for (int c = tid; c < nqi; c += 256) {
@@ -262,21 +283,32 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
   }
 }
 
-template <int WM, int WN, int MODE, int REFINE>
+template <int WM, int WN, int PF, int MODE, int REFINE>
 __global__ __launch_bounds__(256) void scan_grid_kernel(ScanArgs a) {
-  using T = Tile<WM, WN>;
+  using T = Tile<WM, WN, PF>;
   extern __shared__ __align__(16) unsigned char smem[];
   int nq = REFINE ? *a.nq_dev : a.nq;
-  int qtile = blockIdx.x / a.parts, part = blockIdx.x - qtile * a.parts;
+  int bid = blockIdx.x;
+  const int G = gridDim.x;
+  // XCD-aware order: consecutive logical ids share an XCD (and its L2)
+  if (a.xcd_map && (G & 7) == 0) bid = (bid & 7) * (G >> 3) + (bid >> 3);
+  int qtile, part;
+  if (a.xcd_map == 2) {  // part-major: an XCD streams one row partition
+    part = bid / a.qtiles;
+    qtile = bid - part * a.qtiles;
+  } else {
+    qtile = bid / a.parts;
+    part = bid - qtile * a.parts;
+  }
   int qoff = qtile * T::BN;
-  if (qoff >= nq) return;
+  if (qoff >= nq || part >= a.parts) return;
   int nqi = min(T::BN, nq - qoff);
   int rb = part * a.rows_per_part;
   int re = min(a.n_rows, rb + a.rows_per_part);
-  scan_item<WM, WN, MODE>(a, qoff, nqi, rb, re, part, smem);
+  scan_item<WM, WN, PF, MODE>(a, qoff, nqi, rb, re, part, smem);
 }
 
-template <int WM, int WN, int MODE>
+template <int WM, int WN, int PF, int MODE>
 __global__ __launch_bounds__(256) void scan_items_kernel(ScanArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_item;
@@ -288,7 +320,7 @@ __global__ __launch_bounds__(256) void scan_items_kernel(ScanArgs a) {
     __syncthreads();
     if (it >= n_items) break;
     const ScanItem w = a.items[it];
-    scan_item<WM, WN, MODE>(a, w.qoff, w.nq, w.row_begin, w.row_end, w.region, smem);
+    scan_item<WM, WN, PF, MODE>(a, w.qoff, w.nq, w.row_begin, w.row_end, w.region, smem);
   }
 }
 
@@ -298,41 +330,45 @@ void set_lds(K kern, size_t lds) {
                                (int)lds));
 }
 
-template <int WM, int WN, int MODE>
+template <int WM, int WN, int PF, int MODE>
 void launch_t(const ScanArgs& a, int grid, int kind, hipStream_t st) {
-  using T = Tile<WM, WN>;
+  using T = Tile<WM, WN, PF>;
   const size_t lds = MODE == SCAN_GATHER ? T::LDS_GATHER : T::LDS_BYTES;
   static bool attr_set = false;  // dynamic LDS above 64 KiB must be opted into
   if (!attr_set) {
-    set_lds(scan_items_kernel<WM, WN, MODE>, lds);
-    set_lds(scan_grid_kernel<WM, WN, MODE, 0>, lds);
-    set_lds(scan_grid_kernel<WM, WN, MODE, 1>, lds);
+    set_lds(scan_items_kernel<WM, WN, PF, MODE>, lds);
+    set_lds(scan_grid_kernel<WM, WN, PF, MODE, 0>, lds);
+    set_lds(scan_grid_kernel<WM, WN, PF, MODE, 1>, lds);
     attr_set = true;
   }
   if (kind == SCAN_KIND_ITEMS)
-    hipLaunchKernelGGL((scan_items_kernel<WM, WN, MODE>), dim3(grid), dim3(256), lds, st, a);
+    hipLaunchKernelGGL((scan_items_kernel<WM, WN, PF, MODE>), dim3(grid), dim3(256), lds, st, a);
   else if (kind == SCAN_KIND_REFINE)
-    hipLaunchKernelGGL((scan_grid_kernel<WM, WN, MODE, 1>), dim3(grid), dim3(256), lds, st, a);
+    hipLaunchKernelGGL((scan_grid_kernel<WM, WN, PF, MODE, 1>), dim3(grid), dim3(256), lds, st, a);
   else
-    hipLaunchKernelGGL((scan_grid_kernel<WM, WN, MODE, 0>), dim3(grid), dim3(256), lds, st, a);
+    hipLaunchKernelGGL((scan_grid_kernel<WM, WN, PF, MODE, 0>), dim3(grid), dim3(256), lds, st, a);
 }
 
 }  // namespace
 
-// Tile configurations (wave tile = 32*WM rows x 32*WN queries).
-static const int kCfgWM[3] = {2, 4, 2};
-static const int kCfgWN[3] = {2, 2, 4};
+// Tile configurations: wave tile 32*WM rows x 32*WN queries, prefetch depth PF.
+struct ScanCfg {
+  int wm, wn, pf;
+};
+static const ScanCfg kCfg[] = {{2, 2, 1}, {4, 2, 1}, {2, 4, 1}, {2, 2, 2}, {4, 2, 2}, {4, 4, 1}};
+constexpr int kNumCfg = sizeof(kCfg) / sizeof(kCfg[0]);
 
-int scan_tile_rows() { return 64 * kCfgWM[tuning().scan_cfg]; }
-int scan_tile_queries() { return 64 * kCfgWN[tuning().scan_cfg]; }
+int scan_num_cfg() { return kNumCfg; }
+int scan_tile_rows() { return 64 * kCfg[tuning().scan_cfg].wm; }
+int scan_tile_queries() { return 64 * kCfg[tuning().scan_cfg].wn; }
 
-template <int WM, int WN>
+template <int WM, int WN, int PF>
 static void launch_cfg(const ScanArgs& a, int mode, int grid, int kind, hipStream_t st) {
   switch (mode) {
-    case SCAN_FILTER: launch_t<WM, WN, SCAN_FILTER>(a, grid, kind, st); break;
-    case SCAN_DENSE: launch_t<WM, WN, SCAN_DENSE>(a, grid, kind, st); break;
-    case SCAN_TOP1: launch_t<WM, WN, SCAN_TOP1>(a, grid, kind, st); break;
-    case SCAN_GATHER: launch_t<WM, WN, SCAN_GATHER>(a, grid, kind, st); break;
+    case SCAN_FILTER: launch_t<WM, WN, PF, SCAN_FILTER>(a, grid, kind, st); break;
+    case SCAN_DENSE: launch_t<WM, WN, PF, SCAN_DENSE>(a, grid, kind, st); break;
+    case SCAN_TOP1: launch_t<WM, WN, PF, SCAN_TOP1>(a, grid, kind, st); break;
+    case SCAN_GATHER: launch_t<WM, WN, PF, SCAN_GATHER>(a, grid, kind, st); break;
     default: break;
   }
 }
@@ -340,9 +376,12 @@ static void launch_cfg(const ScanArgs& a, int mode, int grid, int kind, hipStrea
 void launch_scan(const ScanArgs& a, int mode, int grid, int kind, hipStream_t st) {
   if (grid <= 0) return;
   switch (tuning().scan_cfg) {
-    case 1: launch_cfg<4, 2>(a, mode, grid, kind, st); break;
-    case 2: launch_cfg<2, 4>(a, mode, grid, kind, st); break;
-    default: launch_cfg<2, 2>(a, mode, grid, kind, st); break;
+    case 1: launch_cfg<4, 2, 1>(a, mode, grid, kind, st); break;
+    case 2: launch_cfg<2, 4, 1>(a, mode, grid, kind, st); break;
+    case 3: launch_cfg<2, 2, 2>(a, mode, grid, kind, st); break;
+    case 4: launch_cfg<4, 2, 2>(a, mode, grid, kind, st); break;
+    case 5: launch_cfg<4, 4, 1>(a, mode, grid, kind, st); break;
+    default: launch_cfg<2, 2, 1>(a, mode, grid, kind, st); break;
   }
 }
 

@@ -16,6 +16,7 @@ namespace {
 constexpr int SEL_CAP = 1024;   // max candidates kept per query (LDS sort size)
 constexpr int EX_BATCH = 256;   // rows per batch in the exact fallback
 constexpr int EX_KMAX = 256;    // max k_req of the exact fallback
+constexpr int EX_SEG = 4096;    // rows per block of the exact fallback
 
 __device__ __forceinline__ void bitonic_sort(uint64_t* s, int P) {
   for (int k = 2; k <= P; k <<= 1) {
@@ -329,7 +330,7 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(SelectArgs a) {
     double T = INFINITY;
     if (n >= k_eff && k_eff > 0) {
       const uint64_t kk = ex[k_eff - 1];
-      T = exclusion_threshold(ord2f((uint32_t)(kk >> 32)), a.metric, a.qn[q], a.qe[q], a.qr[q],
+      T = exclusion_threshold(ord2f((uint32_t)(kk >> 32)), a.metric, a.qe[q], a.qr[q],
                               a.ex, a.rx, a.d);
     }
     return T;
@@ -416,43 +417,100 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(SelectArgs a) {
   if (a.out_count && lane == 0) a.out_count[q] = count;
 }
 
-__global__ __launch_bounds__(256) void exact_topk_kernel(ExactArgs a) {
-  const int nq = *a.nq_dev;
-  if ((int)blockIdx.x >= nq) return;
-  const int q = a.qlist[blockIdx.x];
-  __shared__ uint64_t m[EX_BATCH + EX_KMAX];
+// Exact fallback, stage 1: block (slot, seg) computes the exact metric of the
+// rows of one segment (at most EX_SEG rows inside one range of the query) and
+// keeps their kr smallest (dist, row) keys.  Rows whose key cannot enter the
+// running top kr are dropped before any sort, so the sorts stop once the
+// threshold has settled.
+__global__ __launch_bounds__(256) void exact_partial_kernel(ExactArgs a) {
+  const int slot = a.slot_base + (int)blockIdx.x;
   const int tid = threadIdx.x, g = tid >> 4, j = tid & 15;
   const int kr = a.k_req < EX_KMAX ? a.k_req : EX_KMAX;
+  uint64_t* out = a.partial + ((int64_t)blockIdx.x * gridDim.y + blockIdx.y) * kr;
+  __shared__ uint64_t m[EX_BATCH + EX_KMAX];
+  __shared__ int s_rb, s_re, s_any;
+  if (slot >= *a.nq_dev) return;
+  const int q = a.qlist[slot];
+  if (tid == 0) {  // locate segment blockIdx.y among the segments of the ranges
+    const int nr = a.range_cnt ? a.range_cnt[q] : 1;
+    int seg = (int)blockIdx.y, rb = 0, re = 0;
+    for (int ri = 0; ri < nr; ++ri) {
+      int lo = 0, hi = a.n_rows;
+      if (a.range_cnt) {
+        const int2 rg = a.ranges[(int64_t)q * a.range_stride + ri];
+        lo = rg.x;
+        hi = rg.y;
+      }
+      const int ns = (hi - lo + EX_SEG - 1) / EX_SEG;
+      if (seg < ns) {
+        rb = lo + seg * EX_SEG;
+        re = min(hi, rb + EX_SEG);
+        break;
+      }
+      seg -= ns;
+    }
+    s_rb = rb;
+    s_re = re;
+  }
   for (int i = tid; i < EX_BATCH + EX_KMAX; i += 256) m[i] = ~0ull;
   __syncthreads();
-  int nr = a.range_cnt ? a.range_cnt[q] : 1;
-  for (int ri = 0; ri < nr; ++ri) {
-    int rb = 0, re = a.n_rows;
-    if (a.range_cnt) {
-      int2 rg = a.ranges[(int64_t)q * a.range_stride + ri];
-      rb = rg.x;
-      re = rg.y;
-    }
-    for (int base = rb; base < re; base += EX_BATCH) {
-      for (int i = g; i < EX_BATCH; i += 16) {
-        int r = base + i;
-        uint64_t key = ~0ull;
-        if (r < re) {
-          float dv = exact_metric(a.x32, a.q32, a.xn2, a.qn2, a.d16, a.metric, r, q, j);
-          key = pack_key(dv, (uint32_t)r);
-        }
-        if (j == 0) m[i] = key;
+  const int rb = s_rb, re = s_re;
+  uint64_t thr = ~0ull;
+  for (int base = rb; base < re; base += EX_BATCH) {
+    if (tid == 0) s_any = 0;
+    __syncthreads();
+    for (int i = g; i < EX_BATCH; i += 16) {
+      const int r = base + i;
+      uint64_t key = ~0ull;
+      if (r < re) {
+        const float dv = exact_metric(a.x32, a.q32, a.xn2, a.qn2, a.d16, a.metric, r, q, j);
+        key = pack_key(dv, (uint32_t)r);
+        if (key >= thr) key = ~0ull;
       }
-      __syncthreads();
+      if (j == 0) {
+        m[i] = key;
+        if (key != ~0ull) s_any = 1;
+      }
+    }
+    __syncthreads();
+    if (s_any) {
       bitonic_sort(m, EX_BATCH + EX_KMAX);
-      // keep the kr smallest at [EX_BATCH, EX_BATCH + kr)
-      uint64_t keep = tid < kr ? m[tid] : ~0ull;
+      const uint64_t keep = tid < kr ? m[tid] : ~0ull;
       __syncthreads();
       for (int i = tid; i < EX_BATCH + EX_KMAX; i += 256) m[i] = ~0ull;
       __syncthreads();
       if (tid < kr) m[EX_BATCH + tid] = keep;
       __syncthreads();
+      thr = m[EX_BATCH + kr - 1];
     }
+  }
+  if (tid < kr) out[tid] = m[EX_BATCH + tid];
+}
+
+// Exact fallback, stage 2: merge the per-segment top kr of one query and write
+// its result (self removal by id for the kNN graph).
+__global__ __launch_bounds__(256) void exact_merge_kernel(ExactArgs a) {
+  const int slot = a.slot_base + (int)blockIdx.x;
+  if (slot >= *a.nq_dev) return;
+  const int q = a.qlist[slot];
+  const int tid = threadIdx.x;
+  const int kr = a.k_req < EX_KMAX ? a.k_req : EX_KMAX;
+  __shared__ uint64_t m[EX_BATCH + EX_KMAX];
+  for (int i = tid; i < EX_BATCH + EX_KMAX; i += 256) m[i] = ~0ull;
+  __syncthreads();
+  const uint64_t* part = a.partial + (int64_t)blockIdx.x * a.nseg * kr;
+  for (int sg = 0; sg < a.nseg; ++sg) {
+    const uint64_t* src = part + (int64_t)sg * kr;
+    if (src[0] == ~0ull || src[0] >= m[EX_BATCH + kr - 1]) continue;  // uniform per block
+    if (tid < kr) m[tid] = src[tid];
+    __syncthreads();
+    bitonic_sort(m, EX_BATCH + EX_KMAX);
+    const uint64_t keep = tid < kr ? m[tid] : ~0ull;
+    __syncthreads();
+    for (int i = tid; i < EX_BATCH + EX_KMAX; i += 256) m[i] = ~0ull;
+    __syncthreads();
+    if (tid < kr) m[EX_BATCH + tid] = keep;
+    __syncthreads();
   }
   if (tid == 0) {
     int o = 0;
@@ -493,9 +551,12 @@ void launch_select_rerank(const SelectArgs& a, int nq_max, hipStream_t st) {
                      a);
 }
 
-void launch_exact_topk(const ExactArgs& a, int grid, hipStream_t st) {
-  if (grid <= 0) return;
-  hipLaunchKernelGGL(exact_topk_kernel, dim3(grid), dim3(256), 0, st, a);
+int exact_seg_rows() { return EX_SEG; }
+
+void launch_exact_topk(const ExactArgs& a, int nslots, hipStream_t st) {
+  if (nslots <= 0 || a.nseg <= 0) return;
+  hipLaunchKernelGGL(exact_partial_kernel, dim3(nslots, a.nseg), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(exact_merge_kernel, dim3(nslots), dim3(256), 0, st, a);
 }
 
 }  // namespace mann

@@ -23,8 +23,8 @@ struct Top1Args {
 };
 
 void launch_top1_select(const Top1Args& a, hipStream_t st);
-void launch_prime_threshold(const Top1Args& a, const float* qn, const float* qe, const float* qr,
-                            float ex, float rx, int d, float* tau, hipStream_t st);
+void launch_prime_threshold(const Top1Args& a, const float* qe, const float* qr, float ex, float rx,
+                            int d, float* tau, hipStream_t st);
 int sort_block();
 void launch_counting_sort(const int* assign, int64_t n, int k, int* table, int* offsets,
                           int* members, hipStream_t st);

@@ -18,11 +18,17 @@ struct ScanItem {
 
 struct ScanArgs {
   const bf16_t* xb;   // base rows, kp bf16 each (padded rows: zeros)
+  const float* qnorm; // |q~|^2 per query (added to every value of its column)
+  const float* col_norm;  // or per item column: [qoff + c] (IVF residual queries)
+  int xcd_map;        // grid mode: 0 plain, 1 query-major per XCD, 2 part-major per XCD
+  int qtiles;         // grid mode: number of query tiles
   const float* xn;    // |x~|^2 per base row, +inf on padding
   const bf16_t* qb;   // query rows, kp bf16 each
   int kp;             // padded K (multiple of 64)
   int d;              // true dimension
   const int* qidx;    // query id list (nullptr: identity)
+  int b_by_slot;      // qb row of column c is its slot qoff + c (IVF residual queries), not
+                      // its query id
   // grid mode
   int nq;             // number of queries (or ...)
   const int* nq_dev;  // ... read from device memory when non-null
@@ -42,7 +48,6 @@ struct ScanArgs {
   float* dense;
   int64_t ldd;
   // TOP1 certification inputs
-  const float* t1_qn;
   const float* t1_qe;
   const float* t1_qr;
   float t1_ex, t1_rx;
@@ -50,20 +55,27 @@ struct ScanArgs {
 
 // Run-time tuning knobs (set through mann_tune_set; chosen by the autotuner).
 struct Tuning {
-  int scan_cfg = 0;          // scan tile: 0 = 128x128, 1 = 256x128, 2 = 128x256 (rows x queries)
+  int scan_cfg = 0;          // scan tile config (index into the table of kernels_scan.hip)
+  int xcd_map = 0;           // grid order: 0 plain, 1 query-major per XCD, 2 part-major per XCD
   int target_groups = 2048;  // workgroups the row partitioning aims for
   int sample_rank = 1;       // threshold rank = sample_rank * k_req in the row sample
   int shortlist_mult = 2;    // first re-rank batch = shortlist_mult * k_req (>= 32)
 };
 Tuning& tuning();
 
+int scan_num_cfg();
 int scan_tile_rows();
 int scan_tile_queries();
 void launch_scan(const ScanArgs& a, int mode, int grid, int kind, hipStream_t st);
 
 void launch_prep_rows(const float* src, int64_t ld, int64_t n, int64_t n_pad, int d, int d16,
                       int kp, int normalize, float* x32, bf16_t* xb, float* xn, float* err,
-                      float* nrm, unsigned* max_err, unsigned* max_nrm, hipStream_t st);
+                      float* nrm, unsigned* max_err, unsigned* max_nrm, hipStream_t st,
+                      const float* center = nullptr, const int* center_row = nullptr,
+                      int64_t center_ld = 0);
+// FP32 mean of n rows (FP64 sums); tmp holds d doubles
+void launch_mean_rows(const float* x, int64_t ld, int64_t n, int d, double* tmp, float* mean,
+                      hipStream_t st);
 void launch_a16_norms(const float* x32, int64_t n, int d16, float* out, hipStream_t st);
 
 // ---- selection / exact re-rank -------------------------------------------------
@@ -98,7 +110,6 @@ struct SelectArgs {
   int metric;          // EUCLIDEAN / SQEUCLIDEAN / COSINE / NORMS_FORM
   const int* perm;     // base row -> original point id (nullptr: identity)
   // certification inputs
-  const float* qn;     // |q~|^2
   const float* qe;     // |q - q~|
   const float* qr;     // |q~|
   float ex, rx;        // dataset bounds
@@ -142,7 +153,14 @@ struct ExactArgs {
   int64_t* out_ids;
   float* out_dist;
   int* out_count;
+  // segment partials: slots [slot_base, slot_base + nslots) of qlist, nseg
+  // segments each (EX_SEG rows inside one range), kr keys per segment
+  int slot_base;
+  int nseg;
+  uint64_t* partial;  // [nslots][nseg][min(k_req, exact_kmax())]
 };
-void launch_exact_topk(const ExactArgs& a, int grid, hipStream_t st);
+// Two launches: per-segment exact top-k, then the per-query merge.
+void launch_exact_topk(const ExactArgs& a, int nslots, hipStream_t st);
+int exact_seg_rows();
 
 }  // namespace mann

@@ -26,6 +26,7 @@ class LearnedProbeRouter:
         h = ctypes.c_void_p()
         check(lib().mann_router_create(d, hidden, nlist, seed, ctypes.byref(h)))
         self.handle = h
+        self.gamma = 0.0
 
     def __del__(self):
         h = getattr(self, "handle", None)
@@ -52,6 +53,56 @@ class LearnedProbeRouter:
         if p.size != self.n_params:
             raise ArgumentError("parameter vector has the wrong length")
         check(lib().mann_router_set_params(self.handle, p.ctypes.data))
+
+    def set_prior(self, index, gamma: float):
+        """Add gamma * (2 q.c - |c|^2) of the index's centroids to the logits
+        (gamma = 0 removes it); the MLP then learns a correction of the
+        centroid routing."""
+        check(lib().mann_router_set_prior(self.handle, index.handle if index is not None else None,
+                                          float(gamma)))
+        self.gamma = float(gamma)
+
+    def zero_output_layer(self):
+        """W2 = 0, b2 = 0: with a prior the untrained router equals it."""
+        p = self.get_params()
+        o = self.hidden * self.d + self.hidden
+        p[o:] = 0.0
+        self.set_params(p)
+
+    def fit_prior(self, index, X, labels, grid=None):
+        """Scale of the centroid prior minimising the prior-only cross-entropy
+        on (X, labels) (golden-section refinement of a log-spaced grid).
+        Leaves the prior attached with that scale and returns it."""
+        saved = self.get_params()
+        self.set_params(np.zeros_like(saved))
+        grid = grid if grid is not None else [2.0 ** e for e in range(-12, 3)]
+
+        def loss_at(g):
+            self.set_prior(index, g)
+            return self.grad(X, labels)[0]
+
+        losses = [loss_at(g) for g in grid]
+        i = int(np.argmin(losses))
+        lo = np.log(grid[max(i - 1, 0)])
+        hi = np.log(grid[min(i + 1, len(grid) - 1)])
+        phi = (np.sqrt(5.0) - 1.0) / 2.0
+        a, b = hi - phi * (hi - lo), lo + phi * (hi - lo)
+        fa, fb = loss_at(np.exp(a)), loss_at(np.exp(b))
+        for _ in range(10):
+            if fa < fb:
+                hi, b, fb = b, a, fa
+                a = hi - phi * (hi - lo)
+                fa = loss_at(np.exp(a))
+            else:
+                lo, a, fa = a, b, fb
+                b = lo + phi * (hi - lo)
+                fb = loss_at(np.exp(b))
+        best = float(np.exp(a if fa < fb else b))
+        if min(fa, fb) > losses[i]:
+            best = float(grid[i])
+        self.set_params(saved)
+        self.set_prior(index, best)
+        return best
 
     def grad(self, X, labels):
         """(loss, gradient) of the mean cross-entropy on one batch; X is d x n."""

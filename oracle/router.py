@@ -37,7 +37,14 @@ def targets(labels, c):
     return T
 
 
-def loss_and_grad(params, X, labels, d, h, c, bf16=False):
+def prior_logits(X, C, gamma):
+    """Centroid prior gamma * (2 x.c - |c|^2) (FP64); C is (c, d)."""
+    X = np.asarray(X, np.float64)
+    C = np.asarray(C, np.float64)
+    return gamma * (2.0 * X @ C.T - (C * C).sum(1))
+
+
+def loss_and_grad(params, X, labels, d, h, c, bf16=False, prior=None):
     r = to_bf16 if bf16 else (lambda a: np.asarray(a, np.float64))
     f32 = (lambda a: np.asarray(a, np.float32)) if bf16 else (lambda a: np.asarray(a, np.float64))
     W1, b1, W2, b2 = split(params, d, h, c)
@@ -46,6 +53,8 @@ def loss_and_grad(params, X, labels, d, h, c, bf16=False):
     H = np.maximum(r(Xf) @ r(f32(W1)).T + b1, 0.0)
     H32 = f32(H)
     Z = r(H32) @ r(f32(W2)).T + b2
+    if prior is not None:
+        Z = Z + prior
     Z32 = np.asarray(f32(Z), np.float64)
     T = targets(labels, c)
     zmax = Z32.max(1, keepdims=True)
@@ -63,8 +72,9 @@ def loss_and_grad(params, X, labels, d, h, c, bf16=False):
     return loss, np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2])
 
 
-def scores(params, X, d, h, c, bf16=True):
+def scores(params, X, d, h, c, bf16=True, prior=None):
     r = to_bf16 if bf16 else (lambda a: np.asarray(a, np.float64))
     W1, b1, W2, b2 = split(params, d, h, c)
     H = np.maximum(r(np.asarray(X, np.float32)) @ r(W1.astype(np.float32)).T + b1, 0.0)
-    return r(H.astype(np.float32)) @ r(W2.astype(np.float32)).T + b2
+    Z = r(H.astype(np.float32)) @ r(W2.astype(np.float32)).T + b2
+    return Z if prior is None else Z + prior

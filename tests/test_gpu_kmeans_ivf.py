@@ -132,6 +132,56 @@ def test_ivf_empty_buckets_and_short_lists():
     assert (dist.view(np.uint32) == od.view(np.uint32)).all()
 
 
+def _ivf_vs_oracle(X, C, a, Q, k, nprobe):
+    index = I.build_index_from_assignment(X.T, C.T, a, I.TopKRouting(nprobe))
+    ids, dist = I.query_arrays(index, X.T, Q.T, k)
+    st = M.last_stats()
+    off, lids, nb = O.partition(a)
+    R = O.route_distances(Q, C)
+    oi, od = O.ivf_search(X, off, lids, nb, Q, R, k, nprobe)
+    assert (ids == oi).all(), np.argwhere(ids != oi)[:5]
+    assert (dist.view(np.uint32) == od.view(np.uint32)).all()
+    return st
+
+
+def test_ivf_refinement_pass():
+    # the nearest list is a shell of 2000 points at distance ~1 around the query,
+    # the true neighbours sit in the second list at ~0.5: the row-sample
+    # threshold admits the whole shell (overflow), the refinement pass re-scans
+    # with the threshold of the candidates already held
+    _banner("ivf refinement pass")
+    rng = np.random.default_rng(21)
+    d = 16
+    shell = rng.standard_normal((2000, d))
+    shell /= np.linalg.norm(shell, axis=1, keepdims=True)
+    near = 0.5 * np.eye(d)[0] + 0.01 * rng.standard_normal((10, d))
+    far = 5.0 * np.eye(d)[0] + 0.3 * rng.standard_normal((490, d))
+    X = np.concatenate([shell, near, far]).astype(np.float32)
+    a = np.concatenate([np.ones(2000, np.int64), np.full(500, 2, np.int64)])
+    C = np.stack([X[:2000].mean(0), X[2000:].mean(0)]).astype(np.float32)
+    Q = (0.01 * rng.standard_normal((8, d))).astype(np.float32)
+    st = _ivf_vs_oracle(X, C, a, Q, 10, 2)
+    print(f"stats={st}", flush=True)
+    assert st[2] >= 1, st  # queries that needed the refinement pass
+    assert st[1] >= 1, st  # first-pass overflow
+
+
+def test_ivf_exact_fallback_on_ties():
+    # 3000 identical rows in the probed list: every candidate ties, both passes
+    # overflow and the segmented exact fallback must return the (dist, bucket,
+    # id) order, i.e. the lowest ids
+    _banner("ivf exact fallback on ties")
+    rng = np.random.default_rng(22)
+    d = 24
+    X = np.concatenate([np.ones((3000, d)), rng.standard_normal((1500, d)) + 4.0]).astype(np.float32)
+    a = np.concatenate([np.ones(3000, np.int64), np.full(1500, 2, np.int64)])
+    C = np.stack([X[:3000].mean(0), X[3000:].mean(0)]).astype(np.float32)
+    Q = np.ones((5, d), np.float32)
+    st = _ivf_vs_oracle(X, C, a, Q, 10, 2)
+    print(f"stats={st}", flush=True)
+    assert st[3] == 5, st  # all five resolved by the exact fallback
+
+
 def test_ivf_external_route_scores():
     _banner("ivf with supplied routing distances")
     X, C, a, index = _build_ivf(n=20_000, d=32, nlist=64, seed=6, iters=4)
