@@ -173,11 +173,14 @@ int mann_ivf_search_routed(mann_ivf_t ivf, mann_router_t r, const float* queries
                            int64_t* out_ids, float* out_dist);
 
 /* Tuning knobs of the search kernels (process-wide): "scan_cfg" (first-pass
- * tile: 0 = 128 rows x 128 queries, 1 = 256 x 128, 2 = 128 x 256),
- * "target_groups" (workgroups the row partitioning aims for), "sample_rank"
- * (emission threshold = that multiple of k in the row sample), "shortlist_mult"
- * (first exact re-rank batch = that multiple of k).  No reference counterpart:
- * the reference has no tunable kernels. */
+ * tile: 0 = 128 rows x 128 queries, 1 = 256 x 128, 2 = 128 x 256,
+ * 3 = 256 x 256), "xcd_map" (grid order: 0 plain, 1 query-major per XCD,
+ * 2 row-partition-major per XCD, 3 bands of 8 row partitions per XCD), "target_groups" (workgroups the row
+ * partitioning aims for), "sample_rank" (emission threshold = that multiple of
+ * k in the row sample), "shortlist_mult" (first exact re-rank batch = that
+ * multiple of k), "tau_rows" (IVF: exact rows sampled per query for the
+ * emission threshold, 16..1024).  No reference counterpart: the reference has
+ * no tunable kernels. */
 int mann_tune_set(const char* key, int64_t value);
 int mann_tune_get(const char* key, int64_t* value);
 

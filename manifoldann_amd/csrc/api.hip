@@ -647,7 +647,7 @@ int mann_tune_set(const char* key, int64_t value) {
       MANN_REQUIRE(value >= 0 && value < scan_num_cfg(), ERR_ARGUMENT, "scan_cfg out of range");
       t.scan_cfg = (int)value;
     } else if (k == "xcd_map") {
-      MANN_REQUIRE(value >= 0 && value <= 2, ERR_ARGUMENT, "xcd_map must be 0, 1 or 2");
+      MANN_REQUIRE(value >= 0 && value <= 3, ERR_ARGUMENT, "xcd_map must be 0..3");
       t.xcd_map = (int)value;
     } else if (k == "target_groups") {
       MANN_REQUIRE(value >= 1, ERR_ARGUMENT, "target_groups must be positive");
@@ -655,6 +655,9 @@ int mann_tune_set(const char* key, int64_t value) {
     } else if (k == "sample_rank") {
       MANN_REQUIRE(value >= 1 && value <= 64, ERR_ARGUMENT, "sample_rank must be in 1..64");
       t.sample_rank = (int)value;
+    } else if (k == "tau_rows") {
+      MANN_REQUIRE(value >= 16 && value <= 1024, ERR_ARGUMENT, "tau_rows must be in [16, 1024]");
+      t.tau_rows = (int)value;
     } else if (k == "shortlist_mult") {
       MANN_REQUIRE(value >= 1 && value <= 16, ERR_ARGUMENT, "shortlist_mult must be in 1..16");
       t.shortlist_mult = (int)value;
@@ -674,6 +677,7 @@ int mann_tune_get(const char* key, int64_t* value) {
     else if (k == "target_groups") *value = t.target_groups;
     else if (k == "sample_rank") *value = t.sample_rank;
     else if (k == "shortlist_mult") *value = t.shortlist_mult;
+    else if (k == "tau_rows") *value = t.tau_rows;
     else throw Error{ERR_ARGUMENT, "unknown tuning key " + k};
   });
 }

@@ -202,6 +202,7 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
   ta.d16 = (int)iv.d16;
   ta.metric = metric;
   ta.emit_all_below = cap;
+  ta.sample_rows = std::max(1, std::min(tuning().tau_rows, ivf_tau_rows_max()));
   ta.probes = probes;
   ta.pcount = pcount;
   ta.prows = prows;

@@ -8,6 +8,7 @@ namespace mann {
 struct IvfTauArgs {
   int nq, nprobe, k, d, d16, metric;
   int emit_all_below;       // queries probing at most this many rows emit everything
+  int sample_rows;          // exact sample size (at most ivf_tau_rows_max())
   const int* probes;        // [nq][nprobe] buckets in probe order
   const int* pcount;        // probed (non-empty) lists per query
   const int* prows;         // rows in the probed lists per query
@@ -24,6 +25,7 @@ struct IvfTauArgs {
 };
 
 int route_max();
+int ivf_tau_rows_max();
 void launch_route_dist(const float* q32, const float* qn2, int nq, const float* c32,
                        const float* cn2, int nlist, int d16, float* R, hipStream_t st);
 void launch_probe_select(const float* R, int nq, int nlist, int avail, int nprobe,

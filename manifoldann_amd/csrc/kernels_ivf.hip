@@ -9,33 +9,65 @@ namespace mann {
 
 namespace {
 
-// R[q][c] = norms form to centroid c (kmeans/distance.jl:213-242), A16 order.
-// One block per query: 16 groups of 16 lanes walk the centroids.
+// R[q][c] = norms form to centroid c (kmeans/distance.jl:213-242), A16 order:
+// lane j of a 16-lane group runs the FMA chain of the dims t = j (mod 16) in
+// ascending t, then the fixed tree.  A block computes a 32 x 32 (query,
+// centroid) tile from LDS; group g owns 8 queries x 8 centroids (centroids
+// g%4 + 4b, so the four groups of a wavefront read four adjacent LDS rows), so
+// each lane holds 64 chains and loads 16 operands per 64 FMAs.  The d axis is
+// staged in chunks of RT_DC floats (the chains continue across chunks, order
+// unchanged).
+constexpr int RT_T = 32;
+constexpr int RT_DC = 256;
+constexpr int RT_LD = RT_DC + 16;  // row stride: adjacent rows 16 banks apart
 __global__ __launch_bounds__(256) void route_dist_kernel(const float* __restrict__ q32,
                                                          const float* __restrict__ qn2, int nq,
                                                          const float* __restrict__ c32,
                                                          const float* __restrict__ cn2, int nlist,
                                                          int d16, float* __restrict__ R) {
-  const int q = blockIdx.x;
-  if (q >= nq) return;
-  const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
-  const float* y = q32 + (int64_t)q * d16;
-  const float qn = qn2[q];
-  for (int c0 = 4 * g; c0 < nlist; c0 += 64) {
-    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    const float* x[4];
+  __shared__ float ys[RT_T][RT_LD];
+  __shared__ float xs[RT_T][RT_LD];
+  const int q0 = blockIdx.x * RT_T, c0 = blockIdx.y * RT_T;
+  const int tid = threadIdx.x, g = tid >> 4, j = tid & 15;
+  const int qa = (g >> 2) * 8, cb = g & 3;
+  float acc[8][8];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) x[u] = c32 + (int64_t)min(c0 + u, nlist - 1) * d16;
-    for (int t = 0; t < d16; t += 16) {
-      const float yv = y[t + j];
+  for (int a = 0; a < 8; ++a)
 #pragma unroll
-      for (int u = 0; u < 4; ++u) acc[u] = __builtin_fmaf(x[u][t + j], yv, acc[u]);
+    for (int b = 0; b < 8; ++b) acc[a][b] = 0.0f;
+  for (int t0 = 0; t0 < d16; t0 += RT_DC) {
+    const int w = min(RT_DC, d16 - t0);  // multiple of 16
+    __syncthreads();
+    for (int i = tid; i < RT_T * (RT_DC / 4); i += 256) {
+      const int r = i / (RT_DC / 4), c4 = (i - r * (RT_DC / 4)) * 4;
+      if (c4 < w) {
+        const int q = min(q0 + r, nq - 1), c = min(c0 + r, nlist - 1);
+        *(float4*)&ys[r][c4] = *(const float4*)(q32 + (int64_t)q * d16 + t0 + c4);
+        *(float4*)&xs[r][c4] = *(const float4*)(c32 + (int64_t)c * d16 + t0 + c4);
+      }
     }
+    __syncthreads();
+    for (int t = j; t < w; t += 16) {
+      float yv[8], xv[8];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const float dot = group16_tree(acc[u]);
-      const int c = c0 + u;
-      if (j == 0 && c < nlist) {
+      for (int a = 0; a < 8; ++a) yv[a] = ys[qa + a][t];
+#pragma unroll
+      for (int b = 0; b < 8; ++b) xv[b] = xs[cb + 4 * b][t];
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) acc[a][b] = __builtin_fmaf(xv[b], yv[a], acc[a][b]);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int q = q0 + qa + a;
+    const float qn = q < nq ? qn2[q] : 0.0f;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const float dot = group16_tree(acc[a][b]);
+      const int c = c0 + cb + 4 * b;
+      if (j == ((a * 8 + b) & 15) && q < nq && c < nlist) {
         float tt = cn2[c] + qn;
         float sq = tt - 2.0f * dot;
         R[(int64_t)q * nlist + c] = __builtin_sqrtf(sq > 0.0f ? sq : 0.0f);
@@ -98,6 +130,56 @@ __global__ __launch_bounds__(256) void probe_select_kernel(const float* __restri
   }
 }
 
+// Same selection for nprobe <= 64 and at most 1024 routing distances: one
+// wavefront per query holds 16 keys per lane and extracts the kk smallest
+// (dist, bucket) keys by repeated wavefront minima, i.e. in sorted order.
+constexpr int PS_KPL = 16;
+__global__ __launch_bounds__(256) void probe_select_wave_kernel(
+    const float* __restrict__ R, int nq, int nlist, int avail, int nprobe,
+    const int* __restrict__ list_off, int* __restrict__ probes, int* __restrict__ pcount,
+    int* __restrict__ prows) {
+  const int lane = threadIdx.x & 63;
+  const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (q >= nq) return;
+  uint64_t key[PS_KPL];
+  uint64_t lmin = ~0ull;
+#pragma unroll
+  for (int u = 0; u < PS_KPL; ++u) {
+    const int i = lane + 64 * u;
+    key[u] = i < avail ? pack_key(R[(int64_t)q * nlist + i], (uint32_t)i) : ~0ull;
+    lmin = key[u] < lmin ? key[u] : lmin;
+  }
+  const int kk = nprobe < avail ? nprobe : avail;
+  int m = 0, rows = 0;
+  for (int r = 0; r < kk; ++r) {
+    uint64_t g = lmin;
+    for (int o = 32; o >= 1; o >>= 1) {
+      const uint64_t t = __shfl_xor(g, o);
+      g = t < g ? t : g;
+    }
+    if (lmin == g) {  // the owner removes it (keys are unique: bucket in the low bits)
+      lmin = ~0ull;
+#pragma unroll
+      for (int u = 0; u < PS_KPL; ++u) {
+        if (key[u] == g) key[u] = ~0ull;
+        lmin = key[u] < lmin ? key[u] : lmin;
+      }
+    }
+    const int b = (int)(g & 0xffffffffu);
+    const int sz = list_off[b + 1] - list_off[b];
+    if (sz > 0) {
+      if (lane == 0) probes[(int64_t)q * nprobe + m] = b;
+      ++m;
+      rows += sz;
+    }
+  }
+  if (lane == 0) {
+    for (int r = m; r < nprobe; ++r) probes[(int64_t)q * nprobe + r] = -1;
+    pcount[q] = m;
+    prows[q] = rows;
+  }
+}
+
 // per-list count of probing queries
 __global__ void list_count_kernel(const int* __restrict__ probes, const int* __restrict__ pcount,
                                   int nq, int nprobe, int* __restrict__ qcount) {
@@ -107,8 +189,33 @@ __global__ void list_count_kernel(const int* __restrict__ probes, const int* __r
   if (r < pcount[q]) atomicAdd(&qcount[probes[i]], 1);
 }
 
+// exclusive scan of one value per thread over a 1024-thread block
+__device__ __forceinline__ int block_exclusive_scan(int v, int* wsum, int* total) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  int inc = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(inc, o);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  if (tid == 0) {
+    int run = 0;
+    for (int i = 0; i < 16; ++i) {
+      const int t = wsum[i];
+      wsum[i] = run;
+      run += t;
+    }
+    *total = run;
+  }
+  __syncthreads();
+  const int r = wsum[w] + inc - v;
+  __syncthreads();
+  return r;
+}
+
 // one block: exclusive scan of qcount (list order) and of the work-item counts
-// (size-descending list order); writes n_items.
+// (size-descending list order); writes n_items and the entry count.
 __global__ __launch_bounds__(1024) void plan_kernel(const int* __restrict__ qcount, int nb,
                                                     const int* __restrict__ list_off,
                                                     const int* __restrict__ lorder, int bn,
@@ -116,52 +223,30 @@ __global__ __launch_bounds__(1024) void plan_kernel(const int* __restrict__ qcou
                                                     int* __restrict__ ioff,
                                                     int* __restrict__ n_items,
                                                     int* __restrict__ n_entries) {
-  __shared__ int part[1024];
+  __shared__ int wsum[16];
+  __shared__ int total;
   const int tid = threadIdx.x;
   const int per = (nb + 1023) / 1024;
-  // queries offsets in bucket order
-  int s0 = tid * per, e0 = min(nb, s0 + per), sum = 0;
+  const int s0 = tid * per, e0 = min(nb, s0 + per);
+  // query offsets in bucket order
+  int sum = 0;
   for (int i = s0; i < e0; ++i) sum += qcount[i];
-  part[tid] = sum;
-  __syncthreads();
-  if (tid == 0) {
-    int run = 0;
-    for (int i = 0; i < 1024; ++i) {
-      int v = part[i];
-      part[i] = run;
-      run += v;
-    }
-    *n_entries = run;
-  }
-  __syncthreads();
-  int run = part[tid];
+  int run = block_exclusive_scan(sum, wsum, &total);
+  if (tid == 0) *n_entries = total;
   for (int i = s0; i < e0; ++i) {
     qoff[i] = run;
     run += qcount[i];
   }
-  __syncthreads();
   // items in size-descending list order
   sum = 0;
   for (int i = s0; i < e0; ++i) {
     const int b = lorder[i];
     const int nqc = (qcount[b] + bn - 1) / bn;
     const int sz = list_off[b + 1] - list_off[b];
-    const int ns = (sz + rows_item - 1) / rows_item;
-    sum += nqc * ns;
+    sum += nqc * ((sz + rows_item - 1) / rows_item);
   }
-  part[tid] = sum;
-  __syncthreads();
-  if (tid == 0) {
-    int r2 = 0;
-    for (int i = 0; i < 1024; ++i) {
-      int v = part[i];
-      part[i] = r2;
-      r2 += v;
-    }
-    *n_items = r2;
-  }
-  __syncthreads();
-  run = part[tid];
+  run = block_exclusive_scan(sum, wsum, &total);
+  if (tid == 0) *n_items = total;
   for (int i = s0; i < e0; ++i) {
     ioff[i] = run;
     const int b = lorder[i];
@@ -299,7 +384,8 @@ __global__ __launch_bounds__(256) void ivf_tau_kernel(IvfTauArgs a) {
   const int tid = threadIdx.x, g = tid >> 4, j = tid & 15;
   const int np = a.pcount[q];
   const int total = a.prows[q];
-  const int m0 = total < TAU_M ? total : TAU_M;
+  const int mmax = a.sample_rows < TAU_M ? a.sample_rows : TAU_M;
+  const int m0 = total < mmax ? total : mmax;
   const int k_eff = a.k < total ? a.k : total;
   if (tid == 0) a.k_eff_q[q] = k_eff;
   if (m0 <= 0 || k_eff <= 0 || total <= a.emit_all_below) {
@@ -429,17 +515,22 @@ int route_max() { return ROUTE_MAX; }
 
 void launch_route_dist(const float* q32, const float* qn2, int nq, const float* c32,
                        const float* cn2, int nlist, int d16, float* R, hipStream_t st) {
-  if (nq <= 0) return;
-  hipLaunchKernelGGL(route_dist_kernel, dim3(nq), dim3(256), 0, st, q32, qn2, nq, c32, cn2, nlist,
-                     d16, R);
+  if (nq <= 0 || nlist <= 0) return;
+  dim3 grid((unsigned)ceil_div(nq, RT_T), (unsigned)ceil_div(nlist, RT_T));
+  hipLaunchKernelGGL(route_dist_kernel, grid, dim3(256), 0, st, q32, qn2, nq, c32, cn2, nlist, d16,
+                     R);
 }
 
 void launch_probe_select(const float* R, int nq, int nlist, int avail, int nprobe,
                          const int* list_off, int* probes, int* pcount, int* prows,
                          hipStream_t st) {
   if (nq <= 0) return;
-  hipLaunchKernelGGL(probe_select_kernel, dim3(nq), dim3(256), 0, st, R, nq, nlist, avail, nprobe,
-                     list_off, probes, pcount, prows);
+  if (nprobe <= 64 && avail <= 64 * PS_KPL)
+    hipLaunchKernelGGL(probe_select_wave_kernel, dim3((unsigned)ceil_div(nq, 4)), dim3(256), 0, st,
+                       R, nq, nlist, avail, nprobe, list_off, probes, pcount, prows);
+  else
+    hipLaunchKernelGGL(probe_select_kernel, dim3(nq), dim3(256), 0, st, R, nq, nlist, avail,
+                       nprobe, list_off, probes, pcount, prows);
 }
 
 void launch_ivf_plan(const int* probes, const int* pcount, int nq, int nprobe, int nb,
@@ -480,6 +571,8 @@ void launch_refine_items(const int* flist, const int* nflag, int nflag_host, int
                      flist, nflag, nprobe, probes, pcount, slot_of, list_off, rows_item, items,
                      n_items);
 }
+
+int ivf_tau_rows_max() { return TAU_M; }
 
 void launch_ivf_tau(const IvfTauArgs& a, hipStream_t st) {
   if (a.nq <= 0) return;

@@ -25,8 +25,31 @@ namespace mann {
 
 namespace {
 
-constexpr int BK = 32;    // K chunk staged per LDS buffer
-constexpr int LPAD = 8;   // bf16 padding per LDS row (conflict-free ds_read_b128)
+typedef __attribute__((address_space(3))) void lds_void;
+
+// s_waitcnt vmcnt(N) lgkmcnt(0) (gfx9 encoding; expcnt left open)
+template <int N>
+__device__ __forceinline__ void wait_vm_lgkm() {
+  __builtin_amdgcn_s_waitcnt((N & 15) | (((N >> 4) & 3) << 14) | (7 << 4));
+}
+// Workgroup barrier without the release fence of __syncthreads (which would
+// drain every outstanding LDS DMA): callers wait for exactly the chunk they
+// need first.  The empty asm statements keep the compiler from moving LDS
+// accesses across it.
+__device__ __forceinline__ void bare_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// XOR swizzle of the 16-byte chunks of LDS row r (SLOTS = 4 or 8 per row): the
+// fragment reads (32 consecutive rows, one logical chunk) hit distinct banks
+// for 32 or 64 banks, and the staging writes are contiguous per wavefront.
+template <int SLOTS>
+__device__ __forceinline__ int swz(int r) {
+  if (SLOTS == 4) return ((r >> 1) ^ (r >> 3)) & 3;
+  return (r ^ (r >> 3)) & 7;
+}
 
 // Exclusion threshold (squared-distance domain) for the smallest norms-form
 // distance, given the first-pass value v of an emitted candidate: any centroid
@@ -48,47 +71,68 @@ __device__ float top1_threshold(float v, float qe, float qr, float ex, float rx,
   return (float)t * (1.0f + 1e-6f);
 }
 
-template <int WM, int WN, int PF>
+// Block tile: WR x WC wavefronts, each owning 32*WM rows x 32*WN queries.
+template <int WM, int WN, int WR, int WC, int BK>
 struct Tile {
-  static constexpr int BM = 64 * WM;
-  static constexpr int BN = 64 * WN;
-  static constexpr int A_CHUNKS = BM * (BK / 8) / 256;  // 16B chunks per thread
-  static constexpr int B_CHUNKS = BN * (BK / 8) / 256;
-  static constexpr int LROW = BK + LPAD;
-  static constexpr size_t LDS_BYTES = (size_t)2 * (BM + BN) * LROW * sizeof(bf16_t) + BN * 4;
+  static constexpr int NT = 64 * WR * WC;               // threads
+  static constexpr int BM = 32 * WM * WR;
+  static constexpr int BN = 32 * WN * WC;
+  // LDS ring depth: chunks s+1 .. s+STAGES-1 are in flight while s is used
+  static constexpr int STAGES = 3;
+  static constexpr int SLOTS = BK / 8;                  // 16-byte chunks per LDS row
+  static constexpr int A_CHUNKS = BM * SLOTS / NT;      // 16B chunks per thread
+  static constexpr int B_CHUNKS = BN * SLOTS / NT;
+  static constexpr int LROW = BK;
+  static constexpr size_t LDS_BYTES =
+      (size_t)STAGES * (BM + BN) * LROW * sizeof(bf16_t) + BN * 4;
+  static constexpr int NPC = A_CHUNKS + B_CHUNKS;       // DMA ops per thread per chunk
   // GATHER mode: per query column a small LDS candidate buffer
-  static constexpr int LCAP = BN > 128 ? 16 : 32;
+  static constexpr int LCAP = 16;
   static constexpr size_t LDS_GATHER = LDS_BYTES + (size_t)BN * LCAP * 8;
 };
 
 // One work item.  Values are in the squared-distance domain:
 //   S^ = |x~|^2 - 2 q~.x~ + |q~|^2   (fma, then the query's norm added)
-// Global loads run PF K-chunks ahead through registers; LDS is double-buffered.
-template <int WM, int WN, int PF, int MODE>
+// Loop nest: row tiles outside, K chunks inside, so the accumulators stay in
+// place across the K loop (no copies at a loop-carried merge).  Chunk s (of
+// the flattened (tile, K chunk) sequence) is multiplied out of LDS stage s%2
+// while the registers holding chunk s+1 go to the other stage (read last in
+// iteration s-1, before its barrier) and the global loads of chunk s+2 are in
+// flight.
+template <int WM, int WN, int WR, int WC, int BK, int MODE>
 __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, int row_begin,
                                           int row_end, int region, unsigned char* smem) {
-  using T = Tile<WM, WN, PF>;
-  constexpr int BM = T::BM, BN = T::BN, LROW = T::LROW;
-  bf16_t* As = (bf16_t*)smem;                   // [2][BM][LROW]
-  bf16_t* Bs = As + 2 * BM * LROW;              // [2][BN][LROW]
-  int* lcnt = (int*)(Bs + 2 * BN * LROW);       // [BN] emitted count per query column
+  using T = Tile<WM, WN, WR, WC, BK>;
+  constexpr int BM = T::BM, BN = T::BN, LROW = T::LROW, SL = T::SLOTS, STAGES = T::STAGES;
+  constexpr int NT = T::NT;
+  bf16_t* As = (bf16_t*)smem;                   // [STAGES][BM][LROW]
+  bf16_t* Bs = As + STAGES * BM * LROW;         // [STAGES][BN][LROW]
+  int* lcnt = (int*)(Bs + STAGES * BN * LROW);  // [BN] emitted count per query column
   uint2* lbuf = (uint2*)(lcnt + BN);            // [BN][LCAP] (GATHER mode)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = wave & 1, wc = wave >> 1;
+  const int wr = wave % WR, wc = wave / WR;
   const int kp = a.kp, nk = kp / BK;
   const int ntiles = (row_end - row_begin + BM - 1) / BM;
   if (ntiles <= 0 || nqi <= 0) return;
-  const int S = ntiles * nk;
 
   const int* qidx = a.qidx;
-  int brow[T::B_CHUNKS];
+  // Staging: thread chunk c = tid + NT p lands at LDS row c/SL, slot c%SL (a
+  // contiguous 16-byte write); it carries logical K chunk slot ^ swz(row).
+  int offA[T::A_CHUNKS], offB[T::B_CHUNKS];
+#pragma unroll
+  for (int p = 0; p < T::A_CHUNKS; ++p) {
+    const int c = tid + NT * p, r = c / SL;
+    offA[p] = r * kp + (((c % SL) ^ swz<SL>(r)) * 8);
+  }
 #pragma unroll
   for (int p = 0; p < T::B_CHUNKS; ++p) {
-    int c = (tid + 256 * p) >> 2;
+    const int c = tid + NT * p, r = c / SL;
+    int brow;
     if (a.b_by_slot)
-      brow[p] = qoff + (c < nqi ? c : 0);
+      brow = qoff + (r < nqi ? r : 0);
     else
-      brow[p] = c < nqi ? (qidx ? qidx[qoff + c] : qoff + c) : (qidx ? qidx[qoff] : qoff);
+      brow = r < nqi ? (qidx ? qidx[qoff + r] : qoff + r) : (qidx ? qidx[qoff] : qoff);
+    offB[p] = brow * kp + (((c % SL) ^ swz<SL>(r)) * 8);
   }
   int gqc[WN], qcl[WN];
   float tau_l[WN], qo[WN];
@@ -104,48 +148,60 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
     qo[bn] = o;
   }
 
-  intx4 ra[PF][T::A_CHUNKS], rb[PF][T::B_CHUNKS];
-  auto gload = [&](int R, int s) {
-    int t = s / nk, kc = s - t * nk;
-    int r0 = row_begin + t * BM;
+  // chunk (t, kc) -> LDS stage st by direct global->LDS loads: wave w's
+  // instruction p fills 1 KiB at chunk slots [NT p + 64w, +64) (lane order)
+  const int wv = __builtin_amdgcn_readfirstlane(wave);
+  auto dma = [&](int t, int kc, int st) {
+    const bf16_t* baseA = a.xb + (int64_t)(row_begin + t * BM) * kp + kc * BK;
+    const bf16_t* baseB = a.qb + kc * BK;
 #pragma unroll
-    for (int p = 0; p < T::A_CHUNKS; ++p) {
-      int c = tid + 256 * p;
-      int r = c >> 2, cc = (c & 3) * 8;
-      ra[R][p] = *(const intx4*)(a.xb + (int64_t)(r0 + r) * kp + kc * BK + cc);
-    }
+    for (int p = 0; p < T::A_CHUNKS; ++p)
+      __builtin_amdgcn_global_load_lds((const void*)(baseA + offA[p]),
+                                       (lds_void*)(As + st * BM * LROW + (NT * p + 64 * wv) * 8),
+                                       16, 0, 0);
 #pragma unroll
-    for (int p = 0; p < T::B_CHUNKS; ++p) {
-      int c = tid + 256 * p;
-      int cc = (c & 3) * 8;
-      rb[R][p] = *(const intx4*)(a.qb + (int64_t)brow[p] * kp + kc * BK + cc);
-    }
+    for (int p = 0; p < T::B_CHUNKS; ++p)
+      __builtin_amdgcn_global_load_lds((const void*)(baseB + offB[p]),
+                                       (lds_void*)(Bs + st * BN * LROW + (NT * p + 64 * wv) * 8),
+                                       16, 0, 0);
   };
-  auto lstore = [&](int R, int buf) {
-#pragma unroll
-    for (int p = 0; p < T::A_CHUNKS; ++p) {
-      int c = tid + 256 * p;
-      int r = c >> 2, cc = (c & 3) * 8;
-      *(intx4*)(As + (buf * BM + r) * LROW + cc) = ra[R][p];
-    }
-#pragma unroll
-    for (int p = 0; p < T::B_CHUNKS; ++p) {
-      int c = tid + 256 * p;
-      int r = c >> 2, cc = (c & 3) * 8;
-      *(intx4*)(Bs + (buf * BN + r) * LROW + cc) = rb[R][p];
-    }
+  // wait until at most `later` chunks issued after the one needed next are in flight
+  auto wait_chunks = [&](int later) {
+    constexpr int N = T::NPC;
+    if (later <= 0) wait_vm_lgkm<0>();
+    else if (later == 1) wait_vm_lgkm<N>();
+    else wait_vm_lgkm<2 * N>();
   };
 
   floatx16 acc[WM][WN];
-#pragma unroll
-  for (int i = 0; i < WM; ++i)
-#pragma unroll
-    for (int j = 0; j < WN; ++j)
-#pragma unroll
-      for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
 
-  // Epilogue of one BM-row tile: per MFMA block, S^ from the accumulators
-  // (not written back), threshold mask, then the rare emissions.
+  auto compute = [&](int buf) {
+    const bf16_t* Ab = As + buf * BM * LROW;
+    const bf16_t* Bb = Bs + buf * BN * LROW;
+#pragma unroll
+    for (int ks = 0; ks < BK / 16; ++ks) {
+      bf16x8 fa[WM], fb[WN];
+      const int kc = ks * 2 + (lane >> 5);  // logical 16-byte chunk of this lane
+#pragma unroll
+      for (int i = 0; i < WM; ++i) {
+        const int r = wr * 32 * WM + i * 32 + (lane & 31);
+        fa[i] = *(const bf16x8*)(Ab + r * LROW + ((kc ^ swz<SL>(r)) * 8));
+      }
+#pragma unroll
+      for (int j = 0; j < WN; ++j) {
+        const int r = wc * 32 * WN + j * 32 + (lane & 31);
+        fb[j] = *(const bf16x8*)(Bb + r * LROW + ((kc ^ swz<SL>(r)) * 8));
+      }
+#pragma unroll
+      for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int j = 0; j < WN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    }
+  };
+
+  // Epilogue of one BM-row tile: per MFMA block, S^ from the accumulators,
+  // threshold mask, then the rare emissions.
   auto epilogue = [&](int t) {
     const int r0 = row_begin + t * BM;
 #pragma unroll
@@ -170,29 +226,26 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
           }
           continue;
         }
+        // keep the scheduler from hoisting the accumulator reads of later
+        // blocks (the register file cannot hold them all at once)
+        __builtin_amdgcn_sched_barrier(0);
+        float v[16];
         unsigned m = 0;
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
-          const float v = __builtin_fmaf(-2.0f, acc[i][j][e], nx[e]) + qo[j];
-          m |= (v < tau_l[j] ? 1u : 0u) << e;
+          v[e] = __builtin_fmaf(-2.0f, acc[i][j][e], nx[e]) + qo[j];
+          m |= (v[e] < tau_l[j] ? 1u : 0u) << e;
         }
         if (m == 0) continue;
         const int gq = gqc[j];
         int pos = atomicAdd(&lcnt[qcl[j]], __builtin_popcount(m));
         unsigned* reg = a.cand + ((int64_t)gq * a.parts + region) * (int64_t)a.cap * 2;
-        while (m) {
-          const int e = __builtin_ctz(m);
-          m &= m - 1;
-          const int row = rb0 + 8 * (e >> 2) + (e & 3);
-          float av = 0.0f, xv = 0.0f;
 #pragma unroll
-          for (int ee = 0; ee < 16; ++ee) {
-            av = ee == e ? acc[i][j][ee] : av;
-            xv = ee == e ? nx[ee] : xv;
-          }
-          const float v = __builtin_fmaf(-2.0f, av, xv) + qo[j];
+        for (int e = 0; e < 16; ++e) {
+          if (!((m >> e) & 1u)) continue;
+          const int row = rb0 + 8 * (e >> 2) + (e & 3);
           uint2 cv;
-          cv.x = __float_as_uint(v);
+          cv.x = __float_as_uint(v[e]);
           cv.y = (unsigned)row;
           if (MODE == SCAN_GATHER) {
             if (pos < T::LCAP) {
@@ -206,66 +259,60 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
           }
           ++pos;
           if (MODE == SCAN_TOP1) {
-            float th = top1_threshold(v, a.t1_qe[gq], a.t1_qr[gq], a.t1_ex, a.t1_rx, a.d);
+            float th = top1_threshold(v[e], a.t1_qe[gq], a.t1_qr[gq], a.t1_ex, a.t1_rx, a.d);
             tau_l[j] = fminf(tau_l[j], th);
           }
         }
       }
     }
+  };
+
+  __syncthreads();  // LDS may still be in use by a previous item of this workgroup
+  for (int c = tid; c < BN; c += NT) lcnt[c] = 0;
+  const int S = ntiles * nk;
+  constexpr int D = STAGES - 1;  // prefetch distance in chunks
+  // issue chunks 0 .. D-1
+  int ti = 0, ki = 0, issued = 0;
+  for (int i = 0; i < D && i < S; ++i) {
+    dma(ti, ki, i);
+    ++issued;
+    if (++ki == nk) {
+      ki = 0;
+      ++ti;
+    }
+  }
+  wait_chunks(issued - 1);
+  __syncthreads();
+  int s = 0, st0 = 0, sti = D;  // chunk s in stage st0; next issue goes to stage sti
+  for (int t = 0; t < ntiles; ++t) {
 #pragma unroll
     for (int i = 0; i < WM; ++i)
 #pragma unroll
       for (int j = 0; j < WN; ++j)
 #pragma unroll
         for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
-  };
-
-  auto compute = [&](int buf) {
-    const bf16_t* Ab = As + buf * BM * LROW;
-    const bf16_t* Bb = Bs + buf * BN * LROW;
-#pragma unroll
-    for (int ks = 0; ks < BK / 16; ++ks) {
-      bf16x8 fa[WM], fb[WN];
-      const int koff = ks * 16 + (lane >> 5) * 8;
-#pragma unroll
-      for (int i = 0; i < WM; ++i)
-        fa[i] = *(const bf16x8*)(Ab + (wr * 32 * WM + i * 32 + (lane & 31)) * LROW + koff);
-#pragma unroll
-      for (int j = 0; j < WN; ++j)
-        fb[j] = *(const bf16x8*)(Bb + (wc * 32 * WN + j * 32 + (lane & 31)) * LROW + koff);
-#pragma unroll
-      for (int i = 0; i < WM; ++i)
-#pragma unroll
-        for (int j = 0; j < WN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    for (int kc = 0; kc < nk; ++kc, ++s) {
+      if (issued < S) {  // chunk s + D into the stage chunk s-1 used
+        dma(ti, ki, sti);
+        ++issued;
+        if (++ki == nk) {
+          ki = 0;
+          ++ti;
+        }
+        sti = sti == STAGES - 1 ? 0 : sti + 1;
+      }
+      compute(st0);
+      // chunk s+1 must have landed; chunks s+2 .. issued-1 may stay in flight
+      wait_chunks(issued - (s + 2));
+      bare_barrier();
+      st0 = st0 == STAGES - 1 ? 0 : st0 + 1;
     }
-  };
-
-  // step s uses register set R = s % PF
-  auto step = [&](int R, int s) {
-    if (s + PF < S) gload(R, s + PF);
-    compute(s & 1);
-    if (s + 1 < S) lstore((R + 1) % PF, (s + 1) & 1);
-    __syncthreads();
-    const int t = s / nk;
-    if (s - t * nk == nk - 1) epilogue(t);
-  };
-
-  __syncthreads();  // LDS may still be in use by a previous item of this workgroup
-  for (int c = tid; c < BN; c += 256) lcnt[c] = 0;
-#pragma unroll
-  for (int r = 0; r < PF; ++r)
-    if (r < S) gload(r, r);
-  lstore(0, 0);
-  __syncthreads();
-  for (int s = 0; s < S; s += PF) {
-    step(0, s);
-    if (PF > 1 && s + 1 < S) step(1 % PF, s + 1);
+    epilogue(t);
   }
 
   if (MODE == SCAN_GATHER) {
     __syncthreads();
-    for (int c = tid; c < nqi; c += 256) {
+    for (int c = tid; c < nqi; c += NT) {
       const int n = min(lcnt[c], T::LCAP);
       if (n == 0) continue;
       const int gq = qidx ? qidx[qoff + c] : qoff + c;
@@ -276,16 +323,18 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
     }
   } else if (MODE != SCAN_DENSE) {
     __syncthreads();
-    for (int c = tid; c < nqi; c += 256) {
+    for (int c = tid; c < nqi; c += NT) {
       int gq = qidx ? qidx[qoff + c] : qoff + c;
       a.cnt[(int64_t)gq * a.parts + region] = lcnt[c];
     }
   }
 }
 
-template <int WM, int WN, int PF, int MODE, int REFINE>
-__global__ __launch_bounds__(256) void scan_grid_kernel(ScanArgs a) {
-  using T = Tile<WM, WN, PF>;
+// Wave tiles of at most 128 x 64 keep two waves per SIMD.
+template <int WM, int WN, int WR, int WC, int BK, int MODE, int REFINE>
+__global__ __launch_bounds__(64 * WR * WC)
+__attribute__((amdgpu_waves_per_eu(WM * WN <= 8 ? 2 : 1))) void scan_grid_kernel(ScanArgs a) {
+  using T = Tile<WM, WN, WR, WC, BK>;
   extern __shared__ __align__(16) unsigned char smem[];
   int nq = REFINE ? *a.nq_dev : a.nq;
   int bid = blockIdx.x;
@@ -296,6 +345,17 @@ __global__ __launch_bounds__(256) void scan_grid_kernel(ScanArgs a) {
   if (a.xcd_map == 2) {  // part-major: an XCD streams one row partition
     part = bid / a.qtiles;
     qtile = bid - part * a.qtiles;
+  } else if (a.xcd_map == 3) {
+    // bands of 8 row partitions: the ~64 workgroups an XCD runs at once cover
+    // 8 partitions x 8 query tiles, so each row chunk and each query chunk
+    // they load is shared by 8 of them through the XCD's L2
+    constexpr int SPB = 8;
+    const int g = bid / (SPB * a.qtiles);
+    const int p0 = g * SPB;
+    const int bp = min(SPB, a.parts - p0);
+    const int idx = bid - g * SPB * a.qtiles;
+    part = p0 + idx % bp;
+    qtile = idx / bp;
   } else {
     qtile = bid / a.parts;
     part = bid - qtile * a.parts;
@@ -305,11 +365,12 @@ __global__ __launch_bounds__(256) void scan_grid_kernel(ScanArgs a) {
   int nqi = min(T::BN, nq - qoff);
   int rb = part * a.rows_per_part;
   int re = min(a.n_rows, rb + a.rows_per_part);
-  scan_item<WM, WN, PF, MODE>(a, qoff, nqi, rb, re, part, smem);
+  scan_item<WM, WN, WR, WC, BK, MODE>(a, qoff, nqi, rb, re, part, smem);
 }
 
-template <int WM, int WN, int PF, int MODE>
-__global__ __launch_bounds__(256) void scan_items_kernel(ScanArgs a) {
+template <int WM, int WN, int WR, int WC, int BK, int MODE>
+__global__ __launch_bounds__(64 * WR * WC)
+__attribute__((amdgpu_waves_per_eu(WM * WN <= 8 ? 2 : 1))) void scan_items_kernel(ScanArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_item;
   const int n_items = *a.n_items_dev;
@@ -320,7 +381,7 @@ __global__ __launch_bounds__(256) void scan_items_kernel(ScanArgs a) {
     __syncthreads();
     if (it >= n_items) break;
     const ScanItem w = a.items[it];
-    scan_item<WM, WN, PF, MODE>(a, w.qoff, w.nq, w.row_begin, w.row_end, w.region, smem);
+    scan_item<WM, WN, WR, WC, BK, MODE>(a, w.qoff, w.nq, w.row_begin, w.row_end, w.region, smem);
   }
 }
 
@@ -330,45 +391,56 @@ void set_lds(K kern, size_t lds) {
                                (int)lds));
 }
 
-template <int WM, int WN, int PF, int MODE>
+template <int WM, int WN, int WR, int WC, int BK, int MODE>
 void launch_t(const ScanArgs& a, int grid, int kind, hipStream_t st) {
-  using T = Tile<WM, WN, PF>;
+  using T = Tile<WM, WN, WR, WC, BK>;
   const size_t lds = MODE == SCAN_GATHER ? T::LDS_GATHER : T::LDS_BYTES;
   static bool attr_set = false;  // dynamic LDS above 64 KiB must be opted into
   if (!attr_set) {
-    set_lds(scan_items_kernel<WM, WN, PF, MODE>, lds);
-    set_lds(scan_grid_kernel<WM, WN, PF, MODE, 0>, lds);
-    set_lds(scan_grid_kernel<WM, WN, PF, MODE, 1>, lds);
+    set_lds(scan_items_kernel<WM, WN, WR, WC, BK, MODE>, lds);
+    set_lds(scan_grid_kernel<WM, WN, WR, WC, BK, MODE, 0>, lds);
+    set_lds(scan_grid_kernel<WM, WN, WR, WC, BK, MODE, 1>, lds);
     attr_set = true;
   }
+  const dim3 block(T::NT);
   if (kind == SCAN_KIND_ITEMS)
-    hipLaunchKernelGGL((scan_items_kernel<WM, WN, PF, MODE>), dim3(grid), dim3(256), lds, st, a);
+    hipLaunchKernelGGL((scan_items_kernel<WM, WN, WR, WC, BK, MODE>), dim3(grid), block, lds, st,
+                       a);
   else if (kind == SCAN_KIND_REFINE)
-    hipLaunchKernelGGL((scan_grid_kernel<WM, WN, PF, MODE, 1>), dim3(grid), dim3(256), lds, st, a);
+    hipLaunchKernelGGL((scan_grid_kernel<WM, WN, WR, WC, BK, MODE, 1>), dim3(grid), block, lds,
+                       st, a);
   else
-    hipLaunchKernelGGL((scan_grid_kernel<WM, WN, PF, MODE, 0>), dim3(grid), dim3(256), lds, st, a);
+    hipLaunchKernelGGL((scan_grid_kernel<WM, WN, WR, WC, BK, MODE, 0>), dim3(grid), block, lds,
+                       st, a);
 }
 
 }  // namespace
 
-// Tile configurations: wave tile 32*WM rows x 32*WN queries, prefetch depth PF.
+// Tile configurations: WR x WC wavefronts of 32*WM rows x 32*WN queries.
 struct ScanCfg {
-  int wm, wn, pf;
+  int wm, wn, wr, wc;
 };
-static const ScanCfg kCfg[] = {{2, 2, 1}, {4, 2, 1}, {2, 4, 1}, {2, 2, 2}, {4, 2, 2}, {4, 4, 1}};
+static const ScanCfg kCfg[] = {{2, 2, 2, 2}, {4, 2, 2, 2}, {2, 4, 2, 2},
+                               {4, 4, 2, 2}, {4, 2, 2, 4}, {2, 4, 4, 2}};
 constexpr int kNumCfg = sizeof(kCfg) / sizeof(kCfg[0]);
 
 int scan_num_cfg() { return kNumCfg; }
-int scan_tile_rows() { return 64 * kCfg[tuning().scan_cfg].wm; }
-int scan_tile_queries() { return 64 * kCfg[tuning().scan_cfg].wn; }
+int scan_tile_rows() {
+  const ScanCfg& c = kCfg[tuning().scan_cfg];
+  return 32 * c.wm * c.wr;
+}
+int scan_tile_queries() {
+  const ScanCfg& c = kCfg[tuning().scan_cfg];
+  return 32 * c.wn * c.wc;
+}
 
-template <int WM, int WN, int PF>
+template <int WM, int WN, int WR, int WC>
 static void launch_cfg(const ScanArgs& a, int mode, int grid, int kind, hipStream_t st) {
   switch (mode) {
-    case SCAN_FILTER: launch_t<WM, WN, PF, SCAN_FILTER>(a, grid, kind, st); break;
-    case SCAN_DENSE: launch_t<WM, WN, PF, SCAN_DENSE>(a, grid, kind, st); break;
-    case SCAN_TOP1: launch_t<WM, WN, PF, SCAN_TOP1>(a, grid, kind, st); break;
-    case SCAN_GATHER: launch_t<WM, WN, PF, SCAN_GATHER>(a, grid, kind, st); break;
+    case SCAN_FILTER: launch_t<WM, WN, WR, WC, 32, SCAN_FILTER>(a, grid, kind, st); break;
+    case SCAN_DENSE: launch_t<WM, WN, WR, WC, 32, SCAN_DENSE>(a, grid, kind, st); break;
+    case SCAN_TOP1: launch_t<WM, WN, WR, WC, 32, SCAN_TOP1>(a, grid, kind, st); break;
+    case SCAN_GATHER: launch_t<WM, WN, WR, WC, 32, SCAN_GATHER>(a, grid, kind, st); break;
     default: break;
   }
 }
@@ -376,12 +448,12 @@ static void launch_cfg(const ScanArgs& a, int mode, int grid, int kind, hipStrea
 void launch_scan(const ScanArgs& a, int mode, int grid, int kind, hipStream_t st) {
   if (grid <= 0) return;
   switch (tuning().scan_cfg) {
-    case 1: launch_cfg<4, 2, 1>(a, mode, grid, kind, st); break;
-    case 2: launch_cfg<2, 4, 1>(a, mode, grid, kind, st); break;
-    case 3: launch_cfg<2, 2, 2>(a, mode, grid, kind, st); break;
-    case 4: launch_cfg<4, 2, 2>(a, mode, grid, kind, st); break;
-    case 5: launch_cfg<4, 4, 1>(a, mode, grid, kind, st); break;
-    default: launch_cfg<2, 2, 1>(a, mode, grid, kind, st); break;
+    case 1: launch_cfg<4, 2, 2, 2>(a, mode, grid, kind, st); break;
+    case 2: launch_cfg<2, 4, 2, 2>(a, mode, grid, kind, st); break;
+    case 3: launch_cfg<4, 4, 2, 2>(a, mode, grid, kind, st); break;
+    case 4: launch_cfg<4, 2, 2, 4>(a, mode, grid, kind, st); break;
+    case 5: launch_cfg<2, 4, 4, 2>(a, mode, grid, kind, st); break;
+    default: launch_cfg<2, 2, 2, 2>(a, mode, grid, kind, st); break;
   }
 }
 

@@ -56,10 +56,12 @@ struct ScanArgs {
 // Run-time tuning knobs (set through mann_tune_set; chosen by the autotuner).
 struct Tuning {
   int scan_cfg = 0;          // scan tile config (index into the table of kernels_scan.hip)
-  int xcd_map = 0;           // grid order: 0 plain, 1 query-major per XCD, 2 part-major per XCD
+  int xcd_map = 0;           // grid order: 0 plain, 1 query-major per XCD, 2 part-major per
+                             // XCD, 3 bands of 8 partitions x query tiles per XCD
   int target_groups = 2048;  // workgroups the row partitioning aims for
   int sample_rank = 1;       // threshold rank = sample_rank * k_req in the row sample
   int shortlist_mult = 2;    // first re-rank batch = shortlist_mult * k_req (>= 32)
+  int tau_rows = 1024;       // IVF: exact rows sampled per query for the emission threshold
 };
 Tuning& tuning();
 
