@@ -462,7 +462,7 @@ int mann_route_distances(const float* centroids, int64_t nlist, int64_t d, const
     MANN_REQUIRE(queries && ldq >= d, ERR_ARGUMENT, "bad query matrix");
     ThreadCtx& c = ctx();
     hipStream_t st = c.stream;
-    const int64_t d16 = round_up(d, 16), kp = round_up(d, 64);
+    const int64_t d16 = round_up(d, 16), kp = round_up(d, 32);
     QueryOps Q = prepare_queries(queries, d, nq, ldq, 0, (int)d16, (int)kp, "rdq");
     float* C = c.buf("rd_c").as<float>((size_t)nlist * d16);
     const float* src = centroids;
@@ -643,9 +643,9 @@ int mann_tune_set(const char* key, int64_t value) {
     MANN_REQUIRE(key != nullptr, ERR_NULL, "null key");
     std::string k(key);
     Tuning& t = tuning();
-    if (k == "scan_cfg") {
-      MANN_REQUIRE(value >= 0 && value < scan_num_cfg(), ERR_ARGUMENT, "scan_cfg out of range");
-      t.scan_cfg = (int)value;
+    if (k == "scan_cfg" || k == "ivf_cfg" || k == "kmeans_cfg") {
+      MANN_REQUIRE(value >= 0 && value < scan_num_cfg(), ERR_ARGUMENT, k + " out of range");
+      (k == "scan_cfg" ? t.scan_cfg : k == "ivf_cfg" ? t.ivf_cfg : t.kmeans_cfg) = (int)value;
     } else if (k == "xcd_map") {
       MANN_REQUIRE(value >= 0 && value <= 3, ERR_ARGUMENT, "xcd_map must be 0..3");
       t.xcd_map = (int)value;
@@ -655,6 +655,10 @@ int mann_tune_set(const char* key, int64_t value) {
     } else if (k == "sample_rank") {
       MANN_REQUIRE(value >= 1 && value <= 64, ERR_ARGUMENT, "sample_rank must be in 1..64");
       t.sample_rank = (int)value;
+    } else if (k == "sample_rows") {
+      MANN_REQUIRE(value >= 256 && value <= 65536, ERR_ARGUMENT,
+                   "sample_rows must be in [256, 65536]");
+      t.sample_rows = (int)value;
     } else if (k == "tau_rows") {
       MANN_REQUIRE(value >= 16 && value <= 1024, ERR_ARGUMENT, "tau_rows must be in [16, 1024]");
       t.tau_rows = (int)value;
@@ -673,11 +677,14 @@ int mann_tune_get(const char* key, int64_t* value) {
     std::string k(key);
     const Tuning& t = tuning();
     if (k == "scan_cfg") *value = t.scan_cfg;
+    else if (k == "ivf_cfg") *value = t.ivf_cfg;
+    else if (k == "kmeans_cfg") *value = t.kmeans_cfg;
     else if (k == "xcd_map") *value = t.xcd_map;
     else if (k == "target_groups") *value = t.target_groups;
     else if (k == "sample_rank") *value = t.sample_rank;
     else if (k == "shortlist_mult") *value = t.shortlist_mult;
     else if (k == "tau_rows") *value = t.tau_rows;
+    else if (k == "sample_rows") *value = t.sample_rows;
     else throw Error{ERR_ARGUMENT, "unknown tuning key " + k};
   });
 }

@@ -98,7 +98,6 @@ mann_dataset::~mann_dataset() {
 
 namespace mann {
 
-static constexpr int SAMPLE_ROWS = 4096;
 
 mann_dataset* create_dataset(const float* data, int64_t d, int64_t n, int64_t ld, int metric) {
   MANN_REQUIRE(d > 0, ERR_ARGUMENT, "Dataset must have at least one dimension");
@@ -112,7 +111,7 @@ mann_dataset* create_dataset(const float* data, int64_t d, int64_t n, int64_t ld
   ds->d = d;
   ds->n = n;
   ds->d16 = round_up(d, 16);
-  ds->kp = round_up(d, 64);
+  ds->kp = round_up(d, 32);  // multiple of the scan K chunk
   ds->n_pad = round_up(n, 256) + 256;
   ds->normalized = metric == COSINE;
   try {
@@ -143,7 +142,7 @@ mann_dataset* create_dataset(const float* data, int64_t d, int64_t n, int64_t ld
                      ds->x32, ds->xb, ds->xn, ds->err, ds->nrm, mx, mx + 1, st, ds->center);
     launch_a16_norms(ds->x32, n, (int)ds->d16, ds->xn2, st);
     // stride sample: rows floor(i * n / m)
-    ds->m = (int)std::min<int64_t>(n, SAMPLE_ROWS);
+    ds->m = (int)std::min<int64_t>(n, tuning().sample_rows);
     ds->m_pad = (int)round_up(ds->m, 256) + 256;
     std::vector<int> sidx(ds->m);
     for (int i = 0; i < ds->m; ++i) sidx[i] = (int)((int64_t)i * n / ds->m);
@@ -244,7 +243,8 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   const int cap_total = select_cap_max();
   MANN_REQUIRE(k_req <= select_exmax() / 2, ERR_UNSUPPORTED,
                "k too large for the GPU selection buffer (max 256)");
-  const int BM = scan_tile_rows(), BN = scan_tile_queries();
+  const int cfg = tuning().scan_cfg;
+  const int BM = scan_tile_rows(cfg), BN = scan_tile_queries(cfg);
   // row partitions: enough workgroups to fill the chip, at most 32 regions
   const int qtiles = (int)ceil_div(nq, BN);
   const int row_tiles = (int)ceil_div(n, BM);
@@ -271,6 +271,7 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   } else {
     float* dense = c.buf("bf_dense").as<float>((size_t)nq * ds.m);
     ScanArgs a{};
+    a.cfg = cfg;
     a.xb = ds.sb;
     a.xn = ds.sxn;
     a.qb = Q.qb;
@@ -294,6 +295,7 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
 
   // ---- filtered first pass over all rows
   ScanArgs a{};
+  a.cfg = cfg;
   a.xb = ds.xb;
   a.xn = ds.xn;
   a.qb = Q.qb;

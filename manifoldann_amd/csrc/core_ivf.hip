@@ -135,7 +135,8 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
   const int cap = select_cap_max();
   MANN_REQUIRE(k <= select_exmax() / 2, ERR_UNSUPPORTED,
                "k too large for the GPU selection buffer (max 256)");
-  const int BN = scan_tile_queries();
+  const int cfg = tuning().ivf_cfg;
+  const int BN = scan_tile_queries(cfg);
   const float* R = route_in;
   if (!R) {
     float* Rb = c.buf("iv_R").as<float>((size_t)nq * iv.nlist);
@@ -217,6 +218,10 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
   ta.rx = iv.rx;
   ta.tau = tau;
   ta.k_eff_q = keff;
+  ta.qlist = qlist;
+  ta.qlist_list = qlist_list;
+  ta.n_entries = ctr + 4;
+  ta.max_entries = max_entries;
   {
     ScopedTime tm("ivf_tau", st);
     launch_ivf_tau(ta, st);
@@ -226,6 +231,7 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
   int* cnt = c.buf("iv_cnt").as<int>(nq);
   fill_i32(cnt, nq, 0, st);
   ScanArgs a{};
+  a.cfg = cfg;
   a.xb = iv.xb;
   a.xn = iv.xn;
   a.qb = qres;

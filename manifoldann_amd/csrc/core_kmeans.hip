@@ -93,6 +93,7 @@ void kmeans_estep(const mann_dataset& ds, const float* C, int k, int* assign, fl
     fill_f32(tau, n, INFINITY, st);
   }
   ScanArgs a{};
+  a.cfg = tuning().kmeans_cfg;
   a.xb = P.cb;
   a.xn = P.cn;
   a.qb = ds.xb;
@@ -100,7 +101,7 @@ void kmeans_estep(const mann_dataset& ds, const float* C, int k, int* assign, fl
   a.d = (int)ds.d;
   a.nq = (int)n;
   a.parts = 1;
-  a.rows_per_part = (int)round_up(k, scan_tile_rows());
+  a.rows_per_part = (int)round_up(k, scan_tile_rows(a.cfg));
   a.n_rows = k;
   a.tau = tau;
   a.cand = cand;
@@ -113,7 +114,7 @@ void kmeans_estep(const mann_dataset& ds, const float* C, int k, int* assign, fl
   a.t1_rx = P.rx;
   {
     ScopedTime tm("kmeans_assign_scan", st);
-    launch_scan(a, SCAN_TOP1, (int)ceil_div(n, scan_tile_queries()), SCAN_KIND_GRID, st);
+    launch_scan(a, SCAN_TOP1, (int)ceil_div(n, scan_tile_queries(a.cfg)), SCAN_KIND_GRID, st);
   }
   {
     ScopedTime tm("kmeans_assign_select", st);

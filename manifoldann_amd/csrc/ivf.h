@@ -22,6 +22,13 @@ struct IvfTauArgs {
   float ex, rx;             // dataset bounds
   float* tau;               // out: emission threshold
   int* k_eff_q;             // out: min(k, rows probed)
+  // block order: one block per (list, query) entry of the inverted probe
+  // lists; the block of a query's first probed list does the work, so
+  // consecutive blocks share their sample rows in L2
+  const int* qlist;
+  const int* qlist_list;
+  const int* n_entries;
+  int64_t max_entries;
 };
 
 int route_max();

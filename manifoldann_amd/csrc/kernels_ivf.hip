@@ -4,6 +4,7 @@
 // the certified emission threshold from an exact row sample.
 #include "bounds.h"
 #include "ivf.h"
+#include "util.h"
 
 namespace mann {
 
@@ -379,8 +380,10 @@ __global__ __launch_bounds__(256) void ivf_tau_kernel(IvfTauArgs a) {
   __shared__ int s_nl, s_m;
   __shared__ unsigned hist[256];
   __shared__ unsigned s_sel, s_want;
-  const int q = blockIdx.x;
-  if (q >= a.nq) return;
+  const int64_t ent = blockIdx.x;
+  if (ent >= *a.n_entries) return;
+  const int q = a.qlist[ent];
+  if (a.probes[(int64_t)q * a.nprobe] != a.qlist_list[ent]) return;
   const int tid = threadIdx.x, g = tid >> 4, j = tid & 15;
   const int np = a.pcount[q];
   const int total = a.prows[q];
@@ -575,8 +578,11 @@ void launch_refine_items(const int* flist, const int* nflag, int nflag_host, int
 int ivf_tau_rows_max() { return TAU_M; }
 
 void launch_ivf_tau(const IvfTauArgs& a, hipStream_t st) {
-  if (a.nq <= 0) return;
-  hipLaunchKernelGGL(ivf_tau_kernel, dim3(a.nq), dim3(256), 0, st, a);
+  if (a.nq <= 0 || a.max_entries <= 0) return;
+  // queries without an entry (nothing probed) keep these defaults
+  fill_f32(a.tau, a.nq, INFINITY, st);
+  fill_i32(a.k_eff_q, a.nq, 0, st);
+  hipLaunchKernelGGL(ivf_tau_kernel, dim3((unsigned)a.max_entries), dim3(256), 0, st, a);
 }
 
 void launch_probe_ranges(const int* probes, const int* pcount, int nq, int nprobe,

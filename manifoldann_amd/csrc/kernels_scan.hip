@@ -425,12 +425,12 @@ static const ScanCfg kCfg[] = {{2, 2, 2, 2}, {4, 2, 2, 2}, {2, 4, 2, 2},
 constexpr int kNumCfg = sizeof(kCfg) / sizeof(kCfg[0]);
 
 int scan_num_cfg() { return kNumCfg; }
-int scan_tile_rows() {
-  const ScanCfg& c = kCfg[tuning().scan_cfg];
+int scan_tile_rows(int cfg) {
+  const ScanCfg& c = kCfg[cfg];
   return 32 * c.wm * c.wr;
 }
-int scan_tile_queries() {
-  const ScanCfg& c = kCfg[tuning().scan_cfg];
+int scan_tile_queries(int cfg) {
+  const ScanCfg& c = kCfg[cfg];
   return 32 * c.wn * c.wc;
 }
 
@@ -447,7 +447,7 @@ static void launch_cfg(const ScanArgs& a, int mode, int grid, int kind, hipStrea
 
 void launch_scan(const ScanArgs& a, int mode, int grid, int kind, hipStream_t st) {
   if (grid <= 0) return;
-  switch (tuning().scan_cfg) {
+  switch (a.cfg) {
     case 1: launch_cfg<4, 2, 2, 2>(a, mode, grid, kind, st); break;
     case 2: launch_cfg<2, 4, 2, 2>(a, mode, grid, kind, st); break;
     case 3: launch_cfg<4, 4, 2, 2>(a, mode, grid, kind, st); break;

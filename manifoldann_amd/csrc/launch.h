@@ -17,6 +17,7 @@ struct ScanItem {
 };
 
 struct ScanArgs {
+  int cfg;            // tile configuration (index into the table of kernels_scan.hip)
   const bf16_t* xb;   // base rows, kp bf16 each (padded rows: zeros)
   const float* qnorm; // |q~|^2 per query (added to every value of its column)
   const float* col_norm;  // or per item column: [qoff + c] (IVF residual queries)
@@ -24,7 +25,7 @@ struct ScanArgs {
   int qtiles;         // grid mode: number of query tiles
   const float* xn;    // |x~|^2 per base row, +inf on padding
   const bf16_t* qb;   // query rows, kp bf16 each
-  int kp;             // padded K (multiple of 64)
+  int kp;             // padded K (multiple of 32)
   int d;              // true dimension
   const int* qidx;    // query id list (nullptr: identity)
   int b_by_slot;      // qb row of column c is its slot qoff + c (IVF residual queries), not
@@ -55,19 +56,22 @@ struct ScanArgs {
 
 // Run-time tuning knobs (set through mann_tune_set; chosen by the autotuner).
 struct Tuning {
-  int scan_cfg = 0;          // scan tile config (index into the table of kernels_scan.hip)
-  int xcd_map = 0;           // grid order: 0 plain, 1 query-major per XCD, 2 part-major per
+  int scan_cfg = 5;          // exact-kNN scan tile config (table of kernels_scan.hip)
+  int ivf_cfg = 0;           // IVF list-scan tile config
+  int kmeans_cfg = 0;        // k-means assignment tile config
+  int xcd_map = 3;           // grid order: 0 plain, 1 query-major per XCD, 2 part-major per
                              // XCD, 3 bands of 8 partitions x query tiles per XCD
   int target_groups = 2048;  // workgroups the row partitioning aims for
   int sample_rank = 1;       // threshold rank = sample_rank * k_req in the row sample
-  int shortlist_mult = 2;    // first re-rank batch = shortlist_mult * k_req (>= 32)
+  int shortlist_mult = 1;    // first re-rank batch = shortlist_mult * k_req (>= 32)
   int tau_rows = 1024;       // IVF: exact rows sampled per query for the emission threshold
+  int sample_rows = 4096;    // exact kNN: stride-sample rows (fixed when a dataset is created)
 };
 Tuning& tuning();
 
 int scan_num_cfg();
-int scan_tile_rows();
-int scan_tile_queries();
+int scan_tile_rows(int cfg);
+int scan_tile_queries(int cfg);
 void launch_scan(const ScanArgs& a, int mode, int grid, int kind, hipStream_t st);
 
 void launch_prep_rows(const float* src, int64_t ld, int64_t n, int64_t n_pad, int d, int d16,
