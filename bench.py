@@ -40,6 +40,23 @@ def host_cores():
         return os.cpu_count() or 1
 
 
+PMC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                           "scan_filter_knn_graph_60000x784_pmc.txt")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the FILTER scan from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), or None."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def kernel_avg_ms(L, check, names):
     out = {}
     for name in names:
@@ -84,16 +101,30 @@ def bench_knn_graph(args, M, datasets, L, check, np, torch):
     check(L.mann_profile_enable(0))
     scan_ms = kernels.get("scan_filter", float("nan"))
     achieved = flop / (scan_ms * 1e-3) / 1e12
+    traffic = measured_traffic()
     roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": MFMA_BF16_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": ("rocprofv3 FETCH_SIZE+WRITE_SIZE, " + os.path.relpath(
+                PMC_SUMMARY, os.path.dirname(os.path.abspath(__file__)))) if traffic else None,
             "kernel": "scan_grid_kernel FILTER (bf16 MFMA distances + threshold emit)",
             "algorithmic_flop_per_launch": flop, "avg_launch_ms": scan_ms}
+    if args.copy_calibration:
+        # a plain device copy of known size: the reference reading for the
+        # FETCH_SIZE / WRITE_SIZE counters of the --pmc runs
+        nb = 1 << 30
+        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {nb} bytes read and written")
+        del src, dst
     cpu = None
     if not args.skip_cpu:
         from oracle import oracle as O
         cores = host_cores()
-        nsq = 256
+        nsq = args.cpu_queries
         rows = np.random.default_rng(1).choice(n, nsq, replace=False)
         O.bf_knn(X, X[rows[:8]], k + 1, nthreads=cores)
         t1 = time.perf_counter()
@@ -212,12 +243,12 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
     if not args.skip_cpu and bc:
         from oracle import oracle as O
         cores = host_cores()
-        m = 200
+        m = min(args.cpu_ivf_queries, nq)
         off = np.concatenate([[0], np.cumsum(index.list_sizes)])
         lids = np.zeros(n, np.int64)
         check(L.mann_ivf_list_ids(index.handle, lids.ctypes.data))
         Cc = np.ascontiguousarray(t.centroids.T)
-        R = O.route_distances(Q[:m], Cc, nthreads=cores)
+        O.route_distances(Q[:8], Cc, nthreads=cores)
         t2 = time.perf_counter()
         R = O.route_distances(Q[:m], Cc, nthreads=cores)
         O.ivf_search(X, off, lids, index.n_buckets, Q[:m], R, 10, bc["nprobe"], nthreads=cores)
@@ -254,6 +285,11 @@ def main():
     ap.add_argument("--train-q", type=int, default=200_000)
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--cpu-queries", type=int, default=16384,
+                    help="self-join queries timed on the CPU oracle (bounded sample)")
+    ap.add_argument("--cpu-ivf-queries", type=int, default=4000)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 1 GiB device copy (counter calibration under rocprofv3)")
     ap.add_argument("--no-prior", action="store_true",
                     help="train the router without the centroid prior")
     ap.add_argument("--reps", type=int, default=5)
