@@ -6,6 +6,10 @@ TFLOP/s = 2 n^2 d / t.  One step is one full build_knn_graph over data already
 resident in HBM (the device dataset is prepared once, outside the timed
 region, like the reference harness's prepare_*).
 
+Also reported (key "large_cosine"): exact cosine kNN over a 10,000,000 x 256
+Float16 base (unit 5,000-component mixture), 100,000 queries, k=100, with a
+sample of queries checked against the CPU oracle.
+
 Also reported (key "ivf"): the IVF workload -- 1,000,000 x 256 Zipf mixture,
 nlist 1,024, 20 Lloyd iterations, 10,000 queries, k=10 -- as QPS at
 recall@10 >= 0.95 for centroid TopKRouting and for the learned probe router
@@ -268,6 +272,96 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
     return out
 
 
+def bench_large_cosine(args, M, L, check, np, torch):
+    """Large cosine search: n unit-normalised rows of a 5,000-component mixture
+    (noise 0.1), stored as Float16, queries from the same mixture, k=100."""
+    from oracle import oracle as O
+    n, d, nq, k = args.large_n, 256, args.large_nq, 100
+    t_all = time.perf_counter()
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(55)
+    C = torch.randn(5000, d, generator=gen, device="cuda")
+    C /= C.norm(dim=1, keepdim=True)
+
+    def draw(count, seed):
+        g = torch.Generator(device="cuda")
+        g.manual_seed(seed)
+        out = torch.empty((count, d), dtype=torch.float16, device="cuda")
+        lab = torch.randint(0, 5000, (count,), generator=g, device="cuda")
+        step = 1 << 20
+        for s in range(0, count, step):
+            e = min(count, s + step)
+            v = C[lab[s:e]] + 0.1 * torch.randn((e - s, d), generator=g, device="cuda")
+            out[s:e] = (v / v.norm(dim=1, keepdim=True)).to(torch.float16)
+        return out
+
+    log(f"large: generating {n}x{d} fp16 unit mixture on the GPU")
+    X16 = draw(n, 5)
+    Q32 = draw(nq, 6).to(torch.float32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ds = M.Dataset(X16.t(), M.cosine_distance)
+    torch.cuda.synchronize()
+    prep_s = time.perf_counter() - t0
+    log(f"large: dataset prepared in {prep_s:.2f} s")
+    ids = torch.zeros((nq, k), dtype=torch.int64, device="cuda")
+    dist = torch.zeros((nq, k), dtype=torch.float32, device="cuda")
+
+    def run():
+        check(L.mann_bf_knn(ds.handle, Q32.data_ptr(), d, nq, d, k, M.COSINE, ids.data_ptr(),
+                            dist.data_ptr()))
+
+    t1 = time.perf_counter()
+    run()
+    torch.cuda.synchronize()
+    log(f"large: first search {time.perf_counter() - t1:.2f} s, stats {M.last_stats()}")
+    ts = []
+    for _ in range(args.large_reps):
+        t1 = time.perf_counter()
+        run()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t1)
+        log(f"large: search {ts[-1]:.3f} s")
+    t = sorted(ts)[len(ts) // 2]
+    stats = M.last_stats()
+    check(L.mann_profile_enable(1))
+    check(L.mann_profile_reset())
+    run()
+    kern = kernel_avg_ms(L, check, ["prep_queries", "scan_filter", "scan_dense", "tau_sample",
+                                    "select_rerank", "scan_filter_pass2", "select_rerank_pass2",
+                                    "exact_fallback"])
+    check(L.mann_profile_enable(0))
+    flop = 2.0 * nq * n * d
+    scan_ms = kern.get("scan_filter", float("nan"))
+    # exactness on a sample of queries against the FP32 oracle on the upcast data
+    m = args.large_check
+    rows = np.random.default_rng(3).choice(nq, m, replace=False)
+    Xh = X16.cpu().numpy().astype(np.float32)
+    Qh = Q32[torch.from_numpy(rows).cuda()].cpu().numpy()
+    log(f"large: checking {m} queries on the CPU oracle")
+    oi, od = O.bf_knn(Xh, Qh, k, metric=O.COSINE, nthreads=host_cores())
+    gi = ids.cpu().numpy()[rows]
+    gd = dist.cpu().numpy()[rows]
+    out = {"workload": f"cosine exact kNN: base {n}x{d} Float16 (unit 5,000-component mixture, "
+                       f"noise 0.1), {nq} queries, k={k}",
+           "first_pass": "bf16 MFMA (FP8 e4m3 first pass not implemented)",
+           "ms": round(1e3 * t, 2), "qps": round(nq / t, 1),
+           "tflops": round(flop / t / 1e12, 2),
+           "scan_filter_tflops": round(flop / (scan_ms * 1e-3) / 1e12, 2),
+           "scan_filter_frac_bf16_peak": round(flop / (scan_ms * 1e-3) / 1e12 /
+                                               MFMA_BF16_PEAK_TFLOPS, 4),
+           "dataset_prep_s": round(prep_s, 3), "kernels_ms": kern,
+           "search_stats": {"reranked": stats[0], "overflow": stats[1], "second_pass": stats[2],
+                            "exact_fallback": stats[3]},
+           "oracle_check": {"queries": m, "ids_equal": bool((gi == oi).all()),
+                            "dist_bit_equal": bool((gd.view(np.uint32) ==
+                                                    od.view(np.uint32)).all())},
+           "total_s": round(time.perf_counter() - t_all, 1)}
+    del ds, X16, Q32, ids, dist
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -290,6 +384,13 @@ def main():
     ap.add_argument("--cpu-ivf-queries", type=int, default=4000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration under rocprofv3)")
+    ap.add_argument("--skip-large", action="store_true",
+                    help="skip the 10M x 256 Float16 cosine search (k=100)")
+    ap.add_argument("--large-n", type=int, default=10_000_000)
+    ap.add_argument("--large-nq", type=int, default=100_000)
+    ap.add_argument("--large-reps", type=int, default=3)
+    ap.add_argument("--large-check", type=int, default=16,
+                    help="queries checked against the CPU oracle")
     ap.add_argument("--no-prior", action="store_true",
                     help="train the router without the centroid prior")
     ap.add_argument("--reps", type=int, default=5)
@@ -312,6 +413,7 @@ def main():
 
     res, roof, cpu = bench_knn_graph(args, M, datasets, L, check, np, torch)
     ivf = None if args.skip_ivf else bench_ivf(args, M, datasets, L, check, np, torch)
+    large = None if args.skip_large else bench_large_cosine(args, M, L, check, np, torch)
     out = {
         "metric": METRIC,
         "value": round(res["value"], 3),
@@ -332,6 +434,7 @@ def main():
         "kernels_ms": res["kernels_ms"],
         "search_stats": res["stats"],
         "ivf": ivf,
+        "large_cosine": large,
     }
     print(json.dumps(out))
 

@@ -55,6 +55,13 @@ int mann_device_count(void);
  * first-pass representation (COSINE keeps unit-normalised bf16 rows). */
 int mann_dataset_create(const float* data, int64_t d, int64_t n, int64_t ld, int metric,
                         mann_dataset_t* out);
+/* The same from IEEE binary16 data (a Matrix{Float16} base, SURVEY config 5):
+ * the rows are upcast exactly to Float32 on the device.  The reference would
+ * compute in Float16 (src/indices/bruteforce.jl:53, S = float(T)); this
+ * library computes in Float32 on the upcast values (the documented oracle of
+ * that configuration). */
+int mann_dataset_create_f16(const void* data, int64_t d, int64_t n, int64_t ld, int metric,
+                            mann_dataset_t* out);
 int mann_dataset_destroy(mann_dataset_t ds);
 int mann_dataset_info(mann_dataset_t ds, int64_t* d, int64_t* n);
 

@@ -23,6 +23,8 @@ SIGNATURES = [
     ("mann_last_error", ctypes.c_char_p, []),
     ("mann_device_count", ctypes.c_int, []),
     ("mann_dataset_create", ctypes.c_int, [_vp, _i64, _i64, _i64, ctypes.c_int, ctypes.POINTER(_vp)]),
+    ("mann_dataset_create_f16", ctypes.c_int, [_vp, _i64, _i64, _i64, ctypes.c_int,
+                                                ctypes.POINTER(_vp)]),
     ("mann_dataset_destroy", ctypes.c_int, [_vp]),
     ("mann_dataset_info", ctypes.c_int, [_vp, _i64p, _i64p]),
     ("mann_bf_knn", ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, ctypes.c_int, _vp, _vp]),

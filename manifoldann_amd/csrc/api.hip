@@ -89,6 +89,36 @@ int mann_dataset_create(const float* data, int64_t d, int64_t n, int64_t ld, int
   });
 }
 
+int mann_dataset_create_f16(const void* data, int64_t d, int64_t n, int64_t ld, int metric,
+                            mann_dataset_t* out) {
+  return guarded([&] {
+    MANN_REQUIRE(out != nullptr, ERR_NULL, "null output handle");
+    MANN_REQUIRE(data != nullptr, ERR_NULL, "null data pointer");
+    MANN_REQUIRE(d > 0 && n > 0, ERR_ARGUMENT, "Dataset must contain at least one point");
+    MANN_REQUIRE(ld >= d, ERR_ARGUMENT, "leading dimension smaller than d");
+    check_metric(metric);
+    ThreadCtx& c = ctx();
+    hipStream_t st = c.stream;
+    const size_t elems = (size_t)(ld * (n - 1) + d);
+    struct Tmp {  // freed on every exit (hipFree waits for the device)
+      void* p = nullptr;
+      ~Tmp() {
+        if (p) (void)hipFree(p);
+      }
+    } staged16, up;
+    const void* src = data;
+    if (!is_device_ptr(data)) {
+      MANN_HIP(hipMalloc(&staged16.p, elems * 2));
+      MANN_HIP(hipMemcpyAsync(staged16.p, data, elems * 2, hipMemcpyHostToDevice, st));
+      src = staged16.p;
+    }
+    // exact upcast to FP32 rows of stride d, then the FP32 preparation
+    MANN_HIP(hipMalloc(&up.p, sizeof(float) * (size_t)n * d));
+    f16_rows_to_f32(src, ld, (float*)up.p, d, (int)d, n, st);
+    *out = create_dataset((const float*)up.p, d, n, d, metric);
+  });
+}
+
 int mann_dataset_destroy(mann_dataset_t ds) {
   return guarded([&] { delete ds; });
 }

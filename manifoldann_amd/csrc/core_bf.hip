@@ -142,7 +142,9 @@ mann_dataset* create_dataset(const float* data, int64_t d, int64_t n, int64_t ld
                      ds->x32, ds->xb, ds->xn, ds->err, ds->nrm, mx, mx + 1, st, ds->center);
     launch_a16_norms(ds->x32, n, (int)ds->d16, ds->xn2, st);
     // stride sample: rows floor(i * n / m)
-    ds->m = (int)std::min<int64_t>(n, tuning().sample_rows);
+    // at least n / 40 rows, so that the sample's order statistics stay close to
+    // the k-th neighbour distance for large n
+    ds->m = (int)std::min<int64_t>(n, std::max<int64_t>(tuning().sample_rows, n / 40));
     ds->m_pad = (int)round_up(ds->m, 256) + 256;
     std::vector<int> sidx(ds->m);
     for (int i = 0; i < ds->m; ++i) sidx[i] = (int)((int64_t)i * n / ds->m);
@@ -255,9 +257,14 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   int rpp = (int)round_up(ceil_div(n, parts), BM);
   parts = (int)ceil_div(n, rpp);
   const int cap = cap_total / parts;
+  // the refinement pass only sees the flagged queries (few query tiles): more,
+  // smaller row partitions keep the chip busy (up to 64 regions per query)
+  int rpp2 = (int)round_up(ceil_div(n, std::min(row_tiles, 64)), BM);
+  const int parts2 = (int)ceil_div(n, rpp2);
+  const int cap2 = cap_total / parts2;
 
-  unsigned* cand = c.buf("bf_cand").as<unsigned>((size_t)nq * parts * cap * 2);
-  int* cnt = c.buf("bf_cnt").as<int>((size_t)nq * parts);
+  unsigned* cand = c.buf("bf_cand").as<unsigned>((size_t)nq * cap_total * 2);
+  int* cnt = c.buf("bf_cnt").as<int>((size_t)nq * std::max(parts, parts2));
   float* tau = c.buf("bf_tau").as<float>(nq);
   float* tau2 = c.buf("bf_tau2").as<float>(nq);
   int* flags = c.buf("bf_flags").as<int>(8);   // [0] pass1 count, [1] pass2 count, [2..] stats
@@ -269,28 +276,47 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   if (rpp <= cap) {
     fill_f32(tau, nq, INFINITY, st);
   } else {
-    float* dense = c.buf("bf_dense").as<float>((size_t)nq * ds.m);
-    ScanArgs a{};
-    a.cfg = cfg;
-    a.xb = ds.sb;
-    a.xn = ds.sxn;
-    a.qb = Q.qb;
-    a.qnorm = Q.qn;
-    a.kp = (int)ds.kp;
-    a.d = (int)ds.d;
-    a.nq = nq;
-    a.parts = 1;
-    a.rows_per_part = (int)round_up(ds.m, BM);
-    a.n_rows = ds.m;
-    a.dense = dense;
-    a.ldd = ds.m;
-    {
-      ScopedTime tm("scan_dense", st);
-      launch_scan(a, SCAN_DENSE, (int)ceil_div(nq, BN), SCAN_KIND_GRID, st);
+    // threshold rank in the sample: the sample's r-th smallest value sits near
+    // the (r n / m)-th smallest row; aim at about 15 k_req emissions per query,
+    // bounded by half the candidate capacity
+    const int64_t target = std::min<int64_t>((int64_t)tuning().sample_rank * 15 * k_req,
+                                             cap_total / 2);
+    const int rank = (int)std::max<int64_t>(
+        1, std::min<int64_t>(ds.m, (target * ds.m + n / 2) / n));
+    // the dense sample block is processed in query chunks of at most 2^28 values
+    const int64_t qc_max = std::max<int64_t>(BN, ((int64_t)1 << 28) / ds.m / BN * BN);
+    const int qc = (int)std::min<int64_t>(nq, qc_max);
+    float* dense = c.buf("bf_dense").as<float>((size_t)qc * ds.m);
+    for (int q0 = 0; q0 < nq; q0 += qc) {
+      const int nqc = std::min(qc, nq - q0);
+      ScanArgs a{};
+      a.cfg = cfg;
+      a.xb = ds.sb;
+      a.xn = ds.sxn;
+      a.qb = Q.qb + (int64_t)q0 * ds.kp;
+      a.qnorm = Q.qn + q0;
+      a.kp = (int)ds.kp;
+      a.d = (int)ds.d;
+      a.nq = nqc;
+      // row partitions only when the query chunk has too few tiles to fill the
+      // chip (a large sample; each partition re-reads the chunk's queries)
+      const int qt = (int)ceil_div(nqc, BN);
+      const int pd = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(ds.m, BM),
+                                                                  ceil_div(256, qt)));
+      a.rows_per_part = (int)round_up(ceil_div(ds.m, pd), BM);
+      a.parts = (int)ceil_div(ds.m, a.rows_per_part);
+      a.qtiles = qt;
+      a.n_rows = ds.m;
+      a.dense = dense;
+      a.ldd = ds.m;
+      {
+        ScopedTime tm("scan_dense", st);
+        launch_scan(a, SCAN_DENSE, qt * a.parts, SCAN_KIND_GRID, st);
+      }
+      TauArgs ta{dense, (int64_t)ds.m, ds.m, rank, tau + q0, nqc};
+      ScopedTime tm("tau_sample", st);
+      launch_tau_from_sample(ta, st);
     }
-    TauArgs ta{dense, (int64_t)ds.m, ds.m, std::min(ds.m, tuning().sample_rank * k_req), tau, nq};
-    ScopedTime tm("tau_sample", st);
-    launch_tau_from_sample(ta, st);
   }
 
   // ---- filtered first pass over all rows
@@ -356,10 +382,15 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   a.qidx = list1;
   a.nq_dev = flags;
   a.tau = tau2;
+  a.parts = parts2;
+  a.rows_per_part = rpp2;
+  a.cap = cap2;
   {
     ScopedTime tm("scan_filter_pass2", st);
-    launch_scan(a, SCAN_FILTER, qtiles * parts, SCAN_KIND_REFINE, st);
+    launch_scan(a, SCAN_FILTER, qtiles * parts2, SCAN_KIND_REFINE, st);
   }
+  s.parts = parts2;
+  s.cap = cap2;
   s.qlist = list1;
   s.nq_dev = flags;
   s.tau = tau2;

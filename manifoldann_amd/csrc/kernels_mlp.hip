@@ -25,14 +25,33 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
   floatx16 acc;
   for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
   for (int k0 = 0; k0 < g.K; k0 += GK) {
-    // stage: 64 x 32 elements per operand, 8 per thread
+    // stage: 64 x 32 elements per operand, 8 per thread; consecutive threads
+    // walk the operand's unit-stride axis so the loads coalesce
     for (int i = tid; i < GT * GK; i += 256) {
-      const int r = i / GK, kk = i - r * GK;
-      const int m = m0 + r, n = n0 + r, k = k0 + kk;
-      float av = (m < g.M && k < g.K) ? g.A[(int64_t)m * g.sam + (int64_t)k * g.sak] : 0.0f;
-      float bv = (n < g.N && k < g.K) ? g.B[(int64_t)n * g.sbn + (int64_t)k * g.sbk] : 0.0f;
-      As[r][kk] = (bf16_t)av;
-      Bs[r][kk] = (bf16_t)bv;
+      int r, kk;
+      if (g.sak == 1) {
+        r = i / GK;
+        kk = i - r * GK;
+      } else {
+        kk = i / GT;
+        r = i - kk * GT;
+      }
+      const int m = m0 + r, k = k0 + kk;
+      As[r][kk] = (bf16_t)((m < g.M && k < g.K) ? g.A[(int64_t)m * g.sam + (int64_t)k * g.sak]
+                                                : 0.0f);
+    }
+    for (int i = tid; i < GT * GK; i += 256) {
+      int r, kk;
+      if (g.sbk == 1) {
+        r = i / GK;
+        kk = i - r * GK;
+      } else {
+        kk = i / GT;
+        r = i - kk * GT;
+      }
+      const int n = n0 + r, k = k0 + kk;
+      Bs[r][kk] = (bf16_t)((n < g.N && k < g.K) ? g.B[(int64_t)n * g.sbn + (int64_t)k * g.sbk]
+                                                : 0.0f);
     }
     __syncthreads();
 #pragma unroll
@@ -111,13 +130,23 @@ __global__ __launch_bounds__(256) void softmax_xent_kernel(float* __restrict__ Z
   if (threadIdx.x == 0) atomicAdd(loss, red[0] / (float)batch);
 }
 
-// out[n] = sum_m G[m][n]
-__global__ void colsum_kernel(const float* __restrict__ G, int M, int N, float* __restrict__ out) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+// out[n] = sum_m G[m][n]: a block owns 32 columns, 8 row groups sum rows
+// m = g (mod 8) with coalesced loads, then the 8 partials add in fixed order
+__global__ __launch_bounds__(256) void colsum_kernel(const float* __restrict__ G, int M, int N,
+                                                     float* __restrict__ out) {
+  __shared__ float part[8][32];
+  const int c = threadIdx.x & 31, gr = threadIdx.x >> 5;
+  const int n = blockIdx.x * 32 + c;
   float s = 0.0f;
-  for (int m = 0; m < M; ++m) s += G[(int64_t)m * N + n];
-  out[n] = s;
+  if (n < N)
+    for (int m = gr; m < M; m += 8) s += G[(int64_t)m * N + n];
+  part[gr][c] = s;
+  __syncthreads();
+  if (gr == 0 && n < N) {
+    float t = part[0][c];
+    for (int i = 1; i < 8; ++i) t += part[i][c];
+    out[n] = t;
+  }
 }
 
 // dH *= (H > 0)
@@ -160,7 +189,8 @@ void launch_softmax_xent(float* Z, int ncls, int batch, const int* labels, int L
 }
 
 void launch_colsum(const float* G, int M, int N, float* out, hipStream_t st) {
-  hipLaunchKernelGGL(colsum_kernel, dim3((unsigned)ceil_div(N, 256)), dim3(256), 0, st, G, M, N,
+  if (N <= 0) return;
+  hipLaunchKernelGGL(colsum_kernel, dim3((unsigned)ceil_div(N, 32)), dim3(256), 0, st, G, M, N,
                      out);
 }
 

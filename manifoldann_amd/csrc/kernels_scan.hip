@@ -221,7 +221,7 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
             for (int e = 0; e < 16; ++e) {
               const int row = rb0 + 8 * (e >> 2) + (e & 3);
               const float v = __builtin_fmaf(-2.0f, acc[i][j][e], nx[e]) + qo[j];
-              if (row < row_end) a.dense[(int64_t)gqc[j] * a.ldd + (row - row_begin)] = v;
+              if (row < row_end) a.dense[(int64_t)gqc[j] * a.ldd + row] = v;
             }
           }
           continue;
@@ -229,14 +229,23 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
         // keep the scheduler from hoisting the accumulator reads of later
         // blocks (the register file cannot hold them all at once)
         __builtin_amdgcn_sched_barrier(0);
+        // fast test: one FMA per value and a min tree; fl(min_e a_e + qo) =
+        // min_e fl(a_e + qo) because rounding is monotone, so "no value of
+        // this block passes" is decided exactly
+        float w[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) w[e] = __builtin_fmaf(-2.0f, acc[i][j][e], nx[e]);
+        float mn = w[0];
+#pragma unroll
+        for (int e = 1; e < 16; ++e) mn = fminf(mn, w[e]);
+        if (!(mn + qo[j] < tau_l[j])) continue;
         float v[16];
         unsigned m = 0;
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
-          v[e] = __builtin_fmaf(-2.0f, acc[i][j][e], nx[e]) + qo[j];
+          v[e] = w[e] + qo[j];
           m |= (v[e] < tau_l[j] ? 1u : 0u) << e;
         }
-        if (m == 0) continue;
         const int gq = gqc[j];
         int pos = atomicAdd(&lcnt[qcl[j]], __builtin_popcount(m));
         unsigned* reg = a.cand + ((int64_t)gq * a.parts + region) * (int64_t)a.cap * 2;

@@ -60,6 +60,21 @@ __global__ void copy_rows_kernel(const float* src, int64_t lds, float* dst, int6
   for (int c = threadIdx.x; c < ldd; c += blockDim.x) dst[r * ldd + c] = c < w ? src[r * lds + c] : 0.0f;
 }
 
+// dst[r][c] = float(src[r][c]) for c < w (exact: every binary16 is a binary32)
+__global__ void f16_rows_to_f32_kernel(const _Float16* src, int64_t lds, float* dst, int64_t ldd,
+                                       int w, int64_t rows) {
+  const int64_t r = blockIdx.x;
+  if (r >= rows) return;
+  for (int c = threadIdx.x; c < w; c += blockDim.x) dst[r * ldd + c] = (float)src[r * lds + c];
+}
+
+void f16_rows_to_f32(const void* src, int64_t lds, float* dst, int64_t ldd, int w, int64_t rows,
+                     hipStream_t st) {
+  if (rows <= 0) return;
+  hipLaunchKernelGGL(f16_rows_to_f32_kernel, dim3((unsigned)rows), dim3(128), 0, st,
+                     (const _Float16*)src, lds, dst, ldd, w, rows);
+}
+
 void ids_to_i64(const int* src, int64_t n, int64_t* dst, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(ids_to_i64_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, src, n,

@@ -45,7 +45,7 @@ struct ScanArgs {
   unsigned* cand;     // pairs (float bits of a', row)
   int* cnt;           // [q * parts + r] emitted count (may exceed cap: overflow)
   int cap;            // capacity of one region
-  // DENSE output
+  // DENSE output: dense[q * ldd + row] (row: absolute index in the scanned rows)
   float* dense;
   int64_t ldd;
   // TOP1 certification inputs

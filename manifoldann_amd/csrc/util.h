@@ -17,6 +17,9 @@ void gather_rows_i32(const int* src, const int* idx, int m, int w, int* dst, hip
 void labels_to_i32(const int64_t* src, int64_t n, int* dst, hipStream_t st);
 
 void ids_to_i64(const int* src, int64_t n, int64_t* dst, hipStream_t st);
+// binary16 rows (stride lds) -> binary32 rows (stride ldd), first w columns
+void f16_rows_to_f32(const void* src, int64_t lds, float* dst, int64_t ldd, int w, int64_t rows,
+                     hipStream_t st);
 // dst[r][c] = src[r][c] for c < w, 0 for w <= c < ldd
 void copy_rows(const float* src, int64_t lds, float* dst, int64_t ldd, int w, int64_t rows,
                hipStream_t st);

@@ -99,6 +99,27 @@ def as_matrix(x, d_expected: Optional[int] = None) -> _Mat:
     return _Mat(a.ctypes.data, d, n, d, a, False)
 
 
+def _is_f16(x) -> bool:
+    if _is_torch(x):
+        import torch
+        return x.dtype == torch.float16
+    return isinstance(x, np.ndarray) and x.dtype == np.float16
+
+
+def _as_matrix_f16(x) -> _Mat:
+    """View a d x n Float16 matrix (numpy or torch) as rows of d halves."""
+    if _is_torch(x):
+        t = x if x.dim() == 2 else x.reshape(-1, 1)
+        d, n = t.shape
+        if n > 1 and t.stride(0) != 1:
+            t = t.t().contiguous().t()
+        ld = t.stride(1) if n > 1 else d
+        return _Mat(t.data_ptr(), d, n, ld, t, t.is_cuda)
+    a = np.asfortranarray(x if x.ndim == 2 else x.reshape(-1, 1))
+    d, n = a.shape
+    return _Mat(a.ctypes.data, d, n, d, a, False)
+
+
 class Dataset:
     """Device-resident copy of a data matrix (explicit, caller-owned).
 
@@ -108,11 +129,16 @@ class Dataset:
     """
 
     def __init__(self, data, metric=default_distance):
-        m = as_matrix(data)
-        self.d, self.n = m.d, m.n
-        self.metric = metric
         h = ctypes.c_void_p()
-        check(lib().mann_dataset_create(m.ptr, m.d, m.n, m.ld, metric.code, ctypes.byref(h)))
+        self.metric = metric
+        if _is_f16(data):  # Matrix{Float16}: upcast exactly on the device
+            m = _as_matrix_f16(data)
+            check(lib().mann_dataset_create_f16(m.ptr, m.d, m.n, m.ld, metric.code,
+                                                ctypes.byref(h)))
+        else:
+            m = as_matrix(data)
+            check(lib().mann_dataset_create(m.ptr, m.d, m.n, m.ld, metric.code, ctypes.byref(h)))
+        self.d, self.n = m.d, m.n
         self.handle = h
 
     def __del__(self):
