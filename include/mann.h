@@ -184,13 +184,15 @@ int mann_ivf_search_routed(mann_ivf_t ivf, mann_router_t r, const float* queries
  * the k-means assignment: 0 = 128 rows x 128 queries, 1 = 256 x 128,
  * 2 = 128 x 256, 3 = 256 x 256 on 4 wavefronts, 4 and 5 = 256 x 256 on 8
  * wavefronts), "xcd_map" (grid order: 0 plain, 1 query-major per XCD,
- * 2 row-partition-major per XCD, 3 bands of 8 row partitions per XCD), "target_groups" (workgroups the row
- * partitioning aims for), "sample_rank" (emission threshold = that multiple of
- * k in the row sample), "shortlist_mult" (first exact re-rank batch = that
- * multiple of k), "tau_rows" (IVF: exact rows sampled per query for the
- * emission threshold, 16..1024), "sample_rows" (exact kNN: rows of the stride
- * sample a dataset keeps for the emission threshold; read when the dataset is
- * created).  No reference counterpart: the reference has
+ * 2 row-partition-major per XCD, 3 bands of 8 row partitions per XCD),
+ * "target_groups" (workgroups the row partitioning aims for), "sample_rank"
+ * (multiplier of the emission target of the exact-kNN threshold sample),
+ * "shortlist_mult" (first exact re-rank batch = that multiple of k),
+ * "ivf_tau_mode" (IVF emission threshold: 0 = exact FP32 sample of the probed
+ * rows, 1 = bf16 scan of each query's first list, the default), "tau_rows"
+ * (rows of that exact sample, 16..1024), "sample_rows" (exact kNN: minimum
+ * rows of the stride sample a dataset keeps for the emission threshold; read
+ * when the dataset is created).  No reference counterpart: the reference has
  * no tunable kernels. */
 int mann_tune_set(const char* key, int64_t value);
 int mann_tune_get(const char* key, int64_t* value);

@@ -689,6 +689,9 @@ int mann_tune_set(const char* key, int64_t value) {
       MANN_REQUIRE(value >= 256 && value <= 65536, ERR_ARGUMENT,
                    "sample_rows must be in [256, 65536]");
       t.sample_rows = (int)value;
+    } else if (k == "ivf_tau_mode") {
+      MANN_REQUIRE(value == 0 || value == 1, ERR_ARGUMENT, "ivf_tau_mode must be 0 or 1");
+      t.ivf_tau_mode = (int)value;
     } else if (k == "tau_rows") {
       MANN_REQUIRE(value >= 16 && value <= 1024, ERR_ARGUMENT, "tau_rows must be in [16, 1024]");
       t.tau_rows = (int)value;
@@ -714,6 +717,7 @@ int mann_tune_get(const char* key, int64_t* value) {
     else if (k == "sample_rank") *value = t.sample_rank;
     else if (k == "shortlist_mult") *value = t.shortlist_mult;
     else if (k == "tau_rows") *value = t.tau_rows;
+    else if (k == "ivf_tau_mode") *value = t.ivf_tau_mode;
     else if (k == "sample_rows") *value = t.sample_rows;
     else throw Error{ERR_ARGUMENT, "unknown tuning key " + k};
   });

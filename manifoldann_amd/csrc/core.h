@@ -30,6 +30,7 @@ struct mann_ivf {
   int device = 0;
   int64_t d = 0, n = 0, d16 = 0, kp = 0, n_pad = 0;
   int nlist = 0;   // routing distances per query (centroids)
+  int64_t max_list = 0;  // rows of the largest list
   int nb = 0;      // buckets = max assignment (partition_by_transform)
   int normalized = 0;
   float* c32 = nullptr;

@@ -59,6 +59,8 @@ mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int
                             hipMemcpyDeviceToHost, st));
     MANN_HIP(hipStreamSynchronize(st));
     iv->h_off[nb] = (int)n;
+    for (int b = 0; b < nb; ++b)
+      iv->max_list = std::max<int64_t>(iv->max_list, iv->h_off[b + 1] - iv->h_off[b]);
     MANN_HIP(hipMemcpyAsync(iv->list_off + nb, &iv->h_off[nb], sizeof(int),
                             hipMemcpyHostToDevice, st));
     std::vector<int> order(nb);
@@ -165,8 +167,14 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
   }
   max_items = std::min<int64_t>(max_items, (int64_t)nq * nprobe * ceil_div(iv.n, ROWS_ITEM));
   ScanItem* items = c.buf("iv_items").as<ScanItem>((size_t)std::max<int64_t>(max_items, 1));
+  // threshold from a bf16 scan of each query's first list (dense S^ values),
+  // unless the dense block would be too large
+  const bool dense_tau = tuning().ivf_tau_mode == 1 &&
+                         (int64_t)nq * iv.max_list <= ((int64_t)1 << 29);
+  int* first_cnt = dense_tau ? c.buf("iv_first_cnt").as<int>(nb) : nullptr;
   // [0] n_items, [1] item counter, [2] flagged after pass 1, [3] flagged after
-  // pass 2, [4] entries, [5] refinement items, [6] refinement item counter
+  // pass 2, [4] entries, [5] threshold-scan / refinement items, [6] their item
+  // counter
   int* ctr = c.buf("iv_ctr").as<int>(8);
   int* stats = c.buf("iv_stats").as<int>(4);
   fill_i32(qcount, nb, 0, st);
@@ -176,7 +184,8 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
   {
     ScopedTime tm("ivf_plan", st);
     launch_ivf_plan(probes, pcount, nq, nprobe, nb, iv.list_off, iv.lorder, BN, ROWS_ITEM, qcount,
-                    qfill, qoff, ioff, qlist, qlist_list, slot_of, items, ctr, ctr + 4, st);
+                    qfill, qoff, ioff, qlist, qlist_list, slot_of, items, ctr, ctr + 4, first_cnt,
+                    st);
   }
   // residual query rows per (list, query) entry and per-query bounds
   bf16_t* qres = c.buf("iv_qres").as<bf16_t>((size_t)max_entries * iv.kp);
@@ -192,37 +201,91 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
   const float* qe = (const float*)qem;
   const float* qr = (const float*)(qem + nq);
 
-  // emission thresholds (certified by an exact row sample)
+  // emission thresholds
   float* tau = c.buf("iv_tau").as<float>(nq);
   int* keff = c.buf("iv_keff").as<int>(nq);
-  IvfTauArgs ta{};
-  ta.nq = nq;
-  ta.nprobe = nprobe;
-  ta.k = k;
-  ta.d = (int)iv.d;
-  ta.d16 = (int)iv.d16;
-  ta.metric = metric;
-  ta.emit_all_below = cap;
-  ta.sample_rows = std::max(1, std::min(tuning().tau_rows, ivf_tau_rows_max()));
-  ta.probes = probes;
-  ta.pcount = pcount;
-  ta.prows = prows;
-  ta.list_off = iv.list_off;
-  ta.x32 = iv.x32;
-  ta.xn2 = iv.xn2;
-  ta.q32 = Q.q32;
-  ta.qn2 = Q.qn2;
-  ta.qe = qe;
-  ta.qr = qr;
-  ta.ex = iv.ex;
-  ta.rx = iv.rx;
-  ta.tau = tau;
-  ta.k_eff_q = keff;
-  ta.qlist = qlist;
-  ta.qlist_list = qlist_list;
-  ta.n_entries = ctr + 4;
-  ta.max_entries = max_entries;
-  {
+  if (dense_tau) {
+    // bf16 residual scan of each query's first list; the k-th smallest S^
+    // bounds the k-th exact value from above (certified by construction)
+    ScanItem* fitems = c.buf("iv_fitems").as<ScanItem>((size_t)std::max<int64_t>(max_items, 1));
+    int* fioff = c.buf("iv_fioff").as<int>(nb);
+    float* dense = c.buf("iv_dense").as<float>((size_t)nq * iv.max_list);
+    {
+      ScopedTime tm("ivf_tau_scan", st);
+      launch_first_items(first_cnt, qoff, nb, iv.list_off, iv.lorder, BN, ROWS_ITEM, fioff, fitems,
+                         ctr + 5, st);
+      fill_i32(ctr + 6, 1, 0, st);
+      ScanArgs a{};
+      a.cfg = cfg;
+      a.xb = iv.xb;
+      a.xn = iv.xn;
+      a.qb = qres;
+      a.col_norm = rnorm;
+      a.b_by_slot = 1;
+      a.kp = (int)iv.kp;
+      a.d = (int)iv.d;
+      a.qidx = qlist;
+      a.items = fitems;
+      a.n_items_dev = ctr + 5;
+      a.item_counter = ctr + 6;
+      a.dense = dense;
+      a.ldd = iv.max_list;
+      a.dense_rel = 1;
+      a.parts = 1;
+      int cus = 256;
+      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device);
+      launch_scan(a, SCAN_DENSE, 2 * cus, SCAN_KIND_ITEMS, st);
+    }
+    IvfDenseTauArgs td{};
+    td.nq = nq;
+    td.nprobe = nprobe;
+    td.k = k;
+    td.d = (int)iv.d;
+    td.metric = metric;
+    td.emit_all_below = cap;
+    td.probes = probes;
+    td.pcount = pcount;
+    td.prows = prows;
+    td.list_off = iv.list_off;
+    td.dense = dense;
+    td.ldd = iv.max_list;
+    td.qe = qe;
+    td.qr = qr;
+    td.ex = iv.ex;
+    td.rx = iv.rx;
+    td.tau = tau;
+    td.k_eff_q = keff;
+    ScopedTime tm("ivf_tau", st);
+    launch_ivf_tau_dense(td, st);
+  } else {
+    // exact FP32 sample of the first rows of the probed lists
+    IvfTauArgs ta{};
+    ta.nq = nq;
+    ta.nprobe = nprobe;
+    ta.k = k;
+    ta.d = (int)iv.d;
+    ta.d16 = (int)iv.d16;
+    ta.metric = metric;
+    ta.emit_all_below = cap;
+    ta.sample_rows = std::max(1, std::min(tuning().tau_rows, ivf_tau_rows_max()));
+    ta.probes = probes;
+    ta.pcount = pcount;
+    ta.prows = prows;
+    ta.list_off = iv.list_off;
+    ta.x32 = iv.x32;
+    ta.xn2 = iv.xn2;
+    ta.q32 = Q.q32;
+    ta.qn2 = Q.qn2;
+    ta.qe = qe;
+    ta.qr = qr;
+    ta.ex = iv.ex;
+    ta.rx = iv.rx;
+    ta.tau = tau;
+    ta.k_eff_q = keff;
+    ta.qlist = qlist;
+    ta.qlist_list = qlist_list;
+    ta.n_entries = ctr + 4;
+    ta.max_entries = max_entries;
     ScopedTime tm("ivf_tau", st);
     launch_ivf_tau(ta, st);
   }
@@ -304,6 +367,7 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
     int* flist2 = c.buf("iv_flist2").as<int>(nq);
     {
       ScopedTime tm("ivf_refine_scan", st);
+      fill_i32(ctr + 5, 2, 0, st);
       launch_refine_items(flist, ctr + 2, nflag, nprobe, probes, pcount, slot_of, iv.list_off,
                           ROWS_ITEM, ritems, ctr + 5, st);
       reset_listed(cnt, flist, ctr + 2, nflag, st);

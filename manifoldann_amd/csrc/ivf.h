@@ -42,7 +42,30 @@ void launch_ivf_plan(const int* probes, const int* pcount, int nq, int nprobe, i
                      const int* list_off, const int* lorder, int bn, int rows_item, int* qcount,
                      int* qfill, int* qoff, int* ioff, int* qlist, int* qlist_list,
                      int* slot_of, ScanItem* items, int* n_items, int* n_entries,
-                     hipStream_t st);
+                     int* first_cnt, hipStream_t st);
+// Work items of the threshold scan: for every list, the queries that probe it
+// first (the leading first_cnt entries of its block) against all its rows;
+// region = the list's first row.  n_items written on the device.
+void launch_first_items(const int* first_cnt, const int* qoff, int nb, const int* list_off,
+                        const int* lorder, int bn, int rows_item, int* ioff, ScanItem* items,
+                        int* n_items, hipStream_t st);
+// Emission threshold from the first-list scan: S^ values dense[q][0..|l1|)
+struct IvfDenseTauArgs {
+  int nq, nprobe, k, d, metric;
+  int emit_all_below;
+  const int* probes;
+  const int* pcount;
+  const int* prows;
+  const int* list_off;
+  const float* dense;
+  int64_t ldd;
+  const float* qe;
+  const float* qr;
+  float ex, rx;
+  float* tau;
+  int* k_eff_q;
+};
+void launch_ivf_tau_dense(const IvfDenseTauArgs& a, hipStream_t st);
 // single-query items over the probed lists of the flagged queries
 void launch_refine_items(const int* flist, const int* nflag, int nflag_host, int nprobe,
                          const int* probes, const int* pcount, const int* slot_of,

@@ -3,6 +3,7 @@
 // probe lists into per-list query lists, the persistent scan's work items, and
 // the certified emission threshold from an exact row sample.
 #include "bounds.h"
+#include "radix.h"
 #include "ivf.h"
 #include "util.h"
 
@@ -257,14 +258,17 @@ __global__ __launch_bounds__(1024) void plan_kernel(const int* __restrict__ qcou
   }
 }
 
+// phase 0 places the entries of first-probed lists, phase 1 the others, so a
+// list's entries for which it is the query's first probe come first
 __global__ void list_fill_kernel(const int* __restrict__ probes, const int* __restrict__ pcount,
                                  int nq, int nprobe, const int* __restrict__ qoff,
                                  int* __restrict__ qfill, int* __restrict__ qlist,
-                                 int* __restrict__ qlist_list, int* __restrict__ slot_of) {
+                                 int* __restrict__ qlist_list, int* __restrict__ slot_of,
+                                 int phase) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)nq * nprobe) return;
   const int q = (int)(i / nprobe), r = (int)(i - (int64_t)q * nprobe);
-  if (r >= pcount[q]) return;
+  if (r >= pcount[q] || (r == 0) != (phase == 0)) return;
   const int b = probes[i];
   const int pos = atomicAdd(&qfill[b], 1);
   qlist[qoff[b] + pos] = q;
@@ -366,6 +370,103 @@ __global__ void items_fill_kernel(const int* __restrict__ qcount, int nb,
       it.region = 0;
       items[w++] = it;
     }
+}
+
+// one block: per list (size-descending order) the item count of its first-probe
+// queries, exclusive scan into ioff, total into *n_items; then per list the items
+__global__ __launch_bounds__(1024) void first_plan_kernel(const int* __restrict__ first_cnt,
+                                                          int nb, const int* __restrict__ list_off,
+                                                          const int* __restrict__ lorder, int bn,
+                                                          int rows_item, int* __restrict__ ioff,
+                                                          int* __restrict__ n_items) {
+  __shared__ int wsum[16];
+  __shared__ int total;
+  const int tid = threadIdx.x;
+  const int per = (nb + 1023) / 1024;
+  const int s0 = tid * per, e0 = min(nb, s0 + per);
+  int sum = 0;
+  for (int i = s0; i < e0; ++i) {
+    const int b = lorder[i];
+    const int sz = list_off[b + 1] - list_off[b];
+    sum += ((first_cnt[b] + bn - 1) / bn) * ((sz + rows_item - 1) / rows_item);
+  }
+  int run = block_exclusive_scan(sum, wsum, &total);
+  if (tid == 0) *n_items = total;
+  for (int i = s0; i < e0; ++i) {
+    ioff[i] = run;
+    const int b = lorder[i];
+    const int sz = list_off[b + 1] - list_off[b];
+    run += ((first_cnt[b] + bn - 1) / bn) * ((sz + rows_item - 1) / rows_item);
+  }
+}
+
+__global__ void first_items_kernel(const int* __restrict__ first_cnt, int nb,
+                                   const int* __restrict__ list_off,
+                                   const int* __restrict__ lorder, const int* __restrict__ qoff,
+                                   const int* __restrict__ ioff, int bn, int rows_item,
+                                   ScanItem* __restrict__ items) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const int b = lorder[i];
+  const int c0 = first_cnt[b];
+  const int nqc = (c0 + bn - 1) / bn;
+  const int lo = list_off[b], hi = list_off[b + 1];
+  const int ns = (hi - lo + rows_item - 1) / rows_item;
+  int w = ioff[i];
+  for (int s = 0; s < ns; ++s)
+    for (int c = 0; c < nqc; ++c) {
+      ScanItem it;
+      it.qoff = qoff[b] + c * bn;
+      it.nq = min(bn, c0 - c * bn);
+      it.row_begin = lo + s * rows_item;
+      it.row_end = min(hi, it.row_begin + rows_item);
+      it.region = lo;
+      items[w++] = it;
+    }
+}
+
+// Upper bound, in the metric's own domain, of the FP32 exact value of a row
+// whose residual first-pass value is S: |q - x| <= sqrt(S + G) + qe + ex, then
+// the FP32 rounding of the exact forms on top.
+__device__ inline double metric_upper_bound(double S, int metric, double qe, double qr,
+                                            double ex, double rx, int d) {
+  const double u = 1.0 / 16777216.0;
+  const double R = qr + rx;
+  const double G = (2.0 * d + 8.0) * u * R * R;
+  const double t = sqrt(S + G > 0.0 ? S + G : 0.0) + qe + ex;
+  const double rho = (d + 8.0) * u;
+  if (metric == EUCLIDEAN) return t * (1.0 + rho) * (1.0 + 4.0 * u);
+  if (metric == SQEUCLIDEAN) return t * t * (1.0 + rho) * (1.0 + 4.0 * u);
+  // cosine: the residual rows are unit vectors minus a common centre, so
+  // |q^ - x^|^2 = 2 (1 - cos); FP32 cosine form error below (2d + 16) u
+  return 0.5 * t * t + (2.0 * d + 16.0) * u;
+}
+
+__global__ __launch_bounds__(256) void ivf_tau_dense_kernel(IvfDenseTauArgs a) {
+  const int q = blockIdx.x;
+  if (q >= a.nq) return;
+  __shared__ RadixScratch sm;
+  const int np = a.pcount[q];
+  const int total = a.prows[q];
+  const int k_eff = a.k < total ? a.k : total;
+  int m = 0;
+  if (np > 0) {
+    const int b = a.probes[(int64_t)q * a.nprobe];
+    m = a.list_off[b + 1] - a.list_off[b];
+  }
+  if (threadIdx.x == 0) a.k_eff_q[q] = k_eff;
+  if (k_eff <= 0 || total <= a.emit_all_below || m < k_eff) {
+    // nothing to bound with (or everything fits): emit every probed row
+    if (threadIdx.x == 0) a.tau[q] = INFINITY;
+    return;
+  }
+  const uint32_t key = select_rank_key(a.dense + (int64_t)q * a.ldd, m, k_eff, sm);
+  if (threadIdx.x == 0) {
+    const double dub = metric_upper_bound((double)ord2f(key), a.metric, a.qe[q], a.qr[q], a.ex,
+                                          a.rx, a.d);
+    const double T = exclusion_threshold(dub, a.metric, a.qe[q], a.qr[q], a.ex, a.rx, a.d);
+    a.tau[q] = T < 3.0e38 ? nextafterf((float)T * (1.0f + 1e-6f), INFINITY) : INFINITY;
+  }
 }
 
 constexpr int TAU_M = 1024;
@@ -540,7 +641,7 @@ void launch_ivf_plan(const int* probes, const int* pcount, int nq, int nprobe, i
                      const int* list_off, const int* lorder, int bn, int rows_item, int* qcount,
                      int* qfill, int* qoff, int* ioff, int* qlist, int* qlist_list,
                      int* slot_of, ScanItem* items, int* n_items, int* n_entries,
-                     hipStream_t st) {
+                     int* first_cnt, hipStream_t st) {
   const int64_t tot = (int64_t)nq * nprobe;
   const unsigned g = (unsigned)ceil_div(tot, 256);
   hipLaunchKernelGGL(list_count_kernel, dim3(g), dim3(256), 0, st, probes, pcount, nq, nprobe,
@@ -548,7 +649,11 @@ void launch_ivf_plan(const int* probes, const int* pcount, int nq, int nprobe, i
   hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(1024), 0, st, qcount, nb, list_off, lorder, bn,
                      rows_item, qoff, ioff, n_items, n_entries);
   hipLaunchKernelGGL(list_fill_kernel, dim3(g), dim3(256), 0, st, probes, pcount, nq, nprobe, qoff,
-                     qfill, qlist, qlist_list, slot_of);
+                     qfill, qlist, qlist_list, slot_of, 0);
+  if (first_cnt)
+    MANN_HIP(hipMemcpyAsync(first_cnt, qfill, sizeof(int) * nb, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(list_fill_kernel, dim3(g), dim3(256), 0, st, probes, pcount, nq, nprobe, qoff,
+                     qfill, qlist, qlist_list, slot_of, 1);
   hipLaunchKernelGGL(items_fill_kernel, dim3((unsigned)ceil_div(nb, 256)), dim3(256), 0, st, qcount,
                      nb, list_off, lorder, qoff, ioff, bn, rows_item, items);
 }
@@ -576,6 +681,21 @@ void launch_refine_items(const int* flist, const int* nflag, int nflag_host, int
 }
 
 int ivf_tau_rows_max() { return TAU_M; }
+
+void launch_first_items(const int* first_cnt, const int* qoff, int nb, const int* list_off,
+                        const int* lorder, int bn, int rows_item, int* ioff, ScanItem* items,
+                        int* n_items, hipStream_t st) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(first_plan_kernel, dim3(1), dim3(1024), 0, st, first_cnt, nb, list_off, lorder,
+                     bn, rows_item, ioff, n_items);
+  hipLaunchKernelGGL(first_items_kernel, dim3((unsigned)ceil_div(nb, 256)), dim3(256), 0, st,
+                     first_cnt, nb, list_off, lorder, qoff, ioff, bn, rows_item, items);
+}
+
+void launch_ivf_tau_dense(const IvfDenseTauArgs& a, hipStream_t st) {
+  if (a.nq <= 0) return;
+  hipLaunchKernelGGL(ivf_tau_dense_kernel, dim3(a.nq), dim3(256), 0, st, a);
+}
 
 void launch_ivf_tau(const IvfTauArgs& a, hipStream_t st) {
   if (a.nq <= 0 || a.max_entries <= 0) return;

@@ -8,6 +8,7 @@
 // (lists contiguous, ascending ids inside a list) that is (dist, bucket, id),
 // the batched reference merge order.
 #include "bounds.h"
+#include "radix.h"
 
 namespace mann {
 
@@ -43,127 +44,12 @@ __device__ __forceinline__ int next_pow2(int x) {
   return p;
 }
 
-// rank-th smallest of 256 x 16 order keys held in registers (4-pass radix
-// select over the whole block; all threads return the same key)
-__device__ uint32_t block_radix_select(const uint32_t (&v)[16], unsigned rank, unsigned* hist,
-                                       unsigned* s_sel, unsigned* s_want) {
-  uint32_t prefix = 0, mask = 0;
-  if (threadIdx.x == 0) *s_want = rank;
-  for (int shift = 24; shift >= 0; shift -= 8) {
-    hist[threadIdx.x] = 0;
-    __syncthreads();
-#pragma unroll
-    for (int p = 0; p < 16; ++p)
-      if ((v[p] & mask) == prefix) atomicAdd(&hist[(v[p] >> shift) & 255u], 1u);
-    __syncthreads();
-    if (threadIdx.x < 64) {
-      const int l = threadIdx.x;
-      unsigned h0 = hist[4 * l], h1 = hist[4 * l + 1], h2 = hist[4 * l + 2], h3 = hist[4 * l + 3];
-      unsigned tot = h0 + h1 + h2 + h3, inc = tot;
-      for (int o = 1; o < 64; o <<= 1) {
-        unsigned t = __shfl_up(inc, o);
-        if (l >= o) inc += t;
-      }
-      unsigned excl = inc - tot, want = *s_want;
-      if (excl < want && inc >= want) {
-        unsigned c = excl, b = 4 * l;
-        if (c + h0 >= want) b = 4 * l;
-        else if (c + h0 + h1 >= want) { b = 4 * l + 1; c += h0; }
-        else if (c + h0 + h1 + h2 >= want) { b = 4 * l + 2; c += h0 + h1; }
-        else { b = 4 * l + 3; c += h0 + h1 + h2; }
-        *s_sel = b;
-        *s_want = want - c;
-      }
-    }
-    __syncthreads();
-    prefix |= *s_sel << shift;
-    mask |= 255u << shift;
-    __syncthreads();
-  }
-  return prefix;
-}
-
-// rank-th smallest of the query's m sample values.  m <= 4096: one register
-// radix select.  Larger m: U = rank-th smallest of the first 4096 values bounds
-// the answer from above; one pass collects the values <= U into LDS (a few
-// rank * m / 4096 of them) and a second register select finishes; if more than
-// 4096 qualify, a 4-pass radix select over all m values runs instead.
+// rank-th smallest of each query's m sample values (one block per query)
 __global__ __launch_bounds__(256) void tau_from_sample_kernel(TauArgs a) {
   const int q = blockIdx.x;
   if (q >= a.nq) return;
-  constexpr int PER = 16, CAPL = 256 * PER;
-  __shared__ unsigned hist[256];
-  __shared__ unsigned s_sel, s_want, s_cnt;
-  __shared__ uint32_t buf[CAPL];
-  const float* row = a.dense + (int64_t)q * a.ldd;
-  uint32_t v[PER];
-#pragma unroll
-  for (int p = 0; p < PER; ++p) {
-    int i = threadIdx.x + 256 * p;
-    v[p] = i < a.m ? f2ord(row[i]) : 0xffffffffu;
-  }
-  const unsigned r0 = (unsigned)min(a.rank, min(a.m, CAPL));
-  uint32_t key = block_radix_select(v, r0, hist, &s_sel, &s_want);
-  if (a.m > CAPL) {
-    const uint32_t U = key;
-    if (threadIdx.x == 0) s_cnt = 0;
-    __syncthreads();
-    for (int i = threadIdx.x; i < a.m; i += 256) {
-      const uint32_t o = f2ord(row[i]);
-      if (o <= U) {
-        const unsigned pos = atomicAdd(&s_cnt, 1u);
-        if (pos < (unsigned)CAPL) buf[pos] = o;
-      }
-    }
-    __syncthreads();
-    const unsigned cnt = s_cnt;
-    if (cnt <= (unsigned)CAPL && cnt >= (unsigned)a.rank) {
-#pragma unroll
-      for (int p = 0; p < PER; ++p) {
-        const unsigned i = threadIdx.x + 256 * p;
-        v[p] = i < cnt ? buf[i] : 0xffffffffu;
-      }
-      __syncthreads();
-      key = block_radix_select(v, (unsigned)a.rank, hist, &s_sel, &s_want);
-    } else {  // rare: too many values at or below U; select over all of them
-      uint32_t prefix = 0, mask = 0;
-      if (threadIdx.x == 0) s_want = (unsigned)a.rank;
-      for (int shift = 24; shift >= 0; shift -= 8) {
-        hist[threadIdx.x] = 0;
-        __syncthreads();
-        for (int i = threadIdx.x; i < a.m; i += 256) {
-          const uint32_t o = f2ord(row[i]);
-          if ((o & mask) == prefix) atomicAdd(&hist[(o >> shift) & 255u], 1u);
-        }
-        __syncthreads();
-        if (threadIdx.x < 64) {
-          const int l = threadIdx.x;
-          unsigned h0 = hist[4 * l], h1 = hist[4 * l + 1], h2 = hist[4 * l + 2],
-                   h3 = hist[4 * l + 3];
-          unsigned tot = h0 + h1 + h2 + h3, inc = tot;
-          for (int o = 1; o < 64; o <<= 1) {
-            unsigned t = __shfl_up(inc, o);
-            if (l >= o) inc += t;
-          }
-          unsigned excl = inc - tot, want = s_want;
-          if (excl < want && inc >= want) {
-            unsigned c = excl, b = 4 * l;
-            if (c + h0 >= want) b = 4 * l;
-            else if (c + h0 + h1 >= want) { b = 4 * l + 1; c += h0; }
-            else if (c + h0 + h1 + h2 >= want) { b = 4 * l + 2; c += h0 + h1; }
-            else { b = 4 * l + 3; c += h0 + h1 + h2; }
-            s_sel = b;
-            s_want = want - c;
-          }
-        }
-        __syncthreads();
-        prefix |= s_sel << shift;
-        mask |= 255u << shift;
-        __syncthreads();
-      }
-      key = prefix;
-    }
-  }
+  __shared__ RadixScratch sm;
+  const uint32_t key = select_rank_key(a.dense + (int64_t)q * a.ldd, a.m, a.rank, sm);
   if (threadIdx.x == 0) a.tau[q] = ord2f(key);
 }
 

@@ -45,9 +45,12 @@ struct ScanArgs {
   unsigned* cand;     // pairs (float bits of a', row)
   int* cnt;           // [q * parts + r] emitted count (may exceed cap: overflow)
   int cap;            // capacity of one region
-  // DENSE output: dense[q * ldd + row] (row: absolute index in the scanned rows)
+  // DENSE output: dense[q * ldd + row] (row: absolute index in the scanned rows),
+  // or dense[q * ldd + row - region] when dense_rel (work items carrying their
+  // list's first row as region)
   float* dense;
   int64_t ldd;
+  int dense_rel;
   // TOP1 certification inputs
   const float* t1_qe;
   const float* t1_qr;
@@ -65,6 +68,7 @@ struct Tuning {
   int sample_rank = 1;       // threshold rank = sample_rank * k_req in the row sample
   int shortlist_mult = 1;    // first re-rank batch = shortlist_mult * k_req (>= 32)
   int tau_rows = 1024;       // IVF: exact rows sampled per query for the emission threshold
+  int ivf_tau_mode = 1;      // IVF threshold: 0 exact FP32 row sample, 1 bf16 scan of the first list
   int sample_rows = 4096;    // exact kNN: stride-sample rows (fixed when a dataset is created)
 };
 Tuning& tuning();
