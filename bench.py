@@ -225,11 +225,40 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
         sweep_l.append({"nprobe": npb, "recall": round(rc, 4), "qps": round(qps, 1)})
         log(f"IVF learned nprobe={npb}: recall {rc:.4f} qps {qps:.0f}")
 
+    # probability-mass probing: lists in router order until their softmax mass
+    # reaches the target (at most 64), for the trained router and, as the
+    # ablation, for its centroid prior alone (zero MLP)
+    def mass_sweep(router, name):
+        out = []
+        for mass in (0.6, 0.8, 0.9, 0.95, 0.97, 0.98, 0.99, 0.995):
+            def run(mass=mass):
+                check(L.mann_ivf_search_routed_mass(index.handle, router.handle, Qd.data_ptr(), d,
+                                                    nq, d, 10, 64, mass, 0, ids.data_ptr(),
+                                                    dist.data_ptr()))
+            qps = qps_of(run, nq, args.reps, torch)
+            rc = recall(ids)
+            probes = RT.last_probe_count() / nq
+            out.append({"mass": mass, "recall": round(rc, 4), "qps": round(qps, 1),
+                        "mean_probes": round(probes, 2)})
+            log(f"IVF {name} mass={mass}: recall {rc:.4f} qps {qps:.0f} probes {probes:.2f}")
+        return out
+
+    sweep_m = mass_sweep(r, "learned")
+    sweep_p = None
+    if gamma is not None:
+        r0 = RT.LearnedProbeRouter(d, nlist, hidden=512, seed=11)
+        r0.set_params(np.zeros(r0.n_params, np.float32))
+        r0.set_prior(index, gamma)
+        sweep_p = mass_sweep(r0, "prior-only")
+        del r0
+
     def at95(sw):
-        ok = [s for s in sw if s["recall"] >= 0.95]
+        ok = [s for s in (sw or []) if s["recall"] >= 0.95]
         return max(ok, key=lambda s: s["qps"]) if ok else None
 
-    bc, bl = at95(sweep_c), at95(sweep_l)
+    bc, bl_fixed, bl_mass = at95(sweep_c), at95(sweep_l), at95(sweep_m)
+    cands = [s for s in (bl_fixed, bl_mass) if s]
+    bl = max(cands, key=lambda s: s["qps"]) if cands else None
     # kernel timing of the list scan at the centroid operating point
     kern = {}
     ivf_stats = None
@@ -264,8 +293,11 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
            "kmeans_fit_s": round(fit_s, 3), "router_train_s": round(train_s, 3),
            "router_final_loss": round(loss, 4), "train_queries": args.train_q,
            "router_prior_scale": gamma, "router_epochs": args.epochs,
-           "centroid": sweep_c, "learned": sweep_l,
+           "centroid": sweep_c, "learned": sweep_l, "learned_mass": sweep_m,
+           "prior_only_mass": sweep_p,
            "centroid_at_0.95": bc, "learned_at_0.95": bl,
+           "learned_fixed_at_0.95": bl_fixed, "learned_mass_at_0.95": bl_mass,
+           "prior_only_mass_at_0.95": at95(sweep_p),
            "learned_over_centroid_qps": (round(bl["qps"] / bc["qps"], 3) if bc and bl else None),
            "kernels_ms": kern, "search_stats": ivf_stats, "cpu_baseline": cpu,
            "total_s": round(time.perf_counter() - t_all, 1)}

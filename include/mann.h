@@ -174,10 +174,25 @@ int mann_router_train(mann_router_t r, const float* X, int64_t d, int64_t n, int
                       uint64_t seed, double* out_loss);
 int mann_router_scores(mann_router_t r, const float* Q, int64_t d, int64_t nq, int64_t ldq,
                        float* out_scores);
+/* Probability-mass probing (no reference counterpart; for a learned router):
+ * route_dist holds negated logits; per query the lists are taken in ascending
+ * route_dist order (ties to the lower bucket, empty buckets dropped) until the
+ * softmax probabilities of the lists taken reach `mass`, at most nprobe_max
+ * non-empty lists (nprobe_max <= 64, at most 1,024 lists).  Output as
+ * mann_ivf_probe: nq x nprobe_max, 1-based buckets, 0 = unused slot. */
+int mann_ivf_probe_mass(mann_ivf_t ivf, const float* route_dist, int64_t nq, int64_t nprobe_max,
+                        double mass, int64_t* out_buckets);
 /* mann_ivf_search with the router's negated scores as routing distances */
 int mann_ivf_search_routed(mann_ivf_t ivf, mann_router_t r, const float* queries, int64_t d,
                            int64_t nq, int64_t ldq, int64_t k, int64_t nprobe, int metric,
                            int64_t* out_ids, float* out_dist);
+/* the same with probability-mass probing of the router's softmax (see
+ * mann_ivf_probe_mass); the per-query list count varies */
+int mann_ivf_search_routed_mass(mann_ivf_t ivf, mann_router_t r, const float* queries, int64_t d,
+                                int64_t nq, int64_t ldq, int64_t k, int64_t nprobe_max,
+                                double mass, int metric, int64_t* out_ids, float* out_dist);
+/* (list, query) pairs scanned by the last IVF search of the calling thread */
+int mann_last_probe_count(int64_t* total);
 
 /* Tuning knobs of the search kernels (process-wide): "scan_cfg", "ivf_cfg",
  * "kmeans_cfg" (first-pass tile of the exact kNN scan, the IVF list scan and

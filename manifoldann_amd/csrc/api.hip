@@ -419,36 +419,39 @@ int mann_ivf_route(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq, 
   });
 }
 
+static void probe_out(mann_ivf_t ivf, const float* route_dist, int64_t nq, int64_t nprobe,
+                      float mass, int64_t* out_buckets) {
+  MANN_REQUIRE(ivf && route_dist && out_buckets, ERR_NULL, "null pointer");
+  MANN_REQUIRE(nprobe > 0, ERR_ARGUMENT, "k must be positive");
+  if (nq == 0) return;
+  ThreadCtx& c = ctx();
+  hipStream_t st = c.stream;
+  const float* R = route_dist;
+  if (!is_device_ptr(route_dist)) {
+    float* rb = c.buf("pr_R").as<float>((size_t)nq * ivf->nlist);
+    MANN_HIP(hipMemcpyAsync(rb, route_dist, sizeof(float) * nq * ivf->nlist,
+                            hipMemcpyHostToDevice, st));
+    R = rb;
+  }
+  int* probes = c.buf("pr_probes").as<int>((size_t)nq * nprobe);
+  int* pcount = c.buf("pr_pcount").as<int>(nq);
+  int* prows = c.buf("pr_prows").as<int>(nq);
+  ivf_probe(*ivf, R, (int)nq, (int)nprobe, probes, pcount, prows, mass);
+  std::vector<int> hp((size_t)nq * nprobe);
+  MANN_HIP(hipMemcpyAsync(hp.data(), probes, sizeof(int) * hp.size(), hipMemcpyDeviceToHost, st));
+  MANN_HIP(hipStreamSynchronize(st));
+  std::vector<int64_t> tmp(hp.size());
+  for (size_t i = 0; i < hp.size(); ++i) tmp[i] = hp[i] + 1;  // -1 -> 0 (empty slot)
+  if (is_device_ptr(out_buckets))
+    MANN_HIP(hipMemcpy(out_buckets, tmp.data(), sizeof(int64_t) * tmp.size(),
+                       hipMemcpyHostToDevice));
+  else
+    memcpy(out_buckets, tmp.data(), sizeof(int64_t) * tmp.size());
+}
+
 int mann_ivf_probe(mann_ivf_t ivf, const float* route_dist, int64_t nq, int64_t nprobe,
                    int64_t* out_buckets) {
-  return guarded([&] {
-    MANN_REQUIRE(ivf && route_dist && out_buckets, ERR_NULL, "null pointer");
-    MANN_REQUIRE(nprobe > 0, ERR_ARGUMENT, "k must be positive");
-    if (nq == 0) return;
-    ThreadCtx& c = ctx();
-    hipStream_t st = c.stream;
-    const float* R = route_dist;
-    if (!is_device_ptr(route_dist)) {
-      float* rb = c.buf("pr_R").as<float>((size_t)nq * ivf->nlist);
-      MANN_HIP(hipMemcpyAsync(rb, route_dist, sizeof(float) * nq * ivf->nlist,
-                              hipMemcpyHostToDevice, st));
-      R = rb;
-    }
-    int* probes = c.buf("pr_probes").as<int>((size_t)nq * nprobe);
-    int* pcount = c.buf("pr_pcount").as<int>(nq);
-    int* prows = c.buf("pr_prows").as<int>(nq);
-    ivf_probe(*ivf, R, (int)nq, (int)nprobe, probes, pcount, prows);
-    std::vector<int> hp((size_t)nq * nprobe);
-    MANN_HIP(hipMemcpyAsync(hp.data(), probes, sizeof(int) * hp.size(), hipMemcpyDeviceToHost, st));
-    MANN_HIP(hipStreamSynchronize(st));
-    std::vector<int64_t> tmp(hp.size());
-    for (size_t i = 0; i < hp.size(); ++i) tmp[i] = hp[i] + 1;  // -1 -> 0 (empty slot)
-    if (is_device_ptr(out_buckets))
-      MANN_HIP(hipMemcpy(out_buckets, tmp.data(), sizeof(int64_t) * tmp.size(),
-                         hipMemcpyHostToDevice));
-    else
-      memcpy(out_buckets, tmp.data(), sizeof(int64_t) * tmp.size());
-  });
+  return guarded([&] { probe_out(ivf, route_dist, nq, nprobe, 0.0f, out_buckets); });
 }
 
 int mann_ivf_search(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq, int64_t ldq,
@@ -638,33 +641,64 @@ int mann_router_scores(mann_router_t r, const float* Q, int64_t d, int64_t nq, i
   });
 }
 
+static void search_routed(mann_ivf_t ivf, mann_router_t r, const float* queries, int64_t d,
+                          int64_t nq, int64_t ldq, int64_t k, int64_t nprobe, float mass,
+                          int metric, int64_t* out_ids, float* out_dist) {
+  MANN_REQUIRE(ivf && r, ERR_NULL, "null handle");
+  MANN_REQUIRE(r->c == ivf->nlist && r->d == ivf->d, ERR_DIMENSION,
+               "router does not match the index");
+  check_metric(metric);
+  MANN_REQUIRE((metric == COSINE) == (ivf->normalized != 0), ERR_ARGUMENT,
+               "index was built for another metric");
+  MANN_REQUIRE(nprobe > 0, ERR_ARGUMENT, "k must be positive");
+  QueryOps Q = ivf_queries(ivf, queries, d, nq, ldq);
+  if (nq == 0 || k <= 0) {
+    last_stats() = SearchStats();
+    return;
+  }
+  MANN_REQUIRE(out_ids && out_dist, ERR_NULL, "null pointer");
+  ThreadCtx& c = ctx();
+  float* S = c.buf("sr_scores").as<float>((size_t)nq * r->c);
+  router_scores(*r, Q.q32, ivf->d16, nq, S);
+  launch_neg(S, S, nq * r->c, c.stream);
+  OutBuf<int64_t> oi(out_ids, (size_t)(nq * k), "out_ids");
+  OutBuf<float> od(out_dist, (size_t)(nq * k), "out_dist");
+  ivf_search(*ivf, Q, (int)k, (int)nprobe, metric, S, oi.dev, od.dev, mass);
+  oi.finish();
+  od.finish();
+  timer_flush();
+}
+
 int mann_ivf_search_routed(mann_ivf_t ivf, mann_router_t r, const float* queries, int64_t d,
                            int64_t nq, int64_t ldq, int64_t k, int64_t nprobe, int metric,
                            int64_t* out_ids, float* out_dist) {
   return guarded([&] {
-    MANN_REQUIRE(ivf && r, ERR_NULL, "null handle");
-    MANN_REQUIRE(r->c == ivf->nlist && r->d == ivf->d, ERR_DIMENSION,
-                 "router does not match the index");
-    check_metric(metric);
-    MANN_REQUIRE((metric == COSINE) == (ivf->normalized != 0), ERR_ARGUMENT,
-                 "index was built for another metric");
-    MANN_REQUIRE(nprobe > 0, ERR_ARGUMENT, "k must be positive");
-    QueryOps Q = ivf_queries(ivf, queries, d, nq, ldq);
-    if (nq == 0 || k <= 0) {
-      last_stats() = SearchStats();
-      return;
-    }
-    MANN_REQUIRE(out_ids && out_dist, ERR_NULL, "null pointer");
-    ThreadCtx& c = ctx();
-    float* S = c.buf("sr_scores").as<float>((size_t)nq * r->c);
-    router_scores(*r, Q.q32, ivf->d16, nq, S);
-    launch_neg(S, S, nq * r->c, c.stream);
-    OutBuf<int64_t> oi(out_ids, (size_t)(nq * k), "out_ids");
-    OutBuf<float> od(out_dist, (size_t)(nq * k), "out_dist");
-    ivf_search(*ivf, Q, (int)k, (int)nprobe, metric, S, oi.dev, od.dev);
-    oi.finish();
-    od.finish();
-    timer_flush();
+    search_routed(ivf, r, queries, d, nq, ldq, k, nprobe, 0.0f, metric, out_ids, out_dist);
+  });
+}
+
+int mann_ivf_search_routed_mass(mann_ivf_t ivf, mann_router_t r, const float* queries, int64_t d,
+                                int64_t nq, int64_t ldq, int64_t k, int64_t nprobe_max,
+                                double mass, int metric, int64_t* out_ids, float* out_dist) {
+  return guarded([&] {
+    MANN_REQUIRE(mass > 0.0 && mass <= 1.0, ERR_ARGUMENT, "mass must be in (0, 1]");
+    search_routed(ivf, r, queries, d, nq, ldq, k, nprobe_max, (float)mass, metric, out_ids,
+                  out_dist);
+  });
+}
+
+int mann_ivf_probe_mass(mann_ivf_t ivf, const float* route_dist, int64_t nq, int64_t nprobe_max,
+                        double mass, int64_t* out_buckets) {
+  return guarded([&] {
+    MANN_REQUIRE(mass > 0.0 && mass <= 1.0, ERR_ARGUMENT, "mass must be in (0, 1]");
+    probe_out(ivf, route_dist, nq, nprobe_max, (float)mass, out_buckets);
+  });
+}
+
+int mann_last_probe_count(int64_t* total) {
+  return guarded([&] {
+    MANN_REQUIRE(total != nullptr, ERR_NULL, "null pointer");
+    *total = last_stats().probes;
   });
 }
 

@@ -102,6 +102,7 @@ mann_dataset* create_dataset(const float* data, int64_t d, int64_t n, int64_t ld
 
 struct SearchStats {
   long long reranked = 0, overflow = 0, pass2 = 0, exact = 0;
+  long long probes = 0;  // IVF: (list, query) pairs scanned
 };
 SearchStats& last_stats();
 
@@ -127,10 +128,11 @@ std::vector<int> random_indices(int64_t n, int k, uint64_t seed);
 // IVF (core_ivf.hip)
 mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int64_t* assign_host);
 void ivf_route(const mann_ivf& iv, const QueryOps& Q, float* R);
+// mass > 0: probability-mass probing (R = negated logits), at most nprobe lists
 void ivf_probe(const mann_ivf& iv, const float* R, int nq, int nprobe, int* probes, int* pcount,
-               int* prows);
+               int* prows, float mass = 0.0f);
 void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int metric,
-                const float* route_in, int64_t* out_ids, float* out_dist);
+                const float* route_in, int64_t* out_ids, float* out_dist, float mass = 0.0f);
 
 // learned router (core_router.hip)
 mann_router* router_create(int d, int h, int c, uint64_t seed);

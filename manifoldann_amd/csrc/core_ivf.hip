@@ -117,17 +117,21 @@ void ivf_route(const mann_ivf& iv, const QueryOps& Q, float* R) {
 
 // Probe lists of each query: probes [nq][nprobe] (buckets, 0-based, -1 pad).
 void ivf_probe(const mann_ivf& iv, const float* R, int nq, int nprobe, int* probes, int* pcount,
-               int* prows) {
+               int* prows, float mass) {
   hipStream_t st = ctx().stream;
   const int avail = std::min(iv.nb, iv.nlist);
   MANN_REQUIRE(avail > 0, ERR_ARGUMENT, "Transform produced no routing distances for TopKRouting");
   MANN_REQUIRE(avail <= route_max(), ERR_UNSUPPORTED, "more than 4096 routing buckets");
+  if (mass > 0.0f)
+    MANN_REQUIRE(nprobe <= 64 && avail <= 1024, ERR_UNSUPPORTED,
+                 "probability-mass probing supports nprobe <= 64 and at most 1024 lists");
   ScopedTime tm("ivf_probe", st);
-  launch_probe_select(R, nq, iv.nlist, avail, nprobe, iv.list_off, probes, pcount, prows, st);
+  launch_probe_select(R, nq, iv.nlist, avail, nprobe, mass, iv.list_off, probes, pcount, prows,
+                      st);
 }
 
 void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int metric,
-                const float* route_in, int64_t* out_ids, float* out_dist) {
+                const float* route_in, int64_t* out_ids, float* out_dist, float mass) {
   ThreadCtx& c = ctx();
   hipStream_t st = c.stream;
   SearchStats& S = last_stats();
@@ -148,7 +152,7 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
   int* probes = c.buf("iv_probes").as<int>((size_t)nq * nprobe);
   int* pcount = c.buf("iv_pcount").as<int>(nq);
   int* prows = c.buf("iv_prows").as<int>(nq);
-  ivf_probe(iv, R, nq, nprobe, probes, pcount, prows);
+  ivf_probe(iv, R, nq, nprobe, probes, pcount, prows, mass);
 
   // invert probe lists into work items
   const int nb = iv.nb;
@@ -357,6 +361,7 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
   int h[8], hs[4];
   MANN_HIP(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
   MANN_HIP(hipStreamSynchronize(st));
+  S.probes = h[4];
   const int nflag = h[2];
   int nflag2 = 0;
   if (nflag > 0) {

@@ -35,7 +35,9 @@ int route_max();
 int ivf_tau_rows_max();
 void launch_route_dist(const float* q32, const float* qn2, int nq, const float* c32,
                        const float* cn2, int nlist, int d16, float* R, hipStream_t st);
-void launch_probe_select(const float* R, int nq, int nlist, int avail, int nprobe,
+// mass > 0: probability-mass probing over negated logits (nprobe <= 64,
+// avail <= 1024), see kernels_ivf.hip
+void launch_probe_select(const float* R, int nq, int nlist, int avail, int nprobe, float mass,
                          const int* list_off, int* probes, int* pcount, int* prows,
                          hipStream_t st);
 void launch_ivf_plan(const int* probes, const int* pcount, int nq, int nprobe, int nb,

@@ -135,9 +135,12 @@ __global__ __launch_bounds__(256) void probe_select_kernel(const float* __restri
 // Same selection for nprobe <= 64 and at most 1024 routing distances: one
 // wavefront per query holds 16 keys per lane and extracts the kk smallest
 // (dist, bucket) keys by repeated wavefront minima, i.e. in sorted order.
+// mass > 0 (learned routing): the distances are negated logits; lists are
+// taken in the same order until their softmax probabilities sum to `mass`
+// (at most nprobe non-empty lists).
 constexpr int PS_KPL = 16;
 __global__ __launch_bounds__(256) void probe_select_wave_kernel(
-    const float* __restrict__ R, int nq, int nlist, int avail, int nprobe,
+    const float* __restrict__ R, int nq, int nlist, int avail, int nprobe, float mass,
     const int* __restrict__ list_off, int* __restrict__ probes, int* __restrict__ pcount,
     int* __restrict__ prows) {
   const int lane = threadIdx.x & 63;
@@ -151,14 +154,32 @@ __global__ __launch_bounds__(256) void probe_select_wave_kernel(
     key[u] = i < avail ? pack_key(R[(int64_t)q * nlist + i], (uint32_t)i) : ~0ull;
     lmin = key[u] < lmin ? key[u] : lmin;
   }
-  const int kk = nprobe < avail ? nprobe : avail;
-  int m = 0, rows = 0;
-  for (int r = 0; r < kk; ++r) {
+  // softmax normaliser over the negated distances (mass mode)
+  float rmin = 0.0f, zsum = 1.0f;
+  if (mass > 0.0f) {
     uint64_t g = lmin;
     for (int o = 32; o >= 1; o >>= 1) {
       const uint64_t t = __shfl_xor(g, o);
       g = t < g ? t : g;
     }
+    rmin = ord2f((uint32_t)(g >> 32));
+    float zs = 0.0f;
+#pragma unroll
+    for (int u = 0; u < PS_KPL; ++u)
+      if (key[u] != ~0ull) zs += __expf(rmin - ord2f((uint32_t)(key[u] >> 32)));
+    for (int o = 32; o >= 1; o >>= 1) zs += __shfl_xor(zs, o);
+    zsum = zs;
+  }
+  const int kk = mass > 0.0f ? avail : (nprobe < avail ? nprobe : avail);
+  int m = 0, rows = 0;
+  float cum = 0.0f;
+  for (int r = 0; r < kk && m < nprobe && !(mass > 0.0f && cum >= mass); ++r) {
+    uint64_t g = lmin;
+    for (int o = 32; o >= 1; o >>= 1) {
+      const uint64_t t = __shfl_xor(g, o);
+      g = t < g ? t : g;
+    }
+    if (mass > 0.0f) cum += __expf(rmin - ord2f((uint32_t)(g >> 32))) / zsum;
     if (lmin == g) {  // the owner removes it (keys are unique: bucket in the low bits)
       lmin = ~0ull;
 #pragma unroll
@@ -625,13 +646,13 @@ void launch_route_dist(const float* q32, const float* qn2, int nq, const float* 
                      R);
 }
 
-void launch_probe_select(const float* R, int nq, int nlist, int avail, int nprobe,
+void launch_probe_select(const float* R, int nq, int nlist, int avail, int nprobe, float mass,
                          const int* list_off, int* probes, int* pcount, int* prows,
                          hipStream_t st) {
   if (nq <= 0) return;
-  if (nprobe <= 64 && avail <= 64 * PS_KPL)
+  if (mass > 0.0f || (nprobe <= 64 && avail <= 64 * PS_KPL))
     hipLaunchKernelGGL(probe_select_wave_kernel, dim3((unsigned)ceil_div(nq, 4)), dim3(256), 0, st,
-                       R, nq, nlist, avail, nprobe, list_off, probes, pcount, prows);
+                       R, nq, nlist, avail, nprobe, mass, list_off, probes, pcount, prows);
   else
     hipLaunchKernelGGL(probe_select_kernel, dim3(nq), dim3(256), 0, st, R, nq, nlist, avail,
                        nprobe, list_off, probes, pcount, prows);

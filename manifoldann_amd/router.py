@@ -167,8 +167,26 @@ def neighbour_list_labels(index, data, train_queries, k=10):
     return bucket_of[ids]
 
 
-def query_arrays_routed(index, router: LearnedProbeRouter, Q, k, nprobe):
-    """IVF batched query with the router's routing (TopKRouting over -scores)."""
+def last_probe_count() -> int:
+    """(list, query) pairs scanned by the calling thread's last IVF search."""
+    out = ctypes.c_int64()
+    check(lib().mann_last_probe_count(ctypes.byref(out)))
+    return int(out.value)
+
+
+def probe_mass(index, route, nprobe_max, mass):
+    """Buckets chosen by probability-mass probing of negated logits `route`
+    (nq x nlist): nq x nprobe_max, 1-based, 0 = unused."""
+    r = np.ascontiguousarray(route, np.float32)
+    out = np.zeros((r.shape[0], nprobe_max), np.int64)
+    check(lib().mann_ivf_probe_mass(index.handle, r.ctypes.data, r.shape[0], nprobe_max,
+                                    float(mass), out.ctypes.data))
+    return out
+
+
+def query_arrays_routed(index, router: LearnedProbeRouter, Q, k, nprobe, mass=None):
+    """IVF batched query with the router's routing: TopKRouting over -scores,
+    or (mass given) probability-mass probing with at most nprobe lists."""
     m = as_matrix(Q)
     if m.device:
         import torch
@@ -179,6 +197,11 @@ def query_arrays_routed(index, router: LearnedProbeRouter, Q, k, nprobe):
         ids = np.zeros((m.n, k), np.int64)
         dist = np.zeros((m.n, k), np.float32)
         pi, pd = ids.ctypes.data, dist.ctypes.data
-    check(lib().mann_ivf_search_routed(index.handle, router.handle, m.ptr, m.d, m.n, m.ld, k,
-                                       nprobe, index.distance.code, pi, pd))
+    if mass is None:
+        check(lib().mann_ivf_search_routed(index.handle, router.handle, m.ptr, m.d, m.n, m.ld, k,
+                                           nprobe, index.distance.code, pi, pd))
+    else:
+        check(lib().mann_ivf_search_routed_mass(index.handle, router.handle, m.ptr, m.d, m.n,
+                                                m.ld, k, nprobe, float(mass),
+                                                index.distance.code, pi, pd))
     return ids, dist

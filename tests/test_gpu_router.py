@@ -138,3 +138,81 @@ def test_router_prior_training_gains_over_centroids():
           flush=True)
     assert gamma > 0
     assert l_trained < l_prior
+
+
+def _mass_reference(route, list_sizes, nprobe_max, mass):
+    """FP64 restatement of probability-mass probing: lists in ascending route
+    order (ties to the lower bucket), softmax of -route, stop once the mass is
+    reached or nprobe_max non-empty lists are taken."""
+    out = []
+    avail = min(len(list_sizes), np.asarray(route).shape[1])
+    for r in np.asarray(route, np.float64)[:, :avail]:
+        order = np.lexsort((np.arange(r.size), r))
+        p = np.exp(r.min() - r)
+        p /= p.sum()
+        cum, taken, cums = 0.0, [], []
+        for b in order:
+            if cum >= mass or len(taken) >= nprobe_max:
+                break
+            cum += p[b]
+            if list_sizes[b] > 0:
+                taken.append(b + 1)
+                cums.append(cum)
+        out.append((taken, cums))
+    return out
+
+
+def test_probability_mass_probing_matches_definition():
+    print("\n[gpu] probability-mass probing", flush=True)
+    X, C, a = _small_ivf_data()
+    index = I.build_index_from_assignment(X.T, C.T, a, I.TopKRouting(4))
+    rng = np.random.default_rng(12)
+    route = (3.0 * rng.standard_normal((300, C.shape[0]))).astype(np.float32)
+    for mass in (0.5, 0.9, 0.99):
+        got = RT.probe_mass(index, route, 16, mass)
+        ref = _mass_reference(route, index.list_sizes, 16, mass)
+        for q in range(route.shape[0]):
+            g = [int(b) for b in got[q] if b > 0]
+            taken, cums = ref[q]
+            # identical list order; counts may differ only where the FP32
+            # running sum sits within 1e-5 of the mass
+            n = min(len(g), len(taken))
+            assert g[:n] == taken[:n]
+            if len(g) != len(taken):
+                edge = cums[min(len(g), len(taken)) - 1]
+                assert abs(edge - mass) < 1e-5, (q, g, taken)
+
+
+def _small_ivf_data():
+    X = datasets.zipf_anisotropic_mixture(20_000, 32, components=30, seed=8)
+    C0 = X[np.random.default_rng(8).choice(20_000, 64, replace=False)]
+    oc, oa, _ = O.lloyd(X, C0, 5)
+    return X, oc, oa
+
+
+def test_mass_routed_ivf_search_matches_oracle():
+    print("\n[gpu] mass-routed ivf search", flush=True)
+    X, C, a = _small_ivf_data()
+    index = I.build_index_from_assignment(X.T, C.T, a, I.TopKRouting(6))
+    r = RT.LearnedProbeRouter(32, 64, hidden=64, seed=9)
+    r.set_prior(index, r.fit_prior(index, *_prior_training(X, index)))
+    Q = datasets.mixture_queries(200, 32, components=30, seed=8, query_seed=13)
+    ids, dist = RT.query_arrays_routed(index, r, Q.T, 10, 12, mass=0.9)
+    probes_total = RT.last_probe_count()
+    route = -r.scores(Q.T)
+    chosen = RT.probe_mass(index, route, 12, 0.9)
+    assert probes_total == int((chosen > 0).sum())
+    off, lids, nb = O.partition(a)
+    for q in range(Q.shape[0]):
+        lists = [int(b) - 1 for b in chosen[q] if b > 0]
+        rq = np.full((1, C.shape[0]), np.inf, np.float32)
+        rq[0, lists] = np.arange(len(lists), dtype=np.float32)
+        oi, od = O.ivf_search(X, off, lids, nb, Q[q:q + 1], rq, 10, len(lists))
+        assert (ids[q] == oi[0]).all(), q
+        assert (dist[q].view(np.uint32) == od[0].view(np.uint32)).all(), q
+
+
+def _prior_training(X, index):
+    Tq = datasets.mixture_queries(2000, 32, components=30, seed=8, query_seed=14)
+    labels = RT.neighbour_list_labels(index, X.T, Tq.T, k=10)
+    return Tq.T, labels
