@@ -193,15 +193,21 @@ int mann_ivf_search_routed_mass(mann_ivf_t ivf, mann_router_t r, const float* qu
                                 double mass, int metric, int64_t* out_ids, float* out_dist);
 /* (list, query) pairs scanned by the last IVF search of the calling thread */
 int mann_last_probe_count(int64_t* total);
+/* First-pass candidates the selection read in the last exact or IVF search of
+ * the calling thread (all passes).  No reference counterpart: a diagnostic of
+ * the emission threshold. */
+int mann_last_emitted(int64_t* total);
 
 /* Tuning knobs of the search kernels (process-wide): "scan_cfg", "ivf_cfg",
  * "kmeans_cfg" (first-pass tile of the exact kNN scan, the IVF list scan and
  * the k-means assignment: 0 = 128 rows x 128 queries, 1 = 256 x 128,
  * 2 = 128 x 256, 3 = 256 x 256 on 4 wavefronts, 4 and 5 = 256 x 256 on 8
- * wavefronts), "xcd_map" (grid order: 0 plain, 1 query-major per XCD,
+ * wavefronts, 6 and 7 = configs 5 and 0 with four LDS stages instead of
+ * three), "xcd_map" (grid order: 0 plain, 1 query-major per XCD,
  * 2 row-partition-major per XCD, 3 bands of 8 row partitions per XCD),
- * "target_groups" (workgroups the row partitioning aims for), "sample_rank"
- * (multiplier of the emission target of the exact-kNN threshold sample),
+ * "target_groups" (workgroups the row partitioning aims for), "emit_per_k"
+ * (exact kNN: first-pass emissions per query the threshold sample aims at, per
+ * neighbour requested), "sample_rank" (multiplier of that target),
  * "shortlist_mult" (first exact re-rank batch = that multiple of k),
  * "ivf_tau_mode" (IVF emission threshold: 0 = exact FP32 sample of the probed
  * rows, 1 = bf16 scan of each query's first list, the default), "tau_rows"

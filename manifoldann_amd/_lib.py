@@ -60,6 +60,7 @@ SIGNATURES = [
     ("mann_ivf_search_routed_mass", ctypes.c_int, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64,
                                                    ctypes.c_double, ctypes.c_int, _vp, _vp]),
     ("mann_last_probe_count", ctypes.c_int, [_i64p]),
+    ("mann_last_emitted", ctypes.c_int, [_i64p]),
     ("mann_tune_set", ctypes.c_int, [ctypes.c_char_p, _i64]),
     ("mann_tune_get", ctypes.c_int, [ctypes.c_char_p, _i64p]),
     ("mann_profile_enable", ctypes.c_int, [ctypes.c_int]),

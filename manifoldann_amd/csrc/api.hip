@@ -702,6 +702,13 @@ int mann_last_probe_count(int64_t* total) {
   });
 }
 
+int mann_last_emitted(int64_t* total) {
+  return guarded([&] {
+    MANN_REQUIRE(total != nullptr, ERR_NULL, "null pointer");
+    *total = last_stats().emitted;
+  });
+}
+
 int mann_tune_set(const char* key, int64_t value) {
   return guarded([&] {
     MANN_REQUIRE(key != nullptr, ERR_NULL, "null key");
@@ -732,6 +739,9 @@ int mann_tune_set(const char* key, int64_t value) {
     } else if (k == "shortlist_mult") {
       MANN_REQUIRE(value >= 1 && value <= 16, ERR_ARGUMENT, "shortlist_mult must be in 1..16");
       t.shortlist_mult = (int)value;
+    } else if (k == "emit_per_k") {
+      MANN_REQUIRE(value >= 1 && value <= 64, ERR_ARGUMENT, "emit_per_k must be in 1..64");
+      t.emit_per_k = (int)value;
     } else {
       throw Error{ERR_ARGUMENT, "unknown tuning key " + k};
     }
@@ -753,6 +763,7 @@ int mann_tune_get(const char* key, int64_t* value) {
     else if (k == "tau_rows") *value = t.tau_rows;
     else if (k == "ivf_tau_mode") *value = t.ivf_tau_mode;
     else if (k == "sample_rows") *value = t.sample_rows;
+    else if (k == "emit_per_k") *value = t.emit_per_k;
     else throw Error{ERR_ARGUMENT, "unknown tuning key " + k};
   });
 }

@@ -103,8 +103,11 @@ mann_dataset* create_dataset(const float* data, int64_t d, int64_t n, int64_t ld
 struct SearchStats {
   long long reranked = 0, overflow = 0, pass2 = 0, exact = 0;
   long long probes = 0;  // IVF: (list, query) pairs scanned
+  long long emitted = 0; // first-pass candidates read by the selection (all passes)
 };
 SearchStats& last_stats();
+// adds the selection counters of a kStatSlots x 4 host copy to s
+void add_stats(const int* slots, SearchStats& s);
 
 // Exact k nearest rows of `ds` for each query (bf16 MFMA first pass, certified
 // FP32 re-rank, refinement pass, exact fallback).  out_* are device pointers,

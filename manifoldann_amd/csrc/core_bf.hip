@@ -85,6 +85,14 @@ SearchStats& last_stats() {
   return s;
 }
 
+void add_stats(const int* slots, SearchStats& s) {
+  for (int i = 0; i < kStatSlots; ++i) {
+    s.reranked += slots[4 * i];
+    s.overflow += slots[4 * i + 1];
+    s.emitted += slots[4 * i + 2];
+  }
+}
+
 // ------------------------------------------------------------------ dataset --
 
 }  // namespace mann
@@ -267,10 +275,12 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   int* cnt = c.buf("bf_cnt").as<int>((size_t)nq * std::max(parts, parts2));
   float* tau = c.buf("bf_tau").as<float>(nq);
   float* tau2 = c.buf("bf_tau2").as<float>(nq);
-  int* flags = c.buf("bf_flags").as<int>(8);   // [0] pass1 count, [1] pass2 count, [2..] stats
+  int* flags = c.buf("bf_flags").as<int>(8);   // [0] pass1 count, [1] pass2 count
+  int* stats = c.buf("bf_stats").as<int>(kStatSlots * 4);
   int* list1 = c.buf("bf_list1").as<int>(nq);
   int* list2 = c.buf("bf_list2").as<int>(nq);
   fill_i32(flags, 8, 0, st);
+  fill_i32(stats, kStatSlots * 4, 0, st);
 
   // ---- emission threshold per query (every row fits its region: emit all)
   if (rpp <= cap) {
@@ -279,7 +289,7 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
     // threshold rank in the sample: the sample's r-th smallest value sits near
     // the (r n / m)-th smallest row; aim at about 15 k_req emissions per query,
     // bounded by half the candidate capacity
-    const int64_t target = std::min<int64_t>((int64_t)tuning().sample_rank * 15 * k_req,
+    const int64_t target = std::min<int64_t>((int64_t)tuning().sample_rank * tuning().emit_per_k * k_req,
                                              cap_total / 2);
     const int rank = (int)std::max<int64_t>(
         1, std::min<int64_t>(ds.m, (target * ds.m + n / 2) / n));
@@ -372,7 +382,7 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   s.flag_count = flags;
   s.flag_list = list1;
   s.tau_new = tau2;
-  s.stats = flags + 2;
+  s.stats = stats;
   {
     ScopedTime tm("select_rerank", st);
     launch_select_rerank(s, nq, st);
@@ -420,8 +430,9 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   e.out_dist = out_dist;
   e.out_count = out_count;
   MANN_REQUIRE(k_req <= exact_kmax(), ERR_UNSUPPORTED, "k too large for the exact fallback");
-  int hf[8];
+  int hf[8], hs[kStatSlots * 4];
   MANN_HIP(hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, st));
+  MANN_HIP(hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, st));
   MANN_HIP(hipStreamSynchronize(st));
   {
     ScopedTime tm("exact_fallback", st);
@@ -431,8 +442,7 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   MANN_HIP(hipGetLastError());
   S.pass2 = hf[0];
   S.exact = hf[1];
-  S.reranked = hf[2];
-  S.overflow = hf[3];
+  add_stats(hs, S);
 }
 
 }  // namespace mann

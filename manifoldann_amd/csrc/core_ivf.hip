@@ -180,11 +180,11 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
   // pass 2, [4] entries, [5] threshold-scan / refinement items, [6] their item
   // counter
   int* ctr = c.buf("iv_ctr").as<int>(8);
-  int* stats = c.buf("iv_stats").as<int>(4);
+  int* stats = c.buf("iv_stats").as<int>(kStatSlots * 4);
   fill_i32(qcount, nb, 0, st);
   fill_i32(qfill, nb, 0, st);
   fill_i32(ctr, 8, 0, st);
-  fill_i32(stats, 4, 0, st);
+  fill_i32(stats, kStatSlots * 4, 0, st);
   {
     ScopedTime tm("ivf_plan", st);
     launch_ivf_plan(probes, pcount, nq, nprobe, nb, iv.list_off, iv.lorder, BN, ROWS_ITEM, qcount,
@@ -358,7 +358,7 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
     ScopedTime tm("ivf_select", st);
     launch_select_rerank(s, nq, st);
   }
-  int h[8], hs[4];
+  int h[8], hs[kStatSlots * 4];
   MANN_HIP(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
   MANN_HIP(hipStreamSynchronize(st));
   S.probes = h[4];
@@ -426,8 +426,7 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
   MANN_HIP(hipGetLastError());
   S.pass2 = nflag;
   S.exact = nflag2;
-  S.reranked = hs[0];
-  S.overflow = hs[1];
+  add_stats(hs, S);
 }
 
 }  // namespace mann

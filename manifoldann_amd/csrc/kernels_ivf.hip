@@ -463,10 +463,13 @@ __device__ inline double metric_upper_bound(double S, int metric, double qe, dou
   return 0.5 * t * t + (2.0 * d + 16.0) * u;
 }
 
+// one wavefront per query, four per workgroup
 __global__ __launch_bounds__(256) void ivf_tau_dense_kernel(IvfDenseTauArgs a) {
-  const int q = blockIdx.x;
+  __shared__ uint32_t buf_all[4][256];
+  __shared__ unsigned hist_all[4][256];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int q = blockIdx.x * 4 + w;
   if (q >= a.nq) return;
-  __shared__ RadixScratch sm;
   const int np = a.pcount[q];
   const int total = a.prows[q];
   const int k_eff = a.k < total ? a.k : total;
@@ -475,14 +478,15 @@ __global__ __launch_bounds__(256) void ivf_tau_dense_kernel(IvfDenseTauArgs a) {
     const int b = a.probes[(int64_t)q * a.nprobe];
     m = a.list_off[b + 1] - a.list_off[b];
   }
-  if (threadIdx.x == 0) a.k_eff_q[q] = k_eff;
+  if (lane == 0) a.k_eff_q[q] = k_eff;
   if (k_eff <= 0 || total <= a.emit_all_below || m < k_eff) {
     // nothing to bound with (or everything fits): emit every probed row
-    if (threadIdx.x == 0) a.tau[q] = INFINITY;
+    if (lane == 0) a.tau[q] = INFINITY;
     return;
   }
-  const uint32_t key = select_rank_key(a.dense + (int64_t)q * a.ldd, m, k_eff, sm);
-  if (threadIdx.x == 0) {
+  const uint32_t key =
+      wave_rank_key(a.dense + (int64_t)q * a.ldd, m, k_eff, buf_all[w], hist_all[w], lane);
+  if (lane == 0) {
     const double dub = metric_upper_bound((double)ord2f(key), a.metric, a.qe[q], a.qr[q], a.ex,
                                           a.rx, a.d);
     const double T = exclusion_threshold(dub, a.metric, a.qe[q], a.qr[q], a.ex, a.rx, a.d);
@@ -715,7 +719,8 @@ void launch_first_items(const int* first_cnt, const int* qoff, int nb, const int
 
 void launch_ivf_tau_dense(const IvfDenseTauArgs& a, hipStream_t st) {
   if (a.nq <= 0) return;
-  hipLaunchKernelGGL(ivf_tau_dense_kernel, dim3(a.nq), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(ivf_tau_dense_kernel, dim3((unsigned)ceil_div(a.nq, 4)), dim3(256), 0, st,
+                     a);
 }
 
 void launch_ivf_tau(const IvfTauArgs& a, hipStream_t st) {

@@ -4,8 +4,8 @@
 // The workgroup streams the rows in BM-row tiles; per tile it computes the
 // BM x BN block of  a' = |x~|^2 - 2 q~.x~  (bf16 operands, FP32 accumulate;
 // a' + |q~|^2 is the squared distance of the bf16 vectors) with operands staged
-// through double-buffered LDS, then runs the epilogue of its mode on the
-// accumulators in registers:
+// through an LDS ring, then runs the epilogue of its mode on the accumulators
+// in registers:
 //   FILTER : emit (a', row) when a' < tau[q] into the query's private region
 //            of this item (no global atomics: counters live in LDS)
 //   DENSE  : store every a' (threshold estimation on a row sample)
@@ -16,7 +16,7 @@
 //            with one atomic per (item, query) at the end of the item
 // Only candidates cross HBM; the BM x BN block never leaves registers.
 //
-// Layout: 4 waves as 2 (rows) x 2 (queries); a wave owns 32*WM rows x 32*WN
+// Layout: WR x WC waves (rows x queries); a wave owns 32*WM rows x 32*WN
 // queries, i.e. WM x WN MFMA blocks.  MFMA 32x32 output: lane l holds query
 // column l%32 and rows 8*(e/4) + 4*(l/32) + e%4 of register e.
 #include "launch.h"
@@ -32,6 +32,8 @@ template <int N>
 __device__ __forceinline__ void wait_vm_lgkm() {
   __builtin_amdgcn_s_waitcnt((N & 15) | (((N >> 4) & 3) << 14) | (7 << 4));
 }
+// s_waitcnt lgkmcnt(0) only (vmcnt and expcnt left open)
+__device__ __forceinline__ void wait_lgkm0() { __builtin_amdgcn_s_waitcnt(0xC07F); }
 // Workgroup barrier without the release fence of __syncthreads (which would
 // drain every outstanding LDS DMA): callers wait for exactly the chunk they
 // need first.  The empty asm statements keep the compiler from moving LDS
@@ -72,19 +74,23 @@ __device__ float top1_threshold(float v, float qe, float qr, float ex, float rx,
 }
 
 // Block tile: WR x WC wavefronts, each owning 32*WM rows x 32*WN queries.
-template <int WM, int WN, int WR, int WC, int BK>
+template <int WM, int WN, int WR, int WC, int BK, int ST>
 struct Tile {
   static constexpr int NT = 64 * WR * WC;               // threads
   static constexpr int BM = 32 * WM * WR;
   static constexpr int BN = 32 * WN * WC;
   // LDS ring depth: chunks s+1 .. s+STAGES-1 are in flight while s is used
-  static constexpr int STAGES = 3;
+  static constexpr int STAGES = ST;
   static constexpr int SLOTS = BK / 8;                  // 16-byte chunks per LDS row
   static constexpr int A_CHUNKS = BM * SLOTS / NT;      // 16B chunks per thread
   static constexpr int B_CHUNKS = BN * SLOTS / NT;
   static constexpr int LROW = BK;
+  // |x~|^2 of the tile's rows, staged by the same DMA as chunk 0 of the tile,
+  // in a ring of NXB tiles (a slot is rewritten only after a barrier that
+  // follows the epilogue which read it, whatever the chunks per tile)
+  static constexpr int NXB = 4;
   static constexpr size_t LDS_BYTES =
-      (size_t)STAGES * (BM + BN) * LROW * sizeof(bf16_t) + BN * 4;
+      (size_t)STAGES * (BM + BN) * LROW * sizeof(bf16_t) + BN * 4 + (size_t)NXB * BM * 4;
   static constexpr int NPC = A_CHUNKS + B_CHUNKS;       // DMA ops per thread per chunk
   // GATHER mode: per query column a small LDS candidate buffer
   static constexpr int LCAP = 16;
@@ -94,21 +100,23 @@ struct Tile {
 // One work item.  Values are in the squared-distance domain:
 //   S^ = |x~|^2 - 2 q~.x~ + |q~|^2   (fma, then the query's norm added)
 // Loop nest: row tiles outside, K chunks inside, so the accumulators stay in
-// place across the K loop (no copies at a loop-carried merge).  Chunk s (of
-// the flattened (tile, K chunk) sequence) is multiplied out of LDS stage s%2
-// while the registers holding chunk s+1 go to the other stage (read last in
-// iteration s-1, before its barrier) and the global loads of chunk s+2 are in
-// flight.
-template <int WM, int WN, int WR, int WC, int BK, int MODE>
+// place across the K loop (no copies at a loop-carried merge).  Chunks of the
+// flattened (tile, K chunk) sequence stream through an LDS ring of STAGES
+// stages by direct global->LDS loads, STAGES-1 chunks ahead; within a chunk the
+// fragment reads of the next K step are in flight while the current one is
+// multiplied out, and the first step's reads of chunk s+1 follow the barrier
+// that ends chunk s.
+template <int WM, int WN, int WR, int WC, int BK, int ST, int MODE>
 __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, int row_begin,
                                           int row_end, int region, unsigned char* smem) {
-  using T = Tile<WM, WN, WR, WC, BK>;
+  using T = Tile<WM, WN, WR, WC, BK, ST>;
   constexpr int BM = T::BM, BN = T::BN, LROW = T::LROW, SL = T::SLOTS, STAGES = T::STAGES;
   constexpr int NT = T::NT;
   bf16_t* As = (bf16_t*)smem;                   // [STAGES][BM][LROW]
   bf16_t* Bs = As + STAGES * BM * LROW;         // [STAGES][BN][LROW]
   int* lcnt = (int*)(Bs + STAGES * BN * LROW);  // [BN] emitted count per query column
-  uint2* lbuf = (uint2*)(lcnt + BN);            // [BN][LCAP] (GATHER mode)
+  float* nxs = (float*)(lcnt + BN);             // [NXB][BM] row norms of the tiles
+  uint2* lbuf = (uint2*)(nxs + T::NXB * BM);    // [BN][LCAP] (GATHER mode)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave % WR, wc = wave / WR;
   const int kp = a.kp, nk = kp / BK;
@@ -151,7 +159,16 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
   // chunk (t, kc) -> LDS stage st by direct global->LDS loads: wave w's
   // instruction p fills 1 KiB at chunk slots [NT p + 64w, +64) (lane order)
   const int wv = __builtin_amdgcn_readfirstlane(wave);
+  // row norms: wave w stages rows (w mod BM/64)*64 + lane of the tile (one
+  // 4-byte DMA per lane; waves beyond BM/64 repeat the same values), with the
+  // row clamped into the item; the epilogue masks rows at or past row_end
+  const int nx_slot = (wv % (BM / 64)) * 64;
   auto dma = [&](int t, int kc, int st) {
+    if (kc == 0) {
+      const int row = min(row_begin + t * BM + nx_slot + lane, row_end - 1);
+      __builtin_amdgcn_global_load_lds((const void*)(a.xn + row),
+                                       (lds_void*)(nxs + (t % T::NXB) * BM + nx_slot), 4, 0, 0);
+    }
     const bf16_t* baseA = a.xb + (int64_t)(row_begin + t * BM) * kp + kc * BK;
     const bf16_t* baseB = a.qb + kc * BK;
 #pragma unroll
@@ -165,39 +182,51 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
                                        (lds_void*)(Bs + st * BN * LROW + (NT * p + 64 * wv) * 8),
                                        16, 0, 0);
   };
-  // wait until at most `later` chunks issued after the one needed next are in flight
+  // wait until at most `later` chunks issued after the one needed next are in
+  // flight (a chunk 0 carries one more op, the row norms: counting NPC per
+  // chunk can only wait longer, never shorter, since loads retire in order)
   auto wait_chunks = [&](int later) {
     constexpr int N = T::NPC;
+    static_assert(STAGES >= 3 && STAGES <= 5 && (STAGES - 2) * N <= 63, "vmcnt range");
     if (later <= 0) wait_vm_lgkm<0>();
-    else if (later == 1) wait_vm_lgkm<N>();
-    else wait_vm_lgkm<2 * N>();
+    else if (later == 1 || STAGES == 3) wait_vm_lgkm<N>();
+    else if (later == 2 || STAGES == 4) wait_vm_lgkm<2 * N>();
+    else wait_vm_lgkm<3 * N>();
   };
 
   floatx16 acc[WM][WN];
 
-  auto compute = [&](int buf) {
+  // Fragments of one 16-wide K step, double-buffered in registers: the reads
+  // of step ks+1 are in flight while step ks is multiplied out.
+  constexpr int NKS = BK / 16;
+  static_assert(NKS % 2 == 0, "K steps per chunk must be even");
+  bf16x8 F[2][WM + WN];
+  auto rd = [&](int b, int buf, int ks) {
     const bf16_t* Ab = As + buf * BM * LROW;
     const bf16_t* Bb = Bs + buf * BN * LROW;
+    const int kc = ks * 2 + (lane >> 5);  // logical 16-byte chunk of this lane
 #pragma unroll
-    for (int ks = 0; ks < BK / 16; ++ks) {
-      bf16x8 fa[WM], fb[WN];
-      const int kc = ks * 2 + (lane >> 5);  // logical 16-byte chunk of this lane
-#pragma unroll
-      for (int i = 0; i < WM; ++i) {
-        const int r = wr * 32 * WM + i * 32 + (lane & 31);
-        fa[i] = *(const bf16x8*)(Ab + r * LROW + ((kc ^ swz<SL>(r)) * 8));
-      }
-#pragma unroll
-      for (int j = 0; j < WN; ++j) {
-        const int r = wc * 32 * WN + j * 32 + (lane & 31);
-        fb[j] = *(const bf16x8*)(Bb + r * LROW + ((kc ^ swz<SL>(r)) * 8));
-      }
-#pragma unroll
-      for (int i = 0; i < WM; ++i)
-#pragma unroll
-        for (int j = 0; j < WN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    for (int i = 0; i < WM; ++i) {
+      const int r = wr * 32 * WM + i * 32 + (lane & 31);
+      F[b][i] = *(const bf16x8*)(Ab + r * LROW + ((kc ^ swz<SL>(r)) * 8));
     }
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+      const int r = wc * 32 * WN + j * 32 + (lane & 31);
+      F[b][WM + j] = *(const bf16x8*)(Bb + r * LROW + ((kc ^ swz<SL>(r)) * 8));
+    }
+  };
+  // zero: first step of a tile, accumulate onto 0 (no register clearing)
+  auto mm = [&](int b, bool zero) {
+    __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+      for (int j = 0; j < WN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(F[b][i], F[b][WM + j],
+                                                             zero ? floatx16{} : acc[i][j], 0,
+                                                             0, 0);
+    __builtin_amdgcn_s_setprio(0);
   };
 
   // Epilogue of one BM-row tile: per MFMA block, S^ from the accumulators,
@@ -207,11 +236,16 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
 #pragma unroll
     for (int i = 0; i < WM; ++i) {
       const int rb0 = r0 + wr * 32 * WM + i * 32 + 4 * (lane >> 5);
+      const float* nxt = nxs + (t % T::NXB) * BM + (rb0 - r0);
       float nx[16];
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int row = rb0 + 8 * (e >> 2) + (e & 3);
-        nx[e] = row < row_end ? a.xn[row] : INFINITY;
+      for (int e4 = 0; e4 < 4; ++e4) {
+        const floatx4 v4 = *(const floatx4*)(nxt + 8 * e4);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int row = rb0 + 8 * e4 + c;
+          nx[4 * e4 + c] = row < row_end ? v4[c] : INFINITY;
+        }
       }
 #pragma unroll
       for (int j = 0; j < WN; ++j) {
@@ -294,30 +328,50 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
   wait_chunks(issued - 1);
   __syncthreads();
   int s = 0, st0 = 0, sti = D;  // chunk s in stage st0; next issue goes to stage sti
-  for (int t = 0; t < ntiles; ++t) {
-#pragma unroll
-    for (int i = 0; i < WM; ++i)
-#pragma unroll
-      for (int j = 0; j < WN; ++j)
-#pragma unroll
-        for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
-    for (int kc = 0; kc < nk; ++kc, ++s) {
-      if (issued < S) {  // chunk s + D into the stage chunk s-1 used
-        dma(ti, ki, sti);
-        ++issued;
-        if (++ki == nk) {
-          ki = 0;
-          ++ti;
-        }
-        sti = sti == STAGES - 1 ? 0 : sti + 1;
+  // One chunk: K steps 0 .. NKS-1 out of stage st0.  The reads of the first
+  // step of the next chunk follow the barrier and overlap the last step's
+  // multiply-adds; for the last chunk of a tile they wait until after the
+  // epilogue, which needs the registers (after the last chunk of the item they
+  // read a stale stage, unused).
+  auto chunk = [&](bool first, bool last) __attribute__((always_inline)) {
+    if (issued < S) {  // chunk s + D into the stage chunk s-1 used
+      dma(ti, ki, sti);
+      ++issued;
+      if (++ki == nk) {
+        ki = 0;
+        ++ti;
       }
-      compute(st0);
-      // chunk s+1 must have landed; chunks s+2 .. issued-1 may stay in flight
-      wait_chunks(issued - (s + 2));
-      bare_barrier();
-      st0 = st0 == STAGES - 1 ? 0 : st0 + 1;
+      sti = sti == STAGES - 1 ? 0 : sti + 1;
+    }
+#pragma unroll
+    for (int ks = 0; ks + 1 < NKS; ++ks) {
+      // step ks's fragments (read a whole K step earlier) have landed; waiting
+      // here, before the next reads are issued, keeps the multiply-adds below
+      // from waiting on those
+      wait_lgkm0();
+      rd((ks + 1) & 1, st0, ks + 1);
+      mm(ks & 1, first && ks == 0);
+    }
+    // chunk s+1 must have landed; chunks s+2 .. issued-1 may stay in flight
+    wait_chunks(issued - (s + 2));
+    bare_barrier();
+    wait_lgkm0();  // no-op (waited above); tells the compiler step NKS-1 has landed
+    st0 = st0 == STAGES - 1 ? 0 : st0 + 1;
+    if (!last) rd(0, st0, 0);
+    mm((NKS - 1) & 1, false);
+    ++s;
+  };
+  rd(0, st0, 0);
+  for (int t = 0; t < ntiles; ++t) {
+    if (nk == 1) {
+      chunk(true, true);
+    } else {
+      chunk(true, false);
+      for (int kc = 1; kc + 1 < nk; ++kc) chunk(false, false);
+      chunk(false, true);
     }
     epilogue(t);
+    rd(0, st0, 0);
   }
 
   if (MODE == SCAN_GATHER) {
@@ -341,10 +395,10 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
 }
 
 // Wave tiles of at most 128 x 64 keep two waves per SIMD.
-template <int WM, int WN, int WR, int WC, int BK, int MODE, int REFINE>
+template <int WM, int WN, int WR, int WC, int BK, int ST, int MODE, int REFINE>
 __global__ __launch_bounds__(64 * WR * WC)
 __attribute__((amdgpu_waves_per_eu(WM * WN <= 8 ? 2 : 1))) void scan_grid_kernel(ScanArgs a) {
-  using T = Tile<WM, WN, WR, WC, BK>;
+  using T = Tile<WM, WN, WR, WC, BK, ST>;
   extern __shared__ __align__(16) unsigned char smem[];
   int nq = REFINE ? *a.nq_dev : a.nq;
   int bid = blockIdx.x;
@@ -375,10 +429,10 @@ __attribute__((amdgpu_waves_per_eu(WM * WN <= 8 ? 2 : 1))) void scan_grid_kernel
   int nqi = min(T::BN, nq - qoff);
   int rb = part * a.rows_per_part;
   int re = min(a.n_rows, rb + a.rows_per_part);
-  scan_item<WM, WN, WR, WC, BK, MODE>(a, qoff, nqi, rb, re, part, smem);
+  scan_item<WM, WN, WR, WC, BK, ST, MODE>(a, qoff, nqi, rb, re, part, smem);
 }
 
-template <int WM, int WN, int WR, int WC, int BK, int MODE>
+template <int WM, int WN, int WR, int WC, int BK, int ST, int MODE>
 __global__ __launch_bounds__(64 * WR * WC)
 __attribute__((amdgpu_waves_per_eu(WM * WN <= 8 ? 2 : 1))) void scan_items_kernel(ScanArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -391,7 +445,7 @@ __attribute__((amdgpu_waves_per_eu(WM * WN <= 8 ? 2 : 1))) void scan_items_kerne
     __syncthreads();
     if (it >= n_items) break;
     const ScanItem w = a.items[it];
-    scan_item<WM, WN, WR, WC, BK, MODE>(a, w.qoff, w.nq, w.row_begin, w.row_end, w.region, smem);
+    scan_item<WM, WN, WR, WC, BK, ST, MODE>(a, w.qoff, w.nq, w.row_begin, w.row_end, w.region, smem);
   }
 }
 
@@ -401,37 +455,38 @@ void set_lds(K kern, size_t lds) {
                                (int)lds));
 }
 
-template <int WM, int WN, int WR, int WC, int BK, int MODE>
+template <int WM, int WN, int WR, int WC, int BK, int ST, int MODE>
 void launch_t(const ScanArgs& a, int grid, int kind, hipStream_t st) {
-  using T = Tile<WM, WN, WR, WC, BK>;
+  using T = Tile<WM, WN, WR, WC, BK, ST>;
   const size_t lds = MODE == SCAN_GATHER ? T::LDS_GATHER : T::LDS_BYTES;
   static bool attr_set = false;  // dynamic LDS above 64 KiB must be opted into
   if (!attr_set) {
-    set_lds(scan_items_kernel<WM, WN, WR, WC, BK, MODE>, lds);
-    set_lds(scan_grid_kernel<WM, WN, WR, WC, BK, MODE, 0>, lds);
-    set_lds(scan_grid_kernel<WM, WN, WR, WC, BK, MODE, 1>, lds);
+    set_lds(scan_items_kernel<WM, WN, WR, WC, BK, ST, MODE>, lds);
+    set_lds(scan_grid_kernel<WM, WN, WR, WC, BK, ST, MODE, 0>, lds);
+    set_lds(scan_grid_kernel<WM, WN, WR, WC, BK, ST, MODE, 1>, lds);
     attr_set = true;
   }
   const dim3 block(T::NT);
   if (kind == SCAN_KIND_ITEMS)
-    hipLaunchKernelGGL((scan_items_kernel<WM, WN, WR, WC, BK, MODE>), dim3(grid), block, lds, st,
+    hipLaunchKernelGGL((scan_items_kernel<WM, WN, WR, WC, BK, ST, MODE>), dim3(grid), block, lds, st,
                        a);
   else if (kind == SCAN_KIND_REFINE)
-    hipLaunchKernelGGL((scan_grid_kernel<WM, WN, WR, WC, BK, MODE, 1>), dim3(grid), block, lds,
+    hipLaunchKernelGGL((scan_grid_kernel<WM, WN, WR, WC, BK, ST, MODE, 1>), dim3(grid), block, lds,
                        st, a);
   else
-    hipLaunchKernelGGL((scan_grid_kernel<WM, WN, WR, WC, BK, MODE, 0>), dim3(grid), block, lds,
+    hipLaunchKernelGGL((scan_grid_kernel<WM, WN, WR, WC, BK, ST, MODE, 0>), dim3(grid), block, lds,
                        st, a);
 }
 
 }  // namespace
 
 // Tile configurations: WR x WC wavefronts of 32*WM rows x 32*WN queries.
+// Configs 6 and 7 repeat 5 and 0 with a 4-deep LDS ring (3 chunks in flight).
 struct ScanCfg {
-  int wm, wn, wr, wc;
+  int wm, wn, wr, wc, stages;
 };
-static const ScanCfg kCfg[] = {{2, 2, 2, 2}, {4, 2, 2, 2}, {2, 4, 2, 2},
-                               {4, 4, 2, 2}, {4, 2, 2, 4}, {2, 4, 4, 2}};
+static const ScanCfg kCfg[] = {{2, 2, 2, 2, 3}, {4, 2, 2, 2, 3}, {2, 4, 2, 2, 3}, {4, 4, 2, 2, 3},
+                               {4, 2, 2, 4, 3}, {2, 4, 4, 2, 3}, {2, 4, 4, 2, 4}, {2, 2, 2, 2, 4}};
 constexpr int kNumCfg = sizeof(kCfg) / sizeof(kCfg[0]);
 
 int scan_num_cfg() { return kNumCfg; }
@@ -444,13 +499,13 @@ int scan_tile_queries(int cfg) {
   return 32 * c.wn * c.wc;
 }
 
-template <int WM, int WN, int WR, int WC>
+template <int WM, int WN, int WR, int WC, int ST>
 static void launch_cfg(const ScanArgs& a, int mode, int grid, int kind, hipStream_t st) {
   switch (mode) {
-    case SCAN_FILTER: launch_t<WM, WN, WR, WC, 32, SCAN_FILTER>(a, grid, kind, st); break;
-    case SCAN_DENSE: launch_t<WM, WN, WR, WC, 32, SCAN_DENSE>(a, grid, kind, st); break;
-    case SCAN_TOP1: launch_t<WM, WN, WR, WC, 32, SCAN_TOP1>(a, grid, kind, st); break;
-    case SCAN_GATHER: launch_t<WM, WN, WR, WC, 32, SCAN_GATHER>(a, grid, kind, st); break;
+    case SCAN_FILTER: launch_t<WM, WN, WR, WC, 32, ST, SCAN_FILTER>(a, grid, kind, st); break;
+    case SCAN_DENSE: launch_t<WM, WN, WR, WC, 32, ST, SCAN_DENSE>(a, grid, kind, st); break;
+    case SCAN_TOP1: launch_t<WM, WN, WR, WC, 32, ST, SCAN_TOP1>(a, grid, kind, st); break;
+    case SCAN_GATHER: launch_t<WM, WN, WR, WC, 32, ST, SCAN_GATHER>(a, grid, kind, st); break;
     default: break;
   }
 }
@@ -458,12 +513,14 @@ static void launch_cfg(const ScanArgs& a, int mode, int grid, int kind, hipStrea
 void launch_scan(const ScanArgs& a, int mode, int grid, int kind, hipStream_t st) {
   if (grid <= 0) return;
   switch (a.cfg) {
-    case 1: launch_cfg<4, 2, 2, 2>(a, mode, grid, kind, st); break;
-    case 2: launch_cfg<2, 4, 2, 2>(a, mode, grid, kind, st); break;
-    case 3: launch_cfg<4, 4, 2, 2>(a, mode, grid, kind, st); break;
-    case 4: launch_cfg<4, 2, 2, 4>(a, mode, grid, kind, st); break;
-    case 5: launch_cfg<2, 4, 4, 2>(a, mode, grid, kind, st); break;
-    default: launch_cfg<2, 2, 2, 2>(a, mode, grid, kind, st); break;
+    case 1: launch_cfg<4, 2, 2, 2, 3>(a, mode, grid, kind, st); break;
+    case 2: launch_cfg<2, 4, 2, 2, 3>(a, mode, grid, kind, st); break;
+    case 3: launch_cfg<4, 4, 2, 2, 3>(a, mode, grid, kind, st); break;
+    case 4: launch_cfg<4, 2, 2, 4, 3>(a, mode, grid, kind, st); break;
+    case 5: launch_cfg<2, 4, 4, 2, 3>(a, mode, grid, kind, st); break;
+    case 6: launch_cfg<2, 4, 4, 2, 4>(a, mode, grid, kind, st); break;
+    case 7: launch_cfg<2, 2, 2, 2, 4>(a, mode, grid, kind, st); break;
+    default: launch_cfg<2, 2, 2, 2, 3>(a, mode, grid, kind, st); break;
   }
 }
 

@@ -45,12 +45,16 @@ __device__ __forceinline__ int next_pow2(int x) {
 }
 
 // rank-th smallest of each query's m sample values (one block per query)
+// one wavefront per query, four per workgroup
 __global__ __launch_bounds__(256) void tau_from_sample_kernel(TauArgs a) {
-  const int q = blockIdx.x;
+  __shared__ uint32_t buf_all[4][256];
+  __shared__ unsigned hist_all[4][256];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int q = blockIdx.x * 4 + w;
   if (q >= a.nq) return;
-  __shared__ RadixScratch sm;
-  const uint32_t key = select_rank_key(a.dense + (int64_t)q * a.ldd, a.m, a.rank, sm);
-  if (threadIdx.x == 0) a.tau[q] = ord2f(key);
+  const uint32_t key =
+      wave_rank_key(a.dense + (int64_t)q * a.ldd, a.m, a.rank, buf_all[w], hist_all[w], lane);
+  if (lane == 0) a.tau[q] = ord2f(key);
 }
 
 // ---- wave-per-query selection ------------------------------------------------
@@ -339,8 +343,10 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(SelectArgs a) {
       a.tau_new[q] = tn;
     }
     if (a.stats) {
-      atomicAdd(a.stats, n);
-      if (bad) atomicAdd(a.stats + 1, 1);
+      int* sp = a.stats + (slot % kStatSlots) * 4;
+      atomicAdd(sp, n);
+      if (bad) atomicAdd(sp + 1, 1);
+      atomicAdd(sp + 2, c);
     }
   }
   const int lim = n < a.k_req ? n : a.k_req;
@@ -502,7 +508,8 @@ int exact_kmax() { return EX_KMAX; }
 
 void launch_tau_from_sample(const TauArgs& a, hipStream_t st) {
   if (a.nq <= 0) return;
-  hipLaunchKernelGGL(tau_from_sample_kernel, dim3(a.nq), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(tau_from_sample_kernel, dim3((unsigned)ceil_div(a.nq, 4)), dim3(256), 0, st,
+                     a);
 }
 
 void launch_select_rerank(const SelectArgs& a, int nq_max, hipStream_t st) {

@@ -65,11 +65,12 @@ struct Tuning {
   int xcd_map = 3;           // grid order: 0 plain, 1 query-major per XCD, 2 part-major per
                              // XCD, 3 bands of 8 partitions x query tiles per XCD
   int target_groups = 2048;  // workgroups the row partitioning aims for
-  int sample_rank = 1;       // threshold rank = sample_rank * k_req in the row sample
+  int sample_rank = 1;       // multiplier of the emission target below
+  int emit_per_k = 15;       // exact kNN: aim at emit_per_k * k_req first-pass emissions per query
   int shortlist_mult = 1;    // first re-rank batch = shortlist_mult * k_req (>= 32)
   int tau_rows = 1024;       // IVF: exact rows sampled per query for the emission threshold
   int ivf_tau_mode = 1;      // IVF threshold: 0 exact FP32 row sample, 1 bf16 scan of the first list
-  int sample_rows = 4096;    // exact kNN: stride-sample rows (fixed when a dataset is created)
+  int sample_rows = 2048;    // exact kNN: stride-sample rows (fixed when a dataset is created)
 };
 Tuning& tuning();
 
@@ -99,6 +100,11 @@ struct TauArgs {
   int nq;
 };
 void launch_tau_from_sample(const TauArgs& a, hipStream_t st);
+
+// Search statistics are accumulated by atomics spread over kStatSlots slots of
+// 4 counters each (query slot % kStatSlots), so that same-address atomics do
+// not serialize the selection; readers sum the slots.
+constexpr int kStatSlots = 64;
 
 struct SelectArgs {
   // queries processed: blockIdx -> qlist[blockIdx] (or identity), count on device
@@ -138,7 +144,7 @@ struct SelectArgs {
   int* flag_count;     // queries that failed certification
   int* flag_list;
   float* tau_new;      // refined threshold for a second pass
-  int* stats;          // [0] reranked candidates total, [1] overflow count
+  int* stats;          // [kStatSlots][4]: reranked, overflowed queries, emitted (see below)
 };
 enum ExtraMetric : int { NORMS_FORM = 3 };
 // one wavefront per query; nq_max bounds the device-side query count

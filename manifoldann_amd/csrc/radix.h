@@ -1,4 +1,4 @@
-// Block-wide (256 threads) rank selection on float order keys, shared by the
+// Rank selection on float order keys by one wavefront, shared by the
 // threshold kernels.
 #pragma once
 
@@ -6,124 +6,147 @@
 
 namespace mann {
 
-struct RadixScratch {
-  unsigned hist[256];
-  unsigned sel, want, cnt;
-  uint32_t buf[4096];
-};
+// ---- one wavefront per row ---------------------------------------------------
 
-// rank-th smallest of 256 x 16 order keys held in registers (4-pass radix
-// select over the whole block; all threads return the same key)
-__device__ inline uint32_t block_radix_select(const uint32_t (&v)[16], unsigned rank, unsigned* hist,
-                                       unsigned* s_sel, unsigned* s_want) {
-  uint32_t prefix = 0, mask = 0;
-  if (threadIdx.x == 0) *s_want = rank;
-  for (int shift = 24; shift >= 0; shift -= 8) {
-    hist[threadIdx.x] = 0;
-    __syncthreads();
+// Bitonic sort (ascending) of 64 R keys across the wavefront: element e lives
+// in lane e % 64, register e / 64.
+template <int R>
+__device__ __forceinline__ void wave_sort_u32(uint32_t (&v)[R], int lane) {
+  constexpr int N = 64 * R;
 #pragma unroll
-    for (int p = 0; p < 16; ++p)
-      if ((v[p] & mask) == prefix) atomicAdd(&hist[(v[p] >> shift) & 255u], 1u);
-    __syncthreads();
-    if (threadIdx.x < 64) {
-      const int l = threadIdx.x;
-      unsigned h0 = hist[4 * l], h1 = hist[4 * l + 1], h2 = hist[4 * l + 2], h3 = hist[4 * l + 3];
-      unsigned tot = h0 + h1 + h2 + h3, inc = tot;
-      for (int o = 1; o < 64; o <<= 1) {
-        unsigned t = __shfl_up(inc, o);
-        if (l >= o) inc += t;
-      }
-      unsigned excl = inc - tot, want = *s_want;
-      if (excl < want && inc >= want) {
-        unsigned c = excl, b = 4 * l;
-        if (c + h0 >= want) b = 4 * l;
-        else if (c + h0 + h1 >= want) { b = 4 * l + 1; c += h0; }
-        else if (c + h0 + h1 + h2 >= want) { b = 4 * l + 2; c += h0 + h1; }
-        else { b = 4 * l + 3; c += h0 + h1 + h2; }
-        *s_sel = b;
-        *s_want = want - c;
+  for (int size = 2; size <= N; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride >= 64) {
+        const int sr = stride >> 6;
+#pragma unroll
+        for (int jj = 0; jj < R; ++jj) {
+          if (jj & sr) continue;
+          const bool asc = ((jj * 64 + lane) & size) == 0;
+          const uint32_t lo = v[jj], hi = v[jj | sr];
+          if ((lo > hi) == asc) {
+            v[jj] = hi;
+            v[jj | sr] = lo;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int jj = 0; jj < R; ++jj) {
+          const uint32_t o = (uint32_t)__shfl_xor((int)v[jj], stride);
+          const bool asc = ((jj * 64 + lane) & size) == 0;
+          const bool lower = (lane & stride) == 0;
+          v[jj] = (lower == asc) ? min(v[jj], o) : max(v[jj], o);
+        }
       }
     }
-    __syncthreads();
-    prefix |= *s_sel << shift;
-    mask |= 255u << shift;
-    __syncthreads();
   }
-  return prefix;
 }
 
-// rank-th smallest (1-based) of m floats at row[0..m) as an order key.
-// m <= 4096: one register radix select.  Larger m: U = rank-th smallest of the
-// first 4096 values bounds the answer from above; one pass collects the values
-// <= U into LDS (about rank * m / 4096 of them) and a second register select
-// finishes; if more than 4096 qualify, a 4-pass select over all m runs instead.
-__device__ inline uint32_t select_rank_key(const float* row, int m, int rank, RadixScratch& sm) {
-  constexpr int PER = 16, CAPL = 256 * PER;
-  uint32_t v[PER];
+__device__ __forceinline__ void wave_fence_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// rank-th smallest (1-based, clamped to [1, m]) of the m floats row[0..m) as
+// an order key, by one wavefront; buf (256 keys) and hist (256 counters) are
+// the wavefront's LDS.  For rank <= 64 the rank-th smallest of the 64 lane
+// minima bounds the answer from above (they are rank distinct elements); the
+// values at or below that bound (usually a few more than rank) are collected
+// and sorted.  Otherwise, or when more than 256 qualify: a 4-pass radix select
+// over all m values.
+__device__ inline uint32_t wave_rank_key(const float* row, int m, int rank, uint32_t* buf,
+                                         unsigned* hist, int lane) {
+  rank = rank < 1 ? 1 : (rank > m ? m : rank);
+  if (rank <= 64) {
+    // rows of up to 4096 values stay in registers between the two passes
+    constexpr int P = 64;
+    const bool cached = m <= 64 * P;
+    uint32_t o[P];
+    uint32_t mn[1] = {0xffffffffu};
+    if (cached) {
 #pragma unroll
-  for (int p = 0; p < PER; ++p) {
-    int i = threadIdx.x + 256 * p;
-    v[p] = i < m ? f2ord(row[i]) : 0xffffffffu;
-  }
-  const unsigned r0 = (unsigned)min(rank, min(m, CAPL));
-  uint32_t key = block_radix_select(v, r0, sm.hist, &sm.sel, &sm.want);
-  if (m <= CAPL) return key;
-  const uint32_t U = key;
-  if (threadIdx.x == 0) sm.cnt = 0;
-  __syncthreads();
-  for (int i = threadIdx.x; i < m; i += 256) {
-    const uint32_t o = f2ord(row[i]);
-    if (o <= U) {
-      const unsigned pos = atomicAdd(&sm.cnt, 1u);
-      if (pos < (unsigned)CAPL) sm.buf[pos] = o;
+      for (int t = 0; t < P; ++t) {
+        const int i = t * 64 + lane;
+        o[t] = i < m ? f2ord(row[i]) : 0xffffffffu;
+        mn[0] = min(mn[0], o[t]);
+      }
+    } else {
+      for (int i = lane; i < m; i += 64) mn[0] = min(mn[0], f2ord(row[i]));
+    }
+    wave_sort_u32<1>(mn, lane);
+    const uint32_t U = (uint32_t)__shfl((int)mn[0], rank - 1);
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    int cnt = 0;
+    auto take = [&](bool p, uint32_t v) {
+      const unsigned long long b = __ballot(p);
+      const int pos = cnt + __popcll(b & lt);
+      if (p && pos < 256) buf[pos] = v;
+      cnt += __popcll(b);
+    };
+    if (cached) {
+      // sentinels (0xffffffff) lie above every finite or infinite value's key
+#pragma unroll
+      for (int t = 0; t < P; ++t)
+        if (t * 64 < m) take(o[t] <= U, o[t]);
+    } else {
+      for (int i0 = 0; i0 < m; i0 += 64) {
+        const int i = i0 + lane;
+        const uint32_t v = i < m ? f2ord(row[i]) : 0xffffffffu;
+        take(i < m && v <= U, v);
+      }
+    }
+    if (cnt <= 256) {
+      wave_fence_sync();
+      uint32_t v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = j * 64 + lane < cnt ? buf[j * 64 + lane] : 0xffffffffu;
+      wave_fence_sync();
+      wave_sort_u32<4>(v, lane);
+      const int e = rank - 1;
+      uint32_t r = v[0];
+#pragma unroll
+      for (int j = 1; j < 4; ++j)
+        if (e / 64 == j) r = v[j];
+      return (uint32_t)__shfl((int)r, e % 64);
     }
   }
-  __syncthreads();
-  const unsigned cnt = sm.cnt;
-  if (cnt <= (unsigned)CAPL && cnt >= (unsigned)rank) {
-#pragma unroll
-    for (int p = 0; p < PER; ++p) {
-      const unsigned i = threadIdx.x + 256 * p;
-      v[p] = i < cnt ? sm.buf[i] : 0xffffffffu;
-    }
-    __syncthreads();
-    return block_radix_select(v, (unsigned)rank, sm.hist, &sm.sel, &sm.want);
-  }
-  // rare: too many values at or below U; 4-pass select over all of them
   uint32_t prefix = 0, mask = 0;
-  if (threadIdx.x == 0) sm.want = (unsigned)rank;
+  unsigned want = (unsigned)rank;
   for (int shift = 24; shift >= 0; shift -= 8) {
-    sm.hist[threadIdx.x] = 0;
-    __syncthreads();
-    for (int i = threadIdx.x; i < m; i += 256) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) hist[4 * lane + b] = 0;
+    wave_fence_sync();
+    for (int i = lane; i < m; i += 64) {
       const uint32_t o = f2ord(row[i]);
-      if ((o & mask) == prefix) atomicAdd(&sm.hist[(o >> shift) & 255u], 1u);
+      if ((o & mask) == prefix) atomicAdd(&hist[(o >> shift) & 255u], 1u);
     }
-    __syncthreads();
-    if (threadIdx.x < 64) {
-      const int l = threadIdx.x;
-      unsigned h0 = sm.hist[4 * l], h1 = sm.hist[4 * l + 1], h2 = sm.hist[4 * l + 2],
-               h3 = sm.hist[4 * l + 3];
-      unsigned tot = h0 + h1 + h2 + h3, inc = tot;
-      for (int o = 1; o < 64; o <<= 1) {
-        unsigned t = __shfl_up(inc, o);
-        if (l >= o) inc += t;
-      }
-      unsigned excl = inc - tot, want = sm.want;
-      if (excl < want && inc >= want) {
-        unsigned c = excl, b = 4 * l;
-        if (c + h0 >= want) b = 4 * l;
-        else if (c + h0 + h1 >= want) { b = 4 * l + 1; c += h0; }
-        else if (c + h0 + h1 + h2 >= want) { b = 4 * l + 2; c += h0 + h1; }
-        else { b = 4 * l + 3; c += h0 + h1 + h2; }
-        sm.sel = b;
-        sm.want = want - c;
-      }
+    wave_fence_sync();
+    const unsigned h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2],
+                   h3 = hist[4 * lane + 3];
+    const unsigned tot = h0 + h1 + h2 + h3;
+    unsigned inc = tot;
+    for (int o = 1; o < 64; o <<= 1) {
+      const unsigned t = __shfl_up(inc, o);
+      if (lane >= o) inc += t;
     }
-    __syncthreads();
-    prefix |= sm.sel << shift;
+    const unsigned excl = inc - tot;
+    const bool here = excl < want && inc >= want;
+    const unsigned long long hm = __ballot(here);
+    const int src = __ffsll((long long)hm) - 1;
+    unsigned bin = 0, nw = 0;
+    if (here) {
+      const unsigned c = excl;
+      if (c + h0 >= want) { bin = 4 * lane; nw = want - c; }
+      else if (c + h0 + h1 >= want) { bin = 4 * lane + 1; nw = want - c - h0; }
+      else if (c + h0 + h1 + h2 >= want) { bin = 4 * lane + 2; nw = want - c - h0 - h1; }
+      else { bin = 4 * lane + 3; nw = want - c - h0 - h1 - h2; }
+    }
+    bin = (unsigned)__shfl((int)bin, src);
+    want = (unsigned)__shfl((int)nw, src);
+    prefix |= bin << shift;
     mask |= 255u << shift;
-    __syncthreads();
+    wave_fence_sync();
   }
   return prefix;
 }

@@ -283,6 +283,13 @@ def last_stats():
     return list(out)
 
 
+def last_emitted():
+    """First-pass candidates the selection read in the last search (all passes)."""
+    out = ctypes.c_int64()
+    check(lib().mann_last_emitted(ctypes.byref(out)))
+    return out.value
+
+
 # ---------------------------------------------------------------------------
 # kNN graph (graphs/knn_graph.jl)
 # ---------------------------------------------------------------------------
