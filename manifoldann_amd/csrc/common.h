@@ -71,11 +71,20 @@ __device__ __forceinline__ uint64_t pack_key(float f, uint32_t r) {
 
 // Sum over a 16-lane group with the fixed tree (xor 8, 4, 2, 1): lane j of the
 // group ends with ((a0+a8)+(a4+a12))+... exactly the A16 tree of the oracle.
+// DPP row rotate right by S within each 16-lane row: lane i reads lane (i - S) mod 16
+template <int S>
+__device__ __forceinline__ float row_ror(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x120 + S, 0xF, 0xF,
+                                                    false));
+}
+// Steps +8, +4, +2, +1 by row rotations: after the step of S the values are
+// S-periodic within the row, so lane i adds the same pair as lane i ^ S would
+// (a + b == b + a exactly) and every lane ends with the tree's sum.
 __device__ __forceinline__ float group16_tree(float v) {
-  v = v + __shfl_xor(v, 8, 16);
-  v = v + __shfl_xor(v, 4, 16);
-  v = v + __shfl_xor(v, 2, 16);
-  v = v + __shfl_xor(v, 1, 16);
+  v = v + row_ror<8>(v);
+  v = v + row_ror<4>(v);
+  v = v + row_ror<2>(v);
+  v = v + row_ror<1>(v);
   return v;
 }
 

@@ -20,7 +20,7 @@ namespace {
 // staged in chunks of RT_DC floats (the chains continue across chunks, order
 // unchanged).
 constexpr int RT_T = 32;
-constexpr int RT_DC = 256;
+constexpr int RT_DC = 64;
 constexpr int RT_LD = RT_DC + 16;  // row stride: adjacent rows 16 banks apart
 __global__ __launch_bounds__(256) void route_dist_kernel(const float* __restrict__ q32,
                                                          const float* __restrict__ qn2, int nq,
