@@ -211,7 +211,10 @@ int mann_last_emitted(int64_t* total);
  * "shortlist_mult" (first exact re-rank batch = that multiple of k),
  * "ivf_tau_mode" (IVF emission threshold: 0 = exact FP32 sample of the probed
  * rows, 1 = bf16 scan of each query's first list, the default), "tau_rows"
- * (rows of that exact sample, 16..1024), "sample_rows" (exact kNN: minimum
+ * (rows of that exact sample, 16..1024), "fold_norms" (exact kNN: 1 = when a
+ * dataset has 6 spare padded columns, carry the norms there so that the MFMA
+ * produces the first-pass distances itself; read when a dataset is created),
+ * "sample_rows" (exact kNN: minimum
  * rows of the stride sample a dataset keeps for the emission threshold; read
  * when the dataset is created).  No reference counterpart: the reference has
  * no tunable kernels. */

@@ -739,6 +739,9 @@ int mann_tune_set(const char* key, int64_t value) {
     } else if (k == "shortlist_mult") {
       MANN_REQUIRE(value >= 1 && value <= 16, ERR_ARGUMENT, "shortlist_mult must be in 1..16");
       t.shortlist_mult = (int)value;
+    } else if (k == "fold_norms") {
+      MANN_REQUIRE(value == 0 || value == 1, ERR_ARGUMENT, "fold_norms must be 0 or 1");
+      t.fold_norms = (int)value;
     } else if (k == "emit_per_k") {
       MANN_REQUIRE(value >= 1 && value <= 64, ERR_ARGUMENT, "emit_per_k must be in 1..64");
       t.emit_per_k = (int)value;
@@ -764,6 +767,7 @@ int mann_tune_get(const char* key, int64_t* value) {
     else if (k == "ivf_tau_mode") *value = t.ivf_tau_mode;
     else if (k == "sample_rows") *value = t.sample_rows;
     else if (k == "emit_per_k") *value = t.emit_per_k;
+    else if (k == "fold_norms") *value = t.fold_norms;
     else throw Error{ERR_ARGUMENT, "unknown tuning key " + k};
   });
 }

@@ -71,6 +71,13 @@ __device__ __forceinline__ uint64_t pack_key(float f, uint32_t r) {
 
 // Sum over a 16-lane group with the fixed tree (xor 8, 4, 2, 1): lane j of the
 // group ends with ((a0+a8)+(a4+a12))+... exactly the A16 tree of the oracle.
+// LDS writes of a wavefront visible to its own later reads (other lanes)
+__device__ __forceinline__ void wave_fence_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // DPP row rotate right by S within each 16-lane row: lane i reads lane (i - S) mod 16
 template <int S>
 __device__ __forceinline__ float row_ror(float v) {

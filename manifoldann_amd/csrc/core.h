@@ -22,6 +22,9 @@ struct mann_dataset {
   int m = 0, m_pad = 0;
   mann::bf16_t* sb = nullptr;
   float* sxn = nullptr;
+  // columns d..d+2 of xb (and sb) hold |x~|^2 as three bf16 parts summing to
+  // it exactly, d+3..d+5 hold 1 (first pass with folded norms)
+  int fold = 0;
   ~mann_dataset();
 };
 

@@ -149,6 +149,10 @@ mann_dataset* create_dataset(const float* data, int64_t d, int64_t n, int64_t ld
     launch_prep_rows(src, ld, n, ds->n_pad, (int)d, (int)ds->d16, (int)ds->kp, ds->normalized,
                      ds->x32, ds->xb, ds->xn, ds->err, ds->nrm, mx, mx + 1, st, ds->center);
     launch_a16_norms(ds->x32, n, (int)ds->d16, ds->xn2, st);
+    if (tuning().fold_norms && ds->kp - d >= 6) {
+      launch_fold_base(ds->xb, ds->xn, n, (int)ds->kp, (int)d, st);
+      ds->fold = 1;
+    }
     // stride sample: rows floor(i * n / m)
     // at least n / 40 rows, so that the sample's order statistics stay close to
     // the k-th neighbour distance for large n
@@ -247,6 +251,14 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   S = SearchStats();
   const int nq = (int)Q.nq;
   if (nq == 0 || k <= 0) return;
+  // folded-norm query operand: -2 q~ and the parts of |q~|^2 (see mann_dataset)
+  const bf16_t* qop = Q.qb;
+  if (ds.fold) {
+    const int64_t nq_pad = round_up(Q.nq, 256);
+    bf16_t* qf = c.buf("bf_qfold").as<bf16_t>((size_t)nq_pad * ds.kp);
+    launch_fold_queries(Q.qb, Q.qn, Q.nq, nq_pad, (int)ds.kp, (int)ds.d, qf, st);
+    qop = qf;
+  }
   const int n = (int)ds.n;
   const int k_req = k + (exclude_self ? 1 : 0);
   const int k_eff = std::min(k_req, n);
@@ -303,8 +315,9 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
       a.cfg = cfg;
       a.xb = ds.sb;
       a.xn = ds.sxn;
-      a.qb = Q.qb + (int64_t)q0 * ds.kp;
+      a.qb = qop + (int64_t)q0 * ds.kp;
       a.qnorm = Q.qn + q0;
+      a.fold = ds.fold;
       a.kp = (int)ds.kp;
       a.d = (int)ds.d;
       a.nq = nqc;
@@ -334,8 +347,9 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   a.cfg = cfg;
   a.xb = ds.xb;
   a.xn = ds.xn;
-  a.qb = Q.qb;
+  a.qb = qop;
   a.qnorm = Q.qn;
+  a.fold = ds.fold;
   a.xcd_map = tuning().xcd_map;
   a.qtiles = qtiles;
   a.kp = (int)ds.kp;
@@ -364,7 +378,9 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   s.q32 = Q.q32;
   s.xn2 = ds.xn2;
   s.qn2 = Q.qn2;
-  s.d = (int)ds.d;
+  // certification band: with folded norms the MFMA sums d + 6 terms (the
+  // bound of d + 3 covers them and the FP32 rounding of the two norms)
+  s.d = (int)ds.d + (ds.fold ? 3 : 0);
   s.d16 = (int)ds.d16;
   s.metric = metric;
   s.qe = Q.qe;

@@ -89,6 +89,63 @@ __global__ __launch_bounds__(256) void a16_norms_kernel(const float* __restrict_
   if (row < n && j == 0) out[row] = acc;
 }
 
+// v = h1 + h2 + h3 exactly, each a bf16 (round to nearest): h1 keeps the top
+// 8 significant bits, the FP32 remainder v - h1 is exact and has at most 16,
+// the next at most 8, so h3 represents the last remainder exactly.
+__device__ __forceinline__ void split3_bf16(float v, bf16_t* o) {
+  const bf16_t h1 = (bf16_t)v;
+  const float r1 = v - (float)h1;
+  const bf16_t h2 = (bf16_t)r1;
+  const float r2 = r1 - (float)h2;
+  o[0] = h1;
+  o[1] = h2;
+  o[2] = (bf16_t)r2;
+}
+
+__global__ __launch_bounds__(256) void fold_base_kernel(bf16_t* __restrict__ xb,
+                                                        const float* __restrict__ xn, int64_t n,
+                                                        int kp, int d) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  bf16_t* row = xb + r * kp + d;
+  split3_bf16(xn[r], row);
+  row[3] = row[4] = row[5] = (bf16_t)1.0f;
+}
+
+// out row q: -2 q~ (exact), then 1, 1, 1, then the parts of |q~|^2; padding
+// rows (q >= nq, up to nq_pad) zero
+__global__ __launch_bounds__(256) void fold_queries_kernel(const bf16_t* __restrict__ qb,
+                                                           const float* __restrict__ qn,
+                                                           int64_t nq, int64_t nq_pad, int kp,
+                                                           int d, bf16_t* __restrict__ out) {
+  const int64_t q = blockIdx.x;
+  if (q >= nq_pad) return;
+  bf16_t* o = out + q * kp;
+  for (int c = threadIdx.x; c < kp; c += 256) {
+    float v = 0.0f;
+    if (q < nq) {
+      if (c < d) v = -2.0f * (float)qb[q * kp + c];
+      else if (c < d + 3) v = 1.0f;
+      else if (c < d + 6) continue;  // the norm's parts, written below
+    }
+    o[c] = (bf16_t)v;
+  }
+  if (q < nq && threadIdx.x == 0) split3_bf16(qn[q], o + d + 3);
+}
+
+void launch_fold_base(bf16_t* xb, const float* xn, int64_t n, int kp, int d, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fold_base_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, xb, xn,
+                     n, kp, d);
+}
+
+void launch_fold_queries(const bf16_t* qb, const float* qn, int64_t nq, int64_t nq_pad, int kp,
+                         int d, bf16_t* out, hipStream_t st) {
+  if (nq_pad <= 0) return;
+  hipLaunchKernelGGL(fold_queries_kernel, dim3((unsigned)nq_pad), dim3(256), 0, st, qb, qn, nq,
+                     nq_pad, kp, d, out);
+}
+
 void launch_prep_rows(const float* src, int64_t ld, int64_t n, int64_t n_pad, int d, int d16,
                       int kp, int normalize, float* x32, bf16_t* xb, float* xn, float* err,
                       float* nrm, unsigned* max_err, unsigned* max_nrm, hipStream_t st,

@@ -126,11 +126,15 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
   const int* qidx = a.qidx;
   // Staging: thread chunk c = tid + NT p lands at LDS row c/SL, slot c%SL (a
   // contiguous 16-byte write); it carries logical K chunk slot ^ swz(row).
-  int offA[T::A_CHUNKS], offB[T::B_CHUNKS];
+  // byte offsets: within a row tile 32 bits suffice (a uniform base plus a
+  // 32-bit lane offset, the scalar-base form of the DMA); query rows can lie
+  // anywhere in a large operand
+  unsigned offA[T::A_CHUNKS];
+  int64_t offB[T::B_CHUNKS];
 #pragma unroll
   for (int p = 0; p < T::A_CHUNKS; ++p) {
     const int c = tid + NT * p, r = c / SL;
-    offA[p] = r * kp + (((c % SL) ^ swz<SL>(r)) * 8);
+    offA[p] = (unsigned)(r * kp + (((c % SL) ^ swz<SL>(r)) * 8)) * 2u;
   }
 #pragma unroll
   for (int p = 0; p < T::B_CHUNKS; ++p) {
@@ -140,16 +144,15 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
       brow = qoff + (r < nqi ? r : 0);
     else
       brow = r < nqi ? (qidx ? qidx[qoff + r] : qoff + r) : (qidx ? qidx[qoff] : qoff);
-    offB[p] = brow * kp + (((c % SL) ^ swz<SL>(r)) * 8);
+    offB[p] = ((int64_t)brow * kp + (((c % SL) ^ swz<SL>(r)) * 8)) * 2;
   }
-  int gqc[WN], qcl[WN];
+  int gqc[WN];
   float tau_l[WN], qo[WN];
 #pragma unroll
   for (int bn = 0; bn < WN; ++bn) {
     int c = wc * 32 * WN + bn * 32 + (lane & 31);
     int gq = c < nqi ? (qidx ? qidx[qoff + c] : qoff + c) : -1;
     gqc[bn] = gq;
-    qcl[bn] = c;
     tau_l[bn] = (MODE != SCAN_DENSE && gq >= 0) ? a.tau[gq] : -INFINITY;
     float o = 0.0f;
     if (gq >= 0) o = a.col_norm ? a.col_norm[qoff + c] : (a.qnorm ? a.qnorm[gq] : 0.0f);
@@ -160,25 +163,32 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
   // instruction p fills 1 KiB at chunk slots [NT p + 64w, +64) (lane order)
   const int wv = __builtin_amdgcn_readfirstlane(wave);
   // row norms: wave w stages rows (w mod BM/64)*64 + lane of the tile (one
-  // 4-byte DMA per lane; waves beyond BM/64 repeat the same values), with the
-  // row clamped into the item; the epilogue masks rows at or past row_end
+  // 4-byte DMA per lane; waves beyond BM/64 repeat the same values).  Like the
+  // row operand, the norms of a whole tile are read: every norm array has at
+  // least 256 padding rows past its last row (+inf), and the epilogue masks
+  // rows at or past row_end.
+  // (A buffer load: scalar row offset + 32-bit lane offset, so no 64-bit
+  // per-lane address is held across the loop.)
   const int nx_slot = (wv % (BM / 64)) * 64;
+  const __amdgpu_buffer_rsrc_t nx_rsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void*)a.xn, 0, 0xffffffffu, 0x00020000);
   auto dma = [&](int t, int kc, int st) {
-    if (kc == 0) {
-      const int row = min(row_begin + t * BM + nx_slot + lane, row_end - 1);
-      __builtin_amdgcn_global_load_lds((const void*)(a.xn + row),
-                                       (lds_void*)(nxs + (t % T::NXB) * BM + nx_slot), 4, 0, 0);
-    }
+    if (kc == 0 && !a.fold)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(nx_rsrc,
+                                               (lds_void*)(nxs + (t % T::NXB) * BM + nx_slot), 4,
+                                               lane * 4,
+                                               (int)((unsigned)(row_begin + t * BM + nx_slot) * 4u),
+                                               0, 0);
     const bf16_t* baseA = a.xb + (int64_t)(row_begin + t * BM) * kp + kc * BK;
     const bf16_t* baseB = a.qb + kc * BK;
 #pragma unroll
     for (int p = 0; p < T::A_CHUNKS; ++p)
-      __builtin_amdgcn_global_load_lds((const void*)(baseA + offA[p]),
+      __builtin_amdgcn_global_load_lds((const void*)((const char*)baseA + offA[p]),
                                        (lds_void*)(As + st * BM * LROW + (NT * p + 64 * wv) * 8),
                                        16, 0, 0);
 #pragma unroll
     for (int p = 0; p < T::B_CHUNKS; ++p)
-      __builtin_amdgcn_global_load_lds((const void*)(baseB + offB[p]),
+      __builtin_amdgcn_global_load_lds((const void*)((const char*)baseB + offB[p]),
                                        (lds_void*)(Bs + st * BN * LROW + (NT * p + 64 * wv) * 8),
                                        16, 0, 0);
   };
@@ -231,20 +241,55 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
 
   // Epilogue of one BM-row tile: per MFMA block, S^ from the accumulators,
   // threshold mask, then the rare emissions.
-  auto epilogue = [&](int t) {
+  // Epilogue of one BM-row tile.  FOLD: the accumulators are S^ already
+  // (folded norms); FULL: no row of the tile lies at or past row_end.
+  auto epilogue_t = [&](int t, auto FOLDC, auto FULLC) {
+    constexpr bool FOLD = decltype(FOLDC)::value, FULL = decltype(FULLC)::value;
+    // lane-derived addresses are recomputed from an opaque copy of the lane
+    // id here, rather than hoisted out of the tile loop into registers that
+    // the K loop needs
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
     const int r0 = row_begin + t * BM;
 #pragma unroll
     for (int i = 0; i < WM; ++i) {
-      const int rb0 = r0 + wr * 32 * WM + i * 32 + 4 * (lane >> 5);
+      const int rb0 = r0 + wr * 32 * WM + i * 32 + 4 * (ln >> 5);
       const float* nxt = nxs + (t % T::NXB) * BM + (rb0 - r0);
+      // non-folded: |x~|^2 of this lane's 16 rows (+inf past row_end), from
+      // LDS; folded: 0, or +inf past row_end
       float nx[16];
 #pragma unroll
       for (int e4 = 0; e4 < 4; ++e4) {
-        const floatx4 v4 = *(const floatx4*)(nxt + 8 * e4);
+        floatx4 v4 = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (!FOLD) v4 = *(const floatx4*)(nxt + 8 * e4);
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           const int row = rb0 + 8 * e4 + c;
           nx[4 * e4 + c] = row < row_end ? v4[c] : INFINITY;
+        }
+      }
+      // first-pass value without the query's norm (folded: with it)
+      auto val = [&](int j, int e) -> float {
+        if (FOLD) return FULL ? acc[i][j][e] : acc[i][j][e] + nx[e];
+        return __builtin_fmaf(-2.0f, acc[i][j][e], nx[e]);
+      };
+      // fast test of every block of this row group first: one FMA per value
+      // and a balanced min tree per block (independent chains); fl(min_e a_e
+      // + qo) = min_e fl(a_e + qo) because rounding is monotone, so "no value
+      // of this block passes" is decided exactly
+      bool pass[WN];
+      if (MODE != SCAN_DENSE) {
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < WN; ++j) {
+          float m4[4];
+#pragma unroll
+          for (int g = 0; g < 4; ++g) {
+            m4[g] = fminf(fminf(val(j, 4 * g), val(j, 4 * g + 1)),
+                          fminf(val(j, 4 * g + 2), val(j, 4 * g + 3)));
+          }
+          const float mn = fminf(fminf(m4[0], m4[1]), fminf(m4[2], m4[3]));
+          pass[j] = (FOLD ? mn : mn + qo[j]) < tau_l[j];
         }
       }
 #pragma unroll
@@ -254,46 +299,37 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
               const int row = rb0 + 8 * (e >> 2) + (e & 3);
-              const float v = __builtin_fmaf(-2.0f, acc[i][j][e], nx[e]) + qo[j];
+              const float v = FOLD ? val(j, e) : val(j, e) + qo[j];
               if (row < row_end)
                 a.dense[(int64_t)gqc[j] * a.ldd + (a.dense_rel ? row - region : row)] = v;
             }
           }
           continue;
         }
-        // keep the scheduler from hoisting the accumulator reads of later
-        // blocks (the register file cannot hold them all at once)
-        __builtin_amdgcn_sched_barrier(0);
-        // fast test: one FMA per value and a min tree; fl(min_e a_e + qo) =
-        // min_e fl(a_e + qo) because rounding is monotone, so "no value of
-        // this block passes" is decided exactly
+        if (!pass[j]) continue;
+        // the rare emissions: S^ of the block again (same operations), mask
         float w[16];
-#pragma unroll
-        for (int e = 0; e < 16; ++e) w[e] = __builtin_fmaf(-2.0f, acc[i][j][e], nx[e]);
-        float mn = w[0];
-#pragma unroll
-        for (int e = 1; e < 16; ++e) mn = fminf(mn, w[e]);
-        if (!(mn + qo[j] < tau_l[j])) continue;
-        float v[16];
         unsigned m = 0;
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
-          v[e] = w[e] + qo[j];
-          m |= (v[e] < tau_l[j] ? 1u : 0u) << e;
+          w[e] = FOLD ? val(j, e) : val(j, e) + qo[j];
+          m |= (w[e] < tau_l[j] ? 1u : 0u) << e;
         }
-        const int gq = gqc[j];
-        int pos = atomicAdd(&lcnt[qcl[j]], __builtin_popcount(m));
+        int gq = gqc[j];
+        asm volatile("" : "+v"(gq));
+        const int qc = wc * 32 * WN + j * 32 + (ln & 31);  // query column in the tile
+        int pos = atomicAdd(&lcnt[qc], __builtin_popcount(m));
         unsigned* reg = a.cand + ((int64_t)gq * a.parts + region) * (int64_t)a.cap * 2;
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
           if (!((m >> e) & 1u)) continue;
           const int row = rb0 + 8 * (e >> 2) + (e & 3);
           uint2 cv;
-          cv.x = __float_as_uint(v[e]);
+          cv.x = __float_as_uint(w[e]);
           cv.y = (unsigned)row;
           if (MODE == SCAN_GATHER) {
             if (pos < T::LCAP) {
-              lbuf[qcl[j] * T::LCAP + pos] = cv;
+              lbuf[qc * T::LCAP + pos] = cv;
             } else {  // spill straight to the query's global buffer
               int gp = atomicAdd(a.cnt + gq, 1);
               if (gp < a.cap) *(uint2*)(a.cand + ((int64_t)gq * a.cap + gp) * 2) = cv;
@@ -303,11 +339,21 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
           }
           ++pos;
           if (MODE == SCAN_TOP1) {
-            float th = top1_threshold(v[e], a.t1_qe[gq], a.t1_qr[gq], a.t1_ex, a.t1_rx, a.d);
+            float th = top1_threshold(w[e], a.t1_qe[gq], a.t1_qr[gq], a.t1_ex, a.t1_rx, a.d);
             tau_l[j] = fminf(tau_l[j], th);
           }
         }
       }
+    }
+  };
+  auto epilogue = [&](int t) {
+    using T1 = std::integral_constant<bool, true>;
+    using T0 = std::integral_constant<bool, false>;
+    if (MODE != SCAN_TOP1 && MODE != SCAN_GATHER && a.fold) {
+      if (row_begin + (t + 1) * BM <= row_end) epilogue_t(t, T1{}, T1{});
+      else epilogue_t(t, T1{}, T0{});
+    } else {
+      epilogue_t(t, T0{}, T0{});
     }
   };
 
@@ -512,6 +558,9 @@ static void launch_cfg(const ScanArgs& a, int mode, int grid, int kind, hipStrea
 
 void launch_scan(const ScanArgs& a, int mode, int grid, int kind, hipStream_t st) {
   if (grid <= 0) return;
+  // row norms are read through a 32-bit byte range
+  MANN_REQUIRE(((int64_t)a.n_rows + 1024) * 4 < ((int64_t)1 << 32), ERR_UNSUPPORTED,
+               "too many rows for one first-pass launch");
   switch (a.cfg) {
     case 1: launch_cfg<4, 2, 2, 2, 3>(a, mode, grid, kind, st); break;
     case 2: launch_cfg<2, 4, 2, 2, 3>(a, mode, grid, kind, st); break;

@@ -55,6 +55,10 @@ struct ScanArgs {
   const float* t1_qe;
   const float* t1_qr;
   float t1_ex, t1_rx;
+  // Folded norms (FILTER / DENSE): the operands carry |x~|^2 and |q~|^2 in
+  // spare K columns and the query side is -2 q~, so the MFMA accumulates S^
+  // itself; xn and qnorm are not read
+  int fold;
 };
 
 // Run-time tuning knobs (set through mann_tune_set; chosen by the autotuner).
@@ -67,6 +71,7 @@ struct Tuning {
   int target_groups = 2048;  // workgroups the row partitioning aims for
   int sample_rank = 1;       // multiplier of the emission target below
   int emit_per_k = 15;       // exact kNN: aim at emit_per_k * k_req first-pass emissions per query
+  int fold_norms = 1;        // exact kNN: fold the norms into spare K columns when there are 6
   int shortlist_mult = 1;    // first re-rank batch = shortlist_mult * k_req (>= 32)
   int tau_rows = 1024;       // IVF: exact rows sampled per query for the emission threshold
   int ivf_tau_mode = 1;      // IVF threshold: 0 exact FP32 row sample, 1 bf16 scan of the first list
@@ -88,6 +93,11 @@ void launch_prep_rows(const float* src, int64_t ld, int64_t n, int64_t n_pad, in
 void launch_mean_rows(const float* x, int64_t ld, int64_t n, int d, double* tmp, float* mean,
                       hipStream_t st);
 void launch_a16_norms(const float* x32, int64_t n, int d16, float* out, hipStream_t st);
+// folded-norm operands (see mann_dataset::fold): base rows in place, queries
+// into out (nq_pad rows)
+void launch_fold_base(bf16_t* xb, const float* xn, int64_t n, int kp, int d, hipStream_t st);
+void launch_fold_queries(const bf16_t* qb, const float* qn, int64_t nq, int64_t nq_pad, int kp,
+                         int d, bf16_t* out, hipStream_t st);
 
 // ---- selection / exact re-rank -------------------------------------------------
 

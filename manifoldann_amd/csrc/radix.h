@@ -42,12 +42,6 @@ __device__ __forceinline__ void wave_sort_u32(uint32_t (&v)[R], int lane) {
   }
 }
 
-__device__ __forceinline__ void wave_fence_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // rank-th smallest (1-based, clamped to [1, m]) of the m floats row[0..m) as
 // an order key, by one wavefront; buf (256 keys) and hist (256 counters) are
 // the wavefront's LDS.  For rank <= 64 the rank-th smallest of the 64 lane

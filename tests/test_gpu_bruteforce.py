@@ -151,3 +151,29 @@ def test_knn_graph_config2_subset_bit_exact():
         assert dist[r].tolist() == [float(v) for _, v in keep]
     assert (adj != (np.arange(60_000)[:, None] + 1)).all()   # no self loops
     assert ((adj >= 1) & (adj <= 60_000)).all()
+
+
+def test_folded_norms_match_unfolded():
+    # d = 45 leaves 19 spare padded columns: the folded first pass (norms in
+    # the operands, S^ straight from the MFMA) against the unfolded one and
+    # the oracle, for L2 and cosine
+    _banner("folded norms")
+    from manifoldann_amd._lib import check, lib
+    L = lib()
+    rng = np.random.default_rng(21)
+    X = (rng.standard_normal((6000, 45)) * rng.uniform(0.2, 3.0, (6000, 1))).astype(np.float32)
+    Q = rng.standard_normal((700, 45)).astype(np.float32)
+    try:
+        for metric in (M.Euclidean(), M.CosineDist()):
+            got = []
+            for fold in (0, 1):
+                check(L.mann_tune_set(b"fold_norms", fold))
+                index = M.build_index(M.BruteForceIndex, X.T, distance=metric)
+                ds = M.Dataset(X.T, metric=metric)
+                got.append(M.query_arrays(index, ds, Q.T, 12))
+            oi, od = O.bf_knn(X, Q, 12, metric.code)
+            for ids, dist in got:
+                assert (ids == oi).all()
+                assert (dist.view(np.uint32) == od.view(np.uint32)).all()
+    finally:
+        check(L.mann_tune_set(b"fold_norms", 1))
