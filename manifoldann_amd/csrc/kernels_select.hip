@@ -45,6 +45,15 @@ __device__ __forceinline__ int next_pow2(int x) {
 }
 
 // rank-th smallest of each query's m sample values (one block per query)
+// one workgroup per query (long rows)
+__global__ __launch_bounds__(256) void tau_from_sample_block_kernel(TauArgs a) {
+  __shared__ BlockRankScratch sm;
+  const int q = blockIdx.x;
+  if (q >= a.nq) return;
+  const uint32_t key = block_rank_key(a.dense + (int64_t)q * a.ldd, a.m, a.rank, sm);
+  if (threadIdx.x == 0) a.tau[q] = ord2f(key);
+}
+
 // one wavefront per query, four per workgroup
 __global__ __launch_bounds__(256) void tau_from_sample_kernel(TauArgs a) {
   __shared__ uint32_t buf_all[4][256];
@@ -508,8 +517,11 @@ int exact_kmax() { return EX_KMAX; }
 
 void launch_tau_from_sample(const TauArgs& a, hipStream_t st) {
   if (a.nq <= 0) return;
-  hipLaunchKernelGGL(tau_from_sample_kernel, dim3((unsigned)ceil_div(a.nq, 4)), dim3(256), 0, st,
-                     a);
+  if (a.m > 8192)  // a wavefront per row would stream too long a row alone
+    hipLaunchKernelGGL(tau_from_sample_block_kernel, dim3((unsigned)a.nq), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(tau_from_sample_kernel, dim3((unsigned)ceil_div(a.nq, 4)), dim3(256), 0,
+                       st, a);
 }
 
 void launch_select_rerank(const SelectArgs& a, int nq_max, hipStream_t st) {

@@ -145,4 +145,120 @@ __device__ inline uint32_t wave_rank_key(const float* row, int m, int rank, uint
   return prefix;
 }
 
+// rank-th smallest (1-based, clamped to [1, m]) of m floats by a whole
+// 256-thread workgroup, for long rows.  An upper bound U of the answer: for
+// rank <= 64 the smallest of the four wavefronts' rank-th smallest values of
+// their 4,096-value slices (each is the rank-th smallest of a subset), else the
+// rank-th smallest of the 256 thread minima (rank <= 256).  The values at or
+// below U are collected in LDS (up to 1024) and sorted by wavefront 0;
+// otherwise one wavefront runs the radix select of wave_rank_key over the row.
+// All threads return the key.
+struct BlockRankScratch {
+  uint32_t mins[256];
+  uint32_t buf[1024];
+  unsigned hist[4][256];
+  unsigned cnt;
+  uint32_t key;
+};
+
+__device__ inline uint32_t block_rank_key(const float* row, int m, int rank, BlockRankScratch& sm) {
+  rank = rank < 1 ? 1 : (rank > m ? m : rank);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (rank <= 64 && m >= 4 * 4096) {
+    const uint32_t uw = wave_rank_key(row + w * 4096, 4096, rank, sm.buf + w * 256, sm.hist[w],
+                                      lane);
+    if (lane == 0) sm.mins[w] = uw;
+    if (tid == 0) sm.cnt = 0;
+    __syncthreads();
+    if (tid == 0) sm.key = min(min(sm.mins[0], sm.mins[1]), min(sm.mins[2], sm.mins[3]));
+  } else if (rank <= 256) {
+    uint32_t mn = 0xffffffffu;
+    for (int i = tid; i < m; i += 256) mn = min(mn, f2ord(row[i]));
+    sm.mins[tid] = mn;
+    if (tid == 0) sm.cnt = 0;
+    __syncthreads();
+    if (w == 0) {
+      uint32_t v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = sm.mins[j * 64 + lane];
+      wave_sort_u32<4>(v, lane);
+      const int e = rank - 1;
+      uint32_t r = v[0];
+#pragma unroll
+      for (int j = 1; j < 4; ++j)
+        if (e / 64 == j) r = v[j];
+      r = (uint32_t)__shfl((int)r, e % 64);
+      if (lane == 0) sm.key = r;
+    }
+  }
+  if (rank <= 256) {
+    __syncthreads();
+    const uint32_t U = sm.key;
+    // eight loads in flight per thread: the pass is bound by memory latency
+    int i = tid;
+    for (; i + 7 * 256 < m; i += 8 * 256) {
+      uint32_t o[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) o[u] = f2ord(row[i + u * 256]);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (o[u] <= U) {
+          const unsigned pos = atomicAdd(&sm.cnt, 1u);
+          if (pos < 1024u) sm.buf[pos] = o[u];
+        }
+    }
+    for (; i < m; i += 256) {
+      const uint32_t o = f2ord(row[i]);
+      if (o <= U) {
+        const unsigned pos = atomicAdd(&sm.cnt, 1u);
+        if (pos < 1024u) sm.buf[pos] = o;
+      }
+    }
+    __syncthreads();
+    const unsigned c = sm.cnt;
+    if (c <= 256u) {
+      if (w == 0) {
+        uint32_t v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          v[j] = (unsigned)(j * 64 + lane) < c ? sm.buf[j * 64 + lane] : 0xffffffffu;
+        wave_sort_u32<4>(v, lane);
+        const int e = rank - 1;
+        uint32_t r = v[0];
+#pragma unroll
+        for (int j = 1; j < 4; ++j)
+          if (e / 64 == j) r = v[j];
+        r = (uint32_t)__shfl((int)r, e % 64);
+        if (lane == 0) sm.key = r;
+      }
+      __syncthreads();
+      return sm.key;
+    }
+    if (c <= 1024u) {
+      if (w == 0) {
+        uint32_t v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+          v[j] = (unsigned)(j * 64 + lane) < c ? sm.buf[j * 64 + lane] : 0xffffffffu;
+        wave_sort_u32<16>(v, lane);
+        const int e = rank - 1;
+        uint32_t r = v[0];
+#pragma unroll
+        for (int j = 1; j < 16; ++j)
+          if (e / 64 == j) r = v[j];
+        r = (uint32_t)__shfl((int)r, e % 64);
+        if (lane == 0) sm.key = r;
+      }
+      __syncthreads();
+      return sm.key;
+    }
+  }
+  if (w == 0) {
+    const uint32_t r = wave_rank_key(row, m, rank, sm.buf, sm.hist[0], lane);
+    if (lane == 0) sm.key = r;
+  }
+  __syncthreads();
+  return sm.key;
+}
+
 }  // namespace mann
