@@ -23,8 +23,11 @@ struct mann_dataset {
   mann::bf16_t* sb = nullptr;
   float* sxn = nullptr;
   // columns d..d+2 of xb (and sb) hold |x~|^2 as three bf16 parts summing to
-  // it exactly, d+3..d+5 hold 1 (first pass with folded norms)
+  // it exactly, d+3..d+5 hold 1 (first pass with folded norms); qfold: the
+  // rows as folded query operands (-2 x~, 1, 1, 1, parts of |x~|^2) for
+  // searches of the dataset against itself
   int fold = 0;
+  mann::bf16_t* qfold = nullptr;
   ~mann_dataset();
 };
 

@@ -99,7 +99,7 @@ void add_stats(const int* slots, SearchStats& s) {
 
 mann_dataset::~mann_dataset() {
   (void)hipSetDevice(device);
-  void* ps[] = {x32, xb, xn, err, nrm, xn2, sb, sxn, center};
+  void* ps[] = {x32, xb, xn, err, nrm, xn2, sb, sxn, center, qfold};
   for (void* p : ps)
     if (p) (void)hipFree(p);
 }
@@ -150,6 +150,8 @@ mann_dataset* create_dataset(const float* data, int64_t d, int64_t n, int64_t ld
                      ds->x32, ds->xb, ds->xn, ds->err, ds->nrm, mx, mx + 1, st, ds->center);
     launch_a16_norms(ds->x32, n, (int)ds->d16, ds->xn2, st);
     if (tuning().fold_norms && ds->kp - d >= 6) {
+      MANN_HIP(hipMalloc(&ds->qfold, sizeof(bf16_t) * ds->n_pad * ds->kp));
+      launch_fold_queries(ds->xb, ds->xn, n, ds->n_pad, (int)ds->kp, (int)d, ds->qfold, st);
       launch_fold_base(ds->xb, ds->xn, n, (int)ds->kp, (int)d, st);
       ds->fold = 1;
     }
@@ -253,7 +255,9 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   if (nq == 0 || k <= 0) return;
   // folded-norm query operand: -2 q~ and the parts of |q~|^2 (see mann_dataset)
   const bf16_t* qop = Q.qb;
-  if (ds.fold) {
+  if (ds.fold && Q.qb == ds.xb && ds.qfold) {
+    qop = ds.qfold;
+  } else if (ds.fold) {
     const int64_t nq_pad = round_up(Q.nq, 256);
     bf16_t* qf = c.buf("bf_qfold").as<bf16_t>((size_t)nq_pad * ds.kp);
     launch_fold_queries(Q.qb, Q.qn, Q.nq, nq_pad, (int)ds.kp, (int)ds.d, qf, st);
