@@ -714,9 +714,12 @@ int mann_tune_set(const char* key, int64_t value) {
     MANN_REQUIRE(key != nullptr, ERR_NULL, "null key");
     std::string k(key);
     Tuning& t = tuning();
-    if (k == "scan_cfg" || k == "ivf_cfg" || k == "kmeans_cfg") {
+    if (k == "scan_cfg" || k == "ivf_cfg" || k == "kmeans_cfg" || k == "ivf_tau_cfg") {
       MANN_REQUIRE(value >= 0 && value < scan_num_cfg(), ERR_ARGUMENT, k + " out of range");
-      (k == "scan_cfg" ? t.scan_cfg : k == "ivf_cfg" ? t.ivf_cfg : t.kmeans_cfg) = (int)value;
+      (k == "scan_cfg"  ? t.scan_cfg
+       : k == "ivf_cfg" ? t.ivf_cfg
+       : k == "ivf_tau_cfg" ? t.ivf_tau_cfg
+                            : t.kmeans_cfg) = (int)value;
     } else if (k == "xcd_map") {
       MANN_REQUIRE(value >= 0 && value <= 3, ERR_ARGUMENT, "xcd_map must be 0..3");
       t.xcd_map = (int)value;
@@ -759,6 +762,7 @@ int mann_tune_get(const char* key, int64_t* value) {
     if (k == "scan_cfg") *value = t.scan_cfg;
     else if (k == "ivf_cfg") *value = t.ivf_cfg;
     else if (k == "kmeans_cfg") *value = t.kmeans_cfg;
+    else if (k == "ivf_tau_cfg") *value = t.ivf_tau_cfg;
     else if (k == "xcd_map") *value = t.xcd_map;
     else if (k == "target_groups") *value = t.target_groups;
     else if (k == "sample_rank") *value = t.sample_rank;

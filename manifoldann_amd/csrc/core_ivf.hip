@@ -211,16 +211,23 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
   if (dense_tau) {
     // bf16 residual scan of each query's first list; the k-th smallest S^
     // bounds the k-th exact value from above (certified by construction)
-    ScanItem* fitems = c.buf("iv_fitems").as<ScanItem>((size_t)std::max<int64_t>(max_items, 1));
+    // a tile config of its own: a list is the first probe of few queries
+    const int tcfg = tuning().ivf_tau_cfg;
+    const int BNt = scan_tile_queries(tcfg);
+    // each query has one first list: sum over lists of ceil(first_cnt / BNt) <=
+    // nb + nq / BNt query tiles, each over at most ceil(max_list / ROWS_ITEM) chunks
+    const int64_t max_fitems =
+        (nb + ceil_div(nq, BNt)) * std::max<int64_t>(1, ceil_div(iv.max_list, ROWS_ITEM));
+    ScanItem* fitems = c.buf("iv_fitems").as<ScanItem>((size_t)max_fitems);
     int* fioff = c.buf("iv_fioff").as<int>(nb);
     float* dense = c.buf("iv_dense").as<float>((size_t)nq * iv.max_list);
     {
       ScopedTime tm("ivf_tau_scan", st);
-      launch_first_items(first_cnt, qoff, nb, iv.list_off, iv.lorder, BN, ROWS_ITEM, fioff, fitems,
+      launch_first_items(first_cnt, qoff, nb, iv.list_off, iv.lorder, BNt, ROWS_ITEM, fioff, fitems,
                          ctr + 5, st);
       fill_i32(ctr + 6, 1, 0, st);
       ScanArgs a{};
-      a.cfg = cfg;
+      a.cfg = tcfg;
       a.xb = iv.xb;
       a.xn = iv.xn;
       a.qb = qres;

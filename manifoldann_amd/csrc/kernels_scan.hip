@@ -527,12 +527,14 @@ void launch_t(const ScanArgs& a, int grid, int kind, hipStream_t st) {
 }  // namespace
 
 // Tile configurations: WR x WC wavefronts of 32*WM rows x 32*WN queries.
-// Configs 6 and 7 repeat 5 and 0 with a 4-deep LDS ring (3 chunks in flight).
+// Configs 6 and 7 repeat 5 and 0 with a 4-deep LDS ring (3 chunks in flight);
+// config 8 (128 rows x 64 queries) suits items with few queries.
 struct ScanCfg {
   int wm, wn, wr, wc, stages;
 };
 static const ScanCfg kCfg[] = {{2, 2, 2, 2, 3}, {4, 2, 2, 2, 3}, {2, 4, 2, 2, 3}, {4, 4, 2, 2, 3},
-                               {4, 2, 2, 4, 3}, {2, 4, 4, 2, 3}, {2, 4, 4, 2, 4}, {2, 2, 2, 2, 4}};
+                               {4, 2, 2, 4, 3}, {2, 4, 4, 2, 3}, {2, 4, 4, 2, 4}, {2, 2, 2, 2, 4},
+                               {2, 1, 2, 2, 3}};
 constexpr int kNumCfg = sizeof(kCfg) / sizeof(kCfg[0]);
 
 int scan_num_cfg() { return kNumCfg; }
@@ -569,6 +571,7 @@ void launch_scan(const ScanArgs& a, int mode, int grid, int kind, hipStream_t st
     case 5: launch_cfg<2, 4, 4, 2, 3>(a, mode, grid, kind, st); break;
     case 6: launch_cfg<2, 4, 4, 2, 4>(a, mode, grid, kind, st); break;
     case 7: launch_cfg<2, 2, 2, 2, 4>(a, mode, grid, kind, st); break;
+    case 8: launch_cfg<2, 1, 2, 2, 3>(a, mode, grid, kind, st); break;
     default: launch_cfg<2, 2, 2, 2, 3>(a, mode, grid, kind, st); break;
   }
 }
