@@ -212,6 +212,66 @@ __global__ void list_count_kernel(const int* __restrict__ probes, const int* __r
   if (r < pcount[q]) atomicAdd(&qcount[probes[i]], 1);
 }
 
+// Block-aggregated forms of the two kernels above, for nb lists that fit in
+// LDS: counts (and slots) are taken in LDS first, then one global atomic per
+// (block, list) reserves the block's range, so a list probed by thousands of
+// queries does not serialize thousands of atomics on one address.
+constexpr int LF_EPB = 1024;  // entries per block
+__global__ __launch_bounds__(256) void list_count_agg_kernel(const int* __restrict__ probes,
+                                                             const int* __restrict__ pcount,
+                                                             int nq, int nprobe, int nb,
+                                                             int* __restrict__ qcount) {
+  extern __shared__ int lh[];
+  for (int b = threadIdx.x; b < nb; b += 256) lh[b] = 0;
+  __syncthreads();
+  const int64_t tot = (int64_t)nq * nprobe, e0 = (int64_t)blockIdx.x * LF_EPB;
+  for (int u = threadIdx.x; u < LF_EPB; u += 256) {
+    const int64_t i = e0 + u;
+    if (i >= tot) break;
+    const int q = (int)(i / nprobe), r = (int)(i - (int64_t)q * nprobe);
+    if (r < pcount[q]) atomicAdd(&lh[probes[i]], 1);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nb; b += 256)
+    if (lh[b]) atomicAdd(&qcount[b], lh[b]);
+}
+
+__global__ __launch_bounds__(256) void list_fill_agg_kernel(
+    const int* __restrict__ probes, const int* __restrict__ pcount, int nq, int nprobe, int nb,
+    const int* __restrict__ qoff, int* __restrict__ qfill, int* __restrict__ qlist,
+    int* __restrict__ qlist_list, int* __restrict__ slot_of, int phase) {
+  extern __shared__ int lh[];  // [nb] local counts, then the block's base slot
+  for (int b = threadIdx.x; b < nb; b += 256) lh[b] = 0;
+  __syncthreads();
+  const int64_t tot = (int64_t)nq * nprobe, e0 = (int64_t)blockIdx.x * LF_EPB;
+  constexpr int PER = LF_EPB / 256;
+  int rank[PER], lst[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int64_t i = e0 + threadIdx.x + 256 * j;
+    lst[j] = -1;
+    if (i >= tot) continue;
+    const int q = (int)(i / nprobe), r = (int)(i - (int64_t)q * nprobe);
+    if (r >= pcount[q] || (r == 0) != (phase == 0)) continue;
+    lst[j] = probes[i];
+    rank[j] = atomicAdd(&lh[lst[j]], 1);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nb; b += 256)
+    if (lh[b]) lh[b] = atomicAdd(&qfill[b], lh[b]);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    if (lst[j] < 0) continue;
+    const int64_t i = e0 + threadIdx.x + 256 * j;
+    const int q = (int)(i / nprobe), b = lst[j];
+    const int pos = qoff[b] + lh[b] + rank[j];
+    qlist[pos] = q;
+    qlist_list[pos] = b;
+    slot_of[i] = pos;
+  }
+}
+
 // exclusive scan of one value per thread over a 1024-thread block
 __device__ __forceinline__ int block_exclusive_scan(int v, int* wsum, int* total) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -669,16 +729,28 @@ void launch_ivf_plan(const int* probes, const int* pcount, int nq, int nprobe, i
                      int* first_cnt, hipStream_t st) {
   const int64_t tot = (int64_t)nq * nprobe;
   const unsigned g = (unsigned)ceil_div(tot, 256);
-  hipLaunchKernelGGL(list_count_kernel, dim3(g), dim3(256), 0, st, probes, pcount, nq, nprobe,
-                     qcount);
+  // LDS-aggregated counting when the lists fit in 32 KiB of LDS
+  const bool agg = nb <= 8192;
+  const unsigned ga = (unsigned)ceil_div(tot, LF_EPB);
+  const size_t lds = sizeof(int) * (size_t)nb;
+  if (agg)
+    hipLaunchKernelGGL(list_count_agg_kernel, dim3(ga), dim3(256), lds, st, probes, pcount, nq,
+                       nprobe, nb, qcount);
+  else
+    hipLaunchKernelGGL(list_count_kernel, dim3(g), dim3(256), 0, st, probes, pcount, nq, nprobe,
+                       qcount);
   hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(1024), 0, st, qcount, nb, list_off, lorder, bn,
                      rows_item, qoff, ioff, n_items, n_entries);
-  hipLaunchKernelGGL(list_fill_kernel, dim3(g), dim3(256), 0, st, probes, pcount, nq, nprobe, qoff,
-                     qfill, qlist, qlist_list, slot_of, 0);
-  if (first_cnt)
-    MANN_HIP(hipMemcpyAsync(first_cnt, qfill, sizeof(int) * nb, hipMemcpyDeviceToDevice, st));
-  hipLaunchKernelGGL(list_fill_kernel, dim3(g), dim3(256), 0, st, probes, pcount, nq, nprobe, qoff,
-                     qfill, qlist, qlist_list, slot_of, 1);
+  for (int phase = 0; phase < 2; ++phase) {
+    if (agg)
+      hipLaunchKernelGGL(list_fill_agg_kernel, dim3(ga), dim3(256), lds, st, probes, pcount, nq,
+                         nprobe, nb, qoff, qfill, qlist, qlist_list, slot_of, phase);
+    else
+      hipLaunchKernelGGL(list_fill_kernel, dim3(g), dim3(256), 0, st, probes, pcount, nq, nprobe,
+                         qoff, qfill, qlist, qlist_list, slot_of, phase);
+    if (phase == 0 && first_cnt)
+      MANN_HIP(hipMemcpyAsync(first_cnt, qfill, sizeof(int) * nb, hipMemcpyDeviceToDevice, st));
+  }
   hipLaunchKernelGGL(items_fill_kernel, dim3((unsigned)ceil_div(nb, 256)), dim3(256), 0, st, qcount,
                      nb, list_off, lorder, qoff, ioff, bn, rows_item, items);
 }
