@@ -34,6 +34,33 @@ __device__ __forceinline__ void wait_vm_lgkm() {
 }
 // s_waitcnt lgkmcnt(0) only (vmcnt and expcnt left open)
 __device__ __forceinline__ void wait_lgkm0() { __builtin_amdgcn_s_waitcnt(0xC07F); }
+
+// LDS accesses of the epilogue, written out.  They never touch the ring
+// stages or norm slots that global->LDS loads in flight are filling, but the
+// compiler cannot tell, and before each LDS access it would wait for every
+// outstanding vector-memory operation: the next tile's chunks and all earlier
+// emission stores.  Each statement waits for its own LDS operation.
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned lds_addr(const void* p) {
+  return (unsigned)(uintptr_t)(const lds_void*)p;
+}
+__device__ __forceinline__ int lds_fetch_add(int* p, int v) {
+  int r;
+  asm volatile("ds_add_rtn_u32 %0, %1, %2\n\ts_waitcnt lgkmcnt(0)"
+               : "=v"(r)
+               : "v"(lds_addr(p)), "v"(v)
+               : "memory");
+  return r;
+}
+__device__ __forceinline__ floatx4 lds_read_f4(const float* p) {
+  floatx4 r;
+  asm volatile("ds_read_b128 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(r) : "v"(lds_addr(p)) : "memory");
+  return r;
+}
+__device__ __forceinline__ void lds_write_u2(uint2* p, uint2 v) {
+  const u32x2 w = {v.x, v.y};
+  asm volatile("ds_write_b64 %0, %1\n\ts_waitcnt lgkmcnt(0)" : : "v"(lds_addr(p)), "v"(w) : "memory");
+}
 // Workgroup barrier without the release fence of __syncthreads (which would
 // drain every outstanding LDS DMA): callers wait for exactly the chunk they
 // need first.  The empty asm statements keep the compiler from moving LDS
@@ -148,6 +175,9 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
   }
   int gqc[WN];
   float tau_l[WN], qo[WN];
+  // TOP1: the query's rounding norms for the certification band, loaded here
+  // (a global load in the epilogue would wait for every store before it)
+  float t1e[MODE == SCAN_TOP1 ? WN : 1], t1r[MODE == SCAN_TOP1 ? WN : 1];
 #pragma unroll
   for (int bn = 0; bn < WN; ++bn) {
     int c = wc * 32 * WN + bn * 32 + (lane & 31);
@@ -157,6 +187,10 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
     float o = 0.0f;
     if (gq >= 0) o = a.col_norm ? a.col_norm[qoff + c] : (a.qnorm ? a.qnorm[gq] : 0.0f);
     qo[bn] = o;
+    if (MODE == SCAN_TOP1) {
+      t1e[bn] = gq >= 0 ? a.t1_qe[gq] : 0.0f;
+      t1r[bn] = gq >= 0 ? a.t1_qr[gq] : 0.0f;
+    }
   }
 
   // chunk (t, kc) -> LDS stage st by direct global->LDS loads: wave w's
@@ -261,7 +295,7 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
 #pragma unroll
       for (int e4 = 0; e4 < 4; ++e4) {
         floatx4 v4 = {0.0f, 0.0f, 0.0f, 0.0f};
-        if (!FOLD) v4 = *(const floatx4*)(nxt + 8 * e4);
+        if (!FOLD) v4 = lds_read_f4(nxt + 8 * e4);
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           const int row = rb0 + 8 * e4 + c;
@@ -318,7 +352,7 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
         int gq = gqc[j];
         asm volatile("" : "+v"(gq));
         const int qc = wc * 32 * WN + j * 32 + (ln & 31);  // query column in the tile
-        int pos = atomicAdd(&lcnt[qc], __builtin_popcount(m));
+        int pos = lds_fetch_add(&lcnt[qc], __builtin_popcount(m));
         unsigned* reg = a.cand + ((int64_t)gq * a.parts + region) * (int64_t)a.cap * 2;
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
@@ -329,7 +363,7 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
           cv.y = (unsigned)row;
           if (MODE == SCAN_GATHER) {
             if (pos < T::LCAP) {
-              lbuf[qc * T::LCAP + pos] = cv;
+              lds_write_u2(&lbuf[qc * T::LCAP + pos], cv);
             } else {  // spill straight to the query's global buffer
               int gp = atomicAdd(a.cnt + gq, 1);
               if (gp < a.cap) *(uint2*)(a.cand + ((int64_t)gq * a.cap + gp) * 2) = cv;
@@ -339,7 +373,8 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
           }
           ++pos;
           if (MODE == SCAN_TOP1) {
-            float th = top1_threshold(w[e], a.t1_qe[gq], a.t1_qr[gq], a.t1_ex, a.t1_rx, a.d);
+            float th = top1_threshold(w[e], t1e[MODE == SCAN_TOP1 ? j : 0],
+                                      t1r[MODE == SCAN_TOP1 ? j : 0], a.t1_ex, a.t1_rx, a.d);
             tau_l[j] = fminf(tau_l[j], th);
           }
         }
