@@ -72,6 +72,9 @@ struct mann_router {
   float* pcn = nullptr;
   int64_t pld = 0;
   float gamma = 0.0f;
+  // the same prior as a GEMM: logits += x.(2 gamma c) + (-gamma |c|^2)
+  float* pc2 = nullptr;
+  float* pb = nullptr;
   float* W1() const { return w; }
   float* b1() const { return w + (int64_t)h * d; }
   float* W2() const { return b1() + h; }
@@ -83,6 +86,8 @@ struct mann_router {
     if (m2) (void)hipFree(m2);
     if (pc) (void)hipFree(pc);
     if (pcn) (void)hipFree(pcn);
+    if (pc2) (void)hipFree(pc2);
+    if (pb) (void)hipFree(pb);
   }
 };
 

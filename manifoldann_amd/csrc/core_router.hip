@@ -50,9 +50,11 @@ mann_router* router_create(int d, int h, int c, uint64_t seed) {
 }
 
 void router_set_prior(mann_router& r, const mann_ivf* iv, float gamma) {
-  if (r.pc) (void)hipFree(r.pc);
-  if (r.pcn) (void)hipFree(r.pcn);
-  r.pc = r.pcn = nullptr;
+  float** bufs[] = {&r.pc, &r.pcn, &r.pc2, &r.pb};
+  for (float** b : bufs) {
+    if (*b) (void)hipFree(*b);
+    *b = nullptr;
+  }
   r.gamma = 0.0f;
   if (gamma == 0.0f || !iv) return;
   MANN_REQUIRE(r.d <= prior_dmax(), ERR_UNSUPPORTED, "router prior supports d <= 1024");
@@ -63,6 +65,18 @@ void router_set_prior(mann_router& r, const mann_ivf* iv, float gamma) {
   MANN_HIP(hipMemcpy(r.pcn, iv->cn2, sizeof(float) * (size_t)r.c, hipMemcpyDeviceToDevice));
   r.pld = iv->d16;
   r.gamma = gamma;
+  // 2 gamma c (one rounding per element) and -gamma |c|^2 for the tiled GEMM
+  const size_t nc = (size_t)r.c * iv->d16;
+  std::vector<float> hc(nc), hn(r.c);
+  MANN_HIP(hipMemcpy(hc.data(), iv->c32, sizeof(float) * nc, hipMemcpyDeviceToHost));
+  MANN_HIP(hipMemcpy(hn.data(), iv->cn2, sizeof(float) * r.c, hipMemcpyDeviceToHost));
+  const float g2 = 2.0f * gamma;
+  for (float& v : hc) v *= g2;
+  for (float& v : hn) v *= -gamma;
+  MANN_HIP(hipMalloc(&r.pc2, sizeof(float) * nc));
+  MANN_HIP(hipMalloc(&r.pb, sizeof(float) * (size_t)r.c));
+  MANN_HIP(hipMemcpy(r.pc2, hc.data(), sizeof(float) * nc, hipMemcpyHostToDevice));
+  MANN_HIP(hipMemcpy(r.pb, hn.data(), sizeof(float) * r.c, hipMemcpyHostToDevice));
 }
 
 // Z = W2 relu(W1 x + b1) + b2 (+ the centroid prior) for B rows of X (row
@@ -99,8 +113,29 @@ static void router_forward(const mann_router& r, const float* X, int64_t ldx, in
   g2.K = r.h;
   g2.bias = r.b2();
   launch_gemm(g2, st);
-  if (r.gamma != 0.0f && r.pc)
-    launch_prior(X, ldx, B, r.d, r.pc, r.pld, r.pcn, r.c, r.gamma, Z, r.c, st);
+  if (r.gamma != 0.0f && r.pc) {
+    // the prior on the MFMA with split operands (FP32-grade products), else
+    // the FP32 kernel
+    GemmArgs gp{};
+    gp.A = X;
+    gp.sam = ldx;
+    gp.sak = 1;
+    gp.B = r.pc2;
+    gp.sbn = r.pld;
+    gp.sbk = 1;
+    gp.C = Z;
+    gp.ldc = r.c;
+    gp.M = B;
+    gp.N = r.c;
+    gp.K = r.d;
+    gp.bias = r.pb;
+    gp.accumulate = 1;
+    gp.split = 1;
+    if (gemm_tiled_ok(gp))
+      launch_gemm(gp, st);
+    else
+      launch_prior(X, ldx, B, r.d, r.pc, r.pld, r.pcn, r.c, r.gamma, Z, r.c, st);
+  }
 }
 
 // Gradients of the mean cross-entropy into grad (same layout as the parameters).
@@ -160,7 +195,9 @@ static void router_backward(const mann_router& r, const float* X, int64_t ldx, i
 void router_scores(const mann_router& r, const float* X, int64_t ldx, int64_t n, float* out) {
   ThreadCtx& c = ctx();
   hipStream_t st = c.stream;
-  const int CH = 8192;
+  // balanced chunks of at most 65,536 rows (the hidden layer is kept per chunk)
+  const int64_t nch = std::max<int64_t>(1, ceil_div(n, (int64_t)65536));
+  const int CH = (int)ceil_div(n, nch);
   float* H = c.buf("rt_H").as<float>((size_t)CH * r.h);
   ScopedTime tm("router_forward", st);
   for (int64_t s = 0; s < n; s += CH) {

@@ -78,6 +78,111 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
   }
 }
 
+// Tiled form for operands whose K axis is unit-stride (the forward pass and
+// the centroid prior): 128 x 128 output tiles on 4 wavefronts (each 64 x 64,
+// 2 x 2 MFMA blocks), K chunks of 64 (32 when split) read as float4 one chunk ahead into
+// registers and rounded to bf16 (RNE) when written to LDS.  SPLIT: each
+// operand also keeps its rounding remainder lo = bf16(x - hi), so x = hi + lo
+// to 16 significant bits, and the product hi.hi + hi.lo + lo.hi is accurate
+// to about 2^-15 relative: an FP32-grade dot product on the MFMA.
+constexpr int TT = 128;
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void gemm_tiled_kernel(GemmArgs g) {
+  constexpr int NS = SPLIT ? 2 : 1, TK = SPLIT ? 32 : 64, TLD = TK + 8;
+  constexpr int NP = TT * TK / 4 / 256;  // float4 per thread and operand
+  constexpr int RS = TK / 4;             // float4 per tile row
+  __shared__ __align__(16) bf16_t As[NS][TT][TLD];
+  __shared__ __align__(16) bf16_t Bs[NS][TT][TLD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int m0 = blockIdx.y * TT, n0 = blockIdx.x * TT;
+  // float4 f = tid + 256 p covers tile row f / RS, k offset 4 (f % RS)
+  float4 ra[NP], rb[NP];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int f = tid + 256 * p, r = f / RS, k = k0 + (f % RS) * 4;
+      const int m = m0 + r, n = n0 + r;
+      ra[p] = (m < g.M && k < g.K) ? *(const float4*)(g.A + (int64_t)m * g.sam + k)
+                                   : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      rb[p] = (n < g.N && k < g.K) ? *(const float4*)(g.B + (int64_t)n * g.sbn + k)
+                                   : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+  };
+  auto put = [&](bf16_t (*L)[TT][TLD], int r, int kk, float4 v) {
+    const float x[4] = {v.x, v.y, v.z, v.w};
+    typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+    bf16x4 h, l;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      h[c] = (bf16_t)x[c];
+      if (SPLIT) l[c] = (bf16_t)(x[c] - (float)h[c]);
+    }
+    *(bf16x4*)&L[0][r][kk] = h;
+    if (SPLIT) *(bf16x4*)&L[1][r][kk] = l;
+  };
+  floatx16 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
+  load(0);
+  for (int k0 = 0; k0 < g.K; k0 += TK) {
+    __syncthreads();  // the previous chunk's fragments have been read
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int f = tid + 256 * p, r = f / RS, kk = (f % RS) * 4;
+      put(As, r, kk, ra[p]);
+      put(Bs, r, kk, rb[p]);
+    }
+    __syncthreads();
+    if (k0 + TK < g.K) load(k0 + TK);
+#pragma unroll
+    for (int ks = 0; ks < TK / 16; ++ks) {
+      const int koff = ks * 16 + (lane >> 5) * 8;
+      bf16x8 fa[NS][2], fb[NS][2];
+#pragma unroll
+      for (int s2 = 0; s2 < NS; ++s2)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          fa[s2][i] = *(const bf16x8*)&As[s2][wm * 64 + i * 32 + (lane & 31)][koff];
+          fb[s2][i] = *(const bf16x8*)&Bs[s2][wn * 64 + i * 32 + (lane & 31)][koff];
+        }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][i], fb[0][j], acc[i][j], 0, 0, 0);
+          if (SPLIT) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][i], fb[NS - 1][j], acc[i][j],
+                                                                 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[NS - 1][i], fb[0][j], acc[i][j],
+                                                                 0, 0, 0);
+          }
+        }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int n = n0 + wn * 64 + j * 32 + (lane & 31);
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int m = m0 + wm * 64 + i * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+        if (m < g.M && n < g.N) {
+          float v = acc[i][j][e];
+          if (g.bias) v += g.bias[n];
+          if (g.relu) v = v > 0.0f ? v : 0.0f;
+          if (g.accumulate) v += g.C[(int64_t)m * g.ldc + n];
+          g.C[(int64_t)m * g.ldc + n] = v;
+        }
+      }
+    }
+}
+
 // One workgroup per row: softmax over the logits, cross-entropy against the
 // label distribution (counts / L of the row's list labels), gradient
 // (softmax - target) / batch written in place of the logits.
@@ -175,8 +280,22 @@ __global__ void neg_kernel(const float* __restrict__ s, float* __restrict__ d, i
 
 }  // namespace
 
+bool gemm_tiled_ok(const GemmArgs& g) {
+  return g.sak == 1 && g.sbk == 1 && g.sam % 4 == 0 && g.sbn % 4 == 0 && g.K % 4 == 0 &&
+         ((uintptr_t)g.A & 15) == 0 && ((uintptr_t)g.B & 15) == 0;
+}
+
 void launch_gemm(const GemmArgs& g, hipStream_t st) {
   if (g.M <= 0 || g.N <= 0) return;
+  if (gemm_tiled_ok(g)) {
+    dim3 grid((unsigned)ceil_div(g.N, TT), (unsigned)ceil_div(g.M, TT));
+    if (g.split)
+      hipLaunchKernelGGL(gemm_tiled_kernel<true>, grid, dim3(256), 0, st, g);
+    else
+      hipLaunchKernelGGL(gemm_tiled_kernel<false>, grid, dim3(256), 0, st, g);
+    return;
+  }
+  MANN_REQUIRE(!g.split, ERR_UNSUPPORTED, "split GEMM needs unit-stride K operands");
   dim3 grid((unsigned)ceil_div(g.N, GT), (unsigned)ceil_div(g.M, GT));
   hipLaunchKernelGGL(gemm_kernel, grid, dim3(256), 0, st, g);
 }

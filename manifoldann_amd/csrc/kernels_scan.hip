@@ -311,8 +311,11 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
       // and a balanced min tree per block (independent chains); fl(min_e a_e
       // + qo) = min_e fl(a_e + qo) because rounding is monotone, so "no value
       // of this block passes" is decided exactly
+      // FILTER with folded norms: the values are S^ themselves, so one compare
+      // per value serves the block test, the emission count and the store masks
+      constexpr bool CMP = FOLD && MODE == SCAN_FILTER;
       bool pass[WN];
-      if (MODE != SCAN_DENSE) {
+      if (MODE != SCAN_DENSE && !CMP) {
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int j = 0; j < WN; ++j) {
@@ -337,6 +340,36 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
               if (row < row_end)
                 a.dense[(int64_t)gqc[j] * a.ldd + (a.dense_rel ? row - region : row)] = v;
             }
+          }
+          continue;
+        }
+        if (CMP) {
+          bool p[16];
+          uint64_t any = 0;
+#pragma unroll
+          for (int e = 0; e < 16; ++e) {
+            p[e] = val(j, e) < tau_l[j];
+            any |= __builtin_amdgcn_ballot_w64(p[e]);
+          }
+          if (any == 0) continue;  // wave-uniform
+          int cnt = 0;
+#pragma unroll
+          for (int e = 0; e < 16; ++e) cnt += p[e] ? 1 : 0;
+          int gq = gqc[j];
+          asm volatile("" : "+v"(gq));
+          const int qc = wc * 32 * WN + j * 32 + (ln & 31);
+          int pos = lds_fetch_add(&lcnt[qc], cnt);
+          unsigned* reg = a.cand + ((int64_t)gq * a.parts + region) * (int64_t)a.cap * 2;
+#pragma unroll
+          for (int e = 0; e < 16; ++e) {
+            if (!p[e]) continue;
+            if (pos < a.cap) {
+              uint2 cv;
+              cv.x = __float_as_uint(val(j, e));
+              cv.y = (unsigned)(rb0 + 8 * (e >> 2) + (e & 3));
+              *(uint2*)(reg + 2 * pos) = cv;
+            }
+            ++pos;
           }
           continue;
         }

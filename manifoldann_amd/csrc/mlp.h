@@ -14,8 +14,11 @@ struct GemmArgs {
   const float* bias;    // optional, per n
   int relu;             // clamp at 0 after the bias
   int accumulate;       // C += result
+  int split;            // FP32-accurate product (bf16 hi/lo split); needs gemm_tiled_ok
 };
 
+// unit-stride K operands, 16-byte aligned rows: the tiled kernel applies
+bool gemm_tiled_ok(const GemmArgs& g);
 void launch_gemm(const GemmArgs& g, hipStream_t st);
 void launch_softmax_xent(float* Z, int ncls, int batch, const int* labels, int L, float* loss,
                          hipStream_t st);

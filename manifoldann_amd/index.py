@@ -142,13 +142,14 @@ class Dataset:
         self.handle = h
 
     def __del__(self):
-        h = getattr(self, "handle", None)
-        if h is not None and h.value and not shutting_down():
-            try:
+        # at interpreter exit the module globals may already be gone
+        try:
+            h = getattr(self, "handle", None)
+            if h is not None and h.value and not shutting_down():
                 lib().mann_dataset_destroy(h)
-            except Exception:
-                pass
-            self.handle = None
+                self.handle = None
+        except Exception:
+            pass
 
 
 def _dataset_for(data, metric) -> Dataset:

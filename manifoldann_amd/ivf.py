@@ -310,13 +310,14 @@ class MultiLevelIndex:
         return [ids[off[b]:off[b + 1]] for b in range(self.n_buckets) if self.list_sizes[b] > 0]
 
     def __del__(self):
-        h = getattr(self, "handle", None)
-        if h is not None and h.value and not shutting_down():
-            try:
+        # at interpreter exit the module globals may already be gone
+        try:
+            h = getattr(self, "handle", None)
+            if h is not None and h.value and not shutting_down():
                 lib().mann_ivf_destroy(h)
-            except Exception:
-                pass
-            self.handle = None
+                self.handle = None
+        except Exception:
+            pass
 
 
 def build_index(index_type, X, config=None, merge_strategy=None, distance=default_distance,

@@ -29,13 +29,14 @@ class LearnedProbeRouter:
         self.gamma = 0.0
 
     def __del__(self):
-        h = getattr(self, "handle", None)
-        if h is not None and h.value and not shutting_down():
-            try:
+        # at interpreter exit the module globals may already be gone
+        try:
+            h = getattr(self, "handle", None)
+            if h is not None and h.value and not shutting_down():
                 lib().mann_router_destroy(h)
-            except Exception:
-                pass
-            self.handle = None
+                self.handle = None
+        except Exception:
+            pass
 
     @property
     def n_params(self) -> int:
