@@ -22,6 +22,8 @@ struct mann_dataset {
   int m = 0, m_pad = 0;
   mann::bf16_t* sb = nullptr;
   float* sxn = nullptr;
+  // the sample as a query operand (folded form; null when not folded: sb)
+  mann::bf16_t* sbq = nullptr;
   // columns d..d+2 of xb (and sb) hold |x~|^2 as three bf16 parts summing to
   // it exactly, d+3..d+5 hold 1 (first pass with folded norms); qfold: the
   // rows as folded query operands (-2 x~, 1, 1, 1, parts of |x~|^2) for

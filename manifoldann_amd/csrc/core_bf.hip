@@ -99,7 +99,7 @@ void add_stats(const int* slots, SearchStats& s) {
 
 mann_dataset::~mann_dataset() {
   (void)hipSetDevice(device);
-  void* ps[] = {x32, xb, xn, err, nrm, xn2, sb, sxn, center, qfold};
+  void* ps[] = {x32, xb, xn, err, nrm, xn2, sb, sxn, center, qfold, sbq};
   for (void* p : ps)
     if (p) (void)hipFree(p);
 }
@@ -167,6 +167,10 @@ mann_dataset* create_dataset(const float* data, int64_t d, int64_t n, int64_t ld
     MANN_HIP(hipMalloc(&ds->sb, sizeof(bf16_t) * (size_t)ds->m_pad * ds->kp));
     MANN_HIP(hipMalloc(&ds->sxn, sizeof(float) * (size_t)ds->m_pad));
     gather_rows_bf16(ds->xb, ds->xn, didx, ds->m, ds->m_pad, (int)ds->kp, ds->sb, ds->sxn, st);
+    if (ds->fold) {  // (reads the first d columns of sb only)
+      MANN_HIP(hipMalloc(&ds->sbq, sizeof(bf16_t) * (size_t)ds->m_pad * ds->kp));
+      launch_fold_queries(ds->sb, ds->sxn, ds->m, ds->m_pad, (int)ds->kp, (int)d, ds->sbq, st);
+    }
     unsigned hm[2];
     MANN_HIP(hipMemcpyAsync(hm, mx, sizeof(hm), hipMemcpyDeviceToHost, st));
     MANN_HIP(hipStreamSynchronize(st));
@@ -313,8 +317,42 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
     const int64_t qc_max = std::max<int64_t>(BN, ((int64_t)1 << 28) / ds.m / BN * BN);
     const int qc = (int)std::min<int64_t>(nq, qc_max);
     float* dense = c.buf("bf_dense").as<float>((size_t)qc * ds.m);
+    // self-join: the rows are the queries and the sample rows the query
+    // columns, so each store instruction writes whole 128-byte row segments of
+    // the [query][sample] block (S^ is the same quantity either way round)
+    const bool swap = Q.qb == ds.xb;
     for (int q0 = 0; q0 < nq; q0 += qc) {
       const int nqc = std::min(qc, nq - q0);
+      if (swap) {
+        ScanArgs a{};
+        a.cfg = cfg;
+        a.xb = ds.xb + (int64_t)q0 * ds.kp;
+        a.xn = ds.xn + q0;
+        a.qb = ds.fold ? ds.sbq : ds.sb;
+        a.qnorm = ds.sxn;
+        a.fold = ds.fold;
+        a.kp = (int)ds.kp;
+        a.d = (int)ds.d;
+        a.nq = ds.m;
+        const int qt = (int)ceil_div(ds.m, BN);
+        const int pd = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nqc, BM),
+                                                                    ceil_div(256, qt)));
+        a.rows_per_part = (int)round_up(ceil_div(nqc, pd), BM);
+        a.parts = (int)ceil_div(nqc, a.rows_per_part);
+        a.qtiles = qt;
+        a.n_rows = nqc;
+        a.dense = dense;
+        a.ldd = ds.m;
+        a.dense_t = 1;
+        {
+          ScopedTime tm("scan_dense", st);
+          launch_scan(a, SCAN_DENSE, qt * a.parts, SCAN_KIND_GRID, st);
+        }
+        TauArgs ta{dense, (int64_t)ds.m, ds.m, rank, tau + q0, nqc};
+        ScopedTime tm("tau_sample", st);
+        launch_tau_from_sample(ta, st);
+        continue;
+      }
       ScanArgs a{};
       a.cfg = cfg;
       a.xb = ds.sb;

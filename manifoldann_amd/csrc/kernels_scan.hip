@@ -337,8 +337,12 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
             for (int e = 0; e < 16; ++e) {
               const int row = rb0 + 8 * (e >> 2) + (e & 3);
               const float v = FOLD ? val(j, e) : val(j, e) + qo[j];
-              if (row < row_end)
-                a.dense[(int64_t)gqc[j] * a.ldd + (a.dense_rel ? row - region : row)] = v;
+              if (row < row_end) {
+                if (a.dense_t)
+                  a.dense[(int64_t)row * a.ldd + gqc[j]] = v;
+                else
+                  a.dense[(int64_t)gqc[j] * a.ldd + (a.dense_rel ? row - region : row)] = v;
+              }
             }
           }
           continue;

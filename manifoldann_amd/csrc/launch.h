@@ -47,10 +47,12 @@ struct ScanArgs {
   int cap;            // capacity of one region
   // DENSE output: dense[q * ldd + row] (row: absolute index in the scanned rows),
   // or dense[q * ldd + row - region] when dense_rel (work items carrying their
-  // list's first row as region)
+  // list's first row as region); dense_t: dense[row * ldd + q] instead (the
+  // scanned rows are the queries, the query columns the sample: coalesced)
   float* dense;
   int64_t ldd;
   int dense_rel;
+  int dense_t;
   // TOP1 certification inputs
   const float* t1_qe;
   const float* t1_qr;

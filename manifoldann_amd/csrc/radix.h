@@ -42,67 +42,103 @@ __device__ __forceinline__ void wave_sort_u32(uint32_t (&v)[R], int lane) {
   }
 }
 
+// The rank-th smallest of the cnt (rank <= cnt <= 256) keys collected in buf.
+__device__ __forceinline__ uint32_t wave_rank_of_collected(const uint32_t* buf, int cnt, int rank,
+                                                           int lane) {
+  wave_fence_sync();
+  if (cnt <= 64) {  // the usual case: a one-register sort
+    uint32_t v1[1] = {lane < cnt ? buf[lane] : 0xffffffffu};
+    wave_fence_sync();
+    wave_sort_u32<1>(v1, lane);
+    return (uint32_t)__shfl((int)v1[0], rank - 1);
+  }
+  uint32_t v[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) v[j] = j * 64 + lane < cnt ? buf[j * 64 + lane] : 0xffffffffu;
+  wave_fence_sync();
+  wave_sort_u32<4>(v, lane);
+  const int e = rank - 1;
+  uint32_t r = v[0];
+#pragma unroll
+  for (int j = 1; j < 4; ++j)
+    if (e / 64 == j) r = v[j];
+  return (uint32_t)__shfl((int)r, e % 64);
+}
+
+// Rows of at most 64 P values (rank <= 64): the row stays in registers
+// between the two passes.  The loads are unconditional (indices clamped into
+// the row), so straight-line code keeps all of them in flight at once.
+// Returns false when more than 256 values qualify (the caller falls back).
+template <int P>
+__device__ __forceinline__ bool wave_rank_cached(const float* row, int m, int rank, uint32_t* buf,
+                                                 int lane, uint32_t& key) {
+  float x[P];
+#pragma unroll
+  for (int t = 0; t < P; ++t) {
+    const int i = t * 64 + lane;
+    x[t] = row[i < m ? i : m - 1];
+  }
+  uint32_t o[P];
+  uint32_t mn[1] = {0xffffffffu};
+#pragma unroll
+  for (int t = 0; t < P; ++t) {
+    o[t] = t * 64 + lane < m ? f2ord(x[t]) : 0xffffffffu;
+    mn[0] = min(mn[0], o[t]);
+  }
+  wave_sort_u32<1>(mn, lane);
+  const uint32_t U = (uint32_t)__shfl((int)mn[0], rank - 1);
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  int cnt = 0;
+  // sentinels (0xffffffff) lie above every finite or infinite value's key
+#pragma unroll
+  for (int t = 0; t < P; ++t) {
+    if (t * 64 >= m) break;
+    const bool p = o[t] <= U;
+    const unsigned long long b = __ballot(p);
+    const int pos = cnt + __popcll(b & lt);
+    if (p && pos < 256) buf[pos] = o[t];
+    cnt += __popcll(b);
+  }
+  if (cnt > 256) return false;
+  key = wave_rank_of_collected(buf, cnt, rank, lane);
+  return true;
+}
+
 // rank-th smallest (1-based, clamped to [1, m]) of the m floats row[0..m) as
 // an order key, by one wavefront; buf (256 keys) and hist (256 counters) are
 // the wavefront's LDS.  For rank <= 64 the rank-th smallest of the 64 lane
 // minima bounds the answer from above (they are rank distinct elements); the
 // values at or below that bound (usually a few more than rank) are collected
 // and sorted.  Otherwise, or when more than 256 qualify: a 4-pass radix select
-// over all m values.
-__device__ inline uint32_t wave_rank_key(const float* row, int m, int rank, uint32_t* buf,
-                                         unsigned* hist, int lane) {
+// over all m values.  (Inlined into its kernels, so the row is known to be
+// global memory and the scratch LDS.)
+__device__ __forceinline__ uint32_t wave_rank_key(const float* row, int m, int rank, uint32_t* buf,
+                                                  unsigned* hist, int lane) {
+  if (m <= 0) return 0xffffffffu;  // no values: the largest key
   rank = rank < 1 ? 1 : (rank > m ? m : rank);
   if (rank <= 64) {
-    // rows of up to 4096 values stay in registers between the two passes
-    constexpr int P = 64;
-    const bool cached = m <= 64 * P;
-    uint32_t o[P];
-    uint32_t mn[1] = {0xffffffffu};
-    if (cached) {
-#pragma unroll
-      for (int t = 0; t < P; ++t) {
-        const int i = t * 64 + lane;
-        o[t] = i < m ? f2ord(row[i]) : 0xffffffffu;
-        mn[0] = min(mn[0], o[t]);
-      }
+    uint32_t key;
+    if (m <= 64 * 32) {
+      if (wave_rank_cached<32>(row, m, rank, buf, lane, key)) return key;
+    } else if (m <= 64 * 64) {
+      if (wave_rank_cached<64>(row, m, rank, buf, lane, key)) return key;
     } else {
+      uint32_t mn[1] = {0xffffffffu};
       for (int i = lane; i < m; i += 64) mn[0] = min(mn[0], f2ord(row[i]));
-    }
-    wave_sort_u32<1>(mn, lane);
-    const uint32_t U = (uint32_t)__shfl((int)mn[0], rank - 1);
-    const unsigned long long lt = (1ull << lane) - 1ull;
-    int cnt = 0;
-    auto take = [&](bool p, uint32_t v) {
-      const unsigned long long b = __ballot(p);
-      const int pos = cnt + __popcll(b & lt);
-      if (p && pos < 256) buf[pos] = v;
-      cnt += __popcll(b);
-    };
-    if (cached) {
-      // sentinels (0xffffffff) lie above every finite or infinite value's key
-#pragma unroll
-      for (int t = 0; t < P; ++t)
-        if (t * 64 < m) take(o[t] <= U, o[t]);
-    } else {
+      wave_sort_u32<1>(mn, lane);
+      const uint32_t U = (uint32_t)__shfl((int)mn[0], rank - 1);
+      const unsigned long long lt = (1ull << lane) - 1ull;
+      int cnt = 0;
       for (int i0 = 0; i0 < m; i0 += 64) {
         const int i = i0 + lane;
         const uint32_t v = i < m ? f2ord(row[i]) : 0xffffffffu;
-        take(i < m && v <= U, v);
+        const bool p = i < m && v <= U;
+        const unsigned long long b = __ballot(p);
+        const int pos = cnt + __popcll(b & lt);
+        if (p && pos < 256) buf[pos] = v;
+        cnt += __popcll(b);
       }
-    }
-    if (cnt <= 256) {
-      wave_fence_sync();
-      uint32_t v[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) v[j] = j * 64 + lane < cnt ? buf[j * 64 + lane] : 0xffffffffu;
-      wave_fence_sync();
-      wave_sort_u32<4>(v, lane);
-      const int e = rank - 1;
-      uint32_t r = v[0];
-#pragma unroll
-      for (int j = 1; j < 4; ++j)
-        if (e / 64 == j) r = v[j];
-      return (uint32_t)__shfl((int)r, e % 64);
+      if (cnt <= 256) return wave_rank_of_collected(buf, cnt, rank, lane);
     }
   }
   uint32_t prefix = 0, mask = 0;
@@ -161,7 +197,8 @@ struct BlockRankScratch {
   uint32_t key;
 };
 
-__device__ inline uint32_t block_rank_key(const float* row, int m, int rank, BlockRankScratch& sm) {
+__device__ __forceinline__ uint32_t block_rank_key(const float* row, int m, int rank,
+                                                   BlockRankScratch& sm) {
   rank = rank < 1 ? 1 : (rank > m ? m : rank);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   if (rank <= 64 && m >= 4 * 4096) {
