@@ -313,14 +313,20 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
                                              cap_total / 2);
     const int rank = (int)std::max<int64_t>(
         1, std::min<int64_t>(ds.m, (target * ds.m + n / 2) / n));
+    // a long sample (rank far below the number of 32-row blocks): only the
+    // block minima are stored; the rank-th smallest of them equals the
+    // sample's rank-th smallest unless two of its rank smallest values share
+    // a block, and is never below it (a threshold estimate either way)
+    const bool bmin = ds.m / 32 >= 64 * rank;
+    const int mcols = bmin ? (int)ceil_div(ds.m, 32) : ds.m;
     // the dense sample block is processed in query chunks of at most 2^28 values
-    const int64_t qc_max = std::max<int64_t>(BN, ((int64_t)1 << 28) / ds.m / BN * BN);
+    const int64_t qc_max = std::max<int64_t>(BN, ((int64_t)1 << 28) / mcols / BN * BN);
     const int qc = (int)std::min<int64_t>(nq, qc_max);
-    float* dense = c.buf("bf_dense").as<float>((size_t)qc * ds.m);
+    float* dense = c.buf("bf_dense").as<float>((size_t)qc * mcols);
     // self-join: the rows are the queries and the sample rows the query
     // columns, so each store instruction writes whole 128-byte row segments of
     // the [query][sample] block (S^ is the same quantity either way round)
-    const bool swap = Q.qb == ds.xb;
+    const bool swap = Q.qb == ds.xb && !bmin;
     for (int q0 = 0; q0 < nq; q0 += qc) {
       const int nqc = std::min(qc, nq - q0);
       if (swap) {
@@ -373,12 +379,13 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
       a.qtiles = qt;
       a.n_rows = ds.m;
       a.dense = dense;
-      a.ldd = ds.m;
+      a.ldd = mcols;
+      a.dense_bmin = bmin;
       {
         ScopedTime tm("scan_dense", st);
         launch_scan(a, SCAN_DENSE, qt * a.parts, SCAN_KIND_GRID, st);
       }
-      TauArgs ta{dense, (int64_t)ds.m, ds.m, rank, tau + q0, nqc};
+      TauArgs ta{dense, (int64_t)mcols, mcols, rank, tau + q0, nqc};
       ScopedTime tm("tau_sample", st);
       launch_tau_from_sample(ta, st);
     }

@@ -332,6 +332,19 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
 #pragma unroll
       for (int j = 0; j < WN; ++j) {
         if (MODE == SCAN_DENSE) {
+          if (a.dense_bmin) {
+            // the minimum of each 32-row block: 16 of its values here, 16 in
+            // lane ^ 32 (fl(min a + qo) = min fl(a + qo): monotone rounding)
+            float mn = val(j, 0);
+#pragma unroll
+            for (int e = 1; e < 16; ++e) mn = fminf(mn, val(j, e));
+            if (!FOLD) mn += qo[j];
+            mn = fminf(mn, __shfl_xor(mn, 32));
+            const int blk = (rb0 - 4 * (ln >> 5)) >> 5;
+            if (ln < 32 && gqc[j] >= 0 && (blk << 5) < row_end)
+              a.dense[(int64_t)gqc[j] * a.ldd + blk] = mn;
+            continue;
+          }
           if (gqc[j] >= 0) {
 #pragma unroll
             for (int e = 0; e < 16; ++e) {

@@ -53,6 +53,9 @@ struct ScanArgs {
   int64_t ldd;
   int dense_rel;
   int dense_t;
+  // dense_bmin: dense[q * ldd + row / 32] = the minimum over each 32-row block
+  // (row partitions start at multiples of 32)
+  int dense_bmin;
   // TOP1 certification inputs
   const float* t1_qe;
   const float* t1_qr;

@@ -2,6 +2,8 @@
 a Float16 base (upcast exactly to Float32), cosine, k=100, and the query-chunked
 threshold estimation.  Ids and distances bit-identical to the oracle's FP32
 brute force on the upcast data."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -35,16 +37,36 @@ def test_chunked_threshold_sample_bit_exact():
     # a 65,536-row sample and 5,000 queries: the dense sample block is computed
     # in query chunks (2^28 values each)
     _banner("chunked threshold sample 100000x32, 5000 queries")
+    ids, dist, X, Q = _search_with_sample(65536, 100_000, 5_000, 16, seed=7)
+    oi, od = O.bf_knn(X, Q, 16)
+    assert (ids == oi).all(), np.argwhere(ids != oi)[:5]
+    assert (dist.view(np.uint32) == od.view(np.uint32)).all()
+
+
+def _search_with_sample(sample_rows, n, nq, k, seed):
     L = lib()
-    check(L.mann_tune_set(b"sample_rows", 65536))
+    old = ctypes.c_int64()
+    check(L.mann_tune_get(b"sample_rows", ctypes.byref(old)))
+    check(L.mann_tune_set(b"sample_rows", sample_rows))
     try:
-        X = datasets.gaussian(100_000, 32, seed=7)
-        Q = datasets.gaussian(5_000, 32, seed=8)
+        X = datasets.gaussian(n, 32, seed=seed)
+        Q = datasets.gaussian(nq, 32, seed=seed + 1)
         ds = M.Dataset(X.T)
         index = M.build_index(M.BruteForceIndex, X.T)
-        ids, dist = M.query_arrays(index, ds, Q.T, 16)
+        ids, dist = M.query_arrays(index, ds, Q.T, k)
     finally:
-        check(L.mann_tune_set(b"sample_rows", 4096))
-    oi, od = O.bf_knn(X, Q, 16)
+        check(L.mann_tune_set(b"sample_rows", old.value))
+    return ids, dist, X, Q
+
+
+def test_block_minimum_threshold_sample_bit_exact():
+    # a 65,536-row sample and k = 2: the threshold rank (about 20) lies far
+    # below the 2,048 blocks of 32 sample rows, so only the block minima are
+    # stored and ranked
+    _banner("block-minimum threshold sample 100000x32, 3000 queries, k=2")
+    ids, dist, X, Q = _search_with_sample(65536, 100_000, 3_000, 2, seed=11)
+    st = M.last_stats()
+    print(f"stats={st}", flush=True)
+    oi, od = O.bf_knn(X, Q, 2)
     assert (ids == oi).all(), np.argwhere(ids != oi)[:5]
     assert (dist.view(np.uint32) == od.view(np.uint32)).all()
