@@ -116,6 +116,7 @@ struct Tile {
   // in a ring of NXB tiles (a slot is rewritten only after a barrier that
   // follows the epilogue which read it, whatever the chunks per tile)
   static constexpr int NXB = 4;
+  static_assert(WR * WC >= BM / 64, "one wavefront stages the norms of each 64 rows");
   static constexpr size_t LDS_BYTES =
       (size_t)STAGES * (BM + BN) * LROW * sizeof(bf16_t) + BN * 4 + (size_t)NXB * BM * 4;
   static constexpr int NPC = A_CHUNKS + B_CHUNKS;       // DMA ops per thread per chunk
@@ -613,13 +614,14 @@ void launch_t(const ScanArgs& a, int grid, int kind, hipStream_t st) {
 
 // Tile configurations: WR x WC wavefronts of 32*WM rows x 32*WN queries.
 // Configs 6 and 7 repeat 5 and 0 with a 4-deep LDS ring (3 chunks in flight);
-// config 8 (128 rows x 64 queries) suits items with few queries.
+// configs 8 (128 rows x 64 queries) and 9 (128 x 32 on two wavefronts) suit
+// items with few queries.
 struct ScanCfg {
   int wm, wn, wr, wc, stages;
 };
 static const ScanCfg kCfg[] = {{2, 2, 2, 2, 3}, {4, 2, 2, 2, 3}, {2, 4, 2, 2, 3}, {4, 4, 2, 2, 3},
                                {4, 2, 2, 4, 3}, {2, 4, 4, 2, 3}, {2, 4, 4, 2, 4}, {2, 2, 2, 2, 4},
-                               {2, 1, 2, 2, 3}};
+                               {2, 1, 2, 2, 3}, {2, 1, 2, 1, 3}};
 constexpr int kNumCfg = sizeof(kCfg) / sizeof(kCfg[0]);
 
 int scan_num_cfg() { return kNumCfg; }
@@ -657,6 +659,7 @@ void launch_scan(const ScanArgs& a, int mode, int grid, int kind, hipStream_t st
     case 6: launch_cfg<2, 4, 4, 2, 4>(a, mode, grid, kind, st); break;
     case 7: launch_cfg<2, 2, 2, 2, 4>(a, mode, grid, kind, st); break;
     case 8: launch_cfg<2, 1, 2, 2, 3>(a, mode, grid, kind, st); break;
+    case 9: launch_cfg<2, 1, 2, 1, 3>(a, mode, grid, kind, st); break;
     default: launch_cfg<2, 2, 2, 2, 3>(a, mode, grid, kind, st); break;
   }
 }

@@ -123,20 +123,41 @@ __device__ __forceinline__ uint32_t wave_rank_key(const float* row, int m, int r
     } else if (m <= 64 * 64) {
       if (wave_rank_cached<64>(row, m, rank, buf, lane, key)) return key;
     } else {
+      // two streaming passes, eight loads in flight per lane (clamped indices)
+      constexpr int U8 = 8;
       uint32_t mn[1] = {0xffffffffu};
-      for (int i = lane; i < m; i += 64) mn[0] = min(mn[0], f2ord(row[i]));
+      for (int i0 = 0; i0 < m; i0 += 64 * U8) {
+        float x[U8];
+#pragma unroll
+        for (int u = 0; u < U8; ++u) {
+          const int i = i0 + 64 * u + lane;
+          x[u] = row[i < m ? i : m - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < U8; ++u)
+          if (i0 + 64 * u + lane < m) mn[0] = min(mn[0], f2ord(x[u]));
+      }
       wave_sort_u32<1>(mn, lane);
       const uint32_t U = (uint32_t)__shfl((int)mn[0], rank - 1);
       const unsigned long long lt = (1ull << lane) - 1ull;
       int cnt = 0;
-      for (int i0 = 0; i0 < m; i0 += 64) {
-        const int i = i0 + lane;
-        const uint32_t v = i < m ? f2ord(row[i]) : 0xffffffffu;
-        const bool p = i < m && v <= U;
-        const unsigned long long b = __ballot(p);
-        const int pos = cnt + __popcll(b & lt);
-        if (p && pos < 256) buf[pos] = v;
-        cnt += __popcll(b);
+      for (int i0 = 0; i0 < m; i0 += 64 * U8) {
+        float x[U8];
+#pragma unroll
+        for (int u = 0; u < U8; ++u) {
+          const int i = i0 + 64 * u + lane;
+          x[u] = row[i < m ? i : m - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < U8; ++u) {
+          const int i = i0 + 64 * u + lane;
+          const uint32_t v = i < m ? f2ord(x[u]) : 0xffffffffu;
+          const bool p = i < m && v <= U;
+          const unsigned long long b = __ballot(p);
+          const int pos = cnt + __popcll(b & lt);
+          if (p && pos < 256) buf[pos] = v;
+          cnt += __popcll(b);
+        }
       }
       if (cnt <= 256) return wave_rank_of_collected(buf, cnt, rank, lane);
     }
@@ -210,7 +231,17 @@ __device__ __forceinline__ uint32_t block_rank_key(const float* row, int m, int 
     if (tid == 0) sm.key = min(min(sm.mins[0], sm.mins[1]), min(sm.mins[2], sm.mins[3]));
   } else if (rank <= 256) {
     uint32_t mn = 0xffffffffu;
-    for (int i = tid; i < m; i += 256) mn = min(mn, f2ord(row[i]));
+    for (int i0 = 0; i0 < m; i0 += 256 * 8) {
+      float x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = i0 + 256 * u + tid;
+        x[u] = row[i < m ? i : m - 1];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (i0 + 256 * u + tid < m) mn = min(mn, f2ord(x[u]));
+    }
     sm.mins[tid] = mn;
     if (tid == 0) sm.cnt = 0;
     __syncthreads();

@@ -40,7 +40,7 @@ def test_status_calls_without_gpu():
     assert L.mann_tune_get(b"scan_cfg", ctypes.byref(v)) == 0
     assert L.mann_tune_set(b"no_such_knob", 1) == -2
     assert b"unknown tuning key" in L.mann_last_error()
-    assert L.mann_tune_set(b"scan_cfg", 9) == -2
+    assert L.mann_tune_set(b"scan_cfg", 10) == -2
 
 
 def test_argument_errors_map_to_reference_exceptions():
