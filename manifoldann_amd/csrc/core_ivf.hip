@@ -20,7 +20,6 @@
 
 namespace mann {
 
-static constexpr int ROWS_ITEM = 2048;
 
 mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int64_t* assign_host) {
   ThreadCtx& c = ctx();
@@ -143,6 +142,7 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
                "k too large for the GPU selection buffer (max 256)");
   const int cfg = tuning().ivf_cfg;
   const int BN = scan_tile_queries(cfg);
+  const int ROWS_ITEM = tuning().ivf_rows_item;  // rows per work item
   const float* R = route_in;
   if (!R) {
     float* Rb = c.buf("iv_R").as<float>((size_t)nq * iv.nlist);

@@ -22,13 +22,16 @@ namespace {
 constexpr int RT_T = 32;
 constexpr int RT_DC = 64;
 constexpr int RT_LD = RT_DC + 16;  // row stride: adjacent rows 16 banks apart
+// Two LDS buffers: chunk c+1 is loaded into registers while chunk c is
+// multiplied out from LDS, then written to the other buffer (one barrier per
+// chunk).
 __global__ __launch_bounds__(256) void route_dist_kernel(const float* __restrict__ q32,
                                                          const float* __restrict__ qn2, int nq,
                                                          const float* __restrict__ c32,
                                                          const float* __restrict__ cn2, int nlist,
                                                          int d16, float* __restrict__ R) {
-  __shared__ float ys[RT_T][RT_LD];
-  __shared__ float xs[RT_T][RT_LD];
+  __shared__ float ys[2][RT_T][RT_LD];
+  __shared__ float xs[2][RT_T][RT_LD];
   const int q0 = blockIdx.x * RT_T, c0 = blockIdx.y * RT_T;
   const int tid = threadIdx.x, g = tid >> 4, j = tid & 15;
   const int qa = (g >> 2) * 8, cb = g & 3;
@@ -37,40 +40,66 @@ __global__ __launch_bounds__(256) void route_dist_kernel(const float* __restrict
   for (int a = 0; a < 8; ++a)
 #pragma unroll
     for (int b = 0; b < 8; ++b) acc[a][b] = 0.0f;
-  for (int t0 = 0; t0 < d16; t0 += RT_DC) {
-    const int w = min(RT_DC, d16 - t0);  // multiple of 16
-    __syncthreads();
-    for (int i = tid; i < RT_T * (RT_DC / 4); i += 256) {
-      const int r = i / (RT_DC / 4), c4 = (i - r * (RT_DC / 4)) * 4;
-      if (c4 < w) {
-        const int q = min(q0 + r, nq - 1), c = min(c0 + r, nlist - 1);
-        *(float4*)&ys[r][c4] = *(const float4*)(q32 + (int64_t)q * d16 + t0 + c4);
-        *(float4*)&xs[r][c4] = *(const float4*)(c32 + (int64_t)c * d16 + t0 + c4);
-      }
+  // staging: float4 i = tid + 256 p (p < 2) covers row i / 16, floats 4 (i % 16)
+  float4 ry[2], rx[2];
+  auto load = [&](int t0) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int i = tid + 256 * p, r = i >> 4, c4 = (i & 15) * 4;
+      const bool in = t0 + c4 < d16;  // d16 is a multiple of 16
+      const int q = min(q0 + r, nq - 1), c = min(c0 + r, nlist - 1);
+      ry[p] = in ? *(const float4*)(q32 + (int64_t)q * d16 + t0 + c4) : make_float4(0, 0, 0, 0);
+      rx[p] = in ? *(const float4*)(c32 + (int64_t)c * d16 + t0 + c4) : make_float4(0, 0, 0, 0);
     }
-    __syncthreads();
+  };
+  auto put = [&](int b) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int i = tid + 256 * p, r = i >> 4, c4 = (i & 15) * 4;
+      *(float4*)&ys[b][r][c4] = ry[p];
+      *(float4*)&xs[b][r][c4] = rx[p];
+    }
+  };
+  // the norms of the lane's queries and centroids, loaded up front (a load
+  // inside each writer's branch of the epilogue would wait out its latency
+  // 64 times in a row)
+  float qnl[8], cnl[8];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) qnl[a] = qn2[min(q0 + qa + a, nq - 1)];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) cnl[b] = cn2[min(c0 + cb + 4 * b, nlist - 1)];
+  const int nch = (d16 + RT_DC - 1) / RT_DC;
+  load(0);
+  put(0);
+  __syncthreads();
+  for (int ch = 0; ch < nch; ++ch) {
+    const int t0 = ch * RT_DC, b = ch & 1;
+    if (ch + 1 < nch) load(t0 + RT_DC);
+    const int w = min(RT_DC, d16 - t0);  // multiple of 16
     for (int t = j; t < w; t += 16) {
       float yv[8], xv[8];
 #pragma unroll
-      for (int a = 0; a < 8; ++a) yv[a] = ys[qa + a][t];
+      for (int a = 0; a < 8; ++a) yv[a] = ys[b][qa + a][t];
 #pragma unroll
-      for (int b = 0; b < 8; ++b) xv[b] = xs[cb + 4 * b][t];
+      for (int bb = 0; bb < 8; ++bb) xv[bb] = xs[b][cb + 4 * bb][t];
 #pragma unroll
       for (int a = 0; a < 8; ++a)
 #pragma unroll
-        for (int b = 0; b < 8; ++b) acc[a][b] = __builtin_fmaf(xv[b], yv[a], acc[a][b]);
+        for (int bb = 0; bb < 8; ++bb) acc[a][bb] = __builtin_fmaf(xv[bb], yv[a], acc[a][bb]);
     }
+    if (ch + 1 < nch) put(b ^ 1);  // the other buffer: last read before the previous barrier
+    __syncthreads();
   }
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
     const int q = q0 + qa + a;
-    const float qn = q < nq ? qn2[q] : 0.0f;
+    const float qn = qnl[a];
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
       const float dot = group16_tree(acc[a][b]);
       const int c = c0 + cb + 4 * b;
       if (j == ((a * 8 + b) & 15) && q < nq && c < nlist) {
-        float tt = cn2[c] + qn;
+        float tt = cnl[b] + qn;
         float sq = tt - 2.0f * dot;
         R[(int64_t)q * nlist + c] = __builtin_sqrtf(sq > 0.0f ? sq : 0.0f);
       }
@@ -146,12 +175,22 @@ __global__ __launch_bounds__(256) void probe_select_wave_kernel(
   const int lane = threadIdx.x & 63;
   const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (q >= nq) return;
+  // the lane's distances and list sizes, every load issued before the first
+  // use (clamped indices)
   uint64_t key[PS_KPL];
+  int lsz[PS_KPL];
+  float rv[PS_KPL];
+#pragma unroll
+  for (int u = 0; u < PS_KPL; ++u) {
+    const int ic = min(lane + 64 * u, avail - 1);
+    rv[u] = R[(int64_t)q * nlist + ic];
+    lsz[u] = list_off[ic + 1] - list_off[ic];
+  }
   uint64_t lmin = ~0ull;
 #pragma unroll
   for (int u = 0; u < PS_KPL; ++u) {
     const int i = lane + 64 * u;
-    key[u] = i < avail ? pack_key(R[(int64_t)q * nlist + i], (uint32_t)i) : ~0ull;
+    key[u] = i < avail ? pack_key(rv[u], (uint32_t)i) : ~0ull;
     lmin = key[u] < lmin ? key[u] : lmin;
   }
   // softmax normaliser over the negated distances (mass mode)
@@ -180,16 +219,21 @@ __global__ __launch_bounds__(256) void probe_select_wave_kernel(
       g = t < g ? t : g;
     }
     if (mass > 0.0f) cum += __expf(rmin - ord2f((uint32_t)(g >> 32))) / zsum;
-    if (lmin == g) {  // the owner removes it (keys are unique: bucket in the low bits)
+    const bool own = lmin == g;
+    int osz = 0;
+    if (own) {  // the owner removes it (keys are unique: bucket in the low bits)
       lmin = ~0ull;
 #pragma unroll
       for (int u = 0; u < PS_KPL; ++u) {
-        if (key[u] == g) key[u] = ~0ull;
+        if (key[u] == g) {
+          key[u] = ~0ull;
+          osz = lsz[u];
+        }
         lmin = key[u] < lmin ? key[u] : lmin;
       }
     }
     const int b = (int)(g & 0xffffffffu);
-    const int sz = list_off[b + 1] - list_off[b];
+    const int sz = __shfl(osz, __ffsll((long long)__ballot(own)) - 1);
     if (sz > 0) {
       if (lane == 0) probes[(int64_t)q * nprobe + m] = b;
       ++m;

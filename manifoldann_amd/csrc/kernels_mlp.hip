@@ -65,16 +65,25 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     __syncthreads();
   }
   const int n = n0 + wn * 32 + (lane & 31);
+  const int nc = min(n, g.N - 1);
+  // every load before the first use (clamped addresses)
+  const float bias = g.bias ? g.bias[nc] : 0.0f;
+  float old[16];
+  if (g.accumulate) {
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int m = min(m0 + wm * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3), g.M - 1);
+      old[e] = g.C[(int64_t)m * g.ldc + nc];
+    }
+  }
 #pragma unroll
   for (int e = 0; e < 16; ++e) {
     const int m = m0 + wm * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-    if (m < g.M && n < g.N) {
-      float v = acc[e];
-      if (g.bias) v += g.bias[n];
-      if (g.relu) v = v > 0.0f ? v : 0.0f;
-      if (g.accumulate) v += g.C[(int64_t)m * g.ldc + n];
-      g.C[(int64_t)m * g.ldc + n] = v;
-    }
+    float v = acc[e];
+    if (g.bias) v += bias;
+    if (g.relu) v = v > 0.0f ? v : 0.0f;
+    if (g.accumulate) v += old[e];
+    if (m < g.M && n < g.N) g.C[(int64_t)m * g.ldc + n] = v;
   }
 }
 
@@ -164,23 +173,43 @@ __global__ __launch_bounds__(256) void gemm_tiled_kernel(GemmArgs g) {
         }
     }
   }
+  // epilogue: every load issued before the first use (clamped addresses),
+  // so the latencies overlap instead of adding up output by output
+  float bias[2] = {0.0f, 0.0f};
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int j = 0; j < 2; ++j) {
+    const int n = min(n0 + wn * 64 + j * 32 + (lane & 31), g.N - 1);
+    if (g.bias) bias[j] = g.bias[n];
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    float old[2][16];
+    if (g.accumulate) {
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int n = min(n0 + wn * 64 + j * 32 + (lane & 31), g.N - 1);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          const int m = min(m0 + wm * 64 + i * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3),
+                            g.M - 1);
+          old[j][e] = g.C[(int64_t)m * g.ldc + n];
+        }
+      }
+    }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int n = n0 + wn * 64 + j * 32 + (lane & 31);
 #pragma unroll
       for (int e = 0; e < 16; ++e) {
         const int m = m0 + wm * 64 + i * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-        if (m < g.M && n < g.N) {
-          float v = acc[i][j][e];
-          if (g.bias) v += g.bias[n];
-          if (g.relu) v = v > 0.0f ? v : 0.0f;
-          if (g.accumulate) v += g.C[(int64_t)m * g.ldc + n];
-          g.C[(int64_t)m * g.ldc + n] = v;
-        }
+        float v = acc[i][j][e];
+        if (g.bias) v += bias[j];
+        if (g.relu) v = v > 0.0f ? v : 0.0f;
+        if (g.accumulate) v += old[j][e];
+        if (m < g.M && n < g.N) g.C[(int64_t)m * g.ldc + n] = v;
       }
     }
+  }
 }
 
 // One workgroup per row: softmax over the logits, cross-entropy against the

@@ -122,18 +122,36 @@ __device__ __forceinline__ void exact_metric4(const SelectArgs& a, const uint32_
 #pragma unroll
   for (int u = 0; u < 4; ++u) x[u] = a.x32 + (int64_t)rows[u < nr ? u : 0] * a.d16;
   const bool diff = a.metric == EUCLIDEAN || a.metric == SQEUCLIDEAN;
-  for (int t = 0; t < a.d16; t += 16) {
-    const float yv = y[t + j];
+  auto step = [&](float yv, const float* xv) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const float xv = x[u][t + j];
       if (diff) {
-        const float df = xv - yv;
+        const float df = xv[u] - yv;
         acc[u] = __builtin_fmaf(df, df, acc[u]);
       } else {
-        acc[u] = __builtin_fmaf(xv, yv, acc[u]);
+        acc[u] = __builtin_fmaf(xv[u], yv, acc[u]);
       }
     }
+  };
+  // four steps per iteration with all 20 loads issued first (the chains stay
+  // in ascending t), so the row gathers overlap instead of waiting step by step
+  int t = 0;
+  for (; t + 64 <= a.d16; t += 64) {
+    float yv[4], xv[4][4];
+#pragma unroll
+    for (int s2 = 0; s2 < 4; ++s2) {
+      yv[s2] = y[t + 16 * s2 + j];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) xv[s2][u] = x[u][t + 16 * s2 + j];
+    }
+#pragma unroll
+    for (int s2 = 0; s2 < 4; ++s2) step(yv[s2], xv[s2]);
+  }
+  for (; t < a.d16; t += 16) {
+    float xv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) xv[u] = x[u][t + j];
+    step(y[t + j], xv);
   }
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
@@ -201,16 +219,29 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(SelectArgs a) {
   if (lane < a.parts) pre[lane] = inc - cp;  // exclusive prefix
   const int c = __shfl(inc, 63);
   wave_sync();
-  uint64_t key[KPL];
+  // candidate i lies in the last region p with pre[p] <= i (binary search;
+  // empty regions repeat a prefix); all addresses first, then all loads
+  int64_t off[KPL];
 #pragma unroll
   for (int t = 0; t < KPL; ++t) {
     const int i = t * 64 + lane;
+    int lo = 0, hi = t * 64 < c ? a.parts - 1 : 0;  // (no search past the candidates)
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (pre[mid] <= i) lo = mid;
+      else hi = mid - 1;
+    }
+    off[t] = i < c ? (((int64_t)q * a.parts + lo) * (int64_t)a.cap + (i - pre[lo])) * 2 : -1;
+  }
+  uint2 ev[KPL];
+#pragma unroll
+  for (int t = 0; t < KPL; ++t) ev[t] = *(const uint2*)(a.cand + (off[t] >= 0 ? off[t] : 0));
+  uint64_t key[KPL];
+#pragma unroll
+  for (int t = 0; t < KPL; ++t) {
     uint64_t kv = ~0ull;
-    if (i < c) {
-      int p = a.parts - 1;
-      while (p > 0 && pre[p] > i) --p;
-      const unsigned* cq = a.cand + ((int64_t)q * a.parts + p) * (int64_t)a.cap * 2;
-      const uint2 e = *(const uint2*)(cq + 2 * (i - pre[p]));
+    if (off[t] >= 0) {
+      const uint2 e = ev[t];
       kv = pack_key(__uint_as_float(e.x), e.y);
     }
     key[t] = kv;
