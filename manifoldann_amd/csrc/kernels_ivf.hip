@@ -453,13 +453,26 @@ __global__ __launch_bounds__(256) void residual_queries_kernel(
     scale = ss > 0.0 ? 1.0 / sqrt(ss) : 0.0;
   }
   double nb = 0.0, ee = 0.0;
-  for (int c = lane; c < kp; c += 64) {
-    const double t = c < d ? (double)y[c] * scale - (double)cen[c] : 0.0;
-    const bf16_t v = (bf16_t)(float)t;
-    qres[e * kp + c] = v;
-    const double bd = (double)(float)v;
-    nb += bd * bd;
-    ee += (t - bd) * (t - bd);
+  // four columns per lane per pass, their loads issued together (clamped)
+  for (int c0 = 0; c0 < kp; c0 += 256) {
+    float yv[4], cv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int cc = min(c0 + 64 * u + lane, d - 1);
+      yv[u] = y[cc];
+      cv[u] = cen[cc];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int c = c0 + 64 * u + lane;
+      if (c >= kp) break;
+      const double t = c < d ? (double)yv[u] * scale - (double)cv[u] : 0.0;
+      const bf16_t v = (bf16_t)(float)t;
+      qres[e * kp + c] = v;
+      const double bd = (double)(float)v;
+      nb += bd * bd;
+      ee += (t - bd) * (t - bd);
+    }
   }
   for (int o = 32; o >= 1; o >>= 1) {
     nb += __shfl_xor(nb, o);

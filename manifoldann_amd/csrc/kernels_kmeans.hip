@@ -12,7 +12,19 @@ namespace {
 __device__ __forceinline__ float a16_dot(const float* __restrict__ x, const float* __restrict__ y,
                                          int d16, int j) {
   float acc = 0.0f;
-  for (int t = 0; t < d16; t += 16) acc = __builtin_fmaf(x[t + j], y[t + j], acc);
+  int t = 0;
+  // eight steps per batch, their loads issued first (the chain keeps its order)
+  for (; t + 128 <= d16; t += 128) {
+    float xv[8], yv[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      xv[s] = x[t + 16 * s + j];
+      yv[s] = y[t + 16 * s + j];
+    }
+#pragma unroll
+    for (int s = 0; s < 8; ++s) acc = __builtin_fmaf(xv[s], yv[s], acc);
+  }
+  for (; t < d16; t += 16) acc = __builtin_fmaf(x[t + j], y[t + j], acc);
   return group16_tree(acc);
 }
 
@@ -38,8 +50,10 @@ __global__ __launch_bounds__(256) void top1_select_kernel(Top1Args a) {
   int bi = 0x7fffffff;
   if (craw <= a.cap) {
     const unsigned* cq = a.cand + p * a.cap * 2;
+    int cj = 0;  // candidate i0 + j of this group, fetched 16 at a time
     for (int i = 0; i < craw; ++i) {
-      const int c = (int)cq[2 * i + 1];
+      if ((i & 15) == 0) cj = i + j < craw ? (int)cq[2 * (i + j) + 1] : 0;
+      const int c = __shfl(cj, (threadIdx.x & 48) + (i & 15));
       float v = norms_form(a.cn2[c], xn, a16_dot(a.c32 + (int64_t)c * a.d16, x, a.d16, j));
       if (v < best || (v == best && c < bi)) {
         best = v;
@@ -272,7 +286,15 @@ __global__ __launch_bounds__(256) void pp_block_sums_kernel(const double* __rest
   if (b >= nb) return;
   int64_t s = (int64_t)b * PP_BLOCK, e = s + PP_BLOCK < n ? s + PP_BLOCK : n;
   double acc = 0.0;
-  for (int64_t i = s; i < e; ++i) acc += mins[i];
+  // ascending order, eight loads in flight (clamped)
+  for (int64_t i0 = s; i0 < e; i0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = mins[i0 + u < e ? i0 + u : e - 1];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (i0 + u < e) acc += v[u];
+  }
   bsum[b] = acc;
 }
 
@@ -285,25 +307,49 @@ __global__ __launch_bounds__(256) void pp_pick_kernel(const double* __restrict__
                                                       int* __restrict__ picked) {
   __shared__ int s_idx;
   if (threadIdx.x == 0) {
+    // sequential FP64 sums in ascending order, loads eight at a time (clamped)
     double prefix = 0.0, total = 0.0;
-    for (int b = 0; b < nb; ++b) total += bsum[b];
+    for (int b0 = 0; b0 < nb; b0 += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = bsum[b0 + u < nb ? b0 + u : nb - 1];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (b0 + u < nb) total += v[u];
+    }
     const double thr = draws[round] * total;
     int64_t idx = n - 1;
-    for (int b = 0; b < nb; ++b) {
-      double end = prefix + bsum[b];
-      if (end >= thr) {
-        int64_t s = (int64_t)b * PP_BLOCK, e = s + PP_BLOCK < n ? s + PP_BLOCK : n;
-        double loc = 0.0;
-        for (int64_t i = s; i < e; ++i) {
-          loc += mins[i];
+    int hit = -1;  // first block whose end reaches the threshold
+    for (int b0 = 0; b0 < nb && hit < 0; b0 += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = bsum[b0 + u < nb ? b0 + u : nb - 1];
+      for (int u = 0; u < 8 && b0 + u < nb; ++u) {
+        const double end = prefix + v[u];
+        if (end >= thr) {
+          hit = b0 + u;
+          break;
+        }
+        prefix = end;
+      }
+    }
+    if (hit >= 0) {
+      const int64_t s = (int64_t)hit * PP_BLOCK, e = s + PP_BLOCK < n ? s + PP_BLOCK : n;
+      double loc = 0.0;
+      bool found = false;
+      for (int64_t i0 = s; i0 < e && !found; i0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = mins[i0 + u < e ? i0 + u : e - 1];
+        for (int u = 0; u < 8 && i0 + u < e; ++u) {
+          loc += v[u];
           if (prefix + loc >= thr) {
-            idx = i;
+            idx = i0 + u;
+            found = true;
             break;
           }
         }
-        break;
       }
-      prefix = end;
     }
     s_idx = (int)idx;
     picked[round] = (int)idx;
