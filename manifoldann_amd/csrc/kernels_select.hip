@@ -133,19 +133,20 @@ __device__ __forceinline__ void exact_metric4(const SelectArgs& a, const uint32_
       }
     }
   };
-  // four steps per iteration with all 20 loads issued first (the chains stay
+  // eight steps per iteration with all 40 loads issued first (the chains stay
   // in ascending t), so the row gathers overlap instead of waiting step by step
+  constexpr int SB = 8;
   int t = 0;
-  for (; t + 64 <= a.d16; t += 64) {
-    float yv[4], xv[4][4];
+  for (; t + 16 * SB <= a.d16; t += 16 * SB) {
+    float yv[SB], xv[SB][4];
 #pragma unroll
-    for (int s2 = 0; s2 < 4; ++s2) {
+    for (int s2 = 0; s2 < SB; ++s2) {
       yv[s2] = y[t + 16 * s2 + j];
 #pragma unroll
       for (int u = 0; u < 4; ++u) xv[s2][u] = x[u][t + 16 * s2 + j];
     }
 #pragma unroll
-    for (int s2 = 0; s2 < 4; ++s2) step(yv[s2], xv[s2]);
+    for (int s2 = 0; s2 < SB; ++s2) step(yv[s2], xv[s2]);
   }
   for (; t < a.d16; t += 16) {
     float xv[4];
