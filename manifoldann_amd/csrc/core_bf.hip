@@ -381,6 +381,7 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
       a.dense = dense;
       a.ldd = mcols;
       a.dense_bmin = bmin;
+      a.dvec = mcols % 4 == 0;  // row partitions start at multiples of BM
       {
         ScopedTime tm("scan_dense", st);
         launch_scan(a, SCAN_DENSE, qt * a.parts, SCAN_KIND_GRID, st);

@@ -220,7 +220,9 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
         (nb + ceil_div(nq, BNt)) * std::max<int64_t>(1, ceil_div(iv.max_list, ROWS_ITEM));
     ScanItem* fitems = c.buf("iv_fitems").as<ScanItem>((size_t)max_fitems);
     int* fioff = c.buf("iv_fioff").as<int>(nb);
-    float* dense = c.buf("iv_dense").as<float>((size_t)nq * iv.max_list);
+    // rows of max_list values, padded to a multiple of 4 for 16-byte stores
+    const int64_t ldd = round_up(iv.max_list, 4);
+    float* dense = c.buf("iv_dense").as<float>((size_t)nq * ldd);
     {
       ScopedTime tm("ivf_tau_scan", st);
       launch_first_items(first_cnt, qoff, nb, iv.list_off, iv.lorder, BNt, ROWS_ITEM, fioff, fitems,
@@ -240,8 +242,9 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
       a.n_items_dev = ctr + 5;
       a.item_counter = ctr + 6;
       a.dense = dense;
-      a.ldd = iv.max_list;
+      a.ldd = ldd;
       a.dense_rel = 1;
+      a.dvec = 1;  // items start at their list's first row plus multiples of 256
       a.parts = 1;
       int cus = 256;
       (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device);
@@ -259,7 +262,7 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
     td.prows = prows;
     td.list_off = iv.list_off;
     td.dense = dense;
-    td.ldd = iv.max_list;
+    td.ldd = ldd;
     td.qe = qe;
     td.qr = qr;
     td.ex = iv.ex;

@@ -347,6 +347,22 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
             continue;
           }
           if (gqc[j] >= 0) {
+            // the lane's 16 rows are four runs of 4 consecutive rows: with
+            // every row inside, four 16-byte stores (the caller keeps ldd, the
+            // base and the runs' offsets multiples of 4 floats when dvec)
+            if (!a.dense_t && a.dvec && rb0 + 27 < row_end) {
+              float* dst =
+                  a.dense + (int64_t)gqc[j] * a.ldd + (a.dense_rel ? rb0 - region : rb0);
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                floatx4 v4;
+#pragma unroll
+                for (int c2 = 0; c2 < 4; ++c2)
+                  v4[c2] = FOLD ? val(j, 4 * u + c2) : val(j, 4 * u + c2) + qo[j];
+                *(floatx4*)(dst + 8 * u) = v4;
+              }
+              continue;
+            }
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
               const int row = rb0 + 8 * (e >> 2) + (e & 3);

@@ -53,6 +53,9 @@ struct ScanArgs {
   int64_t ldd;
   int dense_rel;
   int dense_t;
+  // dvec: ldd and the base are multiples of 4 floats and every row tile starts
+  // 4-aligned relative to the stored origin (16-byte stores allowed)
+  int dvec;
   // dense_bmin: dense[q * ldd + row / 32] = the minimum over each 32-row block
   // (row partitions start at multiples of 32)
   int dense_bmin;
