@@ -37,8 +37,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
         r = i - kk * GT;
       }
       const int m = m0 + r, k = k0 + kk;
-      As[r][kk] = (bf16_t)((m < g.M && k < g.K) ? g.A[(int64_t)m * g.sam + (int64_t)k * g.sak]
-                                                : 0.0f);
+      const float v = g.A[(int64_t)min(m, g.M - 1) * g.sam + (int64_t)min(k, g.K - 1) * g.sak];
+      As[r][kk] = (bf16_t)((m < g.M && k < g.K) ? v : 0.0f);
     }
     for (int i = tid; i < GT * GK; i += 256) {
       int r, kk;
@@ -50,8 +50,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
         r = i - kk * GT;
       }
       const int n = n0 + r, k = k0 + kk;
-      Bs[r][kk] = (bf16_t)((n < g.N && k < g.K) ? g.B[(int64_t)n * g.sbn + (int64_t)k * g.sbk]
-                                                : 0.0f);
+      const float v = g.B[(int64_t)min(n, g.N - 1) * g.sbn + (int64_t)min(k, g.K - 1) * g.sbk];
+      Bs[r][kk] = (bf16_t)((n < g.N && k < g.K) ? v : 0.0f);
     }
     __syncthreads();
 #pragma unroll
@@ -78,12 +78,17 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
   }
 #pragma unroll
   for (int e = 0; e < 16; ++e) {
-    const int m = m0 + wm * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
     float v = acc[e];
     if (g.bias) v += bias;
     if (g.relu) v = v > 0.0f ? v : 0.0f;
     if (g.accumulate) v += old[e];
-    if (m < g.M && n < g.N) g.C[(int64_t)m * g.ldc + n] = v;
+    acc[e] = v;
+    asm volatile("" : "+v"(acc[e]));  // computed here, not sunk into the stores
+  }
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const int m = m0 + wm * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+    if (m < g.M && n < g.N) g.C[(int64_t)m * g.ldc + n] = acc[e];
   }
 }
 
@@ -112,10 +117,14 @@ __global__ __launch_bounds__(256) void gemm_tiled_kernel(GemmArgs g) {
     for (int p = 0; p < NP; ++p) {
       const int f = tid + 256 * p, r = f / RS, k = k0 + (f % RS) * 4;
       const int m = m0 + r, n = n0 + r;
-      ra[p] = (m < g.M && k < g.K) ? *(const float4*)(g.A + (int64_t)m * g.sam + k)
-                                   : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      rb[p] = (n < g.N && k < g.K) ? *(const float4*)(g.B + (int64_t)n * g.sbn + k)
-                                   : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      // unconditional loads at clamped addresses (K is a multiple of 4), so
+      // the eight loads are in flight together; out-of-range values are zeroed
+      const int kc = min(k, g.K - 4);
+      const float4 va = *(const float4*)(g.A + (int64_t)min(m, g.M - 1) * g.sam + kc);
+      const float4 vb = *(const float4*)(g.B + (int64_t)min(n, g.N - 1) * g.sbn + kc);
+      const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      ra[p] = (m < g.M && k < g.K) ? va : z;
+      rb[p] = (n < g.N && k < g.K) ? vb : z;
     }
   };
   auto put = [&](bf16_t (*L)[TT][TLD], int r, int kk, float4 v) {
@@ -197,19 +206,30 @@ __global__ __launch_bounds__(256) void gemm_tiled_kernel(GemmArgs g) {
       }
     }
 #pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        float v = acc[i][j][e];
+        if (g.bias) v += bias[j];
+        if (g.relu) v = v > 0.0f ? v : 0.0f;
+        if (g.accumulate) v += old[j][e];
+        acc[i][j][e] = v;
+        asm volatile("" : "+v"(acc[i][j][e]));  // computed here, not sunk into the stores
+      }
+  }
+  // all values final before the first store (a load used after a store would
+  // make the compiler wait for that store)
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int n = n0 + wn * 64 + j * 32 + (lane & 31);
 #pragma unroll
       for (int e = 0; e < 16; ++e) {
         const int m = m0 + wm * 64 + i * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-        float v = acc[i][j][e];
-        if (g.bias) v += bias[j];
-        if (g.relu) v = v > 0.0f ? v : 0.0f;
-        if (g.accumulate) v += old[j][e];
-        if (m < g.M && n < g.N) g.C[(int64_t)m * g.ldc + n] = v;
+        if (m < g.M && n < g.N) g.C[(int64_t)m * g.ldc + n] = acc[i][j][e];
       }
     }
-  }
 }
 
 // One workgroup per row: softmax over the logits, cross-entropy against the
