@@ -659,8 +659,7 @@ static void search_routed(mann_ivf_t ivf, mann_router_t r, const float* queries,
   MANN_REQUIRE(out_ids && out_dist, ERR_NULL, "null pointer");
   ThreadCtx& c = ctx();
   float* S = c.buf("sr_scores").as<float>((size_t)nq * r->c);
-  router_scores(*r, Q.q32, ivf->d16, nq, S);
-  launch_neg(S, S, nq * r->c, c.stream);
+  router_scores(*r, Q.q32, ivf->d16, nq, S, /*negate=*/true);
   OutBuf<int64_t> oi(out_ids, (size_t)(nq * k), "out_ids");
   OutBuf<float> od(out_dist, (size_t)(nq * k), "out_dist");
   ivf_search(*ivf, Q, (int)k, (int)nprobe, metric, S, oi.dev, od.dev, mass);

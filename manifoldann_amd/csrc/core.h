@@ -154,7 +154,9 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
 mann_router* router_create(int d, int h, int c, uint64_t seed);
 // attach (gamma != 0) or drop (gamma == 0) the centroid prior of an IVF index
 void router_set_prior(mann_router& r, const mann_ivf* iv, float gamma);
-void router_scores(const mann_router& r, const float* X, int64_t ldx, int64_t n, float* out);
+// negate: out = -scores (the routing distances of a routed search)
+void router_scores(const mann_router& r, const float* X, int64_t ldx, int64_t n, float* out,
+                   bool negate = false);
 void router_grad(const mann_router& r, const float* X, int64_t ldx, int B, const int* labels,
                  int L, float* grad, float* loss);
 double router_train(mann_router& r, const float* X, int64_t ldx, int64_t n, const int* labels,

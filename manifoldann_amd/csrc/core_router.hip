@@ -83,7 +83,7 @@ void router_set_prior(mann_router& r, const mann_ivf* iv, float gamma) {
 // stride ldx); H kept for backward.  The prior has no parameters, so the
 // backward pass is unchanged.
 static void router_forward(const mann_router& r, const float* X, int64_t ldx, int B, float* H,
-                           float* Z, hipStream_t st) {
+                           float* Z, hipStream_t st, bool negate = false) {
   GemmArgs g{};
   g.A = X;
   g.sam = ldx;
@@ -112,8 +112,12 @@ static void router_forward(const mann_router& r, const float* X, int64_t ldx, in
   g2.N = r.c;
   g2.K = r.h;
   g2.bias = r.b2();
+  const bool prior = r.gamma != 0.0f && r.pc;
+  // the last GEMM stores -Z when asked (and when the tiled kernel takes it)
+  g2.negate = negate && !prior && gemm_tiled_ok(g2);
   launch_gemm(g2, st);
-  if (r.gamma != 0.0f && r.pc) {
+  bool negated = g2.negate;
+  if (prior) {
     // the prior on the MFMA with split operands (FP32-grade products), else
     // the FP32 kernel
     GemmArgs gp{};
@@ -131,11 +135,15 @@ static void router_forward(const mann_router& r, const float* X, int64_t ldx, in
     gp.bias = r.pb;
     gp.accumulate = 1;
     gp.split = 1;
-    if (gemm_tiled_ok(gp))
+    if (gemm_tiled_ok(gp)) {
+      gp.negate = negate;
       launch_gemm(gp, st);
-    else
+      negated = negate;
+    } else {
       launch_prior(X, ldx, B, r.d, r.pc, r.pld, r.pcn, r.c, r.gamma, Z, r.c, st);
+    }
   }
+  if (negate && !negated) launch_neg(Z, Z, (int64_t)B * r.c, st);
 }
 
 // Gradients of the mean cross-entropy into grad (same layout as the parameters).
@@ -192,7 +200,8 @@ static void router_backward(const mann_router& r, const float* X, int64_t ldx, i
   launch_colsum(dH, B, r.h, gb1, st);
 }
 
-void router_scores(const mann_router& r, const float* X, int64_t ldx, int64_t n, float* out) {
+void router_scores(const mann_router& r, const float* X, int64_t ldx, int64_t n, float* out,
+                   bool negate) {
   ThreadCtx& c = ctx();
   hipStream_t st = c.stream;
   // balanced chunks of at most 65,536 rows (the hidden layer is kept per chunk)
@@ -202,7 +211,7 @@ void router_scores(const mann_router& r, const float* X, int64_t ldx, int64_t n,
   ScopedTime tm("router_forward", st);
   for (int64_t s = 0; s < n; s += CH) {
     const int B = (int)std::min<int64_t>(CH, n - s);
-    router_forward(r, X + s * ldx, ldx, B, H, out + s * r.c, st);
+    router_forward(r, X + s * ldx, ldx, B, H, out + s * r.c, st, negate);
   }
 }
 

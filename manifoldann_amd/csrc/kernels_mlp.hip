@@ -213,6 +213,7 @@ __global__ __launch_bounds__(256) void gemm_tiled_kernel(GemmArgs g) {
         if (g.bias) v += bias[j];
         if (g.relu) v = v > 0.0f ? v : 0.0f;
         if (g.accumulate) v += old[j][e];
+        if (g.negate) v = -v;
         acc[i][j][e] = v;
         asm volatile("" : "+v"(acc[i][j][e]));  // computed here, not sunk into the stores
       }
@@ -344,7 +345,8 @@ void launch_gemm(const GemmArgs& g, hipStream_t st) {
       hipLaunchKernelGGL(gemm_tiled_kernel<false>, grid, dim3(256), 0, st, g);
     return;
   }
-  MANN_REQUIRE(!g.split, ERR_UNSUPPORTED, "split GEMM needs unit-stride K operands");
+  MANN_REQUIRE(!g.split && !g.negate, ERR_UNSUPPORTED,
+               "split or negated GEMM needs unit-stride K operands");
   dim3 grid((unsigned)ceil_div(g.N, GT), (unsigned)ceil_div(g.M, GT));
   hipLaunchKernelGGL(gemm_kernel, grid, dim3(256), 0, st, g);
 }

@@ -15,6 +15,7 @@ struct GemmArgs {
   int relu;             // clamp at 0 after the bias
   int accumulate;       // C += result
   int split;            // FP32-accurate product (bf16 hi/lo split); needs gemm_tiled_ok
+  int negate;           // store -result (after bias, ReLU and accumulate); tiled kernel only
 };
 
 // unit-stride K operands, 16-byte aligned rows: the tiled kernel applies
