@@ -268,9 +268,21 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
         check(L.mann_ivf_search(index.handle, Qd.data_ptr(), d, nq, d, 10, bc["nprobe"], 0, None,
                                 ids.data_ptr(), dist.data_ptr()))
         kern = kernel_avg_ms(L, check, ["prep_queries", "ivf_route", "ivf_probe", "ivf_plan",
-                                        "ivf_residual", "ivf_tau", "ivf_scan", "ivf_select",
-                                        "ivf_exact_fallback"])
+                                        "ivf_residual", "ivf_tau_scan", "ivf_tau", "ivf_scan",
+                                        "ivf_select", "ivf_exact_fallback"])
         ivf_stats = M.last_stats()
+        check(L.mann_profile_enable(0))
+    # and at the learned router's operating point (probability mass)
+    kern_l = {}
+    if bl_mass:
+        check(L.mann_profile_enable(1))
+        check(L.mann_profile_reset())
+        check(L.mann_ivf_search_routed_mass(index.handle, r.handle, Qd.data_ptr(), d, nq, d, 10,
+                                            64, bl_mass["mass"], 0, ids.data_ptr(),
+                                            dist.data_ptr()))
+        kern_l = kernel_avg_ms(L, check, ["prep_queries", "router_forward", "ivf_probe",
+                                          "ivf_plan", "ivf_residual", "ivf_tau_scan", "ivf_tau",
+                                          "ivf_scan", "ivf_select", "ivf_exact_fallback"])
         check(L.mann_profile_enable(0))
     cpu = None
     if not args.skip_cpu and bc:
@@ -299,7 +311,8 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
            "learned_fixed_at_0.95": bl_fixed, "learned_mass_at_0.95": bl_mass,
            "prior_only_mass_at_0.95": at95(sweep_p),
            "learned_over_centroid_qps": (round(bl["qps"] / bc["qps"], 3) if bc and bl else None),
-           "kernels_ms": kern, "search_stats": ivf_stats, "cpu_baseline": cpu,
+           "kernels_ms": kern, "kernels_ms_learned_mass": kern_l, "search_stats": ivf_stats,
+           "cpu_baseline": cpu,
            "total_s": round(time.perf_counter() - t_all, 1)}
     return out
 
