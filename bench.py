@@ -189,6 +189,15 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
         return float(np.mean([len(set(a.tolist()) & set(b.tolist())) / 10.0
                               for a, b in zip(f, gt)]))
 
+    if args.ivf_search_only:
+        # only the centroid search at one nprobe, three times, for counter runs
+        for _ in range(3):
+            check(L.mann_ivf_search(index.handle, Qd.data_ptr(), d, nq, d, 10,
+                                    args.ivf_search_only, 0, None, ids.data_ptr(),
+                                    dist.data_ptr()))
+        torch.cuda.synchronize()
+        return {"workload": f"IVF n={n} d={d} nlist={nlist} nq={nq} k=10",
+                "search_only_nprobe": args.ivf_search_only, "recall": round(recall(ids), 4)}
     sweep_c = []
     for npb in (1, 2, 4, 8, 16, 32, 64):
         def run(npb=npb):
@@ -414,6 +423,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-ivf", action="store_true")
+    ap.add_argument("--ivf-search-only", type=int, default=0, metavar="NPROBE",
+                    help="IVF: build the index, then only three centroid searches at NPROBE "
+                         "(for counter collection)")
     ap.add_argument("--n", type=int, default=60_000)
     ap.add_argument("--d", type=int, default=784)
     ap.add_argument("--k", type=int, default=20)
