@@ -32,7 +32,8 @@ __global__ __launch_bounds__(256) void route_dist_kernel(const float* __restrict
                                                          int d16, float* __restrict__ R) {
   __shared__ float ys[2][RT_T][RT_LD];
   __shared__ float xs[2][RT_T][RT_LD];
-  const int q0 = blockIdx.x * RT_T, c0 = blockIdx.y * RT_T;
+  // consecutive workgroups share a query tile (the centroids stay L2-resident)
+  const int c0 = blockIdx.x * RT_T, q0 = blockIdx.y * RT_T;
   const int tid = threadIdx.x, g = tid >> 4, j = tid & 15;
   const int qa = (g >> 2) * 8, cb = g & 3;
   float acc[8][8];
@@ -762,7 +763,7 @@ int route_max() { return ROUTE_MAX; }
 void launch_route_dist(const float* q32, const float* qn2, int nq, const float* c32,
                        const float* cn2, int nlist, int d16, float* R, hipStream_t st) {
   if (nq <= 0 || nlist <= 0) return;
-  dim3 grid((unsigned)ceil_div(nq, RT_T), (unsigned)ceil_div(nlist, RT_T));
+  dim3 grid((unsigned)ceil_div(nlist, RT_T), (unsigned)ceil_div(nq, RT_T));
   hipLaunchKernelGGL(route_dist_kernel, grid, dim3(256), 0, st, q32, qn2, nq, c32, cn2, nlist, d16,
                      R);
 }
