@@ -205,7 +205,9 @@ int mann_last_emitted(int64_t* total);
  * wavefronts, 6 and 7 = configs 5 and 0 with four LDS stages instead of
  * three, 8 = 128 x 64, 9 = 128 x 32 on 2 wavefronts), "ivf_tau_cfg" (the
  * same for the IVF first-list threshold scan, default 9), "ivf_rows_item"
- * (rows of one IVF work item, a multiple of 256, default 2,048), "xcd_map"
+ * (rows of one IVF work item, a multiple of 256, default 2,048),
+ * "ivf_first_rows" (rows of each query's first list that the IVF threshold
+ * scan covers: 0 = the whole list, else at least 16; default 2,048), "xcd_map"
  * (grid order: 0 plain, 1 query-major per XCD,
  * 2 row-partition-major per XCD, 3 bands of 8 row partitions per XCD),
  * "target_groups" (workgroups the row partitioning aims for), "emit_per_k"

@@ -719,6 +719,10 @@ int mann_tune_set(const char* key, int64_t value) {
        : k == "ivf_cfg" ? t.ivf_cfg
        : k == "ivf_tau_cfg" ? t.ivf_tau_cfg
                             : t.kmeans_cfg) = (int)value;
+    } else if (k == "ivf_first_rows") {
+      MANN_REQUIRE(value == 0 || (value >= 16 && value <= (1 << 30)), ERR_ARGUMENT,
+                   "ivf_first_rows must be 0 (all) or at least 16");
+      t.ivf_first_rows = (int)value;
     } else if (k == "ivf_rows_item") {
       MANN_REQUIRE(value >= 256 && value <= (1 << 20) && value % 256 == 0, ERR_ARGUMENT,
                    "ivf_rows_item must be a multiple of 256 in 256..2^20");
@@ -767,6 +771,7 @@ int mann_tune_get(const char* key, int64_t* value) {
     else if (k == "kmeans_cfg") *value = t.kmeans_cfg;
     else if (k == "ivf_tau_cfg") *value = t.ivf_tau_cfg;
     else if (k == "ivf_rows_item") *value = t.ivf_rows_item;
+    else if (k == "ivf_first_rows") *value = t.ivf_first_rows;
     else if (k == "xcd_map") *value = t.xcd_map;
     else if (k == "target_groups") *value = t.target_groups;
     else if (k == "sample_rank") *value = t.sample_rank;

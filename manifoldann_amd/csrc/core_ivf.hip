@@ -225,8 +225,8 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
     float* dense = c.buf("iv_dense").as<float>((size_t)nq * ldd);
     {
       ScopedTime tm("ivf_tau_scan", st);
-      launch_first_items(first_cnt, qoff, nb, iv.list_off, iv.lorder, BNt, ROWS_ITEM, fioff, fitems,
-                         ctr + 5, st);
+      launch_first_items(first_cnt, qoff, nb, iv.list_off, iv.lorder, BNt, ROWS_ITEM,
+                         tuning().ivf_first_rows, fioff, fitems, ctr + 5, st);
       fill_i32(ctr + 6, 1, 0, st);
       ScanArgs a{};
       a.cfg = tcfg;
@@ -269,6 +269,7 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
     td.rx = iv.rx;
     td.tau = tau;
     td.k_eff_q = keff;
+    td.max_rows = tuning().ivf_first_rows;
     ScopedTime tm("ivf_tau", st);
     launch_ivf_tau_dense(td, st);
   } else {

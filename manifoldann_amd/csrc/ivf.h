@@ -49,8 +49,8 @@ void launch_ivf_plan(const int* probes, const int* pcount, int nq, int nprobe, i
 // first (the leading first_cnt entries of its block) against all its rows;
 // region = the list's first row.  n_items written on the device.
 void launch_first_items(const int* first_cnt, const int* qoff, int nb, const int* list_off,
-                        const int* lorder, int bn, int rows_item, int* ioff, ScanItem* items,
-                        int* n_items, hipStream_t st);
+                        const int* lorder, int bn, int rows_item, int max_rows, int* ioff,
+                        ScanItem* items, int* n_items, hipStream_t st);
 // Emission threshold from the first-list scan: S^ values dense[q][0..|l1|)
 struct IvfDenseTauArgs {
   int nq, nprobe, k, d, metric;
@@ -66,6 +66,7 @@ struct IvfDenseTauArgs {
   float ex, rx;
   float* tau;
   int* k_eff_q;
+  int max_rows;  // rows of the first list the DENSE scan covered (0: all)
 };
 void launch_ivf_tau_dense(const IvfDenseTauArgs& a, hipStream_t st);
 // single-query items over the probed lists of the flagged queries

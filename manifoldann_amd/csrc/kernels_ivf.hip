@@ -516,26 +516,30 @@ __global__ void items_fill_kernel(const int* __restrict__ qcount, int nb,
 __global__ __launch_bounds__(1024) void first_plan_kernel(const int* __restrict__ first_cnt,
                                                           int nb, const int* __restrict__ list_off,
                                                           const int* __restrict__ lorder, int bn,
-                                                          int rows_item, int* __restrict__ ioff,
+                                                          int rows_item, int max_rows,
+                                                          int* __restrict__ ioff,
                                                           int* __restrict__ n_items) {
   __shared__ int wsum[16];
   __shared__ int total;
   const int tid = threadIdx.x;
   const int per = (nb + 1023) / 1024;
   const int s0 = tid * per, e0 = min(nb, s0 + per);
+  // rows of a list taken for the threshold: all, or its first max_rows
+  auto rows_of = [&](int b) {
+    const int sz = list_off[b + 1] - list_off[b];
+    return max_rows > 0 && sz > max_rows ? max_rows : sz;
+  };
   int sum = 0;
   for (int i = s0; i < e0; ++i) {
     const int b = lorder[i];
-    const int sz = list_off[b + 1] - list_off[b];
-    sum += ((first_cnt[b] + bn - 1) / bn) * ((sz + rows_item - 1) / rows_item);
+    sum += ((first_cnt[b] + bn - 1) / bn) * ((rows_of(b) + rows_item - 1) / rows_item);
   }
   int run = block_exclusive_scan(sum, wsum, &total);
   if (tid == 0) *n_items = total;
   for (int i = s0; i < e0; ++i) {
     ioff[i] = run;
     const int b = lorder[i];
-    const int sz = list_off[b + 1] - list_off[b];
-    run += ((first_cnt[b] + bn - 1) / bn) * ((sz + rows_item - 1) / rows_item);
+    run += ((first_cnt[b] + bn - 1) / bn) * ((rows_of(b) + rows_item - 1) / rows_item);
   }
 }
 
@@ -543,13 +547,14 @@ __global__ void first_items_kernel(const int* __restrict__ first_cnt, int nb,
                                    const int* __restrict__ list_off,
                                    const int* __restrict__ lorder, const int* __restrict__ qoff,
                                    const int* __restrict__ ioff, int bn, int rows_item,
-                                   ScanItem* __restrict__ items) {
+                                   int max_rows, ScanItem* __restrict__ items) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nb) return;
   const int b = lorder[i];
   const int c0 = first_cnt[b];
   const int nqc = (c0 + bn - 1) / bn;
-  const int lo = list_off[b], hi = list_off[b + 1];
+  const int lo = list_off[b];
+  const int hi = max_rows > 0 ? min(list_off[b + 1], lo + max_rows) : list_off[b + 1];
   const int ns = (hi - lo + rows_item - 1) / rows_item;
   int w = ioff[i];
   for (int s = 0; s < ns; ++s)
@@ -595,6 +600,7 @@ __global__ __launch_bounds__(256) void ivf_tau_dense_kernel(IvfDenseTauArgs a) {
   if (np > 0) {
     const int b = a.probes[(int64_t)q * a.nprobe];
     m = a.list_off[b + 1] - a.list_off[b];
+    if (a.max_rows > 0 && m > a.max_rows) m = a.max_rows;  // the rows the scan covered
   }
   if (lane == 0) a.k_eff_q[q] = k_eff;
   if (k_eff <= 0 || total <= a.emit_all_below || m < k_eff) {
@@ -838,13 +844,13 @@ void launch_refine_items(const int* flist, const int* nflag, int nflag_host, int
 int ivf_tau_rows_max() { return TAU_M; }
 
 void launch_first_items(const int* first_cnt, const int* qoff, int nb, const int* list_off,
-                        const int* lorder, int bn, int rows_item, int* ioff, ScanItem* items,
-                        int* n_items, hipStream_t st) {
+                        const int* lorder, int bn, int rows_item, int max_rows, int* ioff,
+                        ScanItem* items, int* n_items, hipStream_t st) {
   if (nb <= 0) return;
   hipLaunchKernelGGL(first_plan_kernel, dim3(1), dim3(1024), 0, st, first_cnt, nb, list_off, lorder,
-                     bn, rows_item, ioff, n_items);
+                     bn, rows_item, max_rows, ioff, n_items);
   hipLaunchKernelGGL(first_items_kernel, dim3((unsigned)ceil_div(nb, 256)), dim3(256), 0, st,
-                     first_cnt, nb, list_off, lorder, qoff, ioff, bn, rows_item, items);
+                     first_cnt, nb, list_off, lorder, qoff, ioff, bn, rows_item, max_rows, items);
 }
 
 void launch_ivf_tau_dense(const IvfDenseTauArgs& a, hipStream_t st) {
