@@ -207,7 +207,7 @@ int mann_last_emitted(int64_t* total);
  * same for the IVF first-list threshold scan, default 9), "ivf_rows_item"
  * (rows of one IVF work item, a multiple of 256, default 2,048),
  * "ivf_first_rows" (rows of each query's first list that the IVF threshold
- * scan covers: 0 = the whole list, else at least 16; default 2,048), "xcd_map"
+ * scan covers: 0 = the whole list, the default, else at least 16), "xcd_map"
  * (grid order: 0 plain, 1 query-major per XCD,
  * 2 row-partition-major per XCD, 3 bands of 8 row partitions per XCD),
  * "target_groups" (workgroups the row partitioning aims for), "emit_per_k"

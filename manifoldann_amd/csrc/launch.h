@@ -75,7 +75,7 @@ struct Tuning {
   int ivf_cfg = 0;           // IVF list-scan tile config
   int ivf_tau_cfg = 9;
   int ivf_rows_item = 2048;  // IVF work item: rows of one list chunk
-  int ivf_first_rows = 2048; // IVF threshold: rows of the first list scanned (0: all)       // IVF first-list threshold scan tile config (few queries per list)
+  int ivf_first_rows = 0;    // IVF threshold: rows of the first list scanned (0: all)       // IVF first-list threshold scan tile config (few queries per list)
   int kmeans_cfg = 0;        // k-means assignment tile config
   int xcd_map = 3;           // grid order: 0 plain, 1 query-major per XCD, 2 part-major per
                              // XCD, 3 bands of 8 partitions x query tiles per XCD
