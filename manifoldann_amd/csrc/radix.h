@@ -120,8 +120,6 @@ __device__ __forceinline__ uint32_t wave_rank_key(const float* row, int m, int r
     uint32_t key;
     if (m <= 64 * 32) {
       if (wave_rank_cached<32>(row, m, rank, buf, lane, key)) return key;
-    } else if (m <= 64 * 64) {
-      if (wave_rank_cached<64>(row, m, rank, buf, lane, key)) return key;
     } else {
       // two streaming passes, eight loads in flight per lane (clamped indices)
       constexpr int U8 = 8;
