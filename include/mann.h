@@ -55,7 +55,8 @@ int mann_device_count(void);
  * first-pass representation (COSINE keeps unit-normalised bf16 rows). */
 int mann_dataset_create(const float* data, int64_t d, int64_t n, int64_t ld, int metric,
                         mann_dataset_t* out);
-/* The same from IEEE binary16 data (a Matrix{Float16} base, SURVEY config 5):
+/* The same from IEEE binary16 data (a Matrix{Float16} base, as in the large
+ * cosine search over 10M x 256 Float16 rows):
  * the rows are upcast exactly to Float32 on the device.  The reference would
  * compute in Float16 (src/indices/bruteforce.jl:53, S = float(T)); this
  * library computes in Float32 on the upcast values (the documented oracle of
