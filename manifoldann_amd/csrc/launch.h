@@ -72,7 +72,7 @@ struct ScanArgs {
 // Run-time tuning knobs (set through mann_tune_set; chosen by the autotuner).
 struct Tuning {
   int scan_cfg = 5;          // exact-kNN scan tile config (table of kernels_scan.hip)
-  int ivf_cfg = 0;           // IVF list-scan tile config
+  int ivf_cfg = 5;           // IVF list-scan tile config
   int ivf_tau_cfg = 9;
   int ivf_rows_item = 2048;  // IVF work item: rows of one list chunk
   int ivf_first_rows = 0;    // IVF threshold: rows of the first list scanned (0: all)       // IVF first-list threshold scan tile config (few queries per list)
