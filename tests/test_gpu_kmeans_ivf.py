@@ -1,6 +1,8 @@
 """GPU parity of the k-means transform and the IVF (MultiLevelIndex) path
 against the CPU oracle: assignments, centroids, routing and neighbour ids
 bit-identical."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -9,6 +11,7 @@ pytestmark = pytest.mark.gpu
 import manifoldann_amd as M
 from manifoldann_amd import datasets
 from manifoldann_amd import ivf as I
+from manifoldann_amd._lib import check, lib
 from oracle import oracle as O
 
 
@@ -191,3 +194,30 @@ def test_ivf_external_route_scores():
     off, lids, nb = O.partition(a)
     oi, od = O.ivf_search(X, off, lids, nb, Q, R, 10, 6)
     assert (ids == oi).all()
+
+
+@pytest.mark.parametrize("settings", [
+    {"ivf_cfg": 0},
+    {"ivf_cfg": 8, "ivf_tau_cfg": 8},
+    {"ivf_first_rows": 64, "ivf_rows_item": 256},
+    {"ivf_tau_mode": 0},
+])
+def test_ivf_tuning_keeps_results_exact(settings):
+    # tile configs, first-list row caps, item sizes and threshold modes change
+    # only the speed: the ids and distance bits stay the oracle's
+    _banner(f"ivf tuning {settings}")
+    L = lib()
+    old = {}
+    for key, v in settings.items():
+        o = ctypes.c_int64()
+        check(L.mann_tune_get(key.encode(), ctypes.byref(o)))
+        old[key] = o.value
+        check(L.mann_tune_set(key.encode(), v))
+    try:
+        X, C, a, _ = _build_ivf()
+        Q = datasets.mixture_queries(300, 64, components=40, seed=4, query_seed=13)
+        st = _ivf_vs_oracle(X, C, a, Q, 10, 8)
+        print(f"stats={st}", flush=True)
+    finally:
+        for key, v in old.items():
+            check(L.mann_tune_set(key.encode(), v))
