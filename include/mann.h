@@ -124,6 +124,14 @@ int mann_route_distances(const float* centroids, int64_t nlist, int64_t d, const
  * in ascending id; the base is copied list-contiguously.  assign is 1-based. */
 int mann_ivf_build(mann_dataset_t X, const float* centroids, int64_t nlist,
                    const int64_t* assign, mann_ivf_t* out);
+/* build_index(IVFFlatIndex, data; nlist, ...) after the k-means fit
+ * (src/indices/ivf_flat.jl:30-93): one list per centroid, nlist lists, empty
+ * ones included (:66-77), each in ascending id.  Queries go through
+ * mann_ivf_search with nprobe clamped to [1, nlist] (:146); the lists probed
+ * are the nprobe nearest centroids, empty ones taking their slots (:149-150).
+ * assign is 1-based and must lie in [1, nlist]. */
+int mann_ivf_flat_build(mann_dataset_t X, const float* centroids, int64_t nlist,
+                        const int64_t* assign, mann_ivf_t* out);
 int mann_ivf_destroy(mann_ivf_t ivf);
 /* number of buckets and (optional) the size of each */
 int mann_ivf_info(mann_ivf_t ivf, int64_t* n_buckets, int64_t* list_sizes);

@@ -1,7 +1,8 @@
 """ManifoldANN.jl's data-parallel search core on MI355X (gfx950 HIP kernels).
 
 The exact scan (BruteForceIndex), kNN-graph materialisation and the
-MultiLevelIndex IVF path (KMeansTransform, TopKRouting, list scans, merge) run
+MultiLevelIndex and IVFFlatIndex IVF paths (KMeansTransform, TopKRouting, list
+scans, merge) run
 in libmann_hip.so through its C ABI (include/mann.h); this package mirrors the
 reference's Julia API on top of it.
 """
@@ -12,5 +13,6 @@ from .index import (COSINE, EUCLIDEAN, SQEUCLIDEAN, BruteForceIndex, CosineDist,
                     euclidean_distance, graph_metadata, has_metadata, insert, last_emitted, last_stats,
                     neighbor_ids, node_metadata, query, query_arrays, recall_at_k,
                     symmetrize_adjacency)
+from .ivf import IVFFlatIndex, MultiLevelIndex
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -35,6 +35,7 @@ SIGNATURES = [
     ("mann_kmeans_fit", ctypes.c_int, [_vp, _i64, _i64, ctypes.c_double, ctypes.c_int,
                                        ctypes.c_uint64, _vp, _vp, _vp, _i64p]),
     ("mann_ivf_build", ctypes.c_int, [_vp, _vp, _i64, _vp, ctypes.POINTER(_vp)]),
+    ("mann_ivf_flat_build", ctypes.c_int, [_vp, _vp, _i64, _vp, ctypes.POINTER(_vp)]),
     ("mann_ivf_destroy", ctypes.c_int, [_vp]),
     ("mann_ivf_info", ctypes.c_int, [_vp, _i64p, _vp]),
     ("mann_ivf_list_ids", ctypes.c_int, [_vp, _vp]),

@@ -372,6 +372,25 @@ int mann_ivf_build(mann_dataset_t X, const float* centroids, int64_t nlist,
   });
 }
 
+int mann_ivf_flat_build(mann_dataset_t X, const float* centroids, int64_t nlist,
+                        const int64_t* assign, mann_ivf_t* out) {
+  return guarded([&] {
+    MANN_REQUIRE(X && centroids && assign && out, ERR_NULL, "null pointer");
+    MANN_REQUIRE(nlist > 0, ERR_ARGUMENT, "nlist must be positive");
+    std::vector<int64_t> ha(X->n);
+    if (is_device_ptr(assign)) {
+      MANN_HIP(hipMemcpy(ha.data(), assign, sizeof(int64_t) * X->n, hipMemcpyDeviceToHost));
+    } else {
+      memcpy(ha.data(), assign, sizeof(int64_t) * X->n);
+    }
+    for (int64_t j = 0; j < X->n; ++j)
+      MANN_REQUIRE(ha[j] >= 1 && ha[j] <= nlist, ERR_ARGUMENT,
+                   "Assignment out of bounds: " + std::to_string(ha[j]));
+    float* C = stage_centroids(centroids, nlist, *X, "ivb_c");
+    *out = ivf_build(*X, C, (int)nlist, ha.data(), true);
+  });
+}
+
 int mann_ivf_destroy(mann_ivf_t ivf) {
   return guarded([&] { delete ivf; });
 }

@@ -142,7 +142,8 @@ void kmeans_pp(const mann_dataset& ds, int k, uint64_t seed, float* C, int* pick
 std::vector<int> random_indices(int64_t n, int k, uint64_t seed);
 
 // IVF (core_ivf.hip)
-mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int64_t* assign_host);
+mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int64_t* assign_host,
+                    bool all_lists = false);
 void ivf_route(const mann_ivf& iv, const QueryOps& Q, float* R);
 // mass > 0: probability-mass probing (R = negated logits), at most nprobe lists
 void ivf_probe(const mann_ivf& iv, const float* R, int nq, int nprobe, int* probes, int* pcount,

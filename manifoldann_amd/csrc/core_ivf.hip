@@ -21,7 +21,8 @@
 namespace mann {
 
 
-mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int64_t* assign_host) {
+mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int64_t* assign_host,
+                    bool all_lists) {
   ThreadCtx& c = ctx();
   hipStream_t st = c.stream;
   const int64_t n = ds.n;
@@ -32,6 +33,9 @@ mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int
     nb = std::max<int>(nb, (int)assign_host[j]);
   }
   MANN_REQUIRE(nb > 0, ERR_ARGUMENT, "Transform produced invalid bucket assignments");
+  // IVF-Flat keeps one list per centroid, empty trailing ones included, so that
+  // they take probe slots as in ivf_flat.jl:68-77,150
+  if (all_lists) nb = std::max(nb, nlist);
   auto* iv = new mann_ivf();
   try {
     iv->device = c.device;

@@ -219,7 +219,7 @@ def _validate(index, data, d_q):
         raise ArgumentError(f"Data contains {n} points but index tracks {index.n_points}")
 
 
-def query_arrays(index, data, Q, k, out=None):
+def query_arrays(index, data, Q, k, out=None, nprobe=None):
     """Batch exact kNN; returns (ids (nq, k) int64 1-based, dist (nq, k) f32).
 
     Device (torch) queries give device results.  Equivalent to the generic
@@ -227,7 +227,9 @@ def query_arrays(index, data, Q, k, out=None):
     """
     if hasattr(index, "root"):
         from .ivf import query_arrays as ivf_query_arrays
-        return ivf_query_arrays(index, data, Q, k)
+        return ivf_query_arrays(index, data, Q, k, nprobe=nprobe)
+    if nprobe is not None:
+        raise ArgumentError("nprobe is a keyword of IVFFlatIndex queries")
     qm = as_matrix(Q)
     d, n = _data_shape(data)
     if qm.d != d:
@@ -262,8 +264,9 @@ def _to_neighbors(ids, dist):
     return out
 
 
-def query(index, data, q, k):
-    """query(index, data, q|Q, k) -> Vector{Neighbor} or Vector{Vector{Neighbor}}."""
+def query(index, data, q, k, nprobe=None):
+    """query(index, data, q|Q, k[; nprobe]) -> Vector{Neighbor} or
+    Vector{Vector{Neighbor}}; nprobe only for IVFFlatIndex (ivf_flat.jl:138-144)."""
     if isinstance(q, (list, tuple)) and q and not np.isscalar(q[0]):
         # vector-of-vectors batch (ann_index.jl:110-120)
         q = np.stack([np.asarray(v, np.float32) for v in q], axis=1)
@@ -271,9 +274,9 @@ def query(index, data, q, k):
     if single:
         d_q = q.shape[0] if _is_torch(q) else np.asarray(q).shape[0]
         _validate(index, data, d_q) if not hasattr(index, "root") else None
-        ids, dist = query_arrays(index, data, q, k)
+        ids, dist = query_arrays(index, data, q, k, nprobe=nprobe)
         return _to_neighbors(ids, dist)[0] if k > 0 else []
-    ids, dist = query_arrays(index, data, q, k)
+    ids, dist = query_arrays(index, data, q, k, nprobe=nprobe)
     return _to_neighbors(ids, dist)
 
 

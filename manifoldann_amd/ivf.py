@@ -321,9 +321,16 @@ class MultiLevelIndex:
 
 
 def build_index(index_type, X, config=None, merge_strategy=None, distance=default_distance,
-                rng=None):
+                rng=None, **kw):
     """build_index(MultiLevelIndex, X, config; merge_strategy, distance, rng)
-    (builder.jl:61-76) for a KMeans transform over BruteForce terminals."""
+    (builder.jl:61-76) for a KMeans transform over BruteForce terminals, or
+    build_index(IVFFlatIndex, X; nlist, ...) (ivf_flat.jl:30-93)."""
+    if index_type is IVFFlatIndex:
+        if config is not None or merge_strategy is not None:
+            raise ArgumentError("IVFFlatIndex takes keyword arguments, not a config")
+        return build_ivf_flat(X, distance=distance, rng=rng, **kw)
+    if kw:
+        raise ArgumentError(f"unexpected keyword arguments {sorted(kw)}")
     if index_type is not MultiLevelIndex:
         raise ArgumentError(f"unsupported index type {index_type}")
     if not isinstance(config, TransformedConfig) or not isinstance(config.transform, KMeansTransform):
@@ -355,6 +362,91 @@ def build_index_from_assignment(X, centroids_dk, assign, routing, merge_strategy
     check(lib().mann_ivf_build(ds.handle, C.ctypes.data, t.k, a.ctypes.data, ctypes.byref(h)))
     return MultiLevelIndex(t, routing, h, ds.d, ds.n, t.k, merge_strategy or SimpleMerge(),
                            distance)
+
+
+class IVFFlatIndex:
+    """IVFFlatIndex (ivf_flat.jl:11-19): k-means centroids and one inverted list
+    of point ids per centroid (empty lists included), held on the GPU with a
+    list-contiguous copy of the base; the data is still passed to every query,
+    as the shared ANN contract asks."""
+
+    def __init__(self, kmeans, handle, d, n, nlist, default_nprobe, distance, centroid_metric):
+        self.kmeans = kmeans
+        self.centroids = kmeans.centroids
+        self.centroid_norms = kmeans.centroid_norms
+        self.handle = handle
+        self.dimension = d
+        self.d, self.n, self.nlist = d, n, nlist
+        self.default_nprobe = default_nprobe
+        self.distance = distance
+        self.centroid_metric = centroid_metric
+        self.routing = TopKRouting(default_nprobe)
+        self.root = self
+        sizes = np.zeros(nlist, np.int64)
+        nb = ctypes.c_int64()
+        check(lib().mann_ivf_info(handle, ctypes.byref(nb), sizes.ctypes.data))
+        self.list_sizes = sizes
+
+    @property
+    def lists(self):
+        """lists[c] = ascending ids of the points assigned to centroid c (ivf_flat.jl:66-77)."""
+        ids = np.zeros(self.n, np.int64)
+        check(lib().mann_ivf_list_ids(self.handle, ids.ctypes.data))
+        off = np.concatenate([[0], np.cumsum(self.list_sizes)])
+        return [ids[off[c]:off[c + 1]] for c in range(self.nlist)]
+
+    def __del__(self):
+        try:
+            h = getattr(self, "handle", None)
+            if h is not None and h.value and not shutting_down():
+                lib().mann_ivf_destroy(h)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def build_ivf_flat(X, nlist, nprobe=10, distance=default_distance, centroid_metric=None,
+                   kmeans_max_iters=25, kmeans_tol=1e-4, kmeans_subsample_size="default",
+                   rng=None):
+    """build_index(IVFFlatIndex, data; nlist, nprobe=10, distance, centroid_metric,
+    kmeans_max_iters=25, kmeans_tol=1e-4, kmeans_subsample_size=max(256 nlist, 32768))
+    (ivf_flat.jl:30-93): k-means++ Lloyd on the subsample, one full assignment
+    pass, one list per centroid.  The routing metric is Euclidean or
+    SqEuclidean (same order; other centroid metrics are not offered here)."""
+    d, n = _data_shape(X)
+    if d <= 0:
+        raise ArgumentError("Dataset must have at least one dimension")
+    if n <= 0:
+        raise ArgumentError("Dataset must contain at least one point")
+    if nlist <= 0:
+        raise ArgumentError("nlist must be positive")
+    centroid_metric = Euclidean() if centroid_metric is None else centroid_metric
+    if type(centroid_metric).__name__ not in ("Euclidean", "SqEuclidean"):
+        raise ArgumentError("centroid_metric must be Euclidean or SqEuclidean")
+    if kmeans_subsample_size == "default":
+        kmeans_subsample_size = max(256 * nlist, 32768)
+    # SqEuclidean orders the centroids as Euclidean does: one k-means serves both
+    t = KMeansTransform(k=nlist, distance=Euclidean(), init="kmeans_plus_plus",
+                        max_iters=kmeans_max_iters, tol=kmeans_tol,
+                        subsample_size=kmeans_subsample_size)
+    fit(t, X, rng=rng)
+    assign = take_pending_assignments(t)
+    if assign is None or len(assign) != n:
+        raise ArgumentError("KMeansTransform did not return assignments")
+    return ivf_flat_from_assignment(X, t, assign, nprobe, distance, centroid_metric)
+
+
+def ivf_flat_from_assignment(X, t, assign, nprobe=10, distance=default_distance,
+                             centroid_metric=None):
+    """IVFFlatIndex from a fitted KMeansTransform and its 1-based assignment."""
+    ds = _dataset_for(X, Euclidean())
+    C = _centroids_rows(t.centroids)
+    a = np.ascontiguousarray(assign, np.int64)
+    h = ctypes.c_void_p()
+    check(lib().mann_ivf_flat_build(ds.handle, C.ctypes.data, t.k, a.ctypes.data,
+                                    ctypes.byref(h)))
+    return IVFFlatIndex(t, h, ds.d, ds.n, t.k, max(1, min(int(nprobe), t.k)), distance,
+                        centroid_metric or Euclidean())
 
 
 def transform(t, x):
@@ -398,6 +490,16 @@ def query_arrays(index: MultiLevelIndex, data, Q, k, route=None, nprobe=None):
     """Batched query (query.jl:279-300) -> (ids (nq, k), dist (nq, k))."""
     qm = as_matrix(Q)
     d, _ = _data_shape(data)
+    if isinstance(index, IVFFlatIndex):
+        # _validate_dimensions and the nprobe clamp (ivf_flat.jl:95-101, 146)
+        if d != index.dimension:
+            raise DimensionMismatch(f"Expected data with {index.dimension} rows")
+        if qm.d != index.dimension:
+            raise DimensionMismatch(f"Expected query of length {index.dimension}")
+        npb = index.default_nprobe if nprobe is None else int(nprobe)
+        nprobe = min(max(npb, 1), index.nlist)
+    elif nprobe is not None:
+        raise ArgumentError("nprobe is a keyword of IVFFlatIndex queries")
     if qm.d != d:
         raise DimensionMismatch(f"Expected queries with {d} rows, got {qm.d}")
     nq = qm.n
@@ -423,8 +525,8 @@ def query_arrays(index: MultiLevelIndex, data, Q, k, route=None, nprobe=None):
     return ids, dist
 
 
-def query(index: MultiLevelIndex, data, q, k):
+def query(index, data, q, k, nprobe=None):
     single = (np.ndim(q) == 1) if not _is_torch(q) else q.dim() == 1
-    ids, dist = query_arrays(index, data, q, k)
+    ids, dist = query_arrays(index, data, q, k, nprobe=nprobe)
     res = _to_neighbors(ids, dist)
     return res[0] if single else res

@@ -664,3 +664,90 @@ double orc_recall_at_k(const int64_t* found, int64_t kf, const int64_t* truth, i
   }
   return total / (double)nq;
 }
+
+/* query(::IVFFlatIndex, data, q, k; nprobe) (src/indices/ivf_flat.jl:138-167),
+ * one query at a time.  nprobe is clamped to [1, nlist] (:146); the probed
+ * lists are the nprobe smallest routing distances (:149-150, ties to the lower
+ * centroid), empty lists included; k_eff = min(k, total probed size) (:156);
+ * every candidate is pushed, in probe order and ascending id inside a list,
+ * into a bounded max-heap of capacity k_eff that replaces its root only on a
+ * strictly smaller distance (BoundedMaxHeap push!, src/utils/neighbor_heaps.jl
+ * :84-100, sift-up :156-168, sift-down :170-190); the survivors are sorted by
+ * (dist, id) (to_sorted_vector :131-134).  offsets has nlist + 1 entries.
+ * Output k slots per query padded with id 0 / +inf, count[q] = k_eff. */
+typedef struct { int64_t id; float dist; } heap_nb_t;
+
+static void heap_sift_up(heap_nb_t* h, int64_t i) { /* 1-based positions */
+  while (i > 1) {
+    int64_t p = i >> 1;
+    if (h[i - 1].dist > h[p - 1].dist) {
+      heap_nb_t t = h[i - 1]; h[i - 1] = h[p - 1]; h[p - 1] = t;
+      i = p;
+    } else {
+      break;
+    }
+  }
+}
+
+static void heap_sift_down(heap_nb_t* h, int64_t len, int64_t i) {
+  for (;;) {
+    int64_t l = i << 1, r = l + 1, big = i;
+    if (l <= len && h[l - 1].dist > h[big - 1].dist) big = l;
+    if (r <= len && h[r - 1].dist > h[big - 1].dist) big = r;
+    if (big == i) break;
+    heap_nb_t t = h[i - 1]; h[i - 1] = h[big - 1]; h[big - 1] = t;
+    i = big;
+  }
+}
+
+static int cmp_heap_nb(const void* a, const void* b) {
+  const heap_nb_t* x = (const heap_nb_t*)a;
+  const heap_nb_t* y = (const heap_nb_t*)b;
+  if (x->dist != y->dist) return x->dist < y->dist ? -1 : 1;
+  return (x->id > y->id) - (x->id < y->id);
+}
+
+int orc_ivf_flat_query(const float* X, int64_t d, int64_t ld, const int64_t* offsets,
+                       const int64_t* ids, int64_t nlist, const float* Q, int64_t nq,
+                       int64_t ldq, const float* route, int64_t k, int64_t nprobe, int metric,
+                       int64_t* out_ids, float* out_dist, int64_t* out_count) {
+  if (nlist <= 0) return -2;
+  int64_t np = nprobe < 1 ? 1 : (nprobe > nlist ? nlist : nprobe);
+  int64_t* sel = (int64_t*)malloc(sizeof(int64_t) * (size_t)np);
+  for (int64_t qi = 0; qi < nq; ++qi) {
+    int64_t ns = orc_select_topk(route + qi * nlist, nlist, nlist, np, sel);
+    int64_t tot = 0;
+    for (int64_t s = 0; s < ns; ++s) tot += offsets[sel[s]] - offsets[sel[s] - 1];
+    int64_t keff = k < tot ? k : tot;
+    for (int64_t r = 0; r < k; ++r) {
+      out_ids[qi * k + r] = 0;
+      out_dist[qi * k + r] = INFINITY;
+    }
+    out_count[qi] = keff > 0 ? keff : 0;
+    if (keff <= 0) continue;
+    heap_nb_t* h = (heap_nb_t*)malloc(sizeof(heap_nb_t) * (size_t)keff);
+    int64_t len = 0;
+    const float* q = Q + qi * ldq;
+    for (int64_t s = 0; s < ns; ++s) {
+      int64_t b = sel[s] - 1;
+      for (int64_t t = offsets[b]; t < offsets[b + 1]; ++t) {
+        heap_nb_t c = {ids[t], orc_metric(metric, X + (ids[t] - 1) * ld, q, d)};
+        if (len < keff) {
+          h[len++] = c;
+          heap_sift_up(h, len);
+        } else if (c.dist < h[0].dist) {
+          h[0] = c;
+          heap_sift_down(h, len, 1);
+        }
+      }
+    }
+    qsort(h, (size_t)len, sizeof(heap_nb_t), cmp_heap_nb);
+    for (int64_t r = 0; r < len; ++r) {
+      out_ids[qi * k + r] = h[r].id;
+      out_dist[qi * k + r] = h[r].dist;
+    }
+    free(h);
+  }
+  free(sel);
+  return 0;
+}

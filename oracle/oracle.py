@@ -76,6 +76,10 @@ def lib():
         L.orc_ivf_search.argtypes = [_f32p, _i64, _i64, _i64p, _i64p, _i64, _f32p, _i64, _i64,
                                      _f32p, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, _i64p, _f32p, ctypes.c_int]
+        L.orc_ivf_flat_query.restype = ctypes.c_int
+        L.orc_ivf_flat_query.argtypes = [_f32p, _i64, _i64, _i64p, _i64p, _i64, _f32p, _i64,
+                                         _i64, _f32p, _i64, _i64, ctypes.c_int, _i64p, _f32p,
+                                         _i64p]
         L.orc_recall_at_k.restype = ctypes.c_double
         L.orc_recall_at_k.argtypes = [_i64p, _i64, _i64p, _i64, _i64]
         _lib = L
@@ -249,6 +253,25 @@ def ivf_search(X, offsets, ids, nb, Q, route, k, nprobe, metric=EUCLIDEAN, merge
     if rc != 0:
         raise ValueError("Transform produced no routing distances for TopKRouting")
     return oi, od
+
+
+def ivf_flat_query(X, offsets, ids, Q, route, k, nprobe, metric=EUCLIDEAN):
+    """query(::IVFFlatIndex) per query: bounded-heap merge in probe order.
+    offsets has nlist + 1 entries (empty lists included).
+    Returns (ids (nq, k) padded with 0, dist (nq, k) padded with inf, k_eff (nq,))."""
+    X, Q, route = _f32(X), _f32(Q), _f32(route)
+    offsets, ids = _i64a(offsets), _i64a(ids)
+    n, d = X.shape
+    nq = Q.shape[0]
+    nlist = offsets.shape[0] - 1
+    oi = np.zeros((nq, max(k, 0)), np.int64)
+    od = np.full((nq, max(k, 0)), np.inf, np.float32)
+    cnt = np.zeros(max(nq, 1), np.int64)
+    rc = lib().orc_ivf_flat_query(_p(X), d, d, _p(offsets), _p(ids), nlist, _p(Q), nq, d,
+                                  _p(route), max(k, 0), nprobe, metric, _p(oi), _p(od), _p(cnt))
+    if rc != 0:
+        raise ValueError("nlist must be positive")
+    return oi, od, cnt[:nq].copy()
 
 
 def recall_at_k(found, truth):
