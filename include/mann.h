@@ -82,7 +82,9 @@ int mann_knn_graph(mann_dataset_t data, int64_t k, int include_self, int metric,
 /* _symmetrize_adjacency (src/graphs/knn_graph.jl:196-210): union of i->j and
  * j->i, each row sorted ascending, as CSR (out_offsets n+1, out_ids 2*n*k
  * capacity).  *out_total receives the edge count; *out_max_degree the new
- * graph.k (:74). */
+ * graph.k (:74).  Runs on the GPU (radix sort of both edge directions, then
+ * duplicate removal); adj and the two output arrays are all host or all
+ * device memory.  A neighbour id outside [1, n] is an ArgumentError. */
 int mann_knn_graph_symmetrize(const int64_t* adj, int64_t n, int64_t k, int64_t* out_offsets,
                               int64_t* out_ids, int64_t* out_total, int64_t* out_max_degree);
 

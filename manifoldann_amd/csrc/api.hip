@@ -199,35 +199,33 @@ int mann_knn_graph_symmetrize(const int64_t* adj, int64_t n, int64_t k, int64_t*
   return guarded([&] {
     MANN_REQUIRE(adj && out_offsets && out_ids, ERR_NULL, "null pointer");
     MANN_REQUIRE(n >= 0 && k >= 0, ERR_ARGUMENT, "negative size");
-    std::vector<int64_t> deg(n, 0);
-    for (int64_t i = 0; i < n; ++i)
-      for (int64_t r = 0; r < k; ++r) {
-        int64_t j = adj[i * k + r];
-        MANN_REQUIRE(j >= 1 && j <= n, ERR_ARGUMENT, "neighbour id out of range");
-        deg[i]++;
-        deg[j - 1]++;
-      }
-    std::vector<int64_t> off(n + 1, 0), fill(n, 0);
-    for (int64_t i = 0; i < n; ++i) off[i + 1] = off[i] + deg[i];
-    std::vector<int64_t> tmp(off[n]);
-    for (int64_t i = 0; i < n; ++i)
-      for (int64_t r = 0; r < k; ++r) {
-        int64_t j = adj[i * k + r] - 1;
-        tmp[off[i] + fill[i]++] = j + 1;
-        tmp[off[j] + fill[j]++] = i + 1;
-      }
-    int64_t w = 0, maxd = 0;
-    out_offsets[0] = 0;
-    for (int64_t i = 0; i < n; ++i) {
-      std::sort(tmp.begin() + off[i], tmp.begin() + off[i + 1]);
-      int64_t start = w;
-      for (int64_t t = off[i]; t < off[i + 1]; ++t)
-        if (t == off[i] || tmp[t] != tmp[t - 1]) out_ids[w++] = tmp[t];
-      maxd = std::max(maxd, w - start);
-      out_offsets[i + 1] = w;
+    const bool dev = is_device_ptr(adj);
+    MANN_REQUIRE(dev == is_device_ptr(out_offsets) && dev == is_device_ptr(out_ids),
+                 ERR_ARGUMENT, "adjacency and outputs must all be host or all be device memory");
+    if (dev) {
+      knn_graph_symmetrize_gpu(adj, n, k, out_offsets, out_ids, out_total, out_max_degree);
+      timer_flush();
+      return;
     }
-    if (out_total) *out_total = w;
+    // host arrays: stage through device buffers
+    ThreadCtx& c = ctx();
+    hipStream_t st = c.stream;
+    const int64_t m = 2 * n * k;
+    int64_t* da = c.buf("sym_adj").as<int64_t>(std::max<int64_t>(n * k, 1));
+    int64_t* doff = c.buf("sym_off").as<int64_t>(n + 1);
+    int64_t* dids = c.buf("sym_ids").as<int64_t>(std::max<int64_t>(m, 1));
+    if (n * k > 0)
+      MANN_HIP(hipMemcpyAsync(da, adj, sizeof(int64_t) * n * k, hipMemcpyHostToDevice, st));
+    int64_t tot = 0, maxd = 0;
+    knn_graph_symmetrize_gpu(da, n, k, doff, dids, &tot, &maxd);
+    MANN_HIP(hipMemcpyAsync(out_offsets, doff, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost,
+                            st));
+    if (tot > 0)
+      MANN_HIP(hipMemcpyAsync(out_ids, dids, sizeof(int64_t) * tot, hipMemcpyDeviceToHost, st));
+    MANN_HIP(hipStreamSynchronize(st));
+    if (out_total) *out_total = tot;
     if (out_max_degree) *out_max_degree = maxd;
+    timer_flush();
   });
 }
 

@@ -142,6 +142,10 @@ void kmeans_pp(const mann_dataset& ds, int k, uint64_t seed, float* C, int* pick
 std::vector<int> random_indices(int64_t n, int k, uint64_t seed);
 
 // IVF (core_ivf.hip)
+// _symmetrize_adjacency on device arrays: adj (n x k, 1-based) -> CSR offsets
+// (n + 1) and ids (capacity 2 n k); *total edges, *max_degree the longest row
+void knn_graph_symmetrize_gpu(const int64_t* adj, int64_t n, int64_t k, int64_t* offsets,
+                              int64_t* ids, int64_t* total, int64_t* max_degree);
 mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int64_t* assign_host,
                     bool all_lists = false);
 void ivf_route(const mann_ivf& iv, const QueryOps& Q, float* R);
