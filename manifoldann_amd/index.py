@@ -229,7 +229,7 @@ def query_arrays(index, data, Q, k, out=None, nprobe=None):
         from .ivf import query_arrays as ivf_query_arrays
         return ivf_query_arrays(index, data, Q, k, nprobe=nprobe)
     if nprobe is not None:
-        raise ArgumentError("nprobe is a keyword of IVFFlatIndex queries")
+        raise ArgumentError("nprobe is a keyword of IVF index queries")
     qm = as_matrix(Q)
     d, n = _data_shape(data)
     if qm.d != d:
@@ -266,7 +266,7 @@ def _to_neighbors(ids, dist):
 
 def query(index, data, q, k, nprobe=None):
     """query(index, data, q|Q, k[; nprobe]) -> Vector{Neighbor} or
-    Vector{Vector{Neighbor}}; nprobe only for IVFFlatIndex (ivf_flat.jl:138-144)."""
+    Vector{Vector{Neighbor}}; nprobe (IVF indexes only) as in ivf_flat.jl:138-144."""
     if isinstance(q, (list, tuple)) and q and not np.isscalar(q[0]):
         # vector-of-vectors batch (ann_index.jl:110-120)
         q = np.stack([np.asarray(v, np.float32) for v in q], axis=1)

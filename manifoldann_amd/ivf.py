@@ -498,8 +498,6 @@ def query_arrays(index: MultiLevelIndex, data, Q, k, route=None, nprobe=None):
             raise DimensionMismatch(f"Expected query of length {index.dimension}")
         npb = index.default_nprobe if nprobe is None else int(nprobe)
         nprobe = min(max(npb, 1), index.nlist)
-    elif nprobe is not None:
-        raise ArgumentError("nprobe is a keyword of IVFFlatIndex queries")
     if qm.d != d:
         raise DimensionMismatch(f"Expected queries with {d} rows, got {qm.d}")
     nq = qm.n
