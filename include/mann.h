@@ -131,6 +131,10 @@ int mann_ivf_build(mann_dataset_t X, const float* centroids, int64_t nlist,
  * ones included (:66-77), each in ascending id.  Queries go through
  * mann_ivf_search with nprobe clamped to [1, nlist] (:146); the lists probed
  * are the nprobe nearest centroids, empty ones taking their slots (:149-150).
+ * mann_ivf_search on such a handle merges as the reference's bounded max-heap
+ * (:158-167): the GPU takes the (dist, id) top-(k+1); a query whose k-th and
+ * (k+1)-th distances tie replays its probed lists through the heap on the GPU,
+ * so ties at the k-th place resolve as the reference's do.  k <= 255.
  * assign is 1-based and must lie in [1, nlist]. */
 int mann_ivf_flat_build(mann_dataset_t X, const float* centroids, int64_t nlist,
                         const int64_t* assign, mann_ivf_t* out);
@@ -232,8 +236,11 @@ int mann_last_emitted(int64_t* total);
  * produces the first-pass distances itself; read when a dataset is created),
  * "sample_rows" (exact kNN: minimum
  * rows of the stride sample a dataset keeps for the emission threshold; read
- * when the dataset is created).  No reference counterpart: the reference has
- * no tunable kernels. */
+ * when the dataset is created).  mann_tune_get also reads
+ * "last_heap_replays": the queries of the last IVF-Flat search on this thread
+ * whose k-th place had a distance tie and went through the bounded-heap merge
+ * (read only).  No reference counterpart: the reference has no tunable
+ * kernels. */
 int mann_tune_set(const char* key, int64_t value);
 int mann_tune_get(const char* key, int64_t* value);
 

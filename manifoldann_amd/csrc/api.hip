@@ -798,6 +798,7 @@ int mann_tune_get(const char* key, int64_t* value) {
     else if (k == "sample_rows") *value = t.sample_rows;
     else if (k == "emit_per_k") *value = t.emit_per_k;
     else if (k == "fold_norms") *value = t.fold_norms;
+    else if (k == "last_heap_replays") *value = last_stats().heap_replays;
     else throw Error{ERR_ARGUMENT, "unknown tuning key " + k};
   });
 }

@@ -40,6 +40,7 @@ struct mann_ivf {
   int nlist = 0;   // routing distances per query (centroids)
   int64_t max_list = 0;  // rows of the largest list
   int nb = 0;      // buckets = max assignment (partition_by_transform)
+  int flat = 0;    // IVF-Flat: nlist lists, bounded-heap merge (ivf_flat.jl:138-167)
   int normalized = 0;
   float* c32 = nullptr;
   float* cn2 = nullptr;
@@ -117,6 +118,7 @@ struct SearchStats {
   long long reranked = 0, overflow = 0, pass2 = 0, exact = 0;
   long long probes = 0;  // IVF: (list, query) pairs scanned
   long long emitted = 0; // first-pass candidates read by the selection (all passes)
+  long long heap_replays = 0;  // IVF-Flat queries merged through the bounded heap
 };
 SearchStats& last_stats();
 // adds the selection counters of a kStatSlots x 4 host copy to s

@@ -21,6 +21,45 @@
 namespace mann {
 
 
+static void ivf_search_sel(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int metric,
+                           const float* route_in, int64_t* out_ids, float* out_dist, float mass);
+
+void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int metric,
+                const float* route_in, int64_t* out_ids, float* out_dist, float mass) {
+  if (!iv.flat || Q.nq == 0 || k <= 0) {
+    ivf_search_sel(iv, Q, k, nprobe, metric, route_in, out_ids, out_dist, mass);
+    return;
+  }
+  // IVF-Flat (ivf_flat.jl:138-167): the (dist, id) top-(k+1), then the heap
+  // merge for the queries with a distance tie at the k-th place
+  ThreadCtx& c = ctx();
+  hipStream_t st = c.stream;
+  MANN_REQUIRE(k + 1 <= select_exmax() / 2, ERR_UNSUPPORTED,
+               "k too large for the IVF-Flat merge (max 255)");
+  const int nq = (int)Q.nq;
+  const float* R = route_in;
+  if (!R) {
+    float* Rb = c.buf("ivf_fR").as<float>((size_t)nq * iv.nlist);
+    ivf_route(iv, Q, Rb);
+    R = Rb;
+  }
+  int64_t* ti = c.buf("ivf_fti").as<int64_t>((size_t)nq * (k + 1));
+  float* td = c.buf("ivf_ftd").as<float>((size_t)nq * (k + 1));
+  ivf_search_sel(iv, Q, k + 1, nprobe, metric, R, ti, td, mass);
+  SearchStats keep = last_stats();
+  int* nrep = c.buf("ivf_frep").as<int>(1);
+  MANN_HIP(hipMemsetAsync(nrep, 0, sizeof(int), st));
+  ScopedTime tm("ivf_flat_merge", st);
+  launch_ivf_flat_merge(ti, td, nq, k, R, iv.nlist, std::min(nprobe, iv.nlist), iv.list_off,
+                        iv.perm, iv.x32, iv.xn2, Q.q32, Q.qn2, (int)iv.d16, metric, out_ids,
+                        out_dist, nrep, st);
+  int h = 0;
+  MANN_HIP(hipMemcpyAsync(&h, nrep, sizeof(int), hipMemcpyDeviceToHost, st));
+  MANN_HIP(hipStreamSynchronize(st));
+  keep.heap_replays = h;
+  last_stats() = keep;
+}
+
 mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int64_t* assign_host,
                     bool all_lists) {
   ThreadCtx& c = ctx();
@@ -36,6 +75,7 @@ mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int
   // IVF-Flat keeps one list per centroid, empty trailing ones included, so that
   // they take probe slots as in ivf_flat.jl:68-77,150
   if (all_lists) nb = std::max(nb, nlist);
+  const int flat = all_lists ? 1 : 0;
   auto* iv = new mann_ivf();
   try {
     iv->device = c.device;
@@ -46,6 +86,7 @@ mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int
     iv->n_pad = ds.n_pad;
     iv->nlist = nlist;
     iv->nb = nb;
+    iv->flat = flat;
     iv->normalized = ds.normalized;
     // stable counting sort of the assignment (transforms/utils.jl:142-174)
     std::vector<int> a32(n);
@@ -133,8 +174,8 @@ void ivf_probe(const mann_ivf& iv, const float* R, int nq, int nprobe, int* prob
                       st);
 }
 
-void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int metric,
-                const float* route_in, int64_t* out_ids, float* out_dist, float mass) {
+static void ivf_search_sel(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int metric,
+                           const float* route_in, int64_t* out_ids, float* out_dist, float mass) {
   ThreadCtx& c = ctx();
   hipStream_t st = c.stream;
   SearchStats& S = last_stats();

@@ -83,4 +83,14 @@ void launch_ivf_tau(const IvfTauArgs& a, hipStream_t st);
 void launch_probe_ranges(const int* probes, const int* pcount, int nq, int nprobe,
                          const int* list_off, int2* ranges, hipStream_t st);
 
+// IVF-Flat merge from a (dist, id) top-(k+1) per query (ti, td: nq x (k+1)):
+// where the k-th and (k+1)-th distances differ the first k are the heap's
+// result; otherwise the query's probed lists are replayed through the
+// bounded max-heap of ivf_flat.jl:158-167.  Writes out (nq x k); k <= 255.
+void launch_ivf_flat_merge(const int64_t* ti, const float* td, int nq, int k, const float* R,
+                           int nlist, int nprobe, const int* list_off, const int* perm,
+                           const float* x32, const float* xn2, const float* q32,
+                           const float* qn2, int d16, int metric, int64_t* out_ids,
+                           float* out_dist, int* n_replayed, hipStream_t st);
+
 }  // namespace mann

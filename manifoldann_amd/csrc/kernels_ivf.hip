@@ -875,4 +875,156 @@ void launch_probe_ranges(const int* probes, const int* pcount, int nq, int nprob
                      probes, pcount, nq, nprobe, list_off, ranges);
 }
 
+// One wave per query.  Heap operations follow BoundedMaxHeap push!
+// (neighbor_heaps.jl:84-100, sift-up :156-168, sift-down :170-190) and run on
+// lane 0; the 4 groups of 16 lanes compute 4 exact distances at a time.
+__global__ __launch_bounds__(64) void ivf_flat_merge_kernel(
+    const int64_t* __restrict__ ti, const float* __restrict__ td, int k,
+    const float* __restrict__ R, int nlist, int nprobe, const int* __restrict__ list_off,
+    const int* __restrict__ perm, const float* __restrict__ x32, const float* __restrict__ xn2,
+    const float* __restrict__ q32, const float* __restrict__ qn2, int d16, int metric,
+    int64_t* __restrict__ out_ids, float* __restrict__ out_dist, int* n_replayed) {
+  __shared__ float hd[256];
+  __shared__ int hid[256];
+  __shared__ float cd[4];
+  __shared__ int cid[4];
+  __shared__ int hlen;
+  const int q = blockIdx.x, lane = threadIdx.x;
+  const int64_t* ri = ti + (int64_t)q * (k + 1);
+  const float* rd = td + (int64_t)q * (k + 1);
+  int64_t* oi = out_ids + (int64_t)q * k;
+  float* od = out_dist + (int64_t)q * k;
+  const int64_t id_next = ri[k];
+  const float d_next = rd[k], d_last = rd[k - 1];
+  int len = 0;  // heap size, meaningful on lane 0
+  const bool replay = id_next != 0 && d_next == d_last;
+  if (!replay) {
+    // the heap keeps exactly these k; only the order among equal distances
+    // changes (by id below, where the list scan ordered them by list)
+    for (int r = lane; r < k; r += 64) {
+      int64_t v = ri[r];
+      float w = rd[r];
+      hid[r] = (int)v;
+      hd[r] = w;
+    }
+    __syncthreads();
+    if (lane == 0)
+      while (len < k && hid[len] != 0) ++len;
+  } else {
+  if (lane == 0) atomicAdd(n_replayed, 1);
+  // a (k+1)-th candidate exists, so the heap's capacity k_eff is k (:156)
+  float pd = -INFINITY;
+  int pi = -1;
+  const int g = lane >> 4, j = lane & 15;
+  for (int s = 0; s < nprobe; ++s) {
+    // next probed list: smallest (distance, centroid) after (pd, pi) -- the
+    // order of partialsortperm (:150)
+    float bd = INFINITY;
+    int bi = 0x7fffffff;
+    for (int c = lane; c < nlist; c += 64) {
+      float v = R[(int64_t)q * nlist + c];
+      bool after = v > pd || (v == pd && c > pi);
+      if (after && (v < bd || (v == bd && c < bi))) {
+        bd = v;
+        bi = c;
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      float ov = __shfl_xor(bd, o);
+      int oc = __shfl_xor(bi, o);
+      if (ov < bd || (ov == bd && oc < bi)) {
+        bd = ov;
+        bi = oc;
+      }
+    }
+    if (bi == 0x7fffffff) break;
+    pd = bd;
+    pi = bi;
+    const int r0 = list_off[bi], r1 = list_off[bi + 1];
+    for (int r = r0; r < r1; r += 4) {
+      const int rr = min(r + g, r1 - 1);
+      float dv = exact_metric(x32, q32, xn2, qn2, d16, metric, rr, q, j);
+      int idv = perm[rr] + 1;
+      if (j == 0) {
+        cd[g] = dv;
+        cid[g] = idv;
+      }
+      __syncthreads();
+      if (lane == 0) {
+        const int nc = min(4, r1 - r);
+        for (int e = 0; e < nc; ++e) {
+          const float dist = cd[e];
+          const int id = cid[e];
+          if (len < k) {
+            int i = ++len;  // 1-based sift-up
+            hd[i - 1] = dist;
+            hid[i - 1] = id;
+            while (i > 1) {
+              int p = i >> 1;
+              if (hd[i - 1] > hd[p - 1]) {
+                float tdv = hd[i - 1]; hd[i - 1] = hd[p - 1]; hd[p - 1] = tdv;
+                int tiv = hid[i - 1]; hid[i - 1] = hid[p - 1]; hid[p - 1] = tiv;
+                i = p;
+              } else {
+                break;
+              }
+            }
+          } else if (dist < hd[0]) {
+            hd[0] = dist;
+            hid[0] = id;
+            int i = 1;
+            for (;;) {
+              int l = i << 1, rgt = l + 1, big = i;
+              if (l <= len && hd[l - 1] > hd[big - 1]) big = l;
+              if (rgt <= len && hd[rgt - 1] > hd[big - 1]) big = rgt;
+              if (big == i) break;
+              float tdv = hd[i - 1]; hd[i - 1] = hd[big - 1]; hd[big - 1] = tdv;
+              int tiv = hid[i - 1]; hid[i - 1] = hid[big - 1]; hid[big - 1] = tiv;
+              i = big;
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  }  // replay
+  if (lane == 0) {
+    // to_sorted_vector (:131-134): insertion sort by (dist, id)
+    for (int a = 1; a < len; ++a) {
+      float dv = hd[a];
+      int iv = hid[a];
+      int b = a - 1;
+      while (b >= 0 && (hd[b] > dv || (hd[b] == dv && hid[b] > iv))) {
+        hd[b + 1] = hd[b];
+        hid[b + 1] = hid[b];
+        --b;
+      }
+      hd[b + 1] = dv;
+      hid[b + 1] = iv;
+    }
+    hlen = len;
+  }
+  __syncthreads();
+  const int n_kept = hlen;
+  for (int r = lane; r < k; r += 64) {
+    const bool have = r < n_kept;
+    float dv = hd[r < 256 ? r : 0];
+    int iv = hid[r < 256 ? r : 0];
+    oi[r] = have ? (int64_t)iv : 0;
+    od[r] = have ? dv : INFINITY;
+  }
+}
+
+void launch_ivf_flat_merge(const int64_t* ti, const float* td, int nq, int k, const float* R,
+                           int nlist, int nprobe, const int* list_off, const int* perm,
+                           const float* x32, const float* xn2, const float* q32,
+                           const float* qn2, int d16, int metric, int64_t* out_ids,
+                           float* out_dist, int* n_replayed, hipStream_t st) {
+  if (nq <= 0 || k <= 0) return;
+  ivf_flat_merge_kernel<<<nq, 64, 0, st>>>(ti, td, k, R, nlist, nprobe, list_off, perm, x32, xn2,
+                                           q32, qn2, d16, metric, out_ids, out_dist, n_replayed);
+  MANN_HIP(hipGetLastError());
+}
+
 }  // namespace mann

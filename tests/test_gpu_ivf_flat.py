@@ -1,10 +1,11 @@
 """IVFFlatIndex on the GPU (ivf_flat.jl:30-168) against the heap-merge oracle.
 
 Without distance ties at the k-th place the reference's bounded heap keeps the
-k nearest candidates of the probed lists, which is what the GPU list scan
-returns (the lower id wins an exact tie there).  The inputs here are continuous
-random data, on which exact FP32 ties at the k-th place do not occur; one would
-show as an id mismatch."""
+k nearest candidates of the probed lists, the GPU list scan's (dist, id) top-k;
+a query with a tie there is replayed through the heap on the GPU.  Continuous
+random data exercises the first case, duplicated points the second."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -12,6 +13,7 @@ pytestmark = pytest.mark.gpu
 
 import manifoldann_amd as M
 from manifoldann_amd import ivf as I
+from manifoldann_amd._lib import check, lib
 from oracle import oracle as O
 
 
@@ -60,7 +62,7 @@ def test_ivf_flat_matches_heap_oracle_and_batch_equals_single():
         assert [n.id for n in vv[i]] == [n.id for n in batch[i]]
 
 
-@pytest.mark.parametrize("k,nprobe", [(1, 1), (10, 3), (100, 2), (256, 1), (10, 0), (10, 99)])
+@pytest.mark.parametrize("k,nprobe", [(1, 1), (10, 3), (100, 2), (255, 1), (10, 0), (10, 99)])
 def test_ivf_flat_probe_and_k_sweep(k, nprobe):
     _banner(f"ivf-flat 3000x24 nlist 16 k {k} nprobe {nprobe}")
     rng = np.random.default_rng(7)
@@ -112,3 +114,54 @@ def test_ivf_flat_dimension_checks():
     bf = M.build_index(M.BruteForceIndex, data)
     with pytest.raises(M.ArgumentError):
         M.query(bf, data, np.zeros(8, np.float32), 3, nprobe=2)
+
+
+def _heap_replays():
+    v = ctypes.c_int64()
+    check(lib().mann_tune_get(b"last_heap_replays", ctypes.byref(v)))
+    return v.value
+
+
+@pytest.mark.parametrize("k,nprobe", [(1, 2), (5, 3), (16, 4), (255, 8)])
+def test_distance_ties_follow_the_heap(k, nprobe):
+    # 150 distinct points, each stored 4 times under different ids: every
+    # k-th place falls in a run of equal distances, where the bounded heap's
+    # survivors depend on push order and heap layout (neighbor_heaps.jl:84-190)
+    _banner(f"ivf-flat duplicated points k {k} nprobe {nprobe}")
+    rng = np.random.default_rng(31 + k)
+    base = rng.standard_normal((150, 12)).astype(np.float32)
+    X = base[rng.permutation(np.repeat(np.arange(150), 4))]
+    data = np.asfortranarray(X.T)
+    C = rng.standard_normal((12, 8)).astype(np.float32)
+    assign = rng.integers(1, 9, size=600)
+    t = I.KMeansTransform(k=8)
+    t.centroids = np.asfortranarray(C)
+    t.centroid_norms = (C.astype(np.float64) ** 2).sum(0).astype(np.float32)
+    idx = I.ivf_flat_from_assignment(data, t, assign)
+    Q = (base[rng.integers(0, 150, size=48)]
+         + 0.05 * rng.standard_normal((48, 12))).astype(np.float32)
+    _check_vs_oracle(idx, data, Q, k, nprobe)
+    reps = _heap_replays()
+    print(f"heap replays {reps} of 48", flush=True)
+    assert reps > 0
+    if k > 1:
+        # the heap's answer is not the (dist, id) one on some query
+        route = I.route_distances(idx, Q.T)
+        off, ids = _oracle_lists(idx)
+        oi, _, cnt = O.ivf_flat_query(X, off, ids, Q, route, k, nprobe)
+        differs = 0
+        for q in range(48):
+            probed = np.argsort(route[q], kind="stable")[:nprobe]
+            cand = np.concatenate([ids[off[c]:off[c + 1]] for c in probed])
+            d32 = np.array([O.metric(O.EUCLIDEAN, X[c - 1], Q[q]) for c in cand], np.float32)
+            order = np.lexsort((cand, d32))
+            differs += cand[order][:cnt[q]].tolist() != oi[q, :cnt[q]].tolist()
+        assert differs > 0
+
+
+def test_ivf_flat_k_limit():
+    rng = np.random.default_rng(2)
+    data = np.asfortranarray(rng.standard_normal((8, 600)).astype(np.float32))
+    idx = M.build_index(M.IVFFlatIndex, data, nlist=2)
+    with pytest.raises(M.MannError):
+        M.query(idx, data, np.zeros(8, np.float32), 256, nprobe=2)
