@@ -911,83 +911,83 @@ __global__ __launch_bounds__(64) void ivf_flat_merge_kernel(
     if (lane == 0)
       while (len < k && hid[len] != 0) ++len;
   } else {
-  if (lane == 0) atomicAdd(n_replayed, 1);
-  // a (k+1)-th candidate exists, so the heap's capacity k_eff is k (:156)
-  float pd = -INFINITY;
-  int pi = -1;
-  const int g = lane >> 4, j = lane & 15;
-  for (int s = 0; s < nprobe; ++s) {
-    // next probed list: smallest (distance, centroid) after (pd, pi) -- the
-    // order of partialsortperm (:150)
-    float bd = INFINITY;
-    int bi = 0x7fffffff;
-    for (int c = lane; c < nlist; c += 64) {
-      float v = R[(int64_t)q * nlist + c];
-      bool after = v > pd || (v == pd && c > pi);
-      if (after && (v < bd || (v == bd && c < bi))) {
-        bd = v;
-        bi = c;
+    if (lane == 0) atomicAdd(n_replayed, 1);
+    // a (k+1)-th candidate exists, so the heap's capacity k_eff is k (:156)
+    float pd = -INFINITY;
+    int pi = -1;
+    const int g = lane >> 4, j = lane & 15;
+    for (int s = 0; s < nprobe; ++s) {
+      // next probed list: smallest (distance, centroid) after (pd, pi) -- the
+      // order of partialsortperm (:150)
+      float bd = INFINITY;
+      int bi = 0x7fffffff;
+      for (int c = lane; c < nlist; c += 64) {
+        float v = R[(int64_t)q * nlist + c];
+        bool after = v > pd || (v == pd && c > pi);
+        if (after && (v < bd || (v == bd && c < bi))) {
+          bd = v;
+          bi = c;
+        }
       }
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-      float ov = __shfl_xor(bd, o);
-      int oc = __shfl_xor(bi, o);
-      if (ov < bd || (ov == bd && oc < bi)) {
-        bd = ov;
-        bi = oc;
+      for (int o = 32; o > 0; o >>= 1) {
+        float ov = __shfl_xor(bd, o);
+        int oc = __shfl_xor(bi, o);
+        if (ov < bd || (ov == bd && oc < bi)) {
+          bd = ov;
+          bi = oc;
+        }
       }
-    }
-    if (bi == 0x7fffffff) break;
-    pd = bd;
-    pi = bi;
-    const int r0 = list_off[bi], r1 = list_off[bi + 1];
-    for (int r = r0; r < r1; r += 4) {
-      const int rr = min(r + g, r1 - 1);
-      float dv = exact_metric(x32, q32, xn2, qn2, d16, metric, rr, q, j);
-      int idv = perm[rr] + 1;
-      if (j == 0) {
-        cd[g] = dv;
-        cid[g] = idv;
-      }
-      __syncthreads();
-      if (lane == 0) {
-        const int nc = min(4, r1 - r);
-        for (int e = 0; e < nc; ++e) {
-          const float dist = cd[e];
-          const int id = cid[e];
-          if (len < k) {
-            int i = ++len;  // 1-based sift-up
-            hd[i - 1] = dist;
-            hid[i - 1] = id;
-            while (i > 1) {
-              int p = i >> 1;
-              if (hd[i - 1] > hd[p - 1]) {
-                float tdv = hd[i - 1]; hd[i - 1] = hd[p - 1]; hd[p - 1] = tdv;
-                int tiv = hid[i - 1]; hid[i - 1] = hid[p - 1]; hid[p - 1] = tiv;
-                i = p;
-              } else {
-                break;
+      if (bi == 0x7fffffff) break;
+      pd = bd;
+      pi = bi;
+      const int r0 = list_off[bi], r1 = list_off[bi + 1];
+      for (int r = r0; r < r1; r += 4) {
+        const int rr = min(r + g, r1 - 1);
+        float dv = exact_metric(x32, q32, xn2, qn2, d16, metric, rr, q, j);
+        int idv = perm[rr] + 1;
+        if (j == 0) {
+          cd[g] = dv;
+          cid[g] = idv;
+        }
+        __syncthreads();
+        if (lane == 0) {
+          const int nc = min(4, r1 - r);
+          for (int e = 0; e < nc; ++e) {
+            const float dist = cd[e];
+            const int id = cid[e];
+            if (len < k) {
+              int i = ++len;  // 1-based sift-up
+              hd[i - 1] = dist;
+              hid[i - 1] = id;
+              while (i > 1) {
+                int p = i >> 1;
+                if (hd[i - 1] > hd[p - 1]) {
+                  float tdv = hd[i - 1]; hd[i - 1] = hd[p - 1]; hd[p - 1] = tdv;
+                  int tiv = hid[i - 1]; hid[i - 1] = hid[p - 1]; hid[p - 1] = tiv;
+                  i = p;
+                } else {
+                  break;
+                }
               }
-            }
-          } else if (dist < hd[0]) {
-            hd[0] = dist;
-            hid[0] = id;
-            int i = 1;
-            for (;;) {
-              int l = i << 1, rgt = l + 1, big = i;
-              if (l <= len && hd[l - 1] > hd[big - 1]) big = l;
-              if (rgt <= len && hd[rgt - 1] > hd[big - 1]) big = rgt;
-              if (big == i) break;
-              float tdv = hd[i - 1]; hd[i - 1] = hd[big - 1]; hd[big - 1] = tdv;
-              int tiv = hid[i - 1]; hid[i - 1] = hid[big - 1]; hid[big - 1] = tiv;
-              i = big;
+            } else if (dist < hd[0]) {
+              hd[0] = dist;
+              hid[0] = id;
+              int i = 1;
+              for (;;) {
+                int l = i << 1, rgt = l + 1, big = i;
+                if (l <= len && hd[l - 1] > hd[big - 1]) big = l;
+                if (rgt <= len && hd[rgt - 1] > hd[big - 1]) big = rgt;
+                if (big == i) break;
+                float tdv = hd[i - 1]; hd[i - 1] = hd[big - 1]; hd[big - 1] = tdv;
+                int tiv = hid[i - 1]; hid[i - 1] = hid[big - 1]; hid[big - 1] = tiv;
+                i = big;
+              }
             }
           }
         }
+        __syncthreads();
       }
-      __syncthreads();
     }
-  }
   }  // replay
   if (lane == 0) {
     // to_sorted_vector (:131-134): insertion sort by (dist, id)
