@@ -208,6 +208,23 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
         sweep_c.append({"nprobe": npb, "recall": round(rc, 4), "qps": round(qps, 1)})
         log(f"IVF centroid nprobe={npb}: recall {rc:.4f} qps {qps:.0f}")
 
+    # IVFFlatIndex over the same fit (ivf_flat.jl:30-168, default nprobe 10):
+    # one list per centroid and the bounded-heap merge
+    flat = I.ivf_flat_from_assignment(ds, t, t.pending_assignments, nprobe=10)
+    sweep_f = []
+    for npb in (10, 16):
+        def run_f(npb=npb):
+            check(L.mann_ivf_search(flat.handle, Qd.data_ptr(), d, nq, d, 10, npb, 0, None,
+                                    ids.data_ptr(), dist.data_ptr()))
+        qps = qps_of(run_f, nq, args.reps, torch)
+        rc = recall(ids)
+        rep = ctypes.c_int64()
+        check(L.mann_tune_get(b"last_heap_replays", ctypes.byref(rep)))
+        sweep_f.append({"nprobe": npb, "recall": round(rc, 4), "qps": round(qps, 1),
+                        "heap_replays": int(rep.value)})
+        log(f"IVF-Flat nprobe={npb}: recall {rc:.4f} qps {qps:.0f} heap replays {rep.value}")
+    del flat
+
     # learned router: labels from the exact kNN of training queries
     t1 = time.perf_counter()
     Tq = datasets.mixture_queries(args.train_q, d, components=1000, seed=3, query_seed=77)
@@ -314,7 +331,7 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
            "kmeans_fit_s": round(fit_s, 3), "router_train_s": round(train_s, 3),
            "router_final_loss": round(loss, 4), "train_queries": args.train_q,
            "router_prior_scale": gamma, "router_epochs": args.epochs,
-           "centroid": sweep_c, "learned": sweep_l, "learned_mass": sweep_m,
+           "centroid": sweep_c, "ivf_flat": sweep_f, "learned": sweep_l, "learned_mass": sweep_m,
            "prior_only_mass": sweep_p,
            "centroid_at_0.95": bc, "learned_at_0.95": bl,
            "learned_fixed_at_0.95": bl_fixed, "learned_mass_at_0.95": bl_mass,

@@ -875,9 +875,47 @@ void launch_probe_ranges(const int* probes, const int* pcount, int nq, int nprob
                      probes, pcount, nq, nprobe, list_off, ranges);
 }
 
+// Exact FP32 metric of reordered row r against query q on one lane: the same
+// A16 order as exact_metric (element i into accumulator i & 15, increasing i;
+// then the pairwise tree +8, +4, +2, +1 that group16_tree forms across lanes).
+__device__ __forceinline__ float exact_metric_lane(const float* __restrict__ x32,
+                                                   const float* __restrict__ q32,
+                                                   const float* __restrict__ xn2,
+                                                   const float* __restrict__ qn2, int d16,
+                                                   int metric, int64_t r, int64_t q) {
+  const float4* x = (const float4*)(x32 + r * d16);
+  const float4* y = (const float4*)(q32 + q * d16);
+  const bool diff = metric == EUCLIDEAN || metric == SQEUCLIDEAN;
+  float a[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) a[e] = 0.0f;
+  for (int t = 0; t < d16 / 4; t += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float4 xv = x[t + u], yv = y[t + u];
+      const float xs[4] = {xv.x, xv.y, xv.z, xv.w}, ys[4] = {yv.x, yv.y, yv.z, yv.w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = 4 * u + e;
+        if (diff) {
+          const float df = xs[e] - ys[e];
+          a[i] = __builtin_fmaf(df, df, a[i]);
+        } else {
+          a[i] = __builtin_fmaf(xs[e], ys[e], a[i]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int sft = 8; sft > 0; sft >>= 1)
+#pragma unroll
+    for (int e = 0; e < sft; ++e) a[e] = a[e] + a[e + sft];
+  return finish_metric(a[0], metric, diff ? 0.0f : xn2[r], diff ? 0.0f : qn2[q]);
+}
+
 // One wave per query.  Heap operations follow BoundedMaxHeap push!
 // (neighbor_heaps.jl:84-100, sift-up :156-168, sift-down :170-190) and run on
-// lane 0; the 4 groups of 16 lanes compute 4 exact distances at a time.
+// lane 0; each lane computes one exact distance, 64 rows at a time.
 __global__ __launch_bounds__(64) void ivf_flat_merge_kernel(
     const int64_t* __restrict__ ti, const float* __restrict__ td, int k,
     const float* __restrict__ R, int nlist, int nprobe, const int* __restrict__ list_off,
@@ -886,8 +924,8 @@ __global__ __launch_bounds__(64) void ivf_flat_merge_kernel(
     int64_t* __restrict__ out_ids, float* __restrict__ out_dist, int* n_replayed) {
   __shared__ float hd[256];
   __shared__ int hid[256];
-  __shared__ float cd[4];
-  __shared__ int cid[4];
+  __shared__ float cd[64];
+  __shared__ int cid[64];
   __shared__ int hlen;
   const int q = blockIdx.x, lane = threadIdx.x;
   const int64_t* ri = ti + (int64_t)q * (k + 1);
@@ -915,7 +953,6 @@ __global__ __launch_bounds__(64) void ivf_flat_merge_kernel(
     // a (k+1)-th candidate exists, so the heap's capacity k_eff is k (:156)
     float pd = -INFINITY;
     int pi = -1;
-    const int g = lane >> 4, j = lane & 15;
     for (int s = 0; s < nprobe; ++s) {
       // next probed list: smallest (distance, centroid) after (pd, pi) -- the
       // order of partialsortperm (:150)
@@ -941,17 +978,15 @@ __global__ __launch_bounds__(64) void ivf_flat_merge_kernel(
       pd = bd;
       pi = bi;
       const int r0 = list_off[bi], r1 = list_off[bi + 1];
-      for (int r = r0; r < r1; r += 4) {
-        const int rr = min(r + g, r1 - 1);
-        float dv = exact_metric(x32, q32, xn2, qn2, d16, metric, rr, q, j);
+      for (int r = r0; r < r1; r += 64) {
+        const int rr = min(r + lane, r1 - 1);
+        float dv = exact_metric_lane(x32, q32, xn2, qn2, d16, metric, rr, q);
         int idv = perm[rr] + 1;
-        if (j == 0) {
-          cd[g] = dv;
-          cid[g] = idv;
-        }
+        cd[lane] = dv;
+        cid[lane] = idv;
         __syncthreads();
         if (lane == 0) {
-          const int nc = min(4, r1 - r);
+          const int nc = min(64, r1 - r);
           for (int e = 0; e < nc; ++e) {
             const float dist = cd[e];
             const int id = cid[e];
