@@ -927,6 +927,8 @@ __global__ __launch_bounds__(64) void ivf_flat_merge_kernel(
   __shared__ float cd[64];
   __shared__ int cid[64];
   __shared__ int hlen;
+  __shared__ int hfull;
+  __shared__ float hroot;
   const int q = blockIdx.x, lane = threadIdx.x;
   const int64_t* ri = ti + (int64_t)q * (k + 1);
   const float* rd = td + (int64_t)q * (k + 1);
@@ -949,7 +951,12 @@ __global__ __launch_bounds__(64) void ivf_flat_merge_kernel(
     if (lane == 0)
       while (len < k && hid[len] != 0) ++len;
   } else {
-    if (lane == 0) atomicAdd(n_replayed, 1);
+    if (lane == 0) {
+      atomicAdd(n_replayed, 1);
+      hfull = 0;
+      hroot = INFINITY;
+    }
+    __syncthreads();
     // a (k+1)-th candidate exists, so the heap's capacity k_eff is k (:156)
     float pd = -INFINITY;
     int pi = -1;
@@ -984,10 +991,13 @@ __global__ __launch_bounds__(64) void ivf_flat_merge_kernel(
         int idv = perm[rr] + 1;
         cd[lane] = dv;
         cid[lane] = idv;
+        // the root only decreases, so a row at or above it now never enters
+        const bool maybe = !(hfull && dv >= hroot) && r + lane < r1;
+        const unsigned long long cand = __builtin_amdgcn_ballot_w64(maybe);
         __syncthreads();
         if (lane == 0) {
-          const int nc = min(64, r1 - r);
-          for (int e = 0; e < nc; ++e) {
+          for (unsigned long long mk = cand; mk != 0; mk &= mk - 1) {
+            const int e = __builtin_ctzll(mk);
             const float dist = cd[e];
             const int id = cid[e];
             if (len < k) {
@@ -1019,6 +1029,8 @@ __global__ __launch_bounds__(64) void ivf_flat_merge_kernel(
               }
             }
           }
+          hfull = len == k;
+          hroot = hd[0];
         }
         __syncthreads();
       }
