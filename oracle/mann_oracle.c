@@ -375,23 +375,27 @@ double orc_uniform(uint64_t* state) {
 
 #define ORC_PP_BLOCK 1024
 
-/* init_kmeans_plus_plus (kmeans/init.jl:56-102, update :110-128).
- * Draw u_0 picks the first centre floor(u_0 * n); draw u_i (i >= 1) is the
- * threshold fraction of round i.  The cumulative sum uses blocks of 1024:
- * sequential FP64 sums inside a block, a sequential exclusive prefix over the
- * block totals, cumsum[j] = prefix[block] + local inclusive sum (the
- * reference's cumsum is one sequential pass).  next = first j with
- * cumsum[j] >= u * total, clamped to the last point. out_idx 1-based. */
-int orc_kmeanspp(const float* X, int64_t d, int64_t n, int64_t ld, int64_t k, uint64_t seed,
-                 float* out_C, int64_t* out_idx, int nthreads) {
+/* init_kmeans_plus_plus (kmeans/init.jl:56-102, update :110-128) driven by
+ * explicit uniform draws u[0..k-1] in [0, 1).
+ * Draw u_0 picks the first centre floor(u_0 * n) (rand(rng, 1:n), :69); draw
+ * u_i (i >= 1) is the threshold fraction of round i (rand(rng) * total, :92).
+ * The squared distances follow _update_distances_euclidean! (:116-127): the
+ * FP32 norms form, clamped at 0, widened to the FP64 running minimum.  The
+ * cumulative sum uses blocks of 1024: sequential FP64 sums inside a block, a
+ * sequential exclusive prefix over the block totals, cumsum[j] =
+ * prefix[block] + local inclusive sum (the reference's cumsum is one
+ * sequential pass, :89).  next = the first j with cumsum[j] >= u * total
+ * (searchsortedfirst, :93), clamped to the last point (:96).  out_idx 1-based. */
+int orc_kmeanspp_draws(const float* X, int64_t d, int64_t n, int64_t ld, int64_t k,
+                       const double* u, float* out_C, int64_t* out_idx, int nthreads) {
   if (k > n) return -3;
-  uint64_t st = seed;
+  if (k <= 0) return 0;
   double* mins = (double*)malloc(sizeof(double) * (size_t)n);
   double* cum = (double*)malloc(sizeof(double) * (size_t)n);
   float* xn = (float*)malloc(sizeof(float) * (size_t)n);
   orc_norms(X, d, n, ld, xn);
   for (int64_t j = 0; j < n; ++j) mins[j] = INFINITY;
-  int64_t first = (int64_t)(orc_uniform(&st) * (double)n);
+  int64_t first = (int64_t)(u[0] * (double)n);
   if (first >= n) first = n - 1;
   memcpy(out_C, X + first * ld, sizeof(float) * (size_t)d);
   if (out_idx) out_idx[0] = first + 1;
@@ -415,7 +419,7 @@ int orc_kmeanspp(const float* X, int64_t d, int64_t n, int64_t ld, int64_t k, ui
       }
       prefix += loc;
     }
-    double thr = orc_uniform(&st) * prefix;
+    double thr = u[i] * prefix;
     int64_t lo = 0, hi = n; /* first j with cum[j] >= thr */
     while (lo < hi) {
       int64_t mid = lo + (hi - lo) / 2;
@@ -430,6 +434,20 @@ int orc_kmeanspp(const float* X, int64_t d, int64_t n, int64_t ld, int64_t k, ui
   free(cum);
   free(xn);
   return 0;
+}
+
+/* The same with the draws taken from the splitmix64 stream of `seed`, in
+ * order u_0, u_1, ... (the library's mann_kmeanspp). */
+int orc_kmeanspp(const float* X, int64_t d, int64_t n, int64_t ld, int64_t k, uint64_t seed,
+                 float* out_C, int64_t* out_idx, int nthreads) {
+  if (k > n) return -3;
+  if (k <= 0) return 0;
+  double* u = (double*)malloc(sizeof(double) * (size_t)k);
+  uint64_t st = seed;
+  for (int64_t i = 0; i < k; ++i) u[i] = orc_uniform(&st);
+  int rc = orc_kmeanspp_draws(X, d, n, ld, k, u, out_C, out_idx, nthreads);
+  free(u);
+  return rc;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -61,6 +61,11 @@ def lib():
                                 _i64p, _i64p, ctypes.c_int]
         L.orc_kmeanspp.argtypes = [_f32p, _i64, _i64, _i64, _i64, ctypes.c_uint64, _f32p, _i64p,
                                    ctypes.c_int]
+        L.orc_kmeanspp_draws.argtypes = [_f32p, _i64, _i64, _i64, _i64,
+                                         ctypes.POINTER(ctypes.c_double), _f32p, _i64p,
+                                         ctypes.c_int]
+        L.orc_splitmix64.restype = ctypes.c_uint64
+        L.orc_splitmix64.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
         L.orc_uniform.restype = ctypes.c_double
         L.orc_uniform.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
         L.orc_route_distances.argtypes = [_f32p, _i64, _i64, _i64, _f32p, _i64, _f32p,
@@ -186,6 +191,28 @@ def kmeanspp(X, k, seed, nthreads=0):
     if rc != 0:
         raise AssertionError("Cannot initialize more clusters than points")
     return C, idx
+
+
+def kmeanspp_draws(X, k, draws, nthreads=0):
+    """kmeans++ with explicit uniform draws u_0..u_{k-1} (u_0 picks the first
+    centre, u_i the threshold fraction of round i) -> (C (k, d), idx (k,) 1-based)."""
+    X = _f32(X)
+    n, d = X.shape
+    u = np.ascontiguousarray(draws, np.float64)
+    assert u.shape[0] >= k
+    C = np.zeros((k, d), np.float32)
+    idx = np.zeros(k, np.int64)
+    rc = lib().orc_kmeanspp_draws(_p(X), d, n, d, k,
+                                  u.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _p(C),
+                                  _p(idx), nthreads)
+    if rc != 0:
+        raise AssertionError("Cannot initialize more clusters than points")
+    return C, idx
+
+
+def splitmix64_stream(seed, count):
+    st = ctypes.c_uint64(seed)
+    return [int(lib().orc_splitmix64(ctypes.byref(st))) for _ in range(count)]
 
 
 def uniform_stream(seed, count):

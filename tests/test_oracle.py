@@ -176,6 +176,96 @@ def test_kmeanspp_properties(orc):
     assert idx[0] == int(u0 * 3000) + 1             # first centre = floor(u0 * n)
 
 
+def test_splitmix64_known_answers(orc):
+    # The published splitmix64 reference outputs for state 0 (Vigna's
+    # splitmix64.c): the stream every seeded draw of the library comes from.
+    assert O.splitmix64_stream(0, 3) == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4,
+                                         0x06C45D188009454F]
+    # uniform = top 53 bits / 2^53
+    assert O.uniform_stream(0, 1)[0] == (0xE220A8397B1DCDAF >> 11) / 2.0**53
+
+
+def test_kmeanspp_closed_form_d2_sampling(orc):
+    # init.jl:56-102 on three points of a line at 0, 1 and 3 with the first
+    # centre at 0: the squared distances are [0, 1, 9], the inclusive
+    # cumulative sum [0, 1, 10], so round 1 picks the first j whose cumsum
+    # reaches u * 10.
+    X = np.array([[0.0], [1.0], [3.0]], np.float32)
+
+    def pick(u1):
+        _, idx = O.kmeanspp_draws(X, 2, [0.0, u1])
+        assert idx[0] == 1
+        return int(idx[1])
+
+    # u in (0.1, 1): threshold in (1, 10] -> point 3.  A D-weighted CDF
+    # ([0, 1, 4] / 4) would pick point 2 for u in (0.1, 0.25).
+    assert pick(0.2) == 3
+    assert pick(0.12) == 3
+    assert pick(0.99) == 3
+    # threshold exactly on the cumulative value 1 (0.1 * 10 == 1.0 in FP64):
+    # searchsortedfirst takes cumsum[j] >= threshold -> point 2 (a strict '>'
+    # search would take point 3)
+    assert pick(0.1) == 2
+    # below the boundary: point 2.  An exclusive cumulative sum ([0, 0, 1])
+    # would take point 3 here.
+    assert pick(0.05) == 2
+    # u = 0: threshold 0 is reached by cumsum[1] = 0 at the first centre itself
+    # (searchsortedfirst of 0 over [0, 1, 10] is 1)
+    assert pick(0.0) == 1
+    # first centre: rand(rng, 1:n) as floor(u_0 n) + 1
+    for u0, want in [(0.0, 1), (0.34, 2), (0.7, 3), (0.999999, 3)]:
+        _, idx = O.kmeanspp_draws(X, 1, [u0])
+        assert idx[0] == want
+    # round 2 after centres 0 and 3: squared distances [0, 1, 0], cumsum
+    # [0, 1, 1]: every u in (0, 1) picks point 2
+    for u2 in (1e-9, 0.5, 0.999):
+        _, idx = O.kmeanspp_draws(X, 3, [0.0, 0.5, u2])
+        assert idx.tolist() == [1, 3, 2]
+
+
+def test_kmeanspp_sampling_frequencies(orc):
+    # D^2 weighting as a distribution: with the first centre at 0 on points
+    # {0, 1, 3, 7}, squared distances [0, 1, 9, 49] -> P = [0, 1, 9, 49] / 59.
+    # 20,000 evenly spaced draws reproduce the CDF to within 1e-3.
+    X = np.array([[0.0], [1.0], [3.0], [7.0]], np.float32)
+    m = 20000
+    counts = np.zeros(5, np.int64)
+    for i in range(m):
+        _, idx = O.kmeanspp_draws(X, 2, [0.0, (i + 0.5) / m])
+        counts[idx[1]] += 1
+    p = counts[1:] / m
+    np.testing.assert_allclose(p, np.array([0, 1, 9, 49]) / 59.0, atol=1e-3)
+
+
+def test_kmeanspp_blocked_cumsum_matches_fp64(orc):
+    # Larger input: every pick of the seeded stream equals the first j whose
+    # FP64 cumulative sum of the running minimum squared distance reaches
+    # u_i * total, recomputed here in FP64 from the points themselves (the
+    # oracle's FP32 norms form differs from it by rounding only; points are
+    # integers so both are exact).
+    rng = np.random.default_rng(5)
+    X = rng.integers(-20, 21, size=(2500, 3)).astype(np.float32)
+    k = 12
+    u = O.uniform_stream(77, k)
+    _, idx = O.kmeanspp_draws(X, k, u)
+    X64 = X.astype(np.float64)
+    first = int(u[0] * len(X))
+    assert idx[0] == first + 1
+    mins = np.full(len(X), np.inf)
+    chosen = [first]
+    for i in range(1, k):
+        c = X64[chosen[-1]]
+        mins = np.minimum(mins, ((X64 - c) ** 2).sum(1))
+        cs = np.cumsum(mins)
+        j = int(np.searchsorted(cs, u[i] * cs[-1], side="left"))
+        j = min(j, len(X) - 1)
+        assert idx[i] == j + 1
+        chosen.append(j)
+    # the seeded entry point draws the same stream in the same order
+    _, idx_seed = orc.kmeanspp(X, k, seed=77)
+    assert (idx_seed == idx).all()
+
+
 # --- routing / partition / merge --------------------------------------------------------
 
 def test_topk_routing_known_answers(orc):
