@@ -135,9 +135,13 @@ int mann_ivf_build(mann_dataset_t X, const float* centroids, int64_t nlist,
  * (:158-167): the GPU takes the (dist, id) top-(k+1); a query whose k-th and
  * (k+1)-th distances tie replays its probed lists through the heap on the GPU,
  * so ties at the k-th place resolve as the reference's do.  k <= 255.
+ * centroid_metric (IVFFlatIndex's centroid_metric, ivf_flat.jl:103-131):
+ * MANN_EUCLIDEAN / MANN_SQEUCLIDEAN (the norms form, with and without the
+ * square root) or MANN_COSINE (CosineDist of centroid and query).  The point
+ * metric is the dataset's (a MANN_COSINE dataset serves CosineDist).
  * assign is 1-based and must lie in [1, nlist]. */
 int mann_ivf_flat_build(mann_dataset_t X, const float* centroids, int64_t nlist,
-                        const int64_t* assign, mann_ivf_t* out);
+                        const int64_t* assign, int centroid_metric, mann_ivf_t* out);
 int mann_ivf_destroy(mann_ivf_t ivf);
 /* number of buckets and (optional) the size of each */
 int mann_ivf_info(mann_ivf_t ivf, int64_t* n_buckets, int64_t* list_sizes);
@@ -160,6 +164,25 @@ int mann_ivf_probe(mann_ivf_t ivf, const float* route_dist, int64_t nq, int64_t 
 int mann_ivf_search(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq, int64_t ldq,
                     int64_t k, int64_t nprobe, int metric, const float* route_dist,
                     int64_t* out_ids, float* out_dist);
+
+/* The same with the index's merge strategy and query form (query.jl:25-39 for
+ * one query, :279-300 for a batch; merge.jl:79-169):
+ *  merge = MANN_MERGE_SIMPLE (SimpleMerge) or MANN_MERGE_DISJOINT
+ *  (DisjointMerge: each probed list's top-min(k, |list|) by (dist, id) pushed
+ *  through the bounded max-heap of neighbor_heaps.jl:84-100, ties at the k-th
+ *  place resolved as that heap resolves them, output by (dist, id));
+ *  probe_order = 1 concatenates the per-list results in probe order, as the
+ *  single-query path does (_query_recursive, query.jl:66-106): SimpleMerge then
+ *  orders ties by (dist, probe rank, id); 0 uses the batched child order
+ *  (dist, bucket, id).
+ * On an IVF-Flat handle both are ignored (its query is the bounded heap over
+ * every probed row in probe order).  k <= 255 when the merge is not the
+ * batched SimpleMerge. */
+#define MANN_MERGE_SIMPLE 0
+#define MANN_MERGE_DISJOINT 1
+int mann_ivf_search_ex(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq, int64_t ldq,
+                       int64_t k, int64_t nprobe, int metric, const float* route_dist, int merge,
+                       int probe_order, int64_t* out_ids, float* out_dist);
 
 /* Learned probe router -- no counterpart in the reference: an MLP
  * d -> hidden (ReLU) -> nlist whose scores rank the IVF lists of a query

@@ -35,7 +35,8 @@ SIGNATURES = [
     ("mann_kmeans_fit", ctypes.c_int, [_vp, _i64, _i64, ctypes.c_double, ctypes.c_int,
                                        ctypes.c_uint64, _vp, _vp, _vp, _i64p]),
     ("mann_ivf_build", ctypes.c_int, [_vp, _vp, _i64, _vp, ctypes.POINTER(_vp)]),
-    ("mann_ivf_flat_build", ctypes.c_int, [_vp, _vp, _i64, _vp, ctypes.POINTER(_vp)]),
+    ("mann_ivf_flat_build", ctypes.c_int, [_vp, _vp, _i64, _vp, ctypes.c_int,
+                                           ctypes.POINTER(_vp)]),
     ("mann_ivf_destroy", ctypes.c_int, [_vp]),
     ("mann_ivf_info", ctypes.c_int, [_vp, _i64p, _vp]),
     ("mann_ivf_list_ids", ctypes.c_int, [_vp, _vp]),
@@ -43,6 +44,8 @@ SIGNATURES = [
     ("mann_ivf_probe", ctypes.c_int, [_vp, _vp, _i64, _i64, _vp]),
     ("mann_ivf_search", ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, _i64, ctypes.c_int, _vp,
                                        _vp, _vp]),
+    ("mann_ivf_search_ex", ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, _i64, ctypes.c_int,
+                                          _vp, ctypes.c_int, ctypes.c_int, _vp, _vp]),
     ("mann_route_distances", ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _i64, _vp]),
     ("mann_router_create", ctypes.c_int, [_i64, _i64, _i64, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     ("mann_router_destroy", ctypes.c_int, [_vp]),

@@ -371,10 +371,11 @@ int mann_ivf_build(mann_dataset_t X, const float* centroids, int64_t nlist,
 }
 
 int mann_ivf_flat_build(mann_dataset_t X, const float* centroids, int64_t nlist,
-                        const int64_t* assign, mann_ivf_t* out) {
+                        const int64_t* assign, int centroid_metric, mann_ivf_t* out) {
   return guarded([&] {
     MANN_REQUIRE(X && centroids && assign && out, ERR_NULL, "null pointer");
     MANN_REQUIRE(nlist > 0, ERR_ARGUMENT, "nlist must be positive");
+    check_metric(centroid_metric);
     std::vector<int64_t> ha(X->n);
     if (is_device_ptr(assign)) {
       MANN_HIP(hipMemcpy(ha.data(), assign, sizeof(int64_t) * X->n, hipMemcpyDeviceToHost));
@@ -385,7 +386,7 @@ int mann_ivf_flat_build(mann_dataset_t X, const float* centroids, int64_t nlist,
       MANN_REQUIRE(ha[j] >= 1 && ha[j] <= nlist, ERR_ARGUMENT,
                    "Assignment out of bounds: " + std::to_string(ha[j]));
     float* C = stage_centroids(centroids, nlist, *X, "ivb_c");
-    *out = ivf_build(*X, C, (int)nlist, ha.data(), true);
+    *out = ivf_build(*X, C, (int)nlist, ha.data(), true, centroid_metric);
   });
 }
 
@@ -439,7 +440,7 @@ int mann_ivf_route(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq, 
 static void probe_out(mann_ivf_t ivf, const float* route_dist, int64_t nq, int64_t nprobe,
                       float mass, int64_t* out_buckets) {
   MANN_REQUIRE(ivf && route_dist && out_buckets, ERR_NULL, "null pointer");
-  MANN_REQUIRE(nprobe > 0, ERR_ARGUMENT, "k must be positive");
+  MANN_REQUIRE(nprobe > 0, ERR_ARGUMENT, "nprobe must be positive (TopKRouting: k must be positive)");
   if (nq == 0) return;
   ThreadCtx& c = ctx();
   hipStream_t st = c.stream;
@@ -471,15 +472,42 @@ int mann_ivf_probe(mann_ivf_t ivf, const float* route_dist, int64_t nq, int64_t 
   return guarded([&] { probe_out(ivf, route_dist, nq, nprobe, 0.0f, out_buckets); });
 }
 
+static void search_ex(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq, int64_t ldq,
+                      int64_t k, int64_t nprobe, int metric, const float* route_dist, int merge,
+                      int probe_order, int64_t* out_ids, float* out_dist);
+
 int mann_ivf_search(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq, int64_t ldq,
                     int64_t k, int64_t nprobe, int metric, const float* route_dist,
                     int64_t* out_ids, float* out_dist) {
   return guarded([&] {
+    search_ex(ivf, queries, d, nq, ldq, k, nprobe, metric, route_dist, MANN_MERGE_SIMPLE, 0,
+              out_ids, out_dist);
+  });
+}
+
+int mann_ivf_search_ex(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq, int64_t ldq,
+                       int64_t k, int64_t nprobe, int metric, const float* route_dist, int merge,
+                       int probe_order, int64_t* out_ids, float* out_dist) {
+  return guarded([&] {
+    MANN_REQUIRE(merge == MANN_MERGE_SIMPLE || merge == MANN_MERGE_DISJOINT, ERR_ARGUMENT,
+                 "unknown merge strategy");
+    search_ex(ivf, queries, d, nq, ldq, k, nprobe, metric, route_dist, merge, probe_order != 0,
+              out_ids, out_dist);
+  });
+}
+
+static void search_ex(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq, int64_t ldq,
+                      int64_t k, int64_t nprobe, int metric, const float* route_dist, int merge,
+                      int probe_order, int64_t* out_ids, float* out_dist) {
+  {
     MANN_REQUIRE(ivf != nullptr, ERR_NULL, "null index");
     check_metric(metric);
     MANN_REQUIRE((metric == COSINE) == (ivf->normalized != 0), ERR_ARGUMENT,
                  "index was built for another metric");
-    MANN_REQUIRE(nprobe > 0, ERR_ARGUMENT, "k must be positive");
+    // IVF-Flat clamps nprobe to [1, nlist] (ivf_flat.jl:146); TopKRouting
+    // rejects k <= 0 (routing.jl:39-42)
+    if (ivf->flat) nprobe = std::max<int64_t>(nprobe, 1);
+    MANN_REQUIRE(nprobe > 0, ERR_ARGUMENT, "nprobe must be positive (TopKRouting: k must be positive)");
     QueryOps Q = ivf_queries(ivf, queries, d, nq, ldq);
     if (nq == 0 || k <= 0) {
       last_stats() = SearchStats();
@@ -496,11 +524,12 @@ int mann_ivf_search(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq,
     }
     OutBuf<int64_t> oi(out_ids, (size_t)(nq * k), "out_ids");
     OutBuf<float> od(out_dist, (size_t)(nq * k), "out_dist");
-    ivf_search(*ivf, Q, (int)k, (int)nprobe, metric, R, oi.dev, od.dev);
+    ivf_search(*ivf, Q, (int)k, (int)nprobe, metric, R, oi.dev, od.dev, 0.0f, merge,
+               probe_order);
     oi.finish();
     od.finish();
     timer_flush();
-  });
+  }
 }
 
 int mann_route_distances(const float* centroids, int64_t nlist, int64_t d, const float* queries,
@@ -667,7 +696,8 @@ static void search_routed(mann_ivf_t ivf, mann_router_t r, const float* queries,
   check_metric(metric);
   MANN_REQUIRE((metric == COSINE) == (ivf->normalized != 0), ERR_ARGUMENT,
                "index was built for another metric");
-  MANN_REQUIRE(nprobe > 0, ERR_ARGUMENT, "k must be positive");
+  if (ivf->flat) nprobe = std::max<int64_t>(nprobe, 1);
+  MANN_REQUIRE(nprobe > 0, ERR_ARGUMENT, "nprobe must be positive (TopKRouting: k must be positive)");
   QueryOps Q = ivf_queries(ivf, queries, d, nq, ldq);
   if (nq == 0 || k <= 0) {
     last_stats() = SearchStats();

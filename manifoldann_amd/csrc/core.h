@@ -42,10 +42,18 @@ struct mann_ivf {
   int nb = 0;      // buckets = max assignment (partition_by_transform)
   int flat = 0;    // IVF-Flat: nlist lists, bounded-heap merge (ivf_flat.jl:138-167)
   int normalized = 0;
+  // routing metric of the centroids: EUCLIDEAN (KMeansTransform's norms form,
+  // kmeans/distance.jl:213-242), or for IVF-Flat its centroid_metric:
+  // SQEUCLIDEAN (no square root) or COSINE (ivf_flat.jl:103-131)
+  int route_metric = 0;
   float* c32 = nullptr;
   float* cn2 = nullptr;
+  // residual centres of the bf16 rows: the centroids, or for cosine (unit
+  // rows) the unit centroids c / |c|
+  float* cres = nullptr;
   int* list_off = nullptr;  // nb + 1
   int* perm = nullptr;      // reordered row -> point id (0-based)
+  int* bucket_of = nullptr; // point id (0-based) -> bucket
   int* lorder = nullptr;    // lists by size, largest first
   std::vector<int> h_off;
   float* x32 = nullptr;
@@ -55,7 +63,8 @@ struct mann_ivf {
   float ex = 0.0f, rx = 0.0f;
   ~mann_ivf() {
     (void)hipSetDevice(device);
-    void* ps[] = {c32, cn2, list_off, perm, lorder, x32, xb, xn, xn2};
+    void* ps[] = {c32, cn2, cres != c32 ? cres : nullptr, list_off, perm, bucket_of, lorder,
+                  x32, xb, xn, xn2};
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
@@ -149,13 +158,17 @@ std::vector<int> random_indices(int64_t n, int k, uint64_t seed);
 void knn_graph_symmetrize_gpu(const int64_t* adj, int64_t n, int64_t k, int64_t* offsets,
                               int64_t* ids, int64_t* total, int64_t* max_degree);
 mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int64_t* assign_host,
-                    bool all_lists = false);
+                    bool all_lists = false, int route_metric = 0);
 void ivf_route(const mann_ivf& iv, const QueryOps& Q, float* R);
 // mass > 0: probability-mass probing (R = negated logits), at most nprobe lists
 void ivf_probe(const mann_ivf& iv, const float* R, int nq, int nprobe, int* probes, int* pcount,
                int* prows, float mass = 0.0f);
+// merge: 0 SimpleMerge, 1 DisjointMerge; probe_order: per-list results in
+// probe order (the single-query path) instead of bucket order (the batch).
+// An IVF-Flat handle always merges through its bounded heap in probe order.
 void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int metric,
-                const float* route_in, int64_t* out_ids, float* out_dist, float mass = 0.0f);
+                const float* route_in, int64_t* out_ids, float* out_dist, float mass = 0.0f,
+                int merge = 0, int probe_order = 0);
 
 // learned router (core_router.hip)
 mann_router* router_create(int d, int h, int c, uint64_t seed);

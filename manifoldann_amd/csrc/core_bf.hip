@@ -271,8 +271,10 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   const int k_req = k + (exclude_self ? 1 : 0);
   const int k_eff = std::min(k_req, n);
   const int cap_total = select_cap_max();
-  MANN_REQUIRE(k_req <= select_exmax() / 2, ERR_UNSUPPORTED,
-               "k too large for the GPU selection buffer (max 256)");
+  MANN_REQUIRE(k_req <= select_exmax() / 2, ERR_ARGUMENT,
+               "k = " + std::to_string(k) + " exceeds the GPU search's maximum of " +
+                   std::to_string(select_exmax() / 2 - (exclude_self ? 1 : 0)) +
+                   " neighbours per query");
   const int cfg = tuning().scan_cfg;
   const int BM = scan_tile_rows(cfg), BN = scan_tile_queries(cfg);
   // row partitions: enough workgroups to fill the chip, at most 32 regions

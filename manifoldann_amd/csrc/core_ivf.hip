@@ -21,47 +21,127 @@
 namespace mann {
 
 
+// The probe lists a search used (for the merges that replay them).
+struct ProbeState {
+  const int* probes = nullptr;  // [nq][nprobe]
+  const int* pcount = nullptr;
+  const int* prows = nullptr;
+  int nprobe = 0;
+};
+
 static void ivf_search_sel(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int metric,
-                           const float* route_in, int64_t* out_ids, float* out_dist, float mass);
+                           const float* route_in, int64_t* out_ids, float* out_dist, float mass,
+                           ProbeState* ps = nullptr);
 
 void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int metric,
-                const float* route_in, int64_t* out_ids, float* out_dist, float mass) {
-  if (!iv.flat || Q.nq == 0 || k <= 0) {
+                const float* route_in, int64_t* out_ids, float* out_dist, float mass, int merge,
+                int probe_order) {
+  // TopKRouting takes min(nprobe, available buckets) lists (routing.jl:108-112):
+  // a larger nprobe changes nothing but the buffers sized by it
+  if (mass <= 0.0f) nprobe = std::max(1, std::min(nprobe, std::min(iv.nb, iv.nlist)));
+  const int kind = iv.flat ? MERGE_HEAP_ROWS
+                           : (merge == 1 ? MERGE_HEAP_LISTS
+                                         : (probe_order ? MERGE_PROBE : MERGE_BATCHED));
+  if (kind == MERGE_BATCHED || Q.nq == 0 || k <= 0) {
     ivf_search_sel(iv, Q, k, nprobe, metric, route_in, out_ids, out_dist, mass);
     return;
   }
-  // IVF-Flat (ivf_flat.jl:138-167): the (dist, id) top-(k+1), then the heap
-  // merge for the queries with a distance tie at the k-th place
+  // the (dist, bucket, id) top-(k+1), then the kind's merge: untied queries
+  // re-sorted, queries tied at the k-th place replayed (kernels_merge.hip)
   ThreadCtx& c = ctx();
   hipStream_t st = c.stream;
-  MANN_REQUIRE(k + 1 <= select_exmax() / 2, ERR_UNSUPPORTED,
-               "k too large for the IVF-Flat merge (max 255)");
+  MANN_REQUIRE(k + 1 <= select_exmax() / 2, ERR_ARGUMENT,
+               "k = " + std::to_string(k) + " exceeds this merge's maximum of " +
+                   std::to_string(select_exmax() / 2 - 1) + " neighbours per query");
   const int nq = (int)Q.nq;
-  const float* R = route_in;
-  if (!R) {
-    float* Rb = c.buf("ivf_fR").as<float>((size_t)nq * iv.nlist);
-    ivf_route(iv, Q, Rb);
-    R = Rb;
-  }
-  int64_t* ti = c.buf("ivf_fti").as<int64_t>((size_t)nq * (k + 1));
-  float* td = c.buf("ivf_ftd").as<float>((size_t)nq * (k + 1));
-  ivf_search_sel(iv, Q, k + 1, nprobe, metric, R, ti, td, mass);
+  int64_t* ti = c.buf("ivm_ti").as<int64_t>((size_t)nq * (k + 1));
+  float* td = c.buf("ivm_td").as<float>((size_t)nq * (k + 1));
+  ProbeState P;
+  ivf_search_sel(iv, Q, k + 1, nprobe, metric, route_in, ti, td, mass, &P);
   SearchStats keep = last_stats();
-  int* nrep = c.buf("ivf_frep").as<int>(1);
-  MANN_HIP(hipMemsetAsync(nrep, 0, sizeof(int), st));
-  ScopedTime tm("ivf_flat_merge", st);
-  launch_ivf_flat_merge(ti, td, nq, k, R, iv.nlist, std::min(nprobe, iv.nlist), iv.list_off,
-                        iv.perm, iv.x32, iv.xn2, Q.q32, Q.qn2, (int)iv.d16, metric, out_ids,
-                        out_dist, nrep, st);
-  int h = 0;
-  MANN_HIP(hipMemcpyAsync(&h, nrep, sizeof(int), hipMemcpyDeviceToHost, st));
+  int* tied = c.buf("ivm_tied").as<int>(nq);
+  int* ntied = c.buf("ivm_ntied").as<int>(1);
+  fill_i32(ntied, 1, 0, st);
+  MergeArgs m{};
+  m.ti = ti;
+  m.td = td;
+  m.nq = nq;
+  m.k = k;
+  m.kind = kind;
+  m.probes = P.probes;
+  m.pcount = P.pcount;
+  m.nprobe = P.nprobe;
+  m.bucket_of = iv.bucket_of;
+  m.perm = iv.perm;
+  m.list_off = iv.list_off;
+  m.out_ids = out_ids;
+  m.out_dist = out_dist;
+  m.tied_list = tied;
+  m.tied_count = ntied;
+  {
+    ScopedTime tm("ivf_merge", st);
+    launch_merge_fix(m, st);
+  }
+  int nt = 0;
+  MANN_HIP(hipMemcpyAsync(&nt, ntied, sizeof(int), hipMemcpyDeviceToHost, st));
   MANN_HIP(hipStreamSynchronize(st));
-  keep.heap_replays = h;
+  if (nt > 0) {
+    // replay the tied queries in batches whose probed rows fit the distance
+    // buffer (each tied query's rows are computed once, in replay order)
+    std::vector<int> ht(nt), hrows(nq);
+    MANN_HIP(hipMemcpyAsync(ht.data(), tied, sizeof(int) * nt, hipMemcpyDeviceToHost, st));
+    MANN_HIP(hipMemcpyAsync(hrows.data(), P.prows, sizeof(int) * nq, hipMemcpyDeviceToHost, st));
+    MANN_HIP(hipStreamSynchronize(st));
+    const int64_t budget = (int64_t)1 << 24;  // distances per batch (64 MiB)
+    int* seq = c.buf("ivm_seq").as<int>((size_t)nt * P.nprobe);
+    int* seqoff = c.buf("ivm_seqoff").as<int>((size_t)nt * (P.nprobe + 1));
+    int64_t* dbase = c.buf("ivm_dbase").as<int64_t>(nt);
+    std::vector<int64_t> hb(nt);
+    ReplayArgs r{};
+    r.tied = tied;
+    r.nprobe = P.nprobe;
+    r.bucket_order = kind == MERGE_HEAP_LISTS && !probe_order;
+    r.probes = P.probes;
+    r.pcount = P.pcount;
+    r.list_off = iv.list_off;
+    r.x32 = iv.x32;
+    r.q32 = Q.q32;
+    r.xn2 = iv.xn2;
+    r.qn2 = Q.qn2;
+    r.d16 = (int)iv.d16;
+    r.metric = metric;
+    ScopedTime tm("ivf_merge_replay", st);
+    for (int s0 = 0; s0 < nt;) {
+      // slots [s0, s1): at least one, more while their rows fit the budget
+      int64_t tot = 0;
+      int s1 = s0, maxr = 0;
+      while (s1 < nt && (s1 == s0 || tot + hrows[ht[s1]] <= budget)) {
+        hb[s1 - s0] = tot;
+        tot += hrows[ht[s1]];
+        maxr = std::max(maxr, hrows[ht[s1]]);
+        ++s1;
+      }
+      float* dist = c.buf("ivm_dist").as<float>((size_t)std::max<int64_t>(tot, 1));
+      MANN_HIP(hipMemcpyAsync(dbase, hb.data(), sizeof(int64_t) * (s1 - s0),
+                              hipMemcpyHostToDevice, st));
+      r.slot0 = s0;
+      r.seq = seq;
+      r.seqoff = seqoff;
+      r.dist = dist;
+      r.dbase = dbase;
+      launch_replay(r, m, s1 - s0, maxr, st);
+      // dbase (a host copy source) and dist are reused by the next batch
+      MANN_HIP(hipStreamSynchronize(st));
+      s0 = s1;
+    }
+  }
+  MANN_HIP(hipGetLastError());
+  keep.heap_replays = nt;
   last_stats() = keep;
 }
 
 mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int64_t* assign_host,
-                    bool all_lists) {
+                    bool all_lists, int route_metric) {
   ThreadCtx& c = ctx();
   hipStream_t st = c.stream;
   const int64_t n = ds.n;
@@ -88,6 +168,7 @@ mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int
     iv->nb = nb;
     iv->flat = flat;
     iv->normalized = ds.normalized;
+    iv->route_metric = route_metric;
     // stable counting sort of the assignment (transforms/utils.jl:142-174)
     std::vector<int> a32(n);
     for (int64_t j = 0; j < n; ++j) a32[j] = (int)assign_host[j] - 1;
@@ -98,6 +179,8 @@ mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int
     MANN_HIP(hipMalloc(&iv->list_off, sizeof(int) * (nb + 1)));
     MANN_HIP(hipMalloc(&iv->perm, sizeof(int) * n));
     launch_counting_sort(da, n, nb, table, iv->list_off, iv->perm, st);
+    MANN_HIP(hipMalloc(&iv->bucket_of, sizeof(int) * n));
+    MANN_HIP(hipMemcpyAsync(iv->bucket_of, da, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
     iv->h_off.resize(nb + 1);
     MANN_HIP(hipMemcpyAsync(iv->h_off.data(), iv->list_off, sizeof(int) * (nb + 1),
                             hipMemcpyDeviceToHost, st));
@@ -123,6 +206,15 @@ mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int
     MANN_HIP(hipMemcpyAsync(iv->c32, C, sizeof(float) * (size_t)nlist * iv->d16,
                             hipMemcpyDeviceToDevice, st));
     launch_a16_norms(iv->c32, nlist, (int)iv->d16, iv->cn2, st);
+    // residual centres: unit rows residualise against the unit centroids
+    // (any centre serves, distances being translation invariant; the nearer
+    // one keeps the residuals, and so the certification band, small)
+    if (iv->normalized) {
+      MANN_HIP(hipMalloc(&iv->cres, sizeof(float) * (size_t)nc * iv->d16));
+      unit_rows(iv->c32, iv->cres, iv->d16, (int)iv->d, nc, st);
+    } else {
+      iv->cres = iv->c32;
+    }
     // list-contiguous copy of the base; bf16 residuals against the list centre
     MANN_HIP(hipMalloc(&iv->x32, sizeof(float) * iv->n_pad * iv->d16));
     MANN_HIP(hipMalloc(&iv->xb, sizeof(bf16_t) * iv->n_pad * iv->kp));
@@ -137,7 +229,7 @@ mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int
     MANN_HIP(hipMemsetAsync(mx, 0, sizeof(unsigned) * 2, st));
     launch_prep_rows(iv->x32, iv->d16, n, iv->n_pad, (int)iv->d, (int)iv->d16, (int)iv->kp,
                      iv->normalized, iv->x32, iv->xb, iv->xn, nullptr, nullptr, mx, mx + 1, st,
-                     iv->c32, rowb, iv->d16);
+                     iv->cres, rowb, iv->d16);
     unsigned hm[2];
     MANN_HIP(hipMemcpyAsync(hm, mx, sizeof(hm), hipMemcpyDeviceToHost, st));
     MANN_HIP(hipStreamSynchronize(st));
@@ -156,7 +248,8 @@ mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int
 void ivf_route(const mann_ivf& iv, const QueryOps& Q, float* R) {
   hipStream_t st = ctx().stream;
   ScopedTime tm("ivf_route", st);
-  launch_route_dist(Q.q32, Q.qn2, (int)Q.nq, iv.c32, iv.cn2, iv.nlist, (int)iv.d16, R, st);
+  launch_route_dist(Q.q32, Q.qn2, (int)Q.nq, iv.c32, iv.cn2, iv.nlist, (int)iv.d16, R, st,
+                    iv.route_metric);
 }
 
 // Probe lists of each query: probes [nq][nprobe] (buckets, 0-based, -1 pad).
@@ -175,7 +268,8 @@ void ivf_probe(const mann_ivf& iv, const float* R, int nq, int nprobe, int* prob
 }
 
 static void ivf_search_sel(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int metric,
-                           const float* route_in, int64_t* out_ids, float* out_dist, float mass) {
+                           const float* route_in, int64_t* out_ids, float* out_dist, float mass,
+                           ProbeState* ps) {
   ThreadCtx& c = ctx();
   hipStream_t st = c.stream;
   SearchStats& S = last_stats();
@@ -183,8 +277,9 @@ static void ivf_search_sel(const mann_ivf& iv, const QueryOps& Q, int k, int npr
   const int nq = (int)Q.nq;
   if (nq == 0 || k <= 0) return;
   const int cap = select_cap_max();
-  MANN_REQUIRE(k <= select_exmax() / 2, ERR_UNSUPPORTED,
-               "k too large for the GPU selection buffer (max 256)");
+  MANN_REQUIRE(k <= select_exmax() / 2, ERR_ARGUMENT,
+               "k = " + std::to_string(k) + " exceeds the IVF search's maximum of " +
+                   std::to_string(select_exmax() / 2) + " neighbours per query");
   const int cfg = tuning().ivf_cfg;
   const int BN = scan_tile_queries(cfg);
   const int ROWS_ITEM = tuning().ivf_rows_item;  // rows per work item
@@ -198,6 +293,12 @@ static void ivf_search_sel(const mann_ivf& iv, const QueryOps& Q, int k, int npr
   int* pcount = c.buf("iv_pcount").as<int>(nq);
   int* prows = c.buf("iv_prows").as<int>(nq);
   ivf_probe(iv, R, nq, nprobe, probes, pcount, prows, mass);
+  if (ps) {
+    ps->probes = probes;
+    ps->pcount = pcount;
+    ps->prows = prows;
+    ps->nprobe = nprobe;
+  }
 
   // invert probe lists into work items
   const int nb = iv.nb;
@@ -243,7 +344,7 @@ static void ivf_search_sel(const mann_ivf& iv, const QueryOps& Q, int k, int npr
   fill_i32((int*)qem, (int64_t)nq * 2, 0, st);
   {
     ScopedTime tm("ivf_residual", st);
-    launch_residual_queries(Q.q32, iv.c32, (int)iv.d, (int)iv.d16, (int)iv.kp, iv.normalized,
+    launch_residual_queries(Q.q32, iv.cres, (int)iv.d, (int)iv.d16, (int)iv.kp, iv.normalized,
                             qlist, qlist_list,
                             ctr + 4, max_entries, qres, rnorm, qem, qem + nq, st);
   }

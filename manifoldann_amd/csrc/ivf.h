@@ -33,8 +33,10 @@ struct IvfTauArgs {
 
 int route_max();
 int ivf_tau_rows_max();
+// metric: EUCLIDEAN (norms form, the default), SQEUCLIDEAN or COSINE
 void launch_route_dist(const float* q32, const float* qn2, int nq, const float* c32,
-                       const float* cn2, int nlist, int d16, float* R, hipStream_t st);
+                       const float* cn2, int nlist, int d16, float* R, hipStream_t st,
+                       int metric = EUCLIDEAN);
 // mass > 0: probability-mass probing over negated logits (nprobe <= 64,
 // avail <= 1024), see kernels_ivf.hip
 void launch_probe_select(const float* R, int nq, int nlist, int avail, int nprobe, float mass,
@@ -83,14 +85,52 @@ void launch_ivf_tau(const IvfTauArgs& a, hipStream_t st);
 void launch_probe_ranges(const int* probes, const int* pcount, int nq, int nprobe,
                          const int* list_off, int2* ranges, hipStream_t st);
 
-// IVF-Flat merge from a (dist, id) top-(k+1) per query (ti, td: nq x (k+1)):
-// where the k-th and (k+1)-th distances differ the first k are the heap's
-// result; otherwise the query's probed lists are replayed through the
-// bounded max-heap of ivf_flat.jl:158-167.  Writes out (nq x k); k <= 255.
-void launch_ivf_flat_merge(const int64_t* ti, const float* td, int nq, int k, const float* R,
-                           int nlist, int nprobe, const int* list_off, const int* perm,
-                           const float* x32, const float* xn2, const float* q32,
-                           const float* qn2, int d16, int metric, int64_t* out_ids,
-                           float* out_dist, int* n_replayed, hipStream_t st);
+// Merge orders other than the batched SimpleMerge order (kernels_merge.hip).
+enum MergeKind : int {
+  MERGE_BATCHED = 0,     // (dist, bucket, id): the selection's own order
+  MERGE_HEAP_ROWS = 1,   // IVF-Flat bounded heap over every probed row, probe order
+  MERGE_HEAP_LISTS = 2,  // DisjointMerge: heap over each list's top-k
+  MERGE_PROBE = 3        // single-query SimpleMerge: (dist, probe rank, id)
+};
+constexpr int ROUTE_MAX_MERGE = 4096;  // probe lists a replay plan can order
+struct MergeArgs {
+  const int64_t* ti;      // [nq][k + 1] the search's top-(k + 1), 1-based ids
+  const float* td;
+  int nq, k, kind;
+  const int* probes;      // [nq][nprobe] the search's probed lists (probe order)
+  const int* pcount;
+  int nprobe;
+  const int* bucket_of;   // point id (0-based) -> its bucket
+  const int* perm;        // reordered row -> point id
+  const int* list_off;
+  int64_t* out_ids;       // [nq][k]
+  float* out_dist;
+  int* tied_list;         // out: queries whose k-th place is tied
+  int* tied_count;
+};
+struct ReplayArgs {
+  const int* tied;        // tied queries; this batch is tied[slot0 ..]
+  int slot0;
+  int nprobe;
+  int bucket_order;       // replay lists in bucket order (else probe order)
+  const int* probes;
+  const int* pcount;
+  const int* list_off;
+  int* seq;               // [slot][nprobe] replay order of the lists
+  int* seqoff;            // [slot][nprobe + 1] row offsets in that order
+  float* dist;            // exact distances of the probed rows in replay order
+  const int64_t* dbase;   // [slot] start of its rows in dist
+  const float* x32;
+  const float* q32;
+  const float* xn2;
+  const float* qn2;
+  int d16, metric;
+};
+// Untied queries are written (their k rows re-sorted by the kind's order);
+// tied ones are appended to tied_list.
+void launch_merge_fix(const MergeArgs& a, hipStream_t st);
+// nslots tied queries from a.slot0; max_rows = most probed rows of any of them
+void launch_replay(const ReplayArgs& a, const MergeArgs& m, int nslots, int max_rows,
+                   hipStream_t st);
 
 }  // namespace mann

@@ -29,7 +29,8 @@ __global__ __launch_bounds__(256) void route_dist_kernel(const float* __restrict
                                                          const float* __restrict__ qn2, int nq,
                                                          const float* __restrict__ c32,
                                                          const float* __restrict__ cn2, int nlist,
-                                                         int d16, float* __restrict__ R) {
+                                                         int d16, int metric,
+                                                         float* __restrict__ R) {
   __shared__ float ys[2][RT_T][RT_LD];
   __shared__ float xs[2][RT_T][RT_LD];
   // consecutive workgroups share a query tile (the centroids stay L2-resident)
@@ -100,9 +101,18 @@ __global__ __launch_bounds__(256) void route_dist_kernel(const float* __restrict
       const float dot = group16_tree(acc[a][b]);
       const int c = c0 + cb + 4 * b;
       if (j == ((a * 8 + b) & 15) && q < nq && c < nlist) {
-        float tt = cnl[b] + qn;
-        float sq = tt - 2.0f * dot;
-        R[(int64_t)q * nlist + c] = __builtin_sqrtf(sq > 0.0f ? sq : 0.0f);
+        float v;
+        if (metric == COSINE) {
+          // CosineDist of centroid and query: max(1 - c.q / (|c| |q|), 0)
+          v = 1.0f - dot / (__builtin_sqrtf(cnl[b]) * __builtin_sqrtf(qn));
+          v = v > 0.0f ? v : 0.0f;
+        } else {
+          float tt = cnl[b] + qn;
+          float sq = tt - 2.0f * dot;
+          sq = sq > 0.0f ? sq : 0.0f;
+          v = metric == SQEUCLIDEAN ? sq : __builtin_sqrtf(sq);
+        }
+        R[(int64_t)q * nlist + c] = v;
       }
     }
   }
@@ -767,11 +777,12 @@ __global__ void probe_ranges_kernel(const int* __restrict__ probes, const int* _
 int route_max() { return ROUTE_MAX; }
 
 void launch_route_dist(const float* q32, const float* qn2, int nq, const float* c32,
-                       const float* cn2, int nlist, int d16, float* R, hipStream_t st) {
+                       const float* cn2, int nlist, int d16, float* R, hipStream_t st,
+                       int metric) {
   if (nq <= 0 || nlist <= 0) return;
   dim3 grid((unsigned)ceil_div(nlist, RT_T), (unsigned)ceil_div(nq, RT_T));
   hipLaunchKernelGGL(route_dist_kernel, grid, dim3(256), 0, st, q32, qn2, nq, c32, cn2, nlist, d16,
-                     R);
+                     metric, R);
 }
 
 void launch_probe_select(const float* R, int nq, int nlist, int avail, int nprobe, float mass,
@@ -873,205 +884,6 @@ void launch_probe_ranges(const int* probes, const int* pcount, int nq, int nprob
   if (tot <= 0) return;
   hipLaunchKernelGGL(probe_ranges_kernel, dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0, st,
                      probes, pcount, nq, nprobe, list_off, ranges);
-}
-
-// Exact FP32 metric of reordered row r against query q on one lane: the same
-// A16 order as exact_metric (element i into accumulator i & 15, increasing i;
-// then the pairwise tree +8, +4, +2, +1 that group16_tree forms across lanes).
-__device__ __forceinline__ float exact_metric_lane(const float* __restrict__ x32,
-                                                   const float* __restrict__ q32,
-                                                   const float* __restrict__ xn2,
-                                                   const float* __restrict__ qn2, int d16,
-                                                   int metric, int64_t r, int64_t q) {
-  const float4* x = (const float4*)(x32 + r * d16);
-  const float4* y = (const float4*)(q32 + q * d16);
-  const bool diff = metric == EUCLIDEAN || metric == SQEUCLIDEAN;
-  float a[16];
-#pragma unroll
-  for (int e = 0; e < 16; ++e) a[e] = 0.0f;
-  for (int t = 0; t < d16 / 4; t += 4) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const float4 xv = x[t + u], yv = y[t + u];
-      const float xs[4] = {xv.x, xv.y, xv.z, xv.w}, ys[4] = {yv.x, yv.y, yv.z, yv.w};
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int i = 4 * u + e;
-        if (diff) {
-          const float df = xs[e] - ys[e];
-          a[i] = __builtin_fmaf(df, df, a[i]);
-        } else {
-          a[i] = __builtin_fmaf(xs[e], ys[e], a[i]);
-        }
-      }
-    }
-  }
-#pragma unroll
-  for (int sft = 8; sft > 0; sft >>= 1)
-#pragma unroll
-    for (int e = 0; e < sft; ++e) a[e] = a[e] + a[e + sft];
-  return finish_metric(a[0], metric, diff ? 0.0f : xn2[r], diff ? 0.0f : qn2[q]);
-}
-
-// One wave per query.  Heap operations follow BoundedMaxHeap push!
-// (neighbor_heaps.jl:84-100, sift-up :156-168, sift-down :170-190) and run on
-// lane 0; each lane computes one exact distance, 64 rows at a time.
-__global__ __launch_bounds__(64) void ivf_flat_merge_kernel(
-    const int64_t* __restrict__ ti, const float* __restrict__ td, int k,
-    const float* __restrict__ R, int nlist, int nprobe, const int* __restrict__ list_off,
-    const int* __restrict__ perm, const float* __restrict__ x32, const float* __restrict__ xn2,
-    const float* __restrict__ q32, const float* __restrict__ qn2, int d16, int metric,
-    int64_t* __restrict__ out_ids, float* __restrict__ out_dist, int* n_replayed) {
-  __shared__ float hd[256];
-  __shared__ int hid[256];
-  __shared__ float cd[64];
-  __shared__ int cid[64];
-  __shared__ int hlen;
-  __shared__ int hfull;
-  __shared__ float hroot;
-  const int q = blockIdx.x, lane = threadIdx.x;
-  const int64_t* ri = ti + (int64_t)q * (k + 1);
-  const float* rd = td + (int64_t)q * (k + 1);
-  int64_t* oi = out_ids + (int64_t)q * k;
-  float* od = out_dist + (int64_t)q * k;
-  const int64_t id_next = ri[k];
-  const float d_next = rd[k], d_last = rd[k - 1];
-  int len = 0;  // heap size, meaningful on lane 0
-  const bool replay = id_next != 0 && d_next == d_last;
-  if (!replay) {
-    // the heap keeps exactly these k; only the order among equal distances
-    // changes (by id below, where the list scan ordered them by list)
-    for (int r = lane; r < k; r += 64) {
-      int64_t v = ri[r];
-      float w = rd[r];
-      hid[r] = (int)v;
-      hd[r] = w;
-    }
-    __syncthreads();
-    if (lane == 0)
-      while (len < k && hid[len] != 0) ++len;
-  } else {
-    if (lane == 0) {
-      atomicAdd(n_replayed, 1);
-      hfull = 0;
-      hroot = INFINITY;
-    }
-    __syncthreads();
-    // a (k+1)-th candidate exists, so the heap's capacity k_eff is k (:156)
-    float pd = -INFINITY;
-    int pi = -1;
-    for (int s = 0; s < nprobe; ++s) {
-      // next probed list: smallest (distance, centroid) after (pd, pi) -- the
-      // order of partialsortperm (:150)
-      float bd = INFINITY;
-      int bi = 0x7fffffff;
-      for (int c = lane; c < nlist; c += 64) {
-        float v = R[(int64_t)q * nlist + c];
-        bool after = v > pd || (v == pd && c > pi);
-        if (after && (v < bd || (v == bd && c < bi))) {
-          bd = v;
-          bi = c;
-        }
-      }
-      for (int o = 32; o > 0; o >>= 1) {
-        float ov = __shfl_xor(bd, o);
-        int oc = __shfl_xor(bi, o);
-        if (ov < bd || (ov == bd && oc < bi)) {
-          bd = ov;
-          bi = oc;
-        }
-      }
-      if (bi == 0x7fffffff) break;
-      pd = bd;
-      pi = bi;
-      const int r0 = list_off[bi], r1 = list_off[bi + 1];
-      for (int r = r0; r < r1; r += 64) {
-        const int rr = min(r + lane, r1 - 1);
-        float dv = exact_metric_lane(x32, q32, xn2, qn2, d16, metric, rr, q);
-        int idv = perm[rr] + 1;
-        cd[lane] = dv;
-        cid[lane] = idv;
-        // the root only decreases, so a row at or above it now never enters
-        const bool maybe = !(hfull && dv >= hroot) && r + lane < r1;
-        const unsigned long long cand = __builtin_amdgcn_ballot_w64(maybe);
-        __syncthreads();
-        if (lane == 0) {
-          for (unsigned long long mk = cand; mk != 0; mk &= mk - 1) {
-            const int e = __builtin_ctzll(mk);
-            const float dist = cd[e];
-            const int id = cid[e];
-            if (len < k) {
-              int i = ++len;  // 1-based sift-up
-              hd[i - 1] = dist;
-              hid[i - 1] = id;
-              while (i > 1) {
-                int p = i >> 1;
-                if (hd[i - 1] > hd[p - 1]) {
-                  float tdv = hd[i - 1]; hd[i - 1] = hd[p - 1]; hd[p - 1] = tdv;
-                  int tiv = hid[i - 1]; hid[i - 1] = hid[p - 1]; hid[p - 1] = tiv;
-                  i = p;
-                } else {
-                  break;
-                }
-              }
-            } else if (dist < hd[0]) {
-              hd[0] = dist;
-              hid[0] = id;
-              int i = 1;
-              for (;;) {
-                int l = i << 1, rgt = l + 1, big = i;
-                if (l <= len && hd[l - 1] > hd[big - 1]) big = l;
-                if (rgt <= len && hd[rgt - 1] > hd[big - 1]) big = rgt;
-                if (big == i) break;
-                float tdv = hd[i - 1]; hd[i - 1] = hd[big - 1]; hd[big - 1] = tdv;
-                int tiv = hid[i - 1]; hid[i - 1] = hid[big - 1]; hid[big - 1] = tiv;
-                i = big;
-              }
-            }
-          }
-          hfull = len == k;
-          hroot = hd[0];
-        }
-        __syncthreads();
-      }
-    }
-  }  // replay
-  if (lane == 0) {
-    // to_sorted_vector (:131-134): insertion sort by (dist, id)
-    for (int a = 1; a < len; ++a) {
-      float dv = hd[a];
-      int iv = hid[a];
-      int b = a - 1;
-      while (b >= 0 && (hd[b] > dv || (hd[b] == dv && hid[b] > iv))) {
-        hd[b + 1] = hd[b];
-        hid[b + 1] = hid[b];
-        --b;
-      }
-      hd[b + 1] = dv;
-      hid[b + 1] = iv;
-    }
-    hlen = len;
-  }
-  __syncthreads();
-  const int n_kept = hlen;
-  for (int r = lane; r < k; r += 64) {
-    const bool have = r < n_kept;
-    float dv = hd[r < 256 ? r : 0];
-    int iv = hid[r < 256 ? r : 0];
-    oi[r] = have ? (int64_t)iv : 0;
-    od[r] = have ? dv : INFINITY;
-  }
-}
-
-void launch_ivf_flat_merge(const int64_t* ti, const float* td, int nq, int k, const float* R,
-                           int nlist, int nprobe, const int* list_off, const int* perm,
-                           const float* x32, const float* xn2, const float* q32,
-                           const float* qn2, int d16, int metric, int64_t* out_ids,
-                           float* out_dist, int* n_replayed, hipStream_t st) {
-  if (nq <= 0 || k <= 0) return;
-  ivf_flat_merge_kernel<<<nq, 64, 0, st>>>(ti, td, k, R, nlist, nprobe, list_off, perm, x32, xn2,
-                                           q32, qn2, d16, metric, out_ids, out_dist, n_replayed);
-  MANN_HIP(hipGetLastError());
 }
 
 }  // namespace mann

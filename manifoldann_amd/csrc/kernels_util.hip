@@ -61,6 +61,19 @@ __global__ void copy_rows_kernel(const float* src, int64_t lds, float* dst, int6
 }
 
 // dst[r][c] = float(src[r][c]) for c < w (exact: every binary16 is a binary32)
+// one wavefront per row: dst = src / |src| (FP64 norm), zero rows stay zero
+__global__ __launch_bounds__(256) void unit_rows_kernel(const float* src, float* dst, int64_t ld,
+                                                        int w, int64_t rows) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= rows) return;
+  double ss = 0.0;
+  for (int c = lane; c < w; c += 64) ss += (double)src[r * ld + c] * (double)src[r * ld + c];
+  for (int o = 32; o >= 1; o >>= 1) ss += __shfl_xor(ss, o);
+  const double sc = ss > 0.0 ? 1.0 / sqrt(ss) : 0.0;
+  for (int c = lane; c < ld; c += 64) dst[r * ld + c] = c < w ? (float)(src[r * ld + c] * sc) : 0.0f;
+}
+
 __global__ void f16_rows_to_f32_kernel(const _Float16* src, int64_t lds, float* dst, int64_t ldd,
                                        int w, int64_t rows) {
   const int64_t r = blockIdx.x;
@@ -123,6 +136,12 @@ void gather_f32(const float* src, const int* idx, int m, float* dst, hipStream_t
   if (m <= 0) return;
   hipLaunchKernelGGL(gather_f32_kernel, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, st, src,
                      idx, m, dst);
+}
+
+void unit_rows(const float* src, float* dst, int64_t ld, int w, int64_t rows, hipStream_t st) {
+  if (rows <= 0) return;
+  hipLaunchKernelGGL(unit_rows_kernel, dim3((unsigned)ceil_div(rows, 4)), dim3(256), 0, st, src,
+                     dst, ld, w, rows);
 }
 
 }  // namespace mann

@@ -24,6 +24,9 @@ void f16_rows_to_f32(const void* src, int64_t lds, float* dst, int64_t ldd, int 
 void copy_rows(const float* src, int64_t lds, float* dst, int64_t ldd, int w, int64_t rows,
                hipStream_t st);
 
+// dst row r = src row r / |src row r| (first w of ld columns; the rest zero)
+void unit_rows(const float* src, float* dst, int64_t ld, int w, int64_t rows, hipStream_t st);
+
 int select_cap_max();
 int select_exmax();   // exactly re-ranked candidates per query
 int exact_kmax();

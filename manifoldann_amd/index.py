@@ -271,6 +271,13 @@ def query(index, data, q, k, nprobe=None):
         # vector-of-vectors batch (ann_index.jl:110-120)
         q = np.stack([np.asarray(v, np.float32) for v in q], axis=1)
     single = (np.ndim(q) == 1) if not _is_torch(q) else q.dim() == 1
+    if hasattr(index, "root"):
+        # IVF: one vector takes the single-query path, whose per-list results
+        # merge in probe order (query.jl:25-106); a matrix the batched one
+        from .ivf import query_arrays as ivf_query_arrays
+        ids, dist = ivf_query_arrays(index, data, q, k, nprobe=nprobe, probe_order=single)
+        res = _to_neighbors(ids, dist)
+        return (res[0] if k > 0 else []) if single else res
     if single:
         d_q = q.shape[0] if _is_torch(q) else np.asarray(q).shape[0]
         _validate(index, data, d_q) if not hasattr(index, "root") else None

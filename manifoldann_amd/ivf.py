@@ -97,6 +97,13 @@ class KMeansTransform:
         self.init_centroids = None      # optional d x k injected initial centroids
 
 
+def _point_dataset(X, distance):
+    """The device dataset the list scan reads: unit bf16 rows for CosineDist,
+    plain rows for Euclidean / SqEuclidean."""
+    from .index import CosineDist
+    return _dataset_for(X, CosineDist() if isinstance(distance, CosineDist) else Euclidean())
+
+
 def _centroids_rows(C_dk):
     """d x k (Julia) -> k x d row-major float32 for the C ABI."""
     return np.ascontiguousarray(np.asarray(C_dk, np.float32).T)
@@ -342,7 +349,7 @@ def build_index(index_type, X, config=None, merge_strategy=None, distance=defaul
     t = copy.deepcopy(config.transform)     # builder.jl:103
     fit(t, X, rng=rng)
     assign = take_pending_assignments(t)
-    ds = _dataset_for(X, Euclidean())
+    ds = _point_dataset(X, distance)
     C = _centroids_rows(t.centroids)
     h = ctypes.c_void_p()
     check(lib().mann_ivf_build(ds.handle, C.ctypes.data, t.k, assign.ctypes.data, ctypes.byref(h)))
@@ -355,7 +362,7 @@ def build_index_from_assignment(X, centroids_dk, assign, routing, merge_strategy
     """The IVF layout from an existing k-means fit (centroids d x k, 1-based assignment)."""
     t = KMeansTransform(k=np.asarray(centroids_dk).shape[1])
     t.centroids = np.asfortranarray(np.asarray(centroids_dk, np.float32))
-    ds = _dataset_for(X, Euclidean())
+    ds = _point_dataset(X, distance)
     C = _centroids_rows(t.centroids)
     a = np.ascontiguousarray(assign, np.int64)
     h = ctypes.c_void_p()
@@ -421,8 +428,10 @@ def build_ivf_flat(X, nlist, nprobe=10, distance=default_distance, centroid_metr
     if nlist <= 0:
         raise ArgumentError("nlist must be positive")
     centroid_metric = Euclidean() if centroid_metric is None else centroid_metric
-    if type(centroid_metric).__name__ not in ("Euclidean", "SqEuclidean"):
-        raise ArgumentError("centroid_metric must be Euclidean or SqEuclidean")
+    if type(centroid_metric).__name__ not in ("Euclidean", "SqEuclidean", "CosineDist"):
+        raise ArgumentError("centroid_metric must be Euclidean, SqEuclidean or CosineDist")
+    if type(distance).__name__ not in ("Euclidean", "SqEuclidean", "CosineDist"):
+        raise ArgumentError("distance must be Euclidean, SqEuclidean or CosineDist")
     if kmeans_subsample_size == "default":
         kmeans_subsample_size = max(256 * nlist, 32768)
     # SqEuclidean orders the centroids as Euclidean does: one k-means serves both
@@ -438,15 +447,18 @@ def build_ivf_flat(X, nlist, nprobe=10, distance=default_distance, centroid_metr
 
 def ivf_flat_from_assignment(X, t, assign, nprobe=10, distance=default_distance,
                              centroid_metric=None):
-    """IVFFlatIndex from a fitted KMeansTransform and its 1-based assignment."""
-    ds = _dataset_for(X, Euclidean())
+    """IVFFlatIndex from a fitted KMeansTransform and its 1-based assignment.
+    The lists hold the rows for the point metric `distance`; routing uses
+    `centroid_metric` (ivf_flat.jl:103-131)."""
+    centroid_metric = centroid_metric or Euclidean()
+    ds = _point_dataset(X, distance)
     C = _centroids_rows(t.centroids)
     a = np.ascontiguousarray(assign, np.int64)
     h = ctypes.c_void_p()
     check(lib().mann_ivf_flat_build(ds.handle, C.ctypes.data, t.k, a.ctypes.data,
-                                    ctypes.byref(h)))
+                                    centroid_metric.code, ctypes.byref(h)))
     return IVFFlatIndex(t, h, ds.d, ds.n, t.k, max(1, min(int(nprobe), t.k)), distance,
-                        centroid_metric or Euclidean())
+                        centroid_metric)
 
 
 def transform(t, x):
@@ -486,8 +498,12 @@ def probe_lists(index: MultiLevelIndex, route):
     return out
 
 
-def query_arrays(index: MultiLevelIndex, data, Q, k, route=None, nprobe=None):
-    """Batched query (query.jl:279-300) -> (ids (nq, k), dist (nq, k))."""
+def query_arrays(index: MultiLevelIndex, data, Q, k, route=None, nprobe=None,
+                 probe_order=False):
+    """Batched query (query.jl:279-300) -> (ids (nq, k), dist (nq, k)).
+    probe_order=True merges each query's per-list results in probe order, as
+    the single-query path does (query.jl:25-106); the index's merge strategy
+    (SimpleMerge / DisjointMerge) applies either way."""
     qm = as_matrix(Q)
     d, _ = _data_shape(data)
     if isinstance(index, IVFFlatIndex):
@@ -518,13 +534,18 @@ def query_arrays(index: MultiLevelIndex, data, Q, k, route=None, nprobe=None):
     if route is not None:
         rp = route.data_ptr() if _is_torch(route) else np.ascontiguousarray(route, np.float32)
         rp = rp if isinstance(rp, int) else rp.ctypes.data
-    check(lib().mann_ivf_search(index.handle, qm.ptr, qm.d, nq, qm.ld, k, npb,
-                                index.distance.code, rp, pi, pd))
+    merge = 1 if isinstance(getattr(index, "merge_strategy", None), DisjointMerge) else 0
+    check(lib().mann_ivf_search_ex(index.handle, qm.ptr, qm.d, nq, qm.ld, k, npb,
+                                   index.distance.code, rp, merge, int(bool(probe_order)), pi,
+                                   pd))
     return ids, dist
 
 
 def query(index, data, q, k, nprobe=None):
+    """query(index, data, q | Q, k): one vector goes through the single-query
+    path (per-list results in probe order, query.jl:25-106), a matrix through
+    the batched one (child order, query.jl:279-399)."""
     single = (np.ndim(q) == 1) if not _is_torch(q) else q.dim() == 1
-    ids, dist = query_arrays(index, data, q, k, nprobe=nprobe)
+    ids, dist = query_arrays(index, data, q, k, nprobe=nprobe, probe_order=single)
     res = _to_neighbors(ids, dist)
     return res[0] if single else res

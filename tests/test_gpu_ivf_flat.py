@@ -163,5 +163,6 @@ def test_ivf_flat_k_limit():
     rng = np.random.default_rng(2)
     data = np.asfortranarray(rng.standard_normal((8, 600)).astype(np.float32))
     idx = M.build_index(M.IVFFlatIndex, data, nlist=2)
-    with pytest.raises(M.MannError):
+    # the merge holds the top-(k+1) in a 256-entry buffer: k > 255 is an ArgumentError
+    with pytest.raises(M.ArgumentError):
         M.query(idx, data, np.zeros(8, np.float32), 256, nprobe=2)

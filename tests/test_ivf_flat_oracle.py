@@ -104,7 +104,11 @@ def test_build_arguments_are_checked():
         M.build_index(M.IVFFlatIndex, np.zeros((0, 10), np.float32), nlist=2)
     with pytest.raises(M.ArgumentError):
         M.build_index(M.IVFFlatIndex, np.zeros((4, 0), np.float32), nlist=2)
+    class Chebyshev:  # a metric the GPU path does not offer
+        pass
     with pytest.raises(M.ArgumentError):
-        M.build_index(M.IVFFlatIndex, X, nlist=2, centroid_metric=M.CosineDist())
+        M.build_index(M.IVFFlatIndex, X, nlist=2, centroid_metric=Chebyshev())
+    with pytest.raises(M.ArgumentError):
+        M.build_index(M.IVFFlatIndex, X, nlist=2, distance=Chebyshev())
     with pytest.raises(M.ArgumentError):
         M.build_index(M.IVFFlatIndex, X, nlist=2, merge_strategy=I.SimpleMerge())
