@@ -72,6 +72,70 @@ def kernel_avg_ms(L, check, names):
     return out
 
 
+def fp64_metric(np, X, Q, ids, metric):
+    """FP64 distances of queries Q (m, d) to rows ids (m, j) (1-based) of X."""
+    X64 = X.astype(np.float64)
+    out = np.empty(ids.shape)
+    for s0 in range(0, ids.shape[0], 512):
+        s1 = min(ids.shape[0], s0 + 512)
+        R = X64[ids[s0:s1] - 1]
+        q = Q[s0:s1].astype(np.float64)[:, None, :]
+        if metric == 2:   # CosineDist
+            dot = (R * q).sum(-1)
+            out[s0:s1] = np.maximum(
+                1.0 - dot / (np.linalg.norm(R, axis=-1) * np.linalg.norm(q, axis=-1)), 0.0)
+        else:
+            out[s0:s1] = np.sqrt(((R - q) ** 2).sum(-1))
+    return out
+
+
+def precision_report(np, X, Q, k, gpu_ids, gpu_dist, o_ids, o_dist, metric):
+    """Parity and FP64 error of one configuration's GPU results on the queries
+    Q: ids and distance bits against the oracle (o_ids / o_dist hold the
+    oracle's top-(k+1)), the GPU distances' error against FP64, and the number
+    of queries whose FP64 k-th and (k+1)-th distances lie within the FP32
+    rounding bound (where FP32 and FP64 may order the neighbours differently)."""
+    d = X.shape[1]
+    u = 2.0 ** -24
+    d64 = fp64_metric(np, X, Q, gpu_ids, metric)
+    err = np.abs(gpu_dist.astype(np.float64) - d64)
+    rel = err / np.maximum(d64, 1e-30)
+    o64 = np.sort(fp64_metric(np, X, Q, o_ids[:, :k + 1], metric), axis=1)
+    gap = o64[:, k] - o64[:, k - 1]
+    bound = (2 * d + 16) * u if metric == 2 else (d + 4) * u * o64[:, k]
+    return {"queries": int(Q.shape[0]),
+            "ids_equal_oracle": bool((gpu_ids == o_ids[:, :k]).all()),
+            "dist_bits_equal_oracle": bool((gpu_dist.view(np.uint32) ==
+                                            o_dist[:, :k].view(np.uint32)).all()),
+            "max_abs_err_vs_fp64": float(err.max()), "max_rel_err_vs_fp64": float(rel.max()),
+            "fp32_ambiguous_queries": int((gap <= bound).sum())}
+
+
+def bench_config1(args, M, datasets, L, check, np, torch):
+    """Plain BruteForceIndex exact kNN, L2: base 10,000 x 128 N(0, 1), 1,000
+    queries, k=10 -- the CPU-runnable parity configuration."""
+    from oracle import oracle as O
+    X = datasets.gaussian(10_000, 128, seed=1)
+    Q = datasets.gaussian(1_000, 128, seed=2)
+    Xd = torch.from_numpy(X).cuda()
+    Qd = torch.from_numpy(Q).cuda()
+    ds = M.Dataset(Xd.t())
+    ids = torch.zeros((1000, 10), dtype=torch.int64, device="cuda")
+    dist = torch.zeros((1000, 10), dtype=torch.float32, device="cuda")
+
+    def run():
+        check(L.mann_bf_knn(ds.handle, Qd.data_ptr(), 128, 1000, 128, 10, M.EUCLIDEAN,
+                            ids.data_ptr(), dist.data_ptr()))
+
+    qps = qps_of(run, 1000, args.reps, torch)
+    oi, od = O.bf_knn(X, Q, 11, nthreads=host_cores())
+    rep = precision_report(np, X, Q, 10, ids.cpu().numpy(), dist.cpu().numpy(), oi, od, 0)
+    rep.update({"workload": "exact kNN L2: base 10000x128 N(0,1), 1000 queries, k=10",
+                "qps": round(qps, 1)})
+    del ds, Xd, Qd
+    return rep
+
+
 def bench_knn_graph(args, M, datasets, L, check, np, torch):
     n, d, k = args.n, args.d, args.k
     log(f"kNN graph: generating {n}x{d} clipped mixture")
@@ -134,11 +198,22 @@ def bench_knn_graph(args, M, datasets, L, check, np, torch):
         t1 = time.perf_counter()
         O.bf_knn(X, X[rows], k + 1, nthreads=cores)
         tc = time.perf_counter() - t1
+        oi, od = O.bf_knn(X, X[rows], k + 2, nthreads=cores)
+        gi, gd = ids.cpu().numpy()[rows], dist.cpu().numpy()[rows]
+        # the oracle's self-join rows without the point itself (by id)
+        keep_i = np.zeros((nsq, k + 1), np.int64)
+        keep_d = np.zeros((nsq, k + 1), np.float32)
+        for t_, r in enumerate(rows):
+            m_ = oi[t_] != r + 1
+            keep_i[t_] = oi[t_][m_][:k + 1]
+            keep_d[t_] = od[t_][m_][:k + 1]
+        precision = precision_report(np, X, X[rows], k, gi, gd, keep_i, keep_d, 0)
         cpu = {"value": round(2.0 * nsq * n * d / tc / 1e12, 6), "unit": "TFLOP/s",
                "cores": cores, "kind": "port",
                "sample": f"{nsq} self-join queries (k=21) against all {n} points in {tc:.2f} s "
                          f"(oracle per-pair A16 FP32 scan + partial sort, OpenMP)"}
     res = {"ms_per_step": 1e3 * t, "value": flop / t / 1e12, "kernels_ms": kernels,
+           "precision": precision if not args.skip_cpu else None,
            "stats": {"reranked": stats[0], "overflow": stats[1], "second_pass": stats[2],
                      "exact_fallback": stats[3]}}
     del ds, Xd
@@ -410,9 +485,10 @@ def bench_large_cosine(args, M, L, check, np, torch):
     Xh = X16.cpu().numpy().astype(np.float32)
     Qh = Q32[torch.from_numpy(rows).cuda()].cpu().numpy()
     log(f"large: checking {m} queries on the CPU oracle")
-    oi, od = O.bf_knn(Xh, Qh, k, metric=O.COSINE, nthreads=host_cores())
+    oi, od = O.bf_knn(Xh, Qh, k + 1, metric=O.COSINE, nthreads=host_cores())
     gi = ids.cpu().numpy()[rows]
     gd = dist.cpu().numpy()[rows]
+    precision = precision_report(np, Xh, Qh, k, gi, gd, oi, od, 2)
     out = {"workload": f"cosine exact kNN: base {n}x{d} Float16 (unit 5,000-component mixture, "
                        f"noise 0.1), {nq} queries, k={k}",
            "first_pass": "bf16 MFMA (FP8 e4m3 first pass not implemented)",
@@ -424,9 +500,7 @@ def bench_large_cosine(args, M, L, check, np, torch):
            "dataset_prep_s": round(prep_s, 3), "kernels_ms": kern,
            "search_stats": {"reranked": stats[0], "overflow": stats[1], "second_pass": stats[2],
                             "exact_fallback": stats[3]},
-           "oracle_check": {"queries": m, "ids_equal": bool((gi == oi).all()),
-                            "dist_bit_equal": bool((gd.view(np.uint32) ==
-                                                    od.view(np.uint32)).all())},
+           "precision": precision,
            "total_s": round(time.perf_counter() - t_all, 1)}
     del ds, X16, Q32, ids, dist
     torch.cuda.empty_cache()
@@ -485,6 +559,7 @@ def main():
     from manifoldann_amd._lib import check, lib
     L = lib()
 
+    config1 = None if args.skip_cpu else bench_config1(args, M, datasets, L, check, np, torch)
     res, roof, cpu = bench_knn_graph(args, M, datasets, L, check, np, torch)
     ivf = None if args.skip_ivf else bench_ivf(args, M, datasets, L, check, np, torch)
     large = None if args.skip_large else bench_large_cosine(args, M, L, check, np, torch)
@@ -507,6 +582,8 @@ def main():
         "cpu_baseline": cpu,
         "kernels_ms": res["kernels_ms"],
         "search_stats": res["stats"],
+        "precision": {"exact_knn_10000x128": config1, "knn_graph_60000x784": res["precision"],
+                      "large_cosine_10m_256": large["precision"] if large else None},
         "ivf": ivf,
         "large_cosine": large,
     }
