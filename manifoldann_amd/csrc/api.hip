@@ -777,6 +777,12 @@ int mann_tune_set(const char* key, int64_t value) {
     } else if (k == "xcd_map") {
       MANN_REQUIRE(value >= 0 && value <= 3, ERR_ARGUMENT, "xcd_map must be 0..3");
       t.xcd_map = (int)value;
+    } else if (k == "xcd_band") {
+      MANN_REQUIRE(value >= 1 && value <= 32, ERR_ARGUMENT, "xcd_band must be in 1..32");
+      t.xcd_band = (int)value;
+    } else if (k == "scan_cpol") {
+      MANN_REQUIRE(value >= 0 && value <= 3, ERR_ARGUMENT, "scan_cpol must be 0..3");
+      t.scan_cpol = (int)value;
     } else if (k == "target_groups") {
       MANN_REQUIRE(value >= 1, ERR_ARGUMENT, "target_groups must be positive");
       t.target_groups = (int)value;
@@ -820,6 +826,8 @@ int mann_tune_get(const char* key, int64_t* value) {
     else if (k == "ivf_rows_item") *value = t.ivf_rows_item;
     else if (k == "ivf_first_rows") *value = t.ivf_first_rows;
     else if (k == "xcd_map") *value = t.xcd_map;
+    else if (k == "xcd_band") *value = t.xcd_band;
+    else if (k == "scan_cpol") *value = t.scan_cpol;
     else if (k == "target_groups") *value = t.target_groups;
     else if (k == "sample_rank") *value = t.sample_rank;
     else if (k == "shortlist_mult") *value = t.shortlist_mult;

@@ -282,8 +282,8 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   const int row_tiles = (int)ceil_div(n, BM);
   int parts =
       std::max(1, std::min(std::min(row_tiles, 32), (int)ceil_div(tuning().target_groups, qtiles)));
-  if (tuning().xcd_map == 3)  // bands of 8 partitions (see the scan kernel's grid order)
-    parts = std::min(std::min(row_tiles, 32), (int)round_up(parts, 8));
+  if (tuning().xcd_map == 3)  // whole bands of partitions (see the scan kernel's grid order)
+    parts = std::min(std::min(row_tiles, 32), (int)round_up(parts, tuning().xcd_band));
   int rpp = (int)round_up(ceil_div(n, parts), BM);
   parts = (int)ceil_div(n, rpp);
   const int cap = cap_total / parts;
@@ -403,6 +403,8 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   a.qnorm = Q.qn;
   a.fold = ds.fold;
   a.xcd_map = tuning().xcd_map;
+  a.xcd_band = tuning().xcd_band;
+  a.cpol = tuning().scan_cpol;
   a.qtiles = qtiles;
   a.kp = (int)ds.kp;
   a.d = (int)ds.d;

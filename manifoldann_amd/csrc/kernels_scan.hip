@@ -71,12 +71,15 @@ __device__ __forceinline__ void bare_barrier() {
   asm volatile("" ::: "memory");
 }
 
-// XOR swizzle of the 16-byte chunks of LDS row r (SLOTS = 4 or 8 per row): the
-// fragment reads (32 consecutive rows, one logical chunk) hit distinct banks
-// for 32 or 64 banks, and the staging writes are contiguous per wavefront.
+// XOR swizzle of the 16-byte chunks of LDS row r (SLOTS = 4 or 8 per row); the
+// staging writes stay contiguous per wavefront whatever the swizzle.  For
+// 64-byte rows the fragment reads (32 consecutive rows, one logical chunk per
+// half-wave) are bank-conflict free on gfx950 with the chunk index XORed by
+// bits 3-4 of the row (measured with SQ_LDS_BANK_CONFLICT: 0 cycles, against
+// half of all LDS cycles for the XOR of bits 1-2 and 3-4).
 template <int SLOTS>
 __device__ __forceinline__ int swz(int r) {
-  if (SLOTS == 4) return ((r >> 1) ^ (r >> 3)) & 3;
+  if (SLOTS == 4) return (r >> 3) & 3;
   return (r ^ (r >> 3)) & 7;
 }
 
@@ -98,6 +101,13 @@ __device__ float top1_threshold(float v, float qe, float qr, float ex, float rx,
   double L = sqrt(f2 * f2 + H) + qe + ex;
   double t = (L * L + G) * (1.0 + 1e-9) + 1e-30;
   return (float)t * (1.0f + 1e-6f);
+}
+
+// 16-byte global->LDS load per lane with cache-policy bits AUX (gfx950: 2 =
+// non-temporal).
+template <int AUX>
+__device__ __forceinline__ void gl_lds16(const void* g, lds_void* l) {
+  __builtin_amdgcn_global_load_lds(g, l, 16, 0, AUX);
 }
 
 // Block tile: WR x WC wavefronts, each owning 32*WM rows x 32*WN queries.
@@ -205,6 +215,7 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
   // (A buffer load: scalar row offset + 32-bit lane offset, so no 64-bit
   // per-lane address is held across the loop.)
   const int nx_slot = (wv % (BM / 64)) * 64;
+  const int cpol = __builtin_amdgcn_readfirstlane(a.cpol);
   const __amdgpu_buffer_rsrc_t nx_rsrc =
       __builtin_amdgcn_make_buffer_rsrc((void*)a.xn, 0, 0xffffffffu, 0x00020000);
   auto dma = [&](int t, int kc, int st) {
@@ -216,16 +227,28 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
                                                0, 0);
     const bf16_t* baseA = a.xb + (int64_t)(row_begin + t * BM) * kp + kc * BK;
     const bf16_t* baseB = a.qb + kc * BK;
+    if (cpol & 1) {
 #pragma unroll
-    for (int p = 0; p < T::A_CHUNKS; ++p)
-      __builtin_amdgcn_global_load_lds((const void*)((const char*)baseA + offA[p]),
-                                       (lds_void*)(As + st * BM * LROW + (NT * p + 64 * wv) * 8),
-                                       16, 0, 0);
+      for (int p = 0; p < T::A_CHUNKS; ++p)
+        gl_lds16<2>((const char*)baseA + offA[p],
+                    (lds_void*)(As + st * BM * LROW + (NT * p + 64 * wv) * 8));
+    } else {
 #pragma unroll
-    for (int p = 0; p < T::B_CHUNKS; ++p)
-      __builtin_amdgcn_global_load_lds((const void*)((const char*)baseB + offB[p]),
-                                       (lds_void*)(Bs + st * BN * LROW + (NT * p + 64 * wv) * 8),
-                                       16, 0, 0);
+      for (int p = 0; p < T::A_CHUNKS; ++p)
+        gl_lds16<0>((const char*)baseA + offA[p],
+                    (lds_void*)(As + st * BM * LROW + (NT * p + 64 * wv) * 8));
+    }
+    if (cpol & 2) {
+#pragma unroll
+      for (int p = 0; p < T::B_CHUNKS; ++p)
+        gl_lds16<2>((const char*)baseB + offB[p],
+                    (lds_void*)(Bs + st * BN * LROW + (NT * p + 64 * wv) * 8));
+    } else {
+#pragma unroll
+      for (int p = 0; p < T::B_CHUNKS; ++p)
+        gl_lds16<0>((const char*)baseB + offB[p],
+                    (lds_void*)(Bs + st * BN * LROW + (NT * p + 64 * wv) * 8));
+    }
   };
   // wait until at most `later` chunks issued after the one needed next are in
   // flight (a chunk 0 carries one more op, the row norms: counting NPC per
@@ -561,7 +584,7 @@ __attribute__((amdgpu_waves_per_eu(WM * WN <= 8 ? 2 : 1))) void scan_grid_kernel
     // bands of 8 row partitions: the ~64 workgroups an XCD runs at once cover
     // 8 partitions x 8 query tiles, so each row chunk and each query chunk
     // they load is shared by 8 of them through the XCD's L2
-    constexpr int SPB = 8;
+    const int SPB = a.xcd_band > 0 ? a.xcd_band : 8;
     const int g = bid / (SPB * a.qtiles);
     const int p0 = g * SPB;
     const int bp = min(SPB, a.parts - p0);

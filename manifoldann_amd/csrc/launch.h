@@ -22,6 +22,9 @@ struct ScanArgs {
   const float* qnorm; // |q~|^2 per query (added to every value of its column)
   const float* col_norm;  // or per item column: [qoff + c] (IVF residual queries)
   int xcd_map;        // grid mode: 0 plain, 1 query-major per XCD, 2 part-major per XCD
+  int xcd_band;       // grid mode 3: row partitions per band
+  int cpol;           // cache policy of the operand loads: bit 0 rows, bit 1 queries
+                      // non-temporal (streamed: inserted for early eviction in L2)
   int qtiles;         // grid mode: number of query tiles
   const float* xn;    // |x~|^2 per base row, +inf on padding
   const bf16_t* qb;   // query rows, kp bf16 each
@@ -78,7 +81,9 @@ struct Tuning {
   int ivf_first_rows = 0;    // IVF threshold: rows of the first list scanned (0: all)       // IVF first-list threshold scan tile config (few queries per list)
   int kmeans_cfg = 0;        // k-means assignment tile config
   int xcd_map = 3;           // grid order: 0 plain, 1 query-major per XCD, 2 part-major per
-                             // XCD, 3 bands of 8 partitions x query tiles per XCD
+                             // XCD, 3 bands of xcd_band partitions x query tiles per XCD
+  int xcd_band = 8;          // row partitions per band of grid order 3
+  int scan_cpol = 0;         // operand loads non-temporal: bit 0 rows, bit 1 queries
   int target_groups = 2048;  // workgroups the row partitioning aims for
   int sample_rank = 1;       // multiplier of the emission target below
   int emit_per_k = 15;       // exact kNN: aim at emit_per_k * k_req first-pass emissions per query
