@@ -72,6 +72,21 @@ int mann_dataset_info(mann_dataset_t ds, int64_t* d, int64_t* n);
 int mann_bf_knn(mann_dataset_t base, const float* queries, int64_t d, int64_t nq, int64_t ldq,
                 int64_t k, int metric, int64_t* out_ids, float* out_dist);
 
+/* mann_bf_knn without allocation -- no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * checks and results as mann_bf_knn (bit-identical ids and distances), but
+ * queries, out_ids and out_dist must be device memory, and every scratch
+ * buffer of the search comes from one 64 MiB array that the library keeps per
+ * GPU for the life of the process: a batch whose scratch would exceed it is
+ * an ArgumentError (the message names the bytes needed; split the batch).
+ * The first call on a host thread and device does one-time set-up (the
+ * thread's stream, the array's device address); no call allocates after that.
+ * Calls on one GPU share the array: the library runs them one at a time, and
+ * work that overlaps them on that GPU by other means races. */
+int mann_bf_knn_noalloc(mann_dataset_t base, const float* queries, int64_t d, int64_t nq,
+                        int64_t ldq, int64_t k, int metric, int64_t* out_ids, float* out_dist);
+
 /* build_knn_graph(index, data; k, include_self) -- directed adjacency
  * (src/graphs/knn_graph.jl:42-81, _prepare_neighbors :105-130): each point is
  * queried with k+1 (k if include_self), itself dropped by id, first k kept.

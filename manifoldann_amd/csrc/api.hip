@@ -2,6 +2,7 @@
 // classes, host/device pointer staging, and dispatch to the pipelines.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <set>
 #include <vector>
 
@@ -159,6 +160,98 @@ int mann_bf_knn(mann_dataset_t base, const float* queries, int64_t d, int64_t nq
     oi.finish();
     od.finish();
     timer_flush();
+  });
+}
+
+// ---- allocation-free exact search ---------------------------------------------
+//
+// The scratch of mann_bf_knn_noalloc: one file-scope device array, which every
+// GPU the library runs on holds until the process ends.  Host code never uses
+// its name as an address: the address is looked up once per device with
+// hipGetSymbolAddress (with that device current) and kept per device.
+constexpr size_t kNoAllocScratch = (size_t)64 << 20;
+constexpr size_t kNoAllocGuard = (size_t)64 << 10;  // never carved: the array's last 64 KiB
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+
+static constexpr int kMaxDevices = 64;
+struct NoAllocDevice {
+  std::once_flag once;
+  char* ptr = nullptr;
+  hipError_t rc = hipSuccess;
+  std::mutex mu;  // calls on one device run one at a time (they share the array)
+};
+static NoAllocDevice g_noalloc[kMaxDevices];
+
+static char* noalloc_scratch(int dev) {
+  MANN_REQUIRE(dev >= 0 && dev < kMaxDevices, ERR_DEVICE, "device index out of range");
+  NoAllocDevice& nd = g_noalloc[dev];
+  std::call_once(nd.once, [&] {
+    void* ptr = nullptr;
+    nd.rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    nd.ptr = (char*)ptr;
+  });
+  MANN_REQUIRE(nd.rc == hipSuccess && nd.ptr != nullptr, ERR_DEVICE,
+               std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(nd.rc));
+  return nd.ptr;
+}
+
+// Upper bound of the scratch an exact search of nq queries carves before its
+// exact fallback (which sizes its batches to what is left, down to one query).
+static size_t noalloc_need(const mann_dataset& ds, int64_t nq, int64_t k) {
+  const size_t nq_pad = (size_t)round_up(nq, 256);
+  size_t b = 0;
+  b += nq_pad * ds.d16 * 4 + nq_pad * ds.kp * 2 + 4 * nq_pad * 4;  // query operands
+  if (ds.fold) b += nq_pad * ds.kp * 2;                              // folded operand
+  b += (size_t)nq * select_cap_max() * 8 + (size_t)nq * 64 * 4;      // candidates, counts
+  b += 4 * (size_t)nq * 4 + 32 + kStatSlots * 16;                    // thresholds, lists
+  b += (size_t)256 * ds.m * 4;                                       // one sample chunk
+  const size_t kr = (size_t)std::min<int64_t>(k, exact_kmax());
+  b += (size_t)ceil_div(ds.n, exact_seg_rows()) * kr * 8;            // one fallback query
+  return b + b / 8 + 40 * 512 + kNoAllocGuard;  // carve slack and alignment, end guard
+}
+
+int mann_bf_knn_noalloc(mann_dataset_t base, const float* queries, int64_t d, int64_t nq,
+                        int64_t ldq, int64_t k, int metric, int64_t* out_ids, float* out_dist) {
+  return guarded([&] {
+    MANN_REQUIRE(base != nullptr, ERR_NULL, "null dataset");
+    check_metric(metric);
+    MANN_REQUIRE(d == base->d, ERR_DIMENSION,
+                 "Expected queries with " + std::to_string(base->d) + " rows, got " +
+                     std::to_string(d));
+    MANN_REQUIRE(nq >= 0, ERR_ARGUMENT, "negative query count");
+    if (nq == 0 || k <= 0) {
+      last_stats() = SearchStats();
+      return;
+    }
+    MANN_REQUIRE(queries && out_ids && out_dist, ERR_NULL, "null pointer");
+    MANN_REQUIRE(ldq >= d, ERR_ARGUMENT, "leading dimension smaller than d");
+    MANN_REQUIRE((metric == COSINE) == (base->normalized != 0), ERR_ARGUMENT,
+                 "dataset was prepared for another metric");
+    MANN_REQUIRE(is_device_ptr(queries) && is_device_ptr(out_ids) && is_device_ptr(out_dist),
+                 ERR_ARGUMENT,
+                 "the allocation-free search takes device queries and outputs only "
+                 "(host buffers would need staging allocations)");
+    const size_t need = noalloc_need(*base, nq, k);
+    MANN_REQUIRE(need <= kNoAllocScratch, ERR_ARGUMENT,
+                 std::to_string(nq) + " queries need " + std::to_string(need) +
+                     " bytes of scratch; the allocation-free search has " +
+                     std::to_string(kNoAllocScratch) + ": split the query batch");
+    ThreadCtx& c = ctx();
+    char* sp = noalloc_scratch(c.device);
+    std::lock_guard<std::mutex> lock(g_noalloc[c.device].mu);
+    Arena arena{sp, kNoAllocScratch - kNoAllocGuard, 0};
+    c.abufs.clear();
+    c.arena = &arena;
+    struct Restore {
+      ThreadCtx& c;
+      ~Restore() {
+        c.arena = nullptr;
+        c.abufs.clear();
+      }
+    } restore{c};
+    QueryOps Q = prepare_queries(queries, d, nq, ldq, base->normalized, (int)base->d16,
+                                 (int)base->kp, "bfq", base->center);
+    bf_search(*base, Q, (int)k, metric, 0, out_ids, out_dist, nullptr);
   });
 }
 

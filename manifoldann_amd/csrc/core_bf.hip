@@ -29,7 +29,7 @@ Timer& timer() {
 
 int timer_begin(const char* name, hipStream_t st) {
   Timer& t = timer();
-  if (!t.enabled) return -1;
+  if (!t.enabled || ctx().arena) return -1;  // (event creation allocates)
   std::lock_guard<std::mutex> g(t.mu);
   Timer::Pending p;
   p.name = name;
@@ -240,7 +240,11 @@ void run_exact_fallback(ExactArgs e, int nflag, int max_segments) {
   if (nflag <= 0) return;
   ThreadCtx& c = ctx();
   const int kr = std::min(e.k_req, exact_kmax());
-  const int CH = 2048;
+  int CH = 2048;
+  if (c.arena) {  // as many queries per launch as the scratch holds partials for
+    const size_t per = (size_t)max_segments * kr * sizeof(uint64_t);
+    CH = (int)std::max<size_t>(1, std::min<size_t>(2048, c.arena->left() / (per + 256)));
+  }
   e.nseg = max_segments;
   e.partial = c.buf("ex_partial").as<uint64_t>((size_t)std::min(nflag, CH) * max_segments * kr);
   for (int base = 0; base < nflag; base += CH) {
@@ -322,7 +326,11 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
     const bool bmin = ds.m / 32 >= 64 * rank;
     const int mcols = bmin ? (int)ceil_div(ds.m, 32) : ds.m;
     // the dense sample block is processed in query chunks of at most 2^28 values
-    const int64_t qc_max = std::max<int64_t>(BN, ((int64_t)1 << 28) / mcols / BN * BN);
+    int64_t qc_max = std::max<int64_t>(BN, ((int64_t)1 << 28) / mcols / BN * BN);
+    if (c.arena)  // the sample block in query chunks the scratch holds
+      qc_max = std::max<int64_t>(
+          BN, std::min<int64_t>(qc_max, (int64_t)(c.arena->left() / 2 / sizeof(float)) / mcols /
+                                            BN * BN));
     const int qc = (int)std::min<int64_t>(nq, qc_max);
     float* dense = c.buf("bf_dense").as<float>((size_t)qc * mcols);
     // self-join: the rows are the queries and the sample rows the query

@@ -10,11 +10,36 @@
 
 namespace mann {
 
-// Grow-only device buffer.
+// A caller-provided block of device memory that scratch buffers are carved
+// from in order (no allocation); carving past its end is an ArgumentError.
+struct Arena {
+  char* base = nullptr;
+  size_t size = 0;
+  size_t used = 0;
+  size_t left() const { return size > used ? size - used : 0; }
+};
+
+// Grow-only device buffer (or, with an arena, a carve of it).
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  Arena* arena = nullptr;
   void* get(size_t need) {
+    if (arena) {
+      if (p && need <= bytes) return p;
+      // the same slack as an allocation below, so both kinds of buffer hold
+      // the same bytes
+      const size_t want = need + need / 8 + 256;
+      const size_t off = (arena->used + 255) / 256 * 256;
+      MANN_REQUIRE(off + want <= arena->size, ERR_ARGUMENT,
+                   "the allocation-free scratch is exhausted (" + std::to_string(off + want) +
+                       " bytes needed, " + std::to_string(arena->size) +
+                       " available): split the query batch");
+      p = arena->base + off;
+      bytes = want;
+      arena->used = off + want;
+      return p;
+    }
     if (need > bytes) {
       if (p) (void)hipFree(p);
       p = nullptr;
@@ -42,7 +67,17 @@ struct ThreadCtx {
   int device = -1;
   hipStream_t stream = nullptr;
   std::map<std::string, DevBuf> bufs;
-  DevBuf& buf(const std::string& name) { return bufs[name]; }
+  // set while an allocation-free call runs: buffers are carved from it
+  Arena* arena = nullptr;
+  std::map<std::string, DevBuf> abufs;
+  DevBuf& buf(const std::string& name) {
+    if (arena) {
+      DevBuf& b = abufs[name];
+      b.arena = arena;
+      return b;
+    }
+    return bufs[name];
+  }
 };
 ThreadCtx& ctx();
 
