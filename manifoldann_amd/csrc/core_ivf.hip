@@ -175,7 +175,7 @@ mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int
     int* da = c.buf("ivb_assign").as<int>(n);
     MANN_HIP(hipMemcpyAsync(da, a32.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
     const int nblk = (int)ceil_div(n, sort_block());
-    int* table = c.buf("ivb_table").as<int>((size_t)nb * nblk);
+    int* table = c.buf("ivb_table").as<int>((size_t)counting_sort_table_ints(n, nb));
     MANN_HIP(hipMalloc(&iv->list_off, sizeof(int) * (nb + 1)));
     MANN_HIP(hipMalloc(&iv->perm, sizeof(int) * n));
     launch_counting_sort(da, n, nb, table, iv->list_off, iv->perm, st);

@@ -136,7 +136,7 @@ int64_t kmeans_lloyd(const mann_dataset& ds, float* C, int k, int64_t max_iters,
   const int64_t n = ds.n;
   const int d = (int)ds.d, d16 = (int)ds.d16;
   const int nb = (int)ceil_div(n, sort_block());
-  int* table = c.buf("km_table").as<int>((size_t)k * nb);
+  int* table = c.buf("km_table").as<int>((size_t)counting_sort_table_ints(n, k));
   int* offsets = c.buf("km_off").as<int>(k + 1);
   int* members = c.buf("km_members").as<int>(n);
   float* newc = c.buf("km_newc").as<float>((size_t)k * d16);
