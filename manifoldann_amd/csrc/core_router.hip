@@ -146,6 +146,30 @@ static void router_forward(const mann_router& r, const float* X, int64_t ldx, in
   if (negate && !negated) launch_neg(Z, Z, (int64_t)B * r.c, st);
 }
 
+// A GEMM whose operands have unit-stride K after transposing the strided ones
+// into thread scratch (a K-major copy: row stride K rounded up to 4), so the
+// tiled MFMA kernel takes it; the values, their bf16 rounding and the K order
+// of the accumulation are those of the strided form.
+static void gemm_k_major(GemmArgs g, const char* ta, const char* tb, hipStream_t st) {
+  ThreadCtx& c = ctx();
+  const int64_t ldk = round_up(g.K, 4);
+  if (g.sak != 1 && g.sam == 1) {
+    float* At = c.buf(ta).as<float>((size_t)g.M * ldk);
+    launch_transpose(g.A, g.sak, g.K, g.M, At, ldk, st);  // A is K x M with row stride sak
+    g.A = At;
+    g.sam = ldk;
+    g.sak = 1;
+  }
+  if (g.sbk != 1 && g.sbn == 1) {
+    float* Bt = c.buf(tb).as<float>((size_t)g.N * ldk);
+    launch_transpose(g.B, g.sbk, g.K, g.N, Bt, ldk, st);
+    g.B = Bt;
+    g.sbn = ldk;
+    g.sbk = 1;
+  }
+  launch_gemm(g, st);
+}
+
 // Gradients of the mean cross-entropy into grad (same layout as the parameters).
 // Z is overwritten with dL/dZ.  Returns nothing; loss accumulated into *loss.
 static void router_backward(const mann_router& r, const float* X, int64_t ldx, int B,
@@ -168,7 +192,7 @@ static void router_backward(const mann_router& r, const float* X, int64_t ldx, i
   a.M = r.c;
   a.N = r.h;
   a.K = B;
-  launch_gemm(a, st);
+  gemm_k_major(a, "rt_tA", "rt_tB", st);
   launch_colsum(Z, B, r.c, gb2, st);
   GemmArgs b{};  // dH[b][j] = sum_i G[b][i] W2[i][j]
   b.A = Z;
@@ -182,7 +206,7 @@ static void router_backward(const mann_router& r, const float* X, int64_t ldx, i
   b.M = B;
   b.N = r.h;
   b.K = r.c;
-  launch_gemm(b, st);
+  gemm_k_major(b, "rt_tA", "rt_tB", st);
   launch_relu_mask(dH, H, (int64_t)B * r.h, st);
   GemmArgs c{};  // dW1[j][t] = sum_b dH[b][j] X[b][t]
   c.A = dH;
@@ -196,7 +220,7 @@ static void router_backward(const mann_router& r, const float* X, int64_t ldx, i
   c.M = r.h;
   c.N = r.d;
   c.K = B;
-  launch_gemm(c, st);
+  gemm_k_major(c, "rt_tA", "rt_tB", st);
   launch_colsum(dH, B, r.h, gb1, st);
 }
 

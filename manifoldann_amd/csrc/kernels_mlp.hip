@@ -323,6 +323,24 @@ __global__ void adam_kernel(float* __restrict__ w, const float* __restrict__ g,
   w[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
 }
 
+// out[c][r] = in[r][c] for r < rows, c < cols (32 x 32 tiles through LDS)
+__global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict__ in, int64_t ldi,
+                                                        int rows, int cols,
+                                                        float* __restrict__ out, int64_t ldo) {
+  __shared__ float t[32][33];
+  const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int i = ty; i < 32; i += 8) {
+    const int r = r0 + i, c = c0 + tx;
+    t[i][tx] = (r < rows && c < cols) ? in[(int64_t)r * ldi + c] : 0.0f;
+  }
+  __syncthreads();
+  for (int i = ty; i < 32; i += 8) {
+    const int c = c0 + i, r = r0 + tx;
+    if (c < cols && r < rows) out[(int64_t)c * ldo + r] = t[tx][i];
+  }
+}
+
 __global__ void neg_kernel(const float* __restrict__ s, float* __restrict__ d, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) d[i] = -s[i];
@@ -418,6 +436,13 @@ void launch_prior(const float* X, int64_t ldx, int B, int d, const float* C, int
   dim3 grid((unsigned)ceil_div(B, PRIOR_ROWS), (unsigned)ceil_div(nlist, 256));
   hipLaunchKernelGGL(prior_kernel, grid, dim3(256), 0, st, X, ldx, B, d, C, ldc, cn2, nlist, gamma,
                      Z, ldz);
+}
+
+void launch_transpose(const float* in, int64_t ldi, int rows, int cols, float* out, int64_t ldo,
+                      hipStream_t st) {
+  if (rows <= 0 || cols <= 0) return;
+  dim3 grid((unsigned)ceil_div(cols, 32), (unsigned)ceil_div(rows, 32));
+  hipLaunchKernelGGL(transpose_kernel, grid, dim3(256), 0, st, in, ldi, rows, cols, out, ldo);
 }
 
 void launch_neg(const float* s, float* d, int64_t n, hipStream_t st) {

@@ -28,6 +28,9 @@ void launch_relu_mask(float* dH, const float* H, int64_t n, hipStream_t st);
 void launch_adam(float* w, const float* g, float* m1, float* m2, int64_t n, float lr, float b1,
                  float b2, float eps, int step, hipStream_t st);
 void launch_neg(const float* s, float* d, int64_t n, hipStream_t st);
+// out (cols x rows, row stride ldo) = transpose of in (rows x cols, row stride ldi)
+void launch_transpose(const float* in, int64_t ldi, int rows, int cols, float* out, int64_t ldo,
+                      hipStream_t st);
 // Z[b][c] += gamma * (2 x_b.c_c - |c_c|^2) (FP32), d <= prior_dmax()
 void launch_prior(const float* X, int64_t ldx, int B, int d, const float* C, int64_t ldc,
                   const float* cn2, int nlist, float gamma, float* Z, int64_t ldz,
