@@ -94,6 +94,17 @@ int mann_bf_knn_noalloc(mann_dataset_t base, const float* queries, int64_t d, in
 int mann_knn_graph(mann_dataset_t data, int64_t k, int include_self, int metric,
                    int64_t* out_ids, float* out_dist);
 
+/* The candidate re-query of build_weighted_graph with candidate_k > k
+ * (src/graphs/weighted_knn_graph.jl:497-503, :549-553): every point queried
+ * with candidate_k + 1, its own id dropped, the first candidate_k kept.  Unlike
+ * build_knn_graph a short row is not an error: a point gets min(candidate_k,
+ * n - 1) candidates (fewer only if it is not among its own first
+ * candidate_k + 1), and empty slots hold id 0.  out_ids (and out_dist when
+ * given) are candidate_k x n; out_count (optional) the candidates per point.
+ * candidate_k <= 255. */
+int mann_knn_candidates(mann_dataset_t data, int64_t candidate_k, int metric, int64_t* out_ids,
+                        float* out_dist, int64_t* out_count);
+
 /* _symmetrize_adjacency (src/graphs/knn_graph.jl:196-210): union of i->j and
  * j->i, each row sorted ascending, as CSR (out_offsets n+1, out_ids 2*n*k
  * capacity).  *out_total receives the edge count; *out_max_degree the new

@@ -10,7 +10,8 @@ keep calling them unchanged:
 - `query(::BruteForceIndex{Float32,D}, data, Q | q, k)`  (bruteforce.jl:46-71,
   ann_index.jl:71-101)
 - `build_knn_graph(::BruteForceIndex{Float32,D}, data; k, include_self,
-  directed, metadata)`  (knn_graph.jl:42-81)
+  directed, metadata)`  (knn_graph.jl:42-81), and the candidate re-query of
+  `build_weighted_graph` (weighted_knn_graph.jl:489-515) as one call
 - `fit!(::KMeansTransform{Euclidean,Float32}, X; rng)`  (KMeansTransform.jl:106-162)
 - `query(::MultiLevelIndex{<:KMeansTransform,<:BruteForceIndex}, data, Q | q, k)`
   (multilevel/query.jl:25-39, :279-300) with the index's merge strategy
@@ -238,6 +239,45 @@ function build_knn_graph(index::BruteForceIndex{Float32,D}, data::StridedMatrix{
     end
     meta = ManifoldANN._create_graph_metadata(original_k, directed, metadata, n)
     return KNNGraph(adjacency, kk, include_self, meta)
+end
+
+"""
+    knn_candidates(index, data, candidate_k) -> Vector{Vector{Int}}
+
+The candidate lists of `build_weighted_graph` with `candidate_k > k`
+(weighted_knn_graph.jl:497-503) in one GPU call: every point queried with
+`candidate_k + 1`, its own id dropped, the first `candidate_k` kept; short
+lists are kept as they are.
+"""
+function knn_candidates(index::BruteForceIndex{Float32,D}, data::StridedMatrix{Float32},
+                        candidate_k::Integer) where {D<:HIPMetric}
+    n = size(data, 2)
+    ids = zeros(Int64, candidate_k, n)
+    cnt = zeros(Int64, n)
+    with_dataset(data, index.distance) do dd
+        GC.@preserve ids cnt check(ccall((:mann_knn_candidates, LIB), Cint,
+            (Ptr{Cvoid}, Int64, Cint, Ptr{Int64}, Ptr{Float32}, Ptr{Int64}),
+            dd.h, candidate_k, metric_code(index.distance), ids, C_NULL, cnt))
+    end
+    return [Vector{Int}(ids[1:cnt[i], i]) for i in 1:n]
+end
+
+# the per-point candidate queries of the unshared geometry fit
+# (weighted_knn_graph.jl:489-515) as one GPU call; the fits are unchanged
+function ManifoldANN._fit_geometries_with_candidates(::ManifoldANN.NoSharing,
+        method::ManifoldANN.AbstractLocalGeometryMethod, graph::KNNGraph,
+        index::BruteForceIndex{Float32,D}, data::StridedMatrix{Float32},
+        candidate_k::Integer) where {D<:HIPMetric}
+    n = length(graph)
+    n == 0 && return ManifoldANN.AbstractLocalGeometry[]
+    cands = knn_candidates(index, data, candidate_k)
+    g1 = ManifoldANN.fit_geometry(method, data, 1, cands[1]; graph = graph)
+    geometries = Vector{typeof(g1)}(undef, n)
+    geometries[1] = g1
+    Threads.@threads for i in 2:n
+        geometries[i] = ManifoldANN.fit_geometry(method, data, i, cands[i]; graph = graph)
+    end
+    return geometries
 end
 
 # ---------------------------------------------------------------------------

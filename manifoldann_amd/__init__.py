@@ -10,7 +10,8 @@ from ._lib import ArgumentError, DimensionMismatch, MannError, lib
 from .index import (COSINE, EUCLIDEAN, SQEUCLIDEAN, BruteForceIndex, CosineDist, Dataset,
                     Euclidean, KNNGraph, Neighbor, SqEuclidean, build_index, build_knn_graph,
                     cosine_distance, default_distance, default_squared_distance,
-                    euclidean_distance, graph_metadata, has_metadata, insert, last_emitted, last_stats,
+                    euclidean_distance, graph_metadata, has_metadata, insert, knn_candidates,
+                    last_emitted, last_stats,
                     neighbor_ids, node_metadata, query, query_arrays, recall_at_k,
                     symmetrize_adjacency)
 from .ivf import IVFFlatIndex, MultiLevelIndex

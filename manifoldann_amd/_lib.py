@@ -31,6 +31,7 @@ SIGNATURES = [
     ("mann_bf_knn_noalloc", ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, ctypes.c_int, _vp,
                                            _vp]),
     ("mann_knn_graph", ctypes.c_int, [_vp, _i64, ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    ("mann_knn_candidates", ctypes.c_int, [_vp, _i64, ctypes.c_int, _vp, _vp, _vp]),
     ("mann_knn_graph_symmetrize", ctypes.c_int, [_vp, _i64, _i64, _vp, _vp, _i64p, _i64p]),
     ("mann_kmeans_assign", ctypes.c_int, [_vp, _vp, _i64, _vp, _vp]),
     ("mann_kmeanspp", ctypes.c_int, [_vp, _i64, ctypes.c_uint64, _vp, _vp]),

@@ -287,6 +287,39 @@ int mann_knn_graph(mann_dataset_t data, int64_t k, int include_self, int metric,
   });
 }
 
+int mann_knn_candidates(mann_dataset_t data, int64_t candidate_k, int metric, int64_t* out_ids,
+                        float* out_dist, int64_t* out_count) {
+  return guarded([&] {
+    MANN_REQUIRE(data != nullptr, ERR_NULL, "null dataset");
+    check_metric(metric);
+    MANN_REQUIRE(candidate_k > 0, ERR_ARGUMENT, "candidate_k must be positive");
+    MANN_REQUIRE(candidate_k + 1 <= select_exmax() / 2, ERR_ARGUMENT,
+                 "candidate_k = " + std::to_string(candidate_k) +
+                     " exceeds the GPU search's maximum of " +
+                     std::to_string(select_exmax() / 2 - 1));
+    MANN_REQUIRE(out_ids != nullptr, ERR_NULL, "null pointer");
+    MANN_REQUIRE((metric == COSINE) == (data->normalized != 0), ERR_ARGUMENT,
+                 "dataset was prepared for another metric");
+    const int64_t n = data->n, k = candidate_k;
+    QueryOps Q = dataset_as_queries(*data);
+    OutBuf<int64_t> oi(out_ids, (size_t)(n * k), "out_ids");
+    OutBuf<float> od(out_dist, (size_t)(n * k), "out_dist");
+    int* cnt = ctx().buf("cand_cnt").as<int>(n);
+    // query k + 1 (at most n results), drop the point itself by id, keep the
+    // first k: a short row is not an error here
+    bf_search(*data, Q, (int)k, metric, 1, oi.dev, od.dev, cnt);
+    if (out_count) {
+      std::vector<int> hc(n);
+      MANN_HIP(hipMemcpy(hc.data(), cnt, sizeof(int) * n, hipMemcpyDeviceToHost));
+      std::vector<int64_t> h64(hc.begin(), hc.end());
+      MANN_HIP(hipMemcpy(out_count, h64.data(), sizeof(int64_t) * n, hipMemcpyDefault));
+    }
+    oi.finish();
+    if (out_dist) od.finish();
+    timer_flush();
+  });
+}
+
 int mann_knn_graph_symmetrize(const int64_t* adj, int64_t n, int64_t k, int64_t* out_offsets,
                               int64_t* out_ids, int64_t* out_total, int64_t* out_max_degree) {
   return guarded([&] {

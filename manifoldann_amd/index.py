@@ -369,6 +369,26 @@ def build_knn_graph(index, data, k=None, include_self=False, directed=True, meta
     return KNNGraph(neighbors, rk, bool(include_self), meta)
 
 
+def knn_candidates(index, data, candidate_k, return_arrays=False):
+    """The candidate lists of build_weighted_graph with candidate_k > k
+    (weighted_knn_graph.jl:497-503): for every point, query candidate_k + 1,
+    drop the point itself by id, keep the first candidate_k; short lists are
+    returned as they are.  -> list of id lists (1-based), or with
+    return_arrays=True (ids (n, candidate_k) 0-padded, dist, counts)."""
+    d, n = _data_shape(data)
+    if n <= 0:
+        raise ArgumentError("Dataset must contain at least one point")
+    ds = _dataset_for(data, index.distance)
+    ids = np.zeros((n, candidate_k), np.int64)
+    dist = np.zeros((n, candidate_k), np.float32)
+    cnt = np.zeros(n, np.int64)
+    check(lib().mann_knn_candidates(ds.handle, int(candidate_k), index.distance.code,
+                                    ids.ctypes.data, dist.ctypes.data, cnt.ctypes.data))
+    if return_arrays:
+        return ids, dist, cnt
+    return [ids[i, :cnt[i]].tolist() for i in range(n)]
+
+
 def symmetrize_adjacency(adj):
     """_symmetrize_adjacency (knn_graph.jl:196-210) -> (lists, max degree)."""
     adj = np.ascontiguousarray(adj, np.int64)

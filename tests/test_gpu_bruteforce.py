@@ -199,3 +199,31 @@ def test_folded_norms_match_unfolded():
                 assert (dist.view(np.uint32) == od.view(np.uint32)).all()
     finally:
         check(L.mann_tune_set(b"fold_norms", 1))
+
+
+@pytest.mark.parametrize("metric", [M.Euclidean(), M.SqEuclidean(), M.CosineDist()])
+def test_knn_candidates_match_oracle(metric):
+    # weighted_knn_graph.jl:497-503: query candidate_k + 1, drop self by id,
+    # keep the first candidate_k, no error when short.  Duplicated points make
+    # some rows lose their own id to lower-id twins; candidate_k >= n leaves
+    # every row short.
+    _banner(f"knn candidates, metric {metric.code}")
+    rng = np.random.default_rng(44 + metric.code)
+    base = rng.standard_normal((500, 24)).astype(np.float32)
+    X = np.concatenate([base, base[:60], base[:20]])          # 580 points, duplicates
+    index = M.build_index(M.BruteForceIndex, X.T, distance=metric)
+    for ck in (5, 31, 255):
+        ids, dist, cnt = M.knn_candidates(index, X.T, ck, return_arrays=True)
+        k1 = min(ck + 1, X.shape[0])
+        oi, od = O.bf_knn(X, X, k1, metric.code)
+        for i in range(X.shape[0]):
+            keep = [(a, b) for a, b in zip(oi[i], od[i]) if a != i + 1][:ck]
+            assert cnt[i] == len(keep), (ck, i)
+            assert ids[i, :cnt[i]].tolist() == [a for a, _ in keep], (ck, i)
+            assert dist[i, :cnt[i]].tolist() == [float(b) for _, b in keep]
+            assert (ids[i, cnt[i]:] == 0).all()
+    small = np.ascontiguousarray(X[:40])
+    idx2 = M.build_index(M.BruteForceIndex, small.T, distance=metric)
+    lists = M.knn_candidates(idx2, small.T, 100)
+    assert all(len(l) <= 39 for l in lists)
+    assert sum(len(l) == 39 for l in lists) >= 20
