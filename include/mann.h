@@ -134,10 +134,16 @@ int mann_kmeanspp(mann_dataset_t X, int64_t k, uint64_t seed, float* out_centroi
  * init_centroids (d x k), 1 = kmeans++ (seed), 2 = random distinct points
  * (seed).  M-step sums in FP64 over a fixed order; empty clusters take the
  * farthest point (lloyd.jl:81-104); stops when the max centroid movement < tol.
+ * subsample (KMeansTransform's subsample_size; 0 = nothing): when 0 < subsample
+ * < n, Lloyd runs on max(subsample, k) distinct points (the first of a
+ * splitmix64 Fisher-Yates shuffle of `seed`, in that order, as
+ * X[:, randperm(rng, n)[1:m]], :113-120), the initialisation then drawing from
+ * the stream of seed * 0x9E3779B97F4A7C15 + 1, and out_assign (n) comes from one
+ * E-step over all points (:137-146).
  * Errors: MANN_ERR_ASSERTION for k <= 0, k > n, max_iters <= 0, tol <= 0. */
 int mann_kmeans_fit(mann_dataset_t X, int64_t k, int64_t max_iters, double tol, int init,
-                    uint64_t seed, const float* init_centroids, float* out_centroids,
-                    int64_t* out_assign, int64_t* out_iters);
+                    uint64_t seed, const float* init_centroids, int64_t subsample,
+                    float* out_centroids, int64_t* out_assign, int64_t* out_iters);
 
 /* transform(::KMeansTransform, x) for a batch (src/transforms/KMeansTransform.jl:
  * 181-190 -> src/transforms/kmeans/distance.jl:213-242): out_dist[q][c] =

@@ -284,9 +284,9 @@ end
 # KMeansTransform
 # ---------------------------------------------------------------------------
 
-# fit! (KMeansTransform.jl:106-162): optional subsample (a draw of
-# max(subsample_size, k) distinct points), init, Lloyd on the GPU, and the
-# full-data E-step after a subsampled fit
+# fit! (KMeansTransform.jl:106-162): the optional subsample (max(subsample_size,
+# k) distinct points in the order drawn), the initialisation, Lloyd and, after a
+# subsampled fit, the full-data E-step all run in the library
 function fit!(t::KMeansTransform{Euclidean,Float32}, X::Matrix{Float32};
               rng::AbstractRNG = default_rng())
     d, n = size(X)
@@ -295,48 +295,20 @@ function fit!(t::KMeansTransform{Euclidean,Float32}, X::Matrix{Float32};
            t.init == :random ? Cint(2) :
            throw(ArgumentError("Unknown initialization strategy: $(t.init)"))
     seed = rand(rng, UInt64)
-    use_sub = t.subsample_size !== nothing && t.subsample_size < n
-    Xtrain = use_sub ? X[:, sort!(subsample(n, max(t.subsample_size, t.k), seed))] : X
+    sub = (t.subsample_size !== nothing && t.subsample_size < n) ? Int64(t.subsample_size) : 0
     C = zeros(Float32, d, t.k)
-    assign = zeros(Int64, size(Xtrain, 2))
+    assign = zeros(Int64, n)
     iters = Ref{Int64}(0)
-    dd = DeviceDataset(Xtrain, false)
-    try
+    with_dataset(X, Euclidean()) do dd
         GC.@preserve C assign check(ccall((:mann_kmeans_fit, LIB), Cint,
-            (Ptr{Cvoid}, Int64, Int64, Float64, Cint, UInt64, Ptr{Float32}, Ptr{Float32},
-             Ptr{Int64}, Ref{Int64}),
-            dd.h, t.k, t.max_iters, t.tol, init, seed, C_NULL, C, assign, iters))
-    finally
-        release!(dd)
-    end
-    if use_sub
-        assign = zeros(Int64, n)
-        with_dataset(X, Euclidean()) do full
-            GC.@preserve C assign check(ccall((:mann_kmeans_assign, LIB), Cint,
-                (Ptr{Cvoid}, Ptr{Float32}, Int64, Ptr{Int64}, Ptr{Float32}),
-                full.h, C, t.k, assign, C_NULL))
-        end
+            (Ptr{Cvoid}, Int64, Int64, Float64, Cint, UInt64, Ptr{Float32}, Int64,
+             Ptr{Float32}, Ptr{Int64}, Ref{Int64}),
+            dd.h, t.k, t.max_iters, t.tol, init, seed, C_NULL, sub, C, assign, iters))
     end
     t.centroids = C
     t.centroid_norms = vec(sum(abs2, C; dims = 1))
     t.pending_assignments = Vector{Int}(assign)
     return t
-end
-
-# m distinct indices of 1:n from a splitmix64-driven Fisher-Yates shuffle
-function subsample(n::Int, m::Int, seed::UInt64)
-    p = collect(1:n)
-    s = seed
-    for i in n:-1:2
-        s += 0x9E3779B97F4A7C15
-        z = s
-        z = (z ⊻ (z >> 30)) * 0xBF58476D1CE4E5B9
-        z = (z ⊻ (z >> 27)) * 0x94D049BB133111EB
-        z = z ⊻ (z >> 31)
-        j = min(Int(floor((z >> 11) * 2.0^-53 * i)) + 1, i)
-        p[i], p[j] = p[j], p[i]
-    end
-    return p[1:m]
 end
 
 # ---------------------------------------------------------------------------

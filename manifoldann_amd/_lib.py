@@ -36,7 +36,7 @@ SIGNATURES = [
     ("mann_kmeans_assign", ctypes.c_int, [_vp, _vp, _i64, _vp, _vp]),
     ("mann_kmeanspp", ctypes.c_int, [_vp, _i64, ctypes.c_uint64, _vp, _vp]),
     ("mann_kmeans_fit", ctypes.c_int, [_vp, _i64, _i64, ctypes.c_double, ctypes.c_int,
-                                       ctypes.c_uint64, _vp, _vp, _vp, _i64p]),
+                                       ctypes.c_uint64, _vp, _i64, _vp, _vp, _i64p]),
     ("mann_ivf_build", ctypes.c_int, [_vp, _vp, _i64, _vp, ctypes.POINTER(_vp)]),
     ("mann_ivf_flat_build", ctypes.c_int, [_vp, _vp, _i64, _vp, ctypes.c_int,
                                            ctypes.POINTER(_vp)]),

@@ -2,6 +2,7 @@
 // classes, host/device pointer staging, and dispatch to the pipelines.
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <set>
 #include <vector>
@@ -20,6 +21,7 @@ template <class F>
 static int guarded(F&& f) {
   try {
     g_err.clear();
+    tuning_refresh();
     f();
     return OK;
   } catch (const Error& e) {
@@ -431,8 +433,8 @@ int mann_kmeanspp(mann_dataset_t X, int64_t k, uint64_t seed, float* out_centroi
 }
 
 int mann_kmeans_fit(mann_dataset_t X, int64_t k, int64_t max_iters, double tol, int init,
-                    uint64_t seed, const float* init_centroids, float* out_centroids,
-                    int64_t* out_assign, int64_t* out_iters) {
+                    uint64_t seed, const float* init_centroids, int64_t subsample,
+                    float* out_centroids, int64_t* out_assign, int64_t* out_iters) {
   return guarded([&] {
     MANN_REQUIRE(X && out_centroids, ERR_NULL, "null pointer");
     // KMeansTransform constructor assertions (KMeansTransform.jl:66-69) and fit! (:110)
@@ -445,9 +447,29 @@ int mann_kmeans_fit(mann_dataset_t X, int64_t k, int64_t max_iters, double tol, 
     MANN_REQUIRE(init >= 0 && init <= 2, ERR_ARGUMENT,
                  "Unknown initialization strategy");
     MANN_REQUIRE(!X->normalized, ERR_ARGUMENT, "k-means runs on a Euclidean dataset");
+    MANN_REQUIRE(subsample >= 0, ERR_ARGUMENT, "subsample_size must be positive or nothing");
     ThreadCtx& c = ctx();
     hipStream_t st = c.stream;
     const int64_t n = X->n;
+    // fit on a uniform subsample of max(subsample, k) distinct points in the
+    // order drawn (X[:, randperm(rng, n)[1:sample_n]], KMeansTransform.jl:113-120),
+    // then assign every point with one E-step (:137-146)
+    const bool use_sub = subsample > 0 && subsample < n;
+    std::unique_ptr<mann_dataset> sub;
+    const mann_dataset* T = X;
+    uint64_t fit_seed = seed;
+    if (use_sub) {
+      const int64_t m = std::max<int64_t>(subsample, k);
+      std::vector<int> idx = random_indices(n, (int)m, seed);
+      int* didx = c.buf("kf_sidx").as<int>(m);
+      float* rows = c.buf("kf_srows").as<float>((size_t)m * X->d16);
+      MANN_HIP(hipMemcpyAsync(didx, idx.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+      gather_rows_f32(X->x32, didx, (int)m, (int)X->d16, rows, st);
+      sub.reset(create_dataset(rows, X->d, m, X->d16, EUCLIDEAN));
+      T = sub.get();
+      fit_seed = seed * 0x9E3779B97F4A7C15ull + 1;  // an independent stream for the fit
+    }
+    const int64_t nt = T->n;
     float* C = c.buf("kf_c").as<float>((size_t)k * X->d16);
     if (init == 0) {
       MANN_REQUIRE(init_centroids != nullptr, ERR_NULL, "null initial centroids");
@@ -455,17 +477,22 @@ int mann_kmeans_fit(mann_dataset_t X, int64_t k, int64_t max_iters, double tol, 
       MANN_HIP(hipMemcpyAsync(C, src, sizeof(float) * k * X->d16, hipMemcpyDeviceToDevice, st));
     } else if (init == 1) {
       int* picked = c.buf("pp_picked").as<int>(k);
-      kmeans_pp(*X, (int)k, seed, C, picked);
+      kmeans_pp(*T, (int)k, fit_seed, C, picked);
     } else {
-      std::vector<int> idx = random_indices(n, (int)k, seed);
+      std::vector<int> idx = random_indices(nt, (int)k, fit_seed);
       int* didx = c.buf("kf_ridx").as<int>(k);
       MANN_HIP(hipMemcpyAsync(didx, idx.data(), sizeof(int) * k, hipMemcpyHostToDevice, st));
-      gather_rows_f32(X->x32, didx, (int)k, (int)X->d16, C, st);
+      gather_rows_f32(T->x32, didx, (int)k, (int)X->d16, C, st);
     }
     int* assign = c.buf("kf_assign").as<int>(n);
     float* md = c.buf("kf_md").as<float>(n);
     long long ovf = 0;
-    int64_t iters = kmeans_lloyd(*X, C, (int)k, max_iters, tol, assign, md, &ovf);
+    int64_t iters = kmeans_lloyd(*T, C, (int)k, max_iters, tol, assign, md, &ovf);
+    if (use_sub) {
+      long long ovf2 = 0;
+      kmeans_estep(*X, C, (int)k, assign, md, false, &ovf2);
+      ovf += ovf2;
+    }
     last_stats() = SearchStats();
     last_stats().overflow = ovf;
     emit_centroids(C, k, *X, out_centroids);
@@ -885,7 +912,7 @@ int mann_tune_set(const char* key, int64_t value) {
   return guarded([&] {
     MANN_REQUIRE(key != nullptr, ERR_NULL, "null key");
     std::string k(key);
-    Tuning& t = tuning();
+    Tuning t = tuning_shared();
     if (k == "scan_cfg" || k == "ivf_cfg" || k == "kmeans_cfg" || k == "ivf_tau_cfg") {
       MANN_REQUIRE(value >= 0 && value < scan_num_cfg(), ERR_ARGUMENT, k + " out of range");
       (k == "scan_cfg"  ? t.scan_cfg
@@ -937,6 +964,7 @@ int mann_tune_set(const char* key, int64_t value) {
     } else {
       throw Error{ERR_ARGUMENT, "unknown tuning key " + k};
     }
+    tuning_publish(t);
   });
 }
 
@@ -944,7 +972,7 @@ int mann_tune_get(const char* key, int64_t* value) {
   return guarded([&] {
     MANN_REQUIRE(key && value, ERR_NULL, "null pointer");
     std::string k(key);
-    const Tuning& t = tuning();
+    const Tuning t = tuning_shared();
     if (k == "scan_cfg") *value = t.scan_cfg;
     else if (k == "ivf_cfg") *value = t.ivf_cfg;
     else if (k == "kmeans_cfg") *value = t.kmeans_cfg;

@@ -2,6 +2,7 @@
 // (BruteForceIndex query and build_knn_graph).
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "core.h"
 
@@ -75,9 +76,51 @@ bool is_device_ptr(const void* p) {
   return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
 }
 
+// The tuning knobs: one process-wide table written under a mutex by
+// mann_tune_set, and per host thread a snapshot that every API call refreshes
+// on entry, so a call reads one consistent set of knobs however other threads
+// retune meanwhile.
+namespace {
+struct TuningStore {
+  std::mutex mu;
+  Tuning table;
+  uint64_t version = 0;
+};
+TuningStore& tuning_store() {
+  static TuningStore s;
+  return s;
+}
+}  // namespace
+
 Tuning& tuning() {
-  static Tuning t;
+  thread_local Tuning t;
   return t;
+}
+
+void tuning_refresh() {
+  thread_local uint64_t seen = ~0ull;
+  TuningStore& s = tuning_store();
+  std::lock_guard<std::mutex> g(s.mu);
+  if (seen != s.version) {
+    tuning() = s.table;
+    seen = s.version;
+  }
+}
+
+Tuning tuning_shared() {
+  TuningStore& s = tuning_store();
+  std::lock_guard<std::mutex> g(s.mu);
+  return s.table;
+}
+
+void tuning_publish(const Tuning& t) {
+  TuningStore& s = tuning_store();
+  {
+    std::lock_guard<std::mutex> g(s.mu);
+    s.table = t;
+    ++s.version;
+  }
+  tuning_refresh();
 }
 
 SearchStats& last_stats() {

@@ -93,7 +93,12 @@ struct Tuning {
   int ivf_tau_mode = 1;      // IVF threshold: 0 exact FP32 row sample, 1 bf16 scan of the first list
   int sample_rows = 2048;    // exact kNN: stride-sample rows (fixed when a dataset is created)
 };
+// the calling thread's snapshot of the knobs (refreshed by tuning_refresh at
+// every API entry)
 Tuning& tuning();
+void tuning_refresh();
+Tuning tuning_shared();             // a copy of the process-wide table
+void tuning_publish(const Tuning& t);  // replace it (and refresh this thread)
 
 int scan_num_cfg();
 int scan_tile_rows(int cfg);

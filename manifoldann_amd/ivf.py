@@ -35,24 +35,6 @@ def _seed_of(rng) -> int:
     raise ArgumentError("rng must be an integer seed or a numpy Generator")
 
 
-def random_indices(n: int, k: int, seed: int) -> np.ndarray:
-    """First k entries of a Fisher-Yates permutation driven by splitmix64
-    (0-based) -- the library's :random init and subsample draw."""
-    p = np.arange(n, dtype=np.int64)
-    s = seed & (2**64 - 1)
-    M = 2**64 - 1
-    for i in range(n - 1, 0, -1):
-        s = (s + 0x9E3779B97F4A7C15) & M
-        z = s
-        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
-        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
-        z = z ^ (z >> 31)
-        u = (z >> 11) * (1.0 / 9007199254740992.0)
-        j = min(int(u * (i + 1)), i)
-        p[i], p[j] = p[j], p[i]
-    return p[:k].copy()
-
-
 # ---------------------------------------------------------------------------
 # transforms (transforms/Transform.jl, KMeansTransform.jl)
 # ---------------------------------------------------------------------------
@@ -114,34 +96,21 @@ def fit(t: KMeansTransform, X, rng=None):
     d, n = _data_shape(X)
     assert t.k <= n, f"Cannot fit {t.k} clusters with only {n} points"
     seed = _seed_of(rng)
+    # the subsample draw, the fit on it and the full assignment pass all run in
+    # the library (subsample = 0: fit on every point)
     use_sub = t.subsample_size is not None and t.subsample_size < n
-    if use_sub:
-        sample_n = max(t.subsample_size, t.k)
-        idx = random_indices(n, sample_n, seed)
-        Xa = X.data if isinstance(X, Dataset) else X
-        if isinstance(X, Dataset):
-            raise ArgumentError("subsample_size needs the data matrix, not a prepared Dataset")
-        Xt = Xa[:, idx] if not _is_torch(Xa) else Xa[:, idx.tolist()]
-        ds_train = Dataset(Xt)
-        seed = (seed * 0x9E3779B97F4A7C15 + 1) & (2**64 - 1)
-    else:
-        ds_train = _dataset_for(X, Euclidean())
+    ds = _dataset_for(X, Euclidean())
     init_code = {"kmeans_plus_plus": 1, "random": 2}[t.init]
     init_ptr = None
     if t.init_centroids is not None:
         init_rows = _centroids_rows(t.init_centroids)
         init_code, init_ptr = 0, init_rows.ctypes.data
     out_c = np.zeros((t.k, d), np.float32)
-    n_train = ds_train.n
-    assign = np.zeros(n_train, np.int64)
+    assign = np.zeros(n, np.int64)
     iters = ctypes.c_int64()
-    check(lib().mann_kmeans_fit(ds_train.handle, t.k, t.max_iters, t.tol, init_code, seed, init_ptr,
-                                out_c.ctypes.data, assign.ctypes.data, ctypes.byref(iters)))
-    if use_sub:
-        ds_full = _dataset_for(X, Euclidean())
-        assign = np.zeros(n, np.int64)
-        check(lib().mann_kmeans_assign(ds_full.handle, out_c.ctypes.data, t.k, assign.ctypes.data,
-                                       None))
+    check(lib().mann_kmeans_fit(ds.handle, t.k, t.max_iters, t.tol, init_code, seed, init_ptr,
+                                int(t.subsample_size) if use_sub else 0, out_c.ctypes.data,
+                                assign.ctypes.data, ctypes.byref(iters)))
     t.centroids = np.asfortranarray(out_c.T)
     t.centroid_norms = (out_c.astype(np.float64) ** 2).sum(1).astype(np.float32)
     t.pending_assignments = assign

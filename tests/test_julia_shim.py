@@ -96,7 +96,7 @@ def test_shim_ccalls_match_the_header():
     names = {c[0] for c in calls}
     # the methods the shim overrides reach these entry points
     for must in ("mann_bf_knn", "mann_knn_graph", "mann_knn_graph_symmetrize", "mann_kmeans_fit",
-                 "mann_kmeans_assign", "mann_ivf_build", "mann_ivf_flat_build",
+                 "mann_knn_candidates", "mann_ivf_build", "mann_ivf_flat_build",
                  "mann_ivf_search_ex", "mann_dataset_create", "mann_dataset_destroy"):
         assert must in names, must
     for name, ret, kinds in calls:
