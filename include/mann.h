@@ -66,6 +66,19 @@ int mann_dataset_create_f16(const void* data, int64_t d, int64_t n, int64_t ld, 
 int mann_dataset_destroy(mann_dataset_t ds);
 int mann_dataset_info(mann_dataset_t ds, int64_t* d, int64_t* n);
 
+/* First-pass arithmetic of the exact searches over `ds` (mann_bf_knn,
+ * mann_knn_graph, mann_knn_candidates).  No counterpart in the reference,
+ * whose searches are FP32 throughout: results are the same either way (the
+ * FP32 re-rank is certified against the bounds of the pass used); only the
+ * speed differs.  MANN_FIRST_PASS_BF16 (default) frees the FP8 copy;
+ * MANN_FIRST_PASS_FP8 builds an OCP E4M3 copy of the rows (one byte per
+ * element, d rounded up to 128) with a power-of-two scale, scanned by the
+ * gfx950 FP8 MFMA; the FP8 survivors are re-scored on bf16 before the
+ * re-rank.  The allocation-free entry point always uses bf16. */
+#define MANN_FIRST_PASS_BF16 0
+#define MANN_FIRST_PASS_FP8 1
+int mann_dataset_set_first_pass(mann_dataset_t ds, int kind);
+
 /* query(::BruteForceIndex, data, Q, k) -- batch form (src/ann_index.jl:71-101
  * over src/indices/bruteforce.jl:46-71, validation src/utils/validation.jl:8-34).
  * k_eff = min(k, n); slots past k_eff are empty.  nq = 0 is valid. */

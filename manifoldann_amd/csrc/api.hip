@@ -134,6 +134,14 @@ int mann_dataset_info(mann_dataset_t ds, int64_t* d, int64_t* n) {
   });
 }
 
+int mann_dataset_set_first_pass(mann_dataset_t ds, int kind) {
+  return guarded([&] {
+    MANN_REQUIRE(ds != nullptr, ERR_NULL, "null dataset");
+    MANN_HIP(hipSetDevice(ds->device));
+    dataset_set_first_pass(*ds, kind);
+  });
+}
+
 int mann_bf_knn(mann_dataset_t base, const float* queries, int64_t d, int64_t nq, int64_t ldq,
                 int64_t k, int metric, int64_t* out_ids, float* out_dist) {
   return guarded([&] {

@@ -13,11 +13,13 @@ namespace mann {
 // (bf16 rounding, triangle inequality), relative error rho of the FP32
 // difference form, absolute error of the cosine form, error H of the norms
 // form.  metric: EUCLIDEAN / SQEUCLIDEAN / COSINE / NORMS_FORM.
+// gcoef > 0 replaces the first-pass error (2d + 8) u by gcoef (another
+// first-pass arithmetic, e.g. FP8).
 __device__ inline double exclusion_threshold(double dk, int metric, double qe, double qr,
-                                             double ex, double rx, int d) {
+                                             double ex, double rx, int d, double gcoef = 0.0) {
   const double u = 1.0 / 16777216.0;
   double R = qr + rx;
-  double G = (2.0 * d + 8.0) * u * R * R;
+  double G = (gcoef > 0.0 ? gcoef : (2.0 * d + 8.0) * u) * R * R;
   double rho = (d + 8.0) * u;
   double L;
   if (metric == EUCLIDEAN) {

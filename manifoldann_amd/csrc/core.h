@@ -30,6 +30,17 @@ struct mann_dataset {
   // searches of the dataset against itself
   int fold = 0;
   mann::bf16_t* qfold = nullptr;
+  // FP8 (E4M3) first pass, when selected: rows x^ = e4m3(t 2^sexp8) of kp8
+  // bytes (t: the vectors of the bf16 copy before rounding), |x~|^2 (+inf on
+  // padding), |t - x~| and |x~| rounded up, and their maxima
+  int first_pass = 0;            // 0 bf16, 1 FP8
+  int64_t kp8 = 0;
+  int sexp8 = 0;
+  unsigned char* x8 = nullptr;
+  float* xn8 = nullptr;
+  float* err8 = nullptr;
+  float* nrm8 = nullptr;
+  float ex8 = 0.0f, rx8 = 0.0f;
   ~mann_dataset();
 };
 
@@ -122,6 +133,9 @@ QueryOps prepare_queries(const float* q, int64_t d, int64_t nq, int64_t ldq, int
 QueryOps dataset_as_queries(const mann_dataset& ds);
 
 mann_dataset* create_dataset(const float* data, int64_t d, int64_t n, int64_t ld, int metric);
+// select the first-pass arithmetic of the exact search: 0 bf16 (frees the FP8
+// copy), 1 FP8 E4M3 (builds it)
+void dataset_set_first_pass(mann_dataset& ds, int kind);
 
 struct SearchStats {
   long long reranked = 0, overflow = 0, pass2 = 0, exact = 0;

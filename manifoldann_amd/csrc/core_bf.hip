@@ -5,6 +5,7 @@
 #include <mutex>
 
 #include "core.h"
+#include "fp8.h"
 
 namespace mann {
 
@@ -142,7 +143,7 @@ void add_stats(const int* slots, SearchStats& s) {
 
 mann_dataset::~mann_dataset() {
   (void)hipSetDevice(device);
-  void* ps[] = {x32, xb, xn, err, nrm, xn2, sb, sxn, center, qfold, sbq};
+  void* ps[] = {x32, xb, xn, err, nrm, xn2, sb, sxn, center, qfold, sbq, x8, xn8, err8, nrm8};
   for (void* p : ps)
     if (p) (void)hipFree(p);
 }
@@ -228,6 +229,77 @@ mann_dataset* create_dataset(const float* data, int64_t d, int64_t n, int64_t ld
   return ds;
 }
 
+void dataset_set_first_pass(mann_dataset& ds, int kind) {
+  MANN_REQUIRE(kind == 0 || kind == 1, ERR_ARGUMENT, "first pass must be 0 (bf16) or 1 (FP8)");
+  ThreadCtx& c = ctx();
+  hipStream_t st = c.stream;
+  if (kind == 0) {
+    ds.first_pass = 0;
+    MANN_HIP(hipStreamSynchronize(st));
+    void* ps[] = {ds.x8, ds.xn8, ds.err8, ds.nrm8};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+    ds.x8 = nullptr;
+    ds.xn8 = ds.err8 = ds.nrm8 = nullptr;
+    return;
+  }
+  if (ds.x8) {
+    ds.first_pass = 1;
+    return;
+  }
+  const int64_t kp8 = round_up(ds.d, 128);  // even number of 64-byte chunks
+  Fp8PrepArgs p{};
+  p.x32 = ds.x32;
+  p.d16 = (int)ds.d16;
+  p.d = (int)ds.d;
+  p.kp8 = (int)kp8;
+  p.n = ds.n;
+  p.n_pad = ds.n_pad;
+  p.normalize = ds.normalized;
+  p.center = ds.center;
+  unsigned* mx = c.buf("ds8_max").as<unsigned>(3);
+  fill_i32((int*)mx, 3, 0, st);
+  launch_fp8_maxabs(p, mx + 2, st);
+  unsigned hm[3];
+  MANN_HIP(hipMemcpyAsync(hm, mx, sizeof(hm), hipMemcpyDeviceToHost, st));
+  MANN_HIP(hipStreamSynchronize(st));
+  float maxabs;
+  memcpy(&maxabs, &hm[2], 4);
+  unsigned char* x8 = nullptr;
+  float *xn8 = nullptr, *err8 = nullptr, *nrm8 = nullptr;
+  try {
+    MANN_HIP(hipMalloc(&x8, (size_t)ds.n_pad * kp8));
+    MANN_HIP(hipMalloc(&xn8, sizeof(float) * ds.n_pad));
+    MANN_HIP(hipMalloc(&err8, sizeof(float) * ds.n_pad));
+    MANN_HIP(hipMalloc(&nrm8, sizeof(float) * ds.n_pad));
+  } catch (...) {
+    void* ps[] = {x8, xn8, err8, nrm8};
+    for (void* q : ps)
+      if (q) (void)hipFree(q);
+    throw;
+  }
+  p.scale_exp = fp8_scale_exp(maxabs);
+  p.x8 = x8;
+  p.xn8 = xn8;
+  p.err8 = err8;
+  p.nrm8 = nrm8;
+  p.max_err = mx;
+  p.max_nrm = mx + 1;
+  launch_fp8_prep(p, st);
+  MANN_HIP(hipMemcpyAsync(hm, mx, sizeof(hm), hipMemcpyDeviceToHost, st));
+  MANN_HIP(hipStreamSynchronize(st));
+  MANN_HIP(hipGetLastError());
+  ds.kp8 = kp8;
+  ds.sexp8 = p.scale_exp;
+  ds.x8 = x8;
+  ds.xn8 = xn8;
+  ds.err8 = err8;
+  ds.nrm8 = nrm8;
+  memcpy(&ds.ex8, &hm[0], 4);
+  memcpy(&ds.rx8, &hm[1], 4);
+  ds.first_pass = 1;
+}
+
 QueryOps prepare_queries(const float* q, int64_t d, int64_t nq, int64_t ldq, int normalize,
                          int d16, int kp, const char* tag, const float* center) {
   ThreadCtx& c = ctx();
@@ -296,6 +368,145 @@ void run_exact_fallback(ExactArgs e, int nflag, int max_segments) {
   }
 }
 
+// Emission threshold per query from the dataset's stride sample: the rank-th
+// smallest bf16 first-pass value over the sample rows, rank chosen so that
+// about `target` rows of the whole base lie below it.
+static void sample_tau(const mann_dataset& ds, const QueryOps& Q, const bf16_t* qop, int nq,
+                       int64_t target, int cfg, float* tau) {
+  ThreadCtx& c = ctx();
+  hipStream_t st = c.stream;
+  const int n = (int)ds.n;
+  const int BM = scan_tile_rows(cfg), BN = scan_tile_queries(cfg);
+  const int rank = (int)std::max<int64_t>(1, std::min<int64_t>(ds.m, (target * ds.m + n / 2) / n));
+  // a long sample (rank far below the number of 32-row blocks): only the
+  // block minima are stored; the rank-th smallest of them equals the
+  // sample's rank-th smallest unless two of its rank smallest values share
+  // a block, and is never below it (a threshold estimate either way)
+  const bool bmin = ds.m / 32 >= 64 * rank;
+  const int mcols = bmin ? (int)ceil_div(ds.m, 32) : ds.m;
+  // the dense sample block is processed in query chunks of at most 2^28 values
+  int64_t qc_max = std::max<int64_t>(BN, ((int64_t)1 << 28) / mcols / BN * BN);
+  if (c.arena)  // the sample block in query chunks the scratch holds
+    qc_max = std::max<int64_t>(
+        BN, std::min<int64_t>(qc_max, (int64_t)(c.arena->left() / 2 / sizeof(float)) / mcols /
+                                          BN * BN));
+  const int qc = (int)std::min<int64_t>(nq, qc_max);
+  float* dense = c.buf("bf_dense").as<float>((size_t)qc * mcols);
+  // self-join: the rows are the queries and the sample rows the query
+  // columns, so each store instruction writes whole 128-byte row segments of
+  // the [query][sample] block (S^ is the same quantity either way round)
+  const bool swap = Q.qb == ds.xb && !bmin;
+  for (int q0 = 0; q0 < nq; q0 += qc) {
+    const int nqc = std::min(qc, nq - q0);
+    if (swap) {
+      ScanArgs a{};
+      a.cfg = cfg;
+      a.xb = ds.xb + (int64_t)q0 * ds.kp;
+      a.xn = ds.xn + q0;
+      a.qb = ds.fold ? ds.sbq : ds.sb;
+      a.qnorm = ds.sxn;
+      a.fold = ds.fold;
+      a.kp = (int)ds.kp;
+      a.d = (int)ds.d;
+      a.nq = ds.m;
+      const int qt = (int)ceil_div(ds.m, BN);
+      const int pd = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nqc, BM),
+                                                                  ceil_div(256, qt)));
+      a.rows_per_part = (int)round_up(ceil_div(nqc, pd), BM);
+      a.parts = (int)ceil_div(nqc, a.rows_per_part);
+      a.qtiles = qt;
+      a.n_rows = nqc;
+      a.dense = dense;
+      a.ldd = ds.m;
+      a.dense_t = 1;
+      {
+        ScopedTime tm("scan_dense", st);
+        launch_scan(a, SCAN_DENSE, qt * a.parts, SCAN_KIND_GRID, st);
+      }
+      TauArgs ta{dense, (int64_t)ds.m, ds.m, rank, tau + q0, nqc};
+      ScopedTime tm("tau_sample", st);
+      launch_tau_from_sample(ta, st);
+      continue;
+    }
+    ScanArgs a{};
+    a.cfg = cfg;
+    a.xb = ds.sb;
+    a.xn = ds.sxn;
+    a.qb = qop + (int64_t)q0 * ds.kp;
+    a.qnorm = Q.qn + q0;
+    a.fold = ds.fold;
+    a.kp = (int)ds.kp;
+    a.d = (int)ds.d;
+    a.nq = nqc;
+    // row partitions only when the query chunk has too few tiles to fill the
+    // chip (a large sample; each partition re-reads the chunk's queries)
+    const int qt = (int)ceil_div(nqc, BN);
+    const int pd = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(ds.m, BM),
+                                                                ceil_div(256, qt)));
+    a.rows_per_part = (int)round_up(ceil_div(ds.m, pd), BM);
+    a.parts = (int)ceil_div(ds.m, a.rows_per_part);
+    a.qtiles = qt;
+    a.n_rows = ds.m;
+    a.dense = dense;
+    a.ldd = mcols;
+    a.dense_bmin = bmin;
+    a.dvec = mcols % 4 == 0;  // row partitions start at multiples of BM
+    {
+      ScopedTime tm("scan_dense", st);
+      launch_scan(a, SCAN_DENSE, qt * a.parts, SCAN_KIND_GRID, st);
+    }
+    TauArgs ta{dense, (int64_t)mcols, mcols, rank, tau + q0, nqc};
+    ScopedTime tm("tau_sample", st);
+    launch_tau_from_sample(ta, st);
+  }
+}
+
+// FP8 operands of a query batch (scratch of the calling thread).
+struct QueryOps8 {
+  const unsigned char* q8 = nullptr;
+  const float* qn8 = nullptr;
+  const float* qe8 = nullptr;
+  const float* qr8 = nullptr;
+  const float* qmul = nullptr;
+};
+
+static QueryOps8 prepare_queries_fp8(const mann_dataset& ds, const QueryOps& Q) {
+  ThreadCtx& c = ctx();
+  const int64_t nq_pad = round_up(Q.nq, 256);
+  QueryOps8 R;
+  unsigned char* q8 = c.buf("bf_q8").as<unsigned char>((size_t)nq_pad * ds.kp8);
+  float* qn8 = c.buf("bf_qn8").as<float>(nq_pad);
+  float* qe8 = c.buf("bf_qe8").as<float>(nq_pad);
+  float* qr8 = c.buf("bf_qr8").as<float>(nq_pad);
+  float* qmul = c.buf("bf_qmul").as<float>(nq_pad);
+  Fp8PrepArgs p{};
+  p.x32 = Q.q32;
+  p.d16 = (int)ds.d16;
+  p.d = (int)ds.d;
+  p.kp8 = (int)ds.kp8;
+  p.n = Q.nq;
+  p.n_pad = nq_pad;
+  p.normalize = ds.normalized;
+  p.center = ds.center;
+  p.scale_exp = -1001;  // a scale per query
+  p.other_exp = ds.sexp8;
+  p.x8 = q8;
+  p.xn8 = qn8;
+  p.err8 = qe8;
+  p.nrm8 = qr8;
+  p.rowmul = qmul;
+  {
+    ScopedTime tm("prep_queries_fp8", c.stream);
+    launch_fp8_prep(p, c.stream);
+  }
+  R.q8 = q8;
+  R.qn8 = qn8;
+  R.qe8 = qe8;
+  R.qr8 = qr8;
+  R.qmul = qmul;
+  return R;
+}
+
 void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int exclude_self,
                int64_t* out_ids, float* out_dist, int* out_count) {
   ThreadCtx& c = ctx();
@@ -304,6 +515,8 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   S = SearchStats();
   const int nq = (int)Q.nq;
   if (nq == 0 || k <= 0) return;
+  // FP8 first pass (selected on the dataset; not in allocation-free calls)
+  const bool fp8 = ds.first_pass == 1 && ds.x8 != nullptr && c.arena == nullptr;
   // folded-norm query operand: -2 q~ and the parts of |q~|^2 (see mann_dataset)
   const bf16_t* qop = Q.qb;
   if (ds.fold && Q.qb == ds.xb && ds.qfold) {
@@ -317,13 +530,18 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   const int n = (int)ds.n;
   const int k_req = k + (exclude_self ? 1 : 0);
   const int k_eff = std::min(k_req, n);
-  const int cap_total = select_cap_max();
   MANN_REQUIRE(k_req <= select_exmax() / 2, ERR_ARGUMENT,
                "k = " + std::to_string(k) + " exceeds the GPU search's maximum of " +
                    std::to_string(select_exmax() / 2 - (exclude_self ? 1 : 0)) +
                    " neighbours per query");
   const int cfg = tuning().scan_cfg;
-  const int BM = scan_tile_rows(cfg), BN = scan_tile_queries(cfg);
+  // first-pass tiles (the bf16 sample pass keeps the bf16 configuration)
+  const int fcfg = fp8 ? std::min(std::max(tuning().scan8_cfg, 0), scan8_num_cfg() - 1) : cfg;
+  const int BM = fp8 ? scan8_tile_rows(fcfg) : scan_tile_rows(cfg);
+  const int BN = fp8 ? scan8_tile_queries(fcfg) : scan_tile_queries(cfg);
+  // candidates per query over all regions: what the selection holds, or for
+  // FP8 what the bf16 re-score holds (it passes the smallest to the selection)
+  const int cap_total = fp8 ? rescore_cap() : select_cap_max();
   // row partitions: enough workgroups to fill the chip, at most 32 regions
   const int qtiles = (int)ceil_div(nq, BN);
   const int row_tiles = (int)ceil_div(n, BM);
@@ -351,113 +569,64 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   fill_i32(flags, 8, 0, st);
   fill_i32(stats, kStatSlots * 4, 0, st);
 
+  QueryOps8 Q8;
+  if (fp8) Q8 = prepare_queries_fp8(ds, Q);
   // ---- emission threshold per query (every row fits its region: emit all)
   if (rpp <= cap) {
     fill_f32(tau, nq, INFINITY, st);
+  } else if (fp8) {
+    // a bf16 threshold near the (fp8_emit_per_k k_req)-th row, carried into
+    // the FP8 domain by both bands
+    const int64_t target =
+        std::min<int64_t>((int64_t)tuning().fp8_emit_per_k * k_req, cap_total / 4);
+    float* tau16 = c.buf("bf_tau16").as<float>(nq);
+    sample_tau(ds, Q, qop, nq, target, cfg, tau16);
+    Fp8TauArgs ta{};
+    ta.nq = nq;
+    ta.d = (int)ds.d + (ds.fold ? 3 : 0);
+    ta.tau16 = tau16;
+    ta.qe16 = Q.qe;
+    ta.qr16 = Q.qr;
+    ta.ex16 = ds.ex;
+    ta.rx16 = ds.rx;
+    ta.qe8 = Q8.qe8;
+    ta.qr8 = Q8.qr8;
+    ta.ex8 = ds.ex8;
+    ta.rx8 = ds.rx8;
+    ta.tau8 = tau;
+    launch_fp8_tau(ta, st);
   } else {
     // threshold rank in the sample: the sample's r-th smallest value sits near
     // the (r n / m)-th smallest row; aim at about 15 k_req emissions per query,
     // bounded by half the candidate capacity
-    const int64_t target = std::min<int64_t>((int64_t)tuning().sample_rank * tuning().emit_per_k * k_req,
-                                             cap_total / 2);
-    const int rank = (int)std::max<int64_t>(
-        1, std::min<int64_t>(ds.m, (target * ds.m + n / 2) / n));
-    // a long sample (rank far below the number of 32-row blocks): only the
-    // block minima are stored; the rank-th smallest of them equals the
-    // sample's rank-th smallest unless two of its rank smallest values share
-    // a block, and is never below it (a threshold estimate either way)
-    const bool bmin = ds.m / 32 >= 64 * rank;
-    const int mcols = bmin ? (int)ceil_div(ds.m, 32) : ds.m;
-    // the dense sample block is processed in query chunks of at most 2^28 values
-    int64_t qc_max = std::max<int64_t>(BN, ((int64_t)1 << 28) / mcols / BN * BN);
-    if (c.arena)  // the sample block in query chunks the scratch holds
-      qc_max = std::max<int64_t>(
-          BN, std::min<int64_t>(qc_max, (int64_t)(c.arena->left() / 2 / sizeof(float)) / mcols /
-                                            BN * BN));
-    const int qc = (int)std::min<int64_t>(nq, qc_max);
-    float* dense = c.buf("bf_dense").as<float>((size_t)qc * mcols);
-    // self-join: the rows are the queries and the sample rows the query
-    // columns, so each store instruction writes whole 128-byte row segments of
-    // the [query][sample] block (S^ is the same quantity either way round)
-    const bool swap = Q.qb == ds.xb && !bmin;
-    for (int q0 = 0; q0 < nq; q0 += qc) {
-      const int nqc = std::min(qc, nq - q0);
-      if (swap) {
-        ScanArgs a{};
-        a.cfg = cfg;
-        a.xb = ds.xb + (int64_t)q0 * ds.kp;
-        a.xn = ds.xn + q0;
-        a.qb = ds.fold ? ds.sbq : ds.sb;
-        a.qnorm = ds.sxn;
-        a.fold = ds.fold;
-        a.kp = (int)ds.kp;
-        a.d = (int)ds.d;
-        a.nq = ds.m;
-        const int qt = (int)ceil_div(ds.m, BN);
-        const int pd = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nqc, BM),
-                                                                    ceil_div(256, qt)));
-        a.rows_per_part = (int)round_up(ceil_div(nqc, pd), BM);
-        a.parts = (int)ceil_div(nqc, a.rows_per_part);
-        a.qtiles = qt;
-        a.n_rows = nqc;
-        a.dense = dense;
-        a.ldd = ds.m;
-        a.dense_t = 1;
-        {
-          ScopedTime tm("scan_dense", st);
-          launch_scan(a, SCAN_DENSE, qt * a.parts, SCAN_KIND_GRID, st);
-        }
-        TauArgs ta{dense, (int64_t)ds.m, ds.m, rank, tau + q0, nqc};
-        ScopedTime tm("tau_sample", st);
-        launch_tau_from_sample(ta, st);
-        continue;
-      }
-      ScanArgs a{};
-      a.cfg = cfg;
-      a.xb = ds.sb;
-      a.xn = ds.sxn;
-      a.qb = qop + (int64_t)q0 * ds.kp;
-      a.qnorm = Q.qn + q0;
-      a.fold = ds.fold;
-      a.kp = (int)ds.kp;
-      a.d = (int)ds.d;
-      a.nq = nqc;
-      // row partitions only when the query chunk has too few tiles to fill the
-      // chip (a large sample; each partition re-reads the chunk's queries)
-      const int qt = (int)ceil_div(nqc, BN);
-      const int pd = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(ds.m, BM),
-                                                                  ceil_div(256, qt)));
-      a.rows_per_part = (int)round_up(ceil_div(ds.m, pd), BM);
-      a.parts = (int)ceil_div(ds.m, a.rows_per_part);
-      a.qtiles = qt;
-      a.n_rows = ds.m;
-      a.dense = dense;
-      a.ldd = mcols;
-      a.dense_bmin = bmin;
-      a.dvec = mcols % 4 == 0;  // row partitions start at multiples of BM
-      {
-        ScopedTime tm("scan_dense", st);
-        launch_scan(a, SCAN_DENSE, qt * a.parts, SCAN_KIND_GRID, st);
-      }
-      TauArgs ta{dense, (int64_t)mcols, mcols, rank, tau + q0, nqc};
-      ScopedTime tm("tau_sample", st);
-      launch_tau_from_sample(ta, st);
-    }
+    const int64_t target = std::min<int64_t>(
+        (int64_t)tuning().sample_rank * tuning().emit_per_k * k_req, cap_total / 2);
+    sample_tau(ds, Q, qop, nq, target, cfg, tau);
   }
 
   // ---- filtered first pass over all rows
   ScanArgs a{};
-  a.cfg = cfg;
-  a.xb = ds.xb;
-  a.xn = ds.xn;
-  a.qb = qop;
-  a.qnorm = Q.qn;
-  a.fold = ds.fold;
+  if (fp8) {
+    a.cfg = fcfg;
+    a.x8 = ds.x8;
+    a.q8 = Q8.q8;
+    a.qmul = Q8.qmul;
+    a.xn = ds.xn8;
+    a.qnorm = Q8.qn8;
+    a.kp = (int)ds.kp8;
+  } else {
+    a.cfg = cfg;
+    a.xb = ds.xb;
+    a.xn = ds.xn;
+    a.qb = qop;
+    a.qnorm = Q.qn;
+    a.fold = ds.fold;
+    a.kp = (int)ds.kp;
+  }
   a.xcd_map = tuning().xcd_map;
   a.xcd_band = tuning().xcd_band;
   a.cpol = tuning().scan_cpol;
   a.qtiles = qtiles;
-  a.kp = (int)ds.kp;
   a.d = (int)ds.d;
   a.nq = nq;
   a.parts = parts;
@@ -467,31 +636,73 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   a.cand = cand;
   a.cnt = cnt;
   a.cap = cap;
-  {
-    ScopedTime tm("scan_filter", st);
-    launch_scan(a, SCAN_FILTER, qtiles * parts, SCAN_KIND_GRID, st);
+  auto first_pass = [&](const char* name, int grid, int kind) {
+    ScopedTime tm(name, st);
+    if (fp8) launch_scan8(a, grid, kind, st);
+    else launch_scan(a, SCAN_FILTER, grid, kind, st);
+  };
+  first_pass("scan_filter", qtiles * parts, SCAN_KIND_GRID);
+
+  // FP8: re-score the survivors on bf16 into one region per query
+  const int keep = select_cap_max();
+  unsigned* cand_sel = cand;
+  int* cnt_sel = cnt;
+  float* dropmin = nullptr;
+  RescoreArgs ra{};
+  if (fp8) {
+    cand_sel = c.buf("bf_cand_sel").as<unsigned>((size_t)nq * keep * 2);
+    cnt_sel = c.buf("bf_cnt_sel").as<int>(nq);
+    dropmin = c.buf("bf_dropmin").as<float>(nq);
+    ra.nq = nq;
+    ra.cand8 = cand;
+    ra.cnt8 = cnt;
+    ra.cap8 = cap;
+    ra.parts8 = parts;
+    ra.xb = ds.xb;
+    ra.xn = ds.xn;
+    ra.qb = Q.qb;
+    ra.qn = Q.qn;
+    ra.kp = (int)ds.kp;
+    ra.d = (int)ds.d;
+    ra.cand = cand_sel;
+    ra.cnt = cnt_sel;
+    ra.keep = keep;
+    ra.dropmin = dropmin;
+    ra.stats = stats;
+    ScopedTime tm("rescore_bf16", st);
+    launch_rescore(ra, nq, st);
   }
 
   SelectArgs s{};
   s.nq = nq;
-  s.cand = cand;
-  s.cnt = cnt;
-  s.cap = cap;
-  s.parts = parts;
+  s.cand = cand_sel;
+  s.cnt = cnt_sel;
+  s.cap = fp8 ? keep : cap;
+  s.parts = fp8 ? 1 : parts;
   s.tau = tau;
   s.x32 = ds.x32;
   s.q32 = Q.q32;
   s.xn2 = ds.xn2;
   s.qn2 = Q.qn2;
   // certification band: with folded norms the MFMA sums d + 6 terms (the
-  // bound of d + 3 covers them and the FP32 rounding of the two norms)
-  s.d = (int)ds.d + (ds.fold ? 3 : 0);
+  // bound of d + 3 covers them and the FP32 rounding of the two norms); the
+  // bf16 re-score of the FP8 path sums the d products only
+  s.d = (int)ds.d + (ds.fold && !fp8 ? 3 : 0);
   s.d16 = (int)ds.d16;
   s.metric = metric;
   s.qe = Q.qe;
   s.qr = Q.qr;
   s.ex = ds.ex;
   s.rx = ds.rx;
+  if (fp8) {
+    s.cert_qe = Q8.qe8;
+    s.cert_qr = Q8.qr8;
+    s.cert_ex = ds.ex8;
+    s.cert_rx = ds.rx8;
+    s.cert_g = kFp8G;
+    s.dropmin = dropmin;
+    s.no_emit_stat = 1;
+  }
   s.k = k;
   s.k_req = k_req;
   s.exclude_self = exclude_self;
@@ -516,12 +727,18 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   a.parts = parts2;
   a.rows_per_part = rpp2;
   a.cap = cap2;
-  {
-    ScopedTime tm("scan_filter_pass2", st);
-    launch_scan(a, SCAN_FILTER, qtiles * parts2, SCAN_KIND_REFINE, st);
+  first_pass("scan_filter_pass2", qtiles * parts2, SCAN_KIND_REFINE);
+  if (fp8) {
+    ra.qlist = list1;
+    ra.nq_dev = flags;
+    ra.cap8 = cap2;
+    ra.parts8 = parts2;
+    ScopedTime tm("rescore_bf16_pass2", st);
+    launch_rescore(ra, nq, st);
+  } else {
+    s.parts = parts2;
+    s.cap = cap2;
   }
-  s.parts = parts2;
-  s.cap = cap2;
   s.qlist = list1;
   s.nq_dev = flags;
   s.tau = tau2;

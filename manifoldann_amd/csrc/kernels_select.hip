@@ -340,6 +340,16 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(SelectArgs a) {
     }
     return T;
   };
+  // exclusion threshold in the first pass's own domain (cert_* bounds)
+  auto cert_threshold = [&](int n) -> double {
+    double T = INFINITY;
+    if (n >= k_eff && k_eff > 0) {
+      const uint64_t kk = ex[k_eff - 1];
+      T = exclusion_threshold(ord2f((uint32_t)(kk >> 32)), a.metric, a.cert_qe[q],
+                              a.cert_qr[q], a.cert_ex, a.cert_rx, a.d, a.cert_g);
+    }
+    return T;
+  };
   sort_ex(n1);
   double T = threshold(n1);
   int n = n1;
@@ -371,23 +381,36 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(SelectArgs a) {
   const float tq = a.tau[q];
   const bool bad = overflow || exovf;
   bool certified;
-  if (bad) certified = false;
-  else if (tq == INFINITY) certified = true;
-  else certified = (n >= k_eff) && ((double)tq > T);
+  double Tc = T;  // exclusion threshold in the domain of tau
+  if (a.cert_qe) {
+    // first pass in its own domain: its threshold covers its exclusion
+    // threshold, and every re-scored value left out lies above the bf16 one
+    Tc = cert_threshold(n);
+    const double dm = (double)a.dropmin[q];
+    certified = !bad && n >= k_eff && (tq == INFINITY || (double)tq > Tc) && dm > T;
+  } else if (bad) {
+    certified = false;
+  } else if (tq == INFINITY) {
+    certified = true;
+  } else {
+    certified = (n >= k_eff) && ((double)tq > T);
+  }
   if (lane == 0) {
     if (!certified) {
       const int f = atomicAdd(a.flag_count, 1);
       a.flag_list[f] = q;
       float tn = INFINITY;
-      if (n >= k_eff && T < 3.0e38) tn = nextafterf((float)T, INFINITY);
-      if (!(tn > tq) && !overflow) tn = INFINITY;  // no progress possible: emit everything
+      if (n >= k_eff && Tc < 3.0e38) tn = nextafterf((float)Tc, INFINITY);
+      // no progress possible: emit everything (with a separate first pass, a
+      // tighter threshold is progress too: it drops candidates)
+      if (!a.cert_qe && !(tn > tq) && !overflow) tn = INFINITY;
       a.tau_new[q] = tn;
     }
     if (a.stats) {
       int* sp = a.stats + (slot % kStatSlots) * 4;
       atomicAdd(sp, n);
       if (bad) atomicAdd(sp + 1, 1);
-      atomicAdd(sp + 2, c);
+      if (!a.no_emit_stat) atomicAdd(sp + 2, c);
     }
   }
   const int lim = n < a.k_req ? n : a.k_req;

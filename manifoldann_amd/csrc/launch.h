@@ -70,6 +70,11 @@ struct ScanArgs {
   // spare K columns and the query side is -2 q~, so the MFMA accumulates S^
   // itself; xn and qnorm are not read
   int fold;
+  // FP8 first pass (launch_scan8): E4M3 rows of kp bytes, and per query the
+  // multiplier -2 * 2^-(s_x + s_q) of its accumulators
+  const unsigned char* x8;
+  const unsigned char* q8;
+  const float* qmul;
 };
 
 // Run-time tuning knobs (set through mann_tune_set; chosen by the autotuner).
@@ -92,6 +97,9 @@ struct Tuning {
   int tau_rows = 1024;       // IVF: exact rows sampled per query for the emission threshold
   int ivf_tau_mode = 1;      // IVF threshold: 0 exact FP32 row sample, 1 bf16 scan of the first list
   int sample_rows = 2048;    // exact kNN: stride-sample rows (fixed when a dataset is created)
+  int scan8_cfg = 0;         // FP8 first-pass tile config (table of kernels_fp8.hip)
+  int fp8_emit_per_k = 2;    // FP8 first pass: bf16 sample rank aimed at fp8_emit_per_k * k_req
+                             // rows, then widened by the FP8 band
 };
 // the calling thread's snapshot of the knobs (refreshed by tuning_refresh at
 // every API entry)
@@ -176,6 +184,18 @@ struct SelectArgs {
   int* flag_list;
   float* tau_new;      // refined threshold for a second pass
   int* stats;          // [kStatSlots][4]: reranked, overflowed queries, emitted (see below)
+  // First pass in another domain (FP8 candidates re-scored on bf16): tau is
+  // then that pass's threshold, certified against the exclusion threshold of
+  // its own bounds (cert_*: |q - q~|, |q~| per query, dataset maxima, error
+  // coefficient G / (|q~| + |x~|)^2), and tau_new is in its domain too.
+  // dropmin: per query, a lower bound of the re-scored values left out of the
+  // candidates (they must lie above the bf16 exclusion threshold).
+  const float* cert_qe;
+  const float* cert_qr;
+  float cert_ex, cert_rx;
+  double cert_g;
+  const float* dropmin;
+  int no_emit_stat;    // emitted counts are recorded by the re-score instead
 };
 enum ExtraMetric : int { NORMS_FORM = 3 };
 // one wavefront per query; nq_max bounds the device-side query count

@@ -128,7 +128,9 @@ class Dataset:
     is this handle, created once and reused across calls.
     """
 
-    def __init__(self, data, metric=default_distance):
+    FIRST_PASS = {"bf16": 0, "fp8": 1}
+
+    def __init__(self, data, metric=default_distance, first_pass="bf16"):
         h = ctypes.c_void_p()
         self.metric = metric
         if _is_f16(data):  # Matrix{Float16}: upcast exactly on the device
@@ -140,6 +142,18 @@ class Dataset:
             check(lib().mann_dataset_create(m.ptr, m.d, m.n, m.ld, metric.code, ctypes.byref(h)))
         self.d, self.n = m.d, m.n
         self.handle = h
+        self.first_pass = "bf16"
+        if first_pass != "bf16":
+            self.set_first_pass(first_pass)
+
+    def set_first_pass(self, kind):
+        """Arithmetic of the exact searches' first pass: "bf16" or "fp8" (OCP
+        E4M3 copy of the rows, one byte per element).  Results are identical
+        either way; only the speed differs."""
+        if kind not in self.FIRST_PASS:
+            raise ArgumentError(f"first_pass must be one of {sorted(self.FIRST_PASS)}")
+        check(lib().mann_dataset_set_first_pass(self.handle, self.FIRST_PASS[kind]))
+        self.first_pass = kind
 
     def __del__(self):
         # at interpreter exit the module globals may already be gone
