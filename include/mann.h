@@ -7,6 +7,10 @@
  *    each d contiguous floats, point stride `ld` (>= d) in elements.
  *  - Pointers may be host memory (staged through the call) or device memory
  *    of the current HIP device (used in place); the library checks which.
+ *    Device memory is read on the calling thread's own non-blocking stream:
+ *    the work that produced an input must be complete when the call is made
+ *    (synchronize the producing stream first).  Outputs are complete when
+ *    the call returns.
  *  - Ids are 1-based int64 like Julia's Neighbor.id; 0 marks an empty slot.
  *    Per-query results are k consecutive slots (a k x nq column-major matrix,
  *    the layout of benchmarking/wrappers/manifoldann.py:57-70), ascending
@@ -304,7 +308,10 @@ int mann_last_emitted(int64_t* total);
  * produces the first-pass distances itself; read when a dataset is created),
  * "sample_rows" (exact kNN: minimum
  * rows of the stride sample a dataset keeps for the emission threshold; read
- * when the dataset is created).  mann_tune_get also reads
+ * when the dataset is created), "scan8_cfg" (FP8 first-pass tile
+ * configuration, 0..4), "fp8_emit_per_k" (FP8 first pass: the bf16 sample
+ * rank it starts from, per neighbour requested, before the FP8 band widens
+ * it).  mann_tune_get also reads
  * "last_heap_replays": the queries of the last IVF-Flat search on this thread
  * whose k-th place had a distance tie and went through the bounded-heap merge
  * (read only).  No reference counterpart: the reference has no tunable

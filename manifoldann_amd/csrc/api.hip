@@ -9,6 +9,7 @@
 
 #include "../../include/mann.h"
 #include "core.h"
+#include "fp8.h"
 #include "ivf.h"
 #include "mlp.h"
 
@@ -969,6 +970,12 @@ int mann_tune_set(const char* key, int64_t value) {
     } else if (k == "emit_per_k") {
       MANN_REQUIRE(value >= 1 && value <= 64, ERR_ARGUMENT, "emit_per_k must be in 1..64");
       t.emit_per_k = (int)value;
+    } else if (k == "scan8_cfg") {
+      MANN_REQUIRE(value >= 0 && value < scan8_num_cfg(), ERR_ARGUMENT, "scan8_cfg out of range");
+      t.scan8_cfg = (int)value;
+    } else if (k == "fp8_emit_per_k") {
+      MANN_REQUIRE(value >= 1 && value <= 64, ERR_ARGUMENT, "fp8_emit_per_k must be in 1..64");
+      t.fp8_emit_per_k = (int)value;
     } else {
       throw Error{ERR_ARGUMENT, "unknown tuning key " + k};
     }
@@ -998,6 +1005,8 @@ int mann_tune_get(const char* key, int64_t* value) {
     else if (k == "sample_rows") *value = t.sample_rows;
     else if (k == "emit_per_k") *value = t.emit_per_k;
     else if (k == "fold_norms") *value = t.fold_norms;
+    else if (k == "scan8_cfg") *value = t.scan8_cfg;
+    else if (k == "fp8_emit_per_k") *value = t.fp8_emit_per_k;
     else if (k == "last_heap_replays") *value = last_stats().heap_replays;
     else throw Error{ERR_ARGUMENT, "unknown tuning key " + k};
   });

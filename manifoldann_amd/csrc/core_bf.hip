@@ -371,18 +371,24 @@ void run_exact_fallback(ExactArgs e, int nflag, int max_segments) {
 // Emission threshold per query from the dataset's stride sample: the rank-th
 // smallest bf16 first-pass value over the sample rows, rank chosen so that
 // about `target` rows of the whole base lie below it.
+// target2 > 0: a second threshold (tau2) for target2 rows from the same sample.
 static void sample_tau(const mann_dataset& ds, const QueryOps& Q, const bf16_t* qop, int nq,
-                       int64_t target, int cfg, float* tau) {
+                       int64_t target, int cfg, float* tau, int64_t target2 = 0,
+                       float* tau2 = nullptr) {
   ThreadCtx& c = ctx();
   hipStream_t st = c.stream;
   const int n = (int)ds.n;
   const int BM = scan_tile_rows(cfg), BN = scan_tile_queries(cfg);
-  const int rank = (int)std::max<int64_t>(1, std::min<int64_t>(ds.m, (target * ds.m + n / 2) / n));
+  auto rank_of = [&](int64_t t) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>(ds.m, (t * ds.m + n / 2) / n));
+  };
+  const int rank = rank_of(target);
+  const int rank2 = tau2 ? rank_of(target2) : 0;
   // a long sample (rank far below the number of 32-row blocks): only the
   // block minima are stored; the rank-th smallest of them equals the
   // sample's rank-th smallest unless two of its rank smallest values share
   // a block, and is never below it (a threshold estimate either way)
-  const bool bmin = ds.m / 32 >= 64 * rank;
+  const bool bmin = ds.m / 32 >= 64 * std::max(rank, rank2);
   const int mcols = bmin ? (int)ceil_div(ds.m, 32) : ds.m;
   // the dense sample block is processed in query chunks of at most 2^28 values
   int64_t qc_max = std::max<int64_t>(BN, ((int64_t)1 << 28) / mcols / BN * BN);
@@ -426,6 +432,10 @@ static void sample_tau(const mann_dataset& ds, const QueryOps& Q, const bf16_t* 
       TauArgs ta{dense, (int64_t)ds.m, ds.m, rank, tau + q0, nqc};
       ScopedTime tm("tau_sample", st);
       launch_tau_from_sample(ta, st);
+      if (tau2) {
+        TauArgs tb{dense, (int64_t)ds.m, ds.m, rank2, tau2 + q0, nqc};
+        launch_tau_from_sample(tb, st);
+      }
       continue;
     }
     ScanArgs a{};
@@ -458,6 +468,10 @@ static void sample_tau(const mann_dataset& ds, const QueryOps& Q, const bf16_t* 
     TauArgs ta{dense, (int64_t)mcols, mcols, rank, tau + q0, nqc};
     ScopedTime tm("tau_sample", st);
     launch_tau_from_sample(ta, st);
+    if (tau2) {
+      TauArgs tb{dense, (int64_t)mcols, mcols, rank2, tau2 + q0, nqc};
+      launch_tau_from_sample(tb, st);
+    }
   }
 }
 
@@ -559,7 +573,7 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   const int cap2 = cap_total / parts2;
 
   unsigned* cand = c.buf("bf_cand").as<unsigned>((size_t)nq * cap_total * 2);
-  int* cnt = c.buf("bf_cnt").as<int>((size_t)nq * std::max(parts, parts2));
+  int* cnt = c.buf("bf_cnt").as<int>((size_t)nq * std::max(std::max(parts, parts2), 64));
   float* tau = c.buf("bf_tau").as<float>(nq);
   float* tau2 = c.buf("bf_tau2").as<float>(nq);
   int* flags = c.buf("bf_flags").as<int>(8);   // [0] pass1 count, [1] pass2 count
@@ -570,6 +584,7 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   fill_i32(stats, kStatSlots * 4, 0, st);
 
   QueryOps8 Q8;
+  float* tau16w = nullptr;  // FP8: bf16 sample threshold for the bf16 second pass
   if (fp8) Q8 = prepare_queries_fp8(ds, Q);
   // ---- emission threshold per query (every row fits its region: emit all)
   if (rpp <= cap) {
@@ -579,8 +594,13 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
     // the FP8 domain by both bands
     const int64_t target =
         std::min<int64_t>((int64_t)tuning().fp8_emit_per_k * k_req, cap_total / 4);
+    // and the bf16 path's own threshold for the bf16 second pass (used when
+    // tighter than the one from the FP8 pass's k-th distance)
+    const int64_t target16 = std::min<int64_t>(
+        (int64_t)tuning().sample_rank * tuning().emit_per_k * k_req, select_cap_max() / 2);
     float* tau16 = c.buf("bf_tau16").as<float>(nq);
-    sample_tau(ds, Q, qop, nq, target, cfg, tau16);
+    tau16w = c.buf("bf_tau16w").as<float>(nq);
+    sample_tau(ds, Q, qop, nq, target, cfg, tau16, target16, tau16w);
     Fp8TauArgs ta{};
     ta.nq = nq;
     ta.d = (int)ds.d + (ds.fold ? 3 : 0);
@@ -702,6 +722,8 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
     s.cert_g = kFp8G;
     s.dropmin = dropmin;
     s.no_emit_stat = 1;
+    s.tau_new_bf16 = 1;  // the second pass runs on bf16
+    s.tau_alt = tau16w;
   }
   s.k = k;
   s.k_req = k_req;
@@ -720,25 +742,66 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
     launch_select_rerank(s, nq, st);
   }
 
-  // ---- second pass for queries that failed certification (device-side count)
-  a.qidx = list1;
-  a.nq_dev = flags;
-  a.tau = tau2;
-  a.parts = parts2;
-  a.rows_per_part = rpp2;
-  a.cap = cap2;
-  first_pass("scan_filter_pass2", qtiles * parts2, SCAN_KIND_REFINE);
+  // ---- second pass for queries that failed certification (device-side
+  // count), always on bf16: after an FP8 first pass its threshold is the bf16
+  // exclusion threshold of the k-th exact distance the FP8 pass found (the
+  // much narrower bf16 band resolves what the FP8 band could not hold)
   if (fp8) {
-    ra.qlist = list1;
-    ra.nq_dev = flags;
-    ra.cap8 = cap2;
-    ra.parts8 = parts2;
-    ScopedTime tm("rescore_bf16_pass2", st);
-    launch_rescore(ra, nq, st);
+    const int BM16 = scan_tile_rows(cfg), BN16 = scan_tile_queries(cfg);
+    const int rt16 = (int)ceil_div(n, BM16);
+    // at most 48 regions: the threshold may be the sample's (hundreds of rows
+    // per query), and each region holds 1/parts of the candidates
+    const int rp2 = (int)round_up(ceil_div(n, std::min(rt16, 48)), BM16);
+    const int p2 = (int)ceil_div(n, rp2);
+    a = ScanArgs{};
+    a.cfg = cfg;
+    a.xb = ds.xb;
+    a.xn = ds.xn;
+    a.qb = qop;
+    a.qnorm = Q.qn;
+    a.fold = ds.fold;
+    a.kp = (int)ds.kp;
+    a.xcd_map = tuning().xcd_map;
+    a.xcd_band = tuning().xcd_band;
+    a.cpol = tuning().scan_cpol;
+    a.qtiles = (int)ceil_div(nq, BN16);
+    a.d = (int)ds.d;
+    a.nq = nq;
+    a.n_rows = n;
+    a.cand = cand;
+    a.cnt = cnt;
+    a.qidx = list1;
+    a.nq_dev = flags;
+    a.tau = tau2;
+    a.parts = p2;
+    a.rows_per_part = rp2;
+    a.cap = select_cap_max() / p2;
+    {
+      ScopedTime tm("scan_filter_pass2", st);
+      launch_scan(a, SCAN_FILTER, a.qtiles * p2, SCAN_KIND_REFINE, st);
+    }
+    s.cand = cand;
+    s.cnt = cnt;
+    s.parts = p2;
+    s.cap = a.cap;
+    s.d = (int)ds.d + (ds.fold ? 3 : 0);
+    s.cert_qe = nullptr;
+    s.cert_qr = nullptr;
+    s.dropmin = nullptr;
+    s.no_emit_stat = 0;
   } else {
+    a.qidx = list1;
+    a.nq_dev = flags;
+    a.tau = tau2;
+    a.parts = parts2;
+    a.rows_per_part = rpp2;
+    a.cap = cap2;
+    first_pass("scan_filter_pass2", qtiles * parts2, SCAN_KIND_REFINE);
     s.parts = parts2;
     s.cap = cap2;
   }
+  s.tau_new_bf16 = 0;
+  s.tau_alt = nullptr;
   s.qlist = list1;
   s.nq_dev = flags;
   s.tau = tau2;

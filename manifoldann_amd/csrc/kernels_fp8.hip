@@ -220,8 +220,10 @@ struct Tile8 {
   static constexpr int NXB = 4;                         // row-norm ring (tiles)
   static_assert(WR * WC >= BM / 64, "one wavefront stages the norms of each 64 rows");
   static_assert(A_CHUNKS >= 1 && B_CHUNKS >= 1, "tile too small for the workgroup");
+  // + per query column {tau, |q~|^2, multiplier, query id}, read by the
+  // epilogue (registers are scarce across the K loop)
   static constexpr size_t LDS_BYTES =
-      (size_t)STAGES * (BM + BN) * LROW + BN * 4 + (size_t)NXB * BM * 4;
+      (size_t)STAGES * (BM + BN) * LROW + BN * 4 + (size_t)NXB * BM * 4 + (size_t)BN * 16;
   static constexpr int NPC = A_CHUNKS + B_CHUNKS;
 };
 
@@ -234,6 +236,7 @@ __device__ __forceinline__ void scan8_item(const ScanArgs& a, int qoff, int nqi,
   unsigned char* Bs = As + STAGES * BM * LROW;         // [STAGES][BN][64]
   int* lcnt = (int*)(Bs + STAGES * BN * LROW);         // [BN]
   float* nxs = (float*)(lcnt + BN);                    // [NXB][BM]
+  floatx4* colv = (floatx4*)(nxs + T::NXB * BM);       // [BN]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave % WR, wc = wave / WR;
   const int kp = a.kp, nk = kp / 64;
@@ -253,17 +256,6 @@ __device__ __forceinline__ void scan8_item(const ScanArgs& a, int qoff, int nqi,
     const int c = tid + NT * p, r = c / 4;
     const int brow = r < nqi ? (qidx ? qidx[qoff + r] : qoff + r) : (qidx ? qidx[qoff] : qoff);
     offB[p] = (int64_t)brow * kp + (((c % 4) ^ swz4(r)) * 16);
-  }
-  int gqc[WN];
-  float tau_l[WN], qo[WN], qm[WN];
-#pragma unroll
-  for (int bn = 0; bn < WN; ++bn) {
-    const int c = wc * 32 * WN + bn * 32 + (lane & 31);
-    const int gq = c < nqi ? (qidx ? qidx[qoff + c] : qoff + c) : -1;
-    gqc[bn] = gq;
-    tau_l[bn] = gq >= 0 ? a.tau[gq] : -INFINITY;
-    qo[bn] = gq >= 0 ? a.qnorm[gq] : 0.0f;
-    qm[bn] = gq >= 0 ? a.qmul[gq] : 0.0f;
   }
 
   const int wv = __builtin_amdgcn_readfirstlane(wave);
@@ -333,6 +325,16 @@ __device__ __forceinline__ void scan8_item(const ScanArgs& a, int qoff, int nqi,
     int ln = lane;
     asm volatile("" : "+v"(ln));
     const int r0 = row_begin + t * BM;
+    int gqc[WN];
+    float tau_l[WN], qo[WN], qm[WN];
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+      const floatx4 v = lds_read_f4((const float*)&colv[wc * 32 * WN + j * 32 + (ln & 31)]);
+      tau_l[j] = v[0];
+      qo[j] = v[1];
+      qm[j] = v[2];
+      gqc[j] = __float_as_int(v[3]);
+    }
 #pragma unroll
     for (int i = 0; i < WM; ++i) {
       const int rb0 = r0 + wr * 32 * WM + i * 32 + 4 * (ln >> 5);
@@ -392,7 +394,16 @@ __device__ __forceinline__ void scan8_item(const ScanArgs& a, int qoff, int nqi,
   };
 
   __syncthreads();
-  for (int c = tid; c < BN; c += NT) lcnt[c] = 0;
+  for (int c = tid; c < BN; c += NT) {
+    lcnt[c] = 0;
+    const int gq = c < nqi ? (qidx ? qidx[qoff + c] : qoff + c) : -1;
+    floatx4 v;
+    v[0] = gq >= 0 ? a.tau[gq] : -INFINITY;
+    v[1] = gq >= 0 ? a.qnorm[gq] : 0.0f;
+    v[2] = gq >= 0 ? a.qmul[gq] : 0.0f;
+    v[3] = __int_as_float(gq);
+    colv[c] = v;
+  }
   const int S = ntiles * nk;
   constexpr int D = STAGES - 1;
   int ti = 0, ki = 0, issued = 0;
@@ -519,7 +530,7 @@ __global__ __launch_bounds__(256) void fp8_tau_kernel(Fp8TauArgs a) {
 }
 
 // ---- bf16 re-score of the FP8 survivors -------------------------------------------
-constexpr int RS_CAP = 4096;   // candidates per query held in LDS
+constexpr int RS_CAP = 8192;   // candidates per query held in LDS
 
 struct RescoreShared {
   float vals[RS_CAP];
@@ -532,7 +543,8 @@ struct RescoreShared {
 };
 
 __global__ __launch_bounds__(256) void rescore_kernel(RescoreArgs a) {
-  __shared__ RescoreShared sm;
+  extern __shared__ __align__(16) unsigned char rs_smem[];
+  RescoreShared& sm = *(RescoreShared*)rs_smem;
   const int tid = threadIdx.x;
   const int nq = a.nq_dev ? *a.nq_dev : a.nq;
   const int slot = blockIdx.x;
@@ -731,7 +743,14 @@ void launch_rescore(const RescoreArgs& a, int nq_max, hipStream_t st) {
   if (nq_max <= 0) return;
   MANN_REQUIRE(a.parts8 >= 1 && a.parts8 <= 64, ERR_ASSERTION, "FP8 candidate regions");
   MANN_REQUIRE(a.kp % 8 == 0, ERR_ASSERTION, "bf16 row length");
-  hipLaunchKernelGGL(rescore_kernel, dim3(nq_max), dim3(256), 0, st, a);
+  static bool attr_set = false;  // dynamic LDS above 64 KiB must be opted into
+  if (!attr_set) {
+    MANN_HIP(hipFuncSetAttribute((const void*)rescore_kernel,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)sizeof(RescoreShared)));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(rescore_kernel, dim3(nq_max), dim3(256), sizeof(RescoreShared), st, a);
 }
 
 }  // namespace mann

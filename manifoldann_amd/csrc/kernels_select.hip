@@ -399,8 +399,14 @@ __global__ __launch_bounds__(256) void select_rerank_kernel(SelectArgs a) {
     if (!certified) {
       const int f = atomicAdd(a.flag_count, 1);
       a.flag_list[f] = q;
+      // the k-th exact distance among real rows bounds the true k-th from
+      // above, so its exclusion threshold is a valid emission threshold
+      const double Tn = a.tau_new_bf16 ? T : Tc;
       float tn = INFINITY;
-      if (n >= k_eff && Tc < 3.0e38) tn = nextafterf((float)Tc, INFINITY);
+      if (n >= k_eff && Tn < 3.0e38) tn = nextafterf((float)Tn, INFINITY);
+      // the sample threshold of the second pass's own arithmetic, when it is
+      // tighter (an overflowed first pass can leave a loose k-th distance)
+      if (a.tau_alt) tn = fminf(tn, a.tau_alt[q]);
       // no progress possible: emit everything (with a separate first pass, a
       // tighter threshold is progress too: it drops candidates)
       if (!a.cert_qe && !(tn > tq) && !overflow) tn = INFINITY;

@@ -97,7 +97,7 @@ struct Tuning {
   int tau_rows = 1024;       // IVF: exact rows sampled per query for the emission threshold
   int ivf_tau_mode = 1;      // IVF threshold: 0 exact FP32 row sample, 1 bf16 scan of the first list
   int sample_rows = 2048;    // exact kNN: stride-sample rows (fixed when a dataset is created)
-  int scan8_cfg = 0;         // FP8 first-pass tile config (table of kernels_fp8.hip)
+  int scan8_cfg = 1;         // FP8 first-pass tile config (table of kernels_fp8.hip)
   int fp8_emit_per_k = 2;    // FP8 first pass: bf16 sample rank aimed at fp8_emit_per_k * k_req
                              // rows, then widened by the FP8 band
 };
@@ -187,7 +187,8 @@ struct SelectArgs {
   // First pass in another domain (FP8 candidates re-scored on bf16): tau is
   // then that pass's threshold, certified against the exclusion threshold of
   // its own bounds (cert_*: |q - q~|, |q~| per query, dataset maxima, error
-  // coefficient G / (|q~| + |x~|)^2), and tau_new is in its domain too.
+  // coefficient G / (|q~| + |x~|)^2); tau_new is in that domain too unless
+  // tau_new_bf16 (a second pass on bf16 follows).
   // dropmin: per query, a lower bound of the re-scored values left out of the
   // candidates (they must lie above the bf16 exclusion threshold).
   const float* cert_qe;
@@ -196,6 +197,11 @@ struct SelectArgs {
   double cert_g;
   const float* dropmin;
   int no_emit_stat;    // emitted counts are recorded by the re-score instead
+  int tau_new_bf16;
+  // a second-pass threshold from the sample (null: none): used when it is
+  // below the one derived from the k-th distance found, or when the first
+  // pass left fewer than k candidates
+  const float* tau_alt;
 };
 enum ExtraMetric : int { NORMS_FORM = 3 };
 // one wavefront per query; nq_max bounds the device-side query count

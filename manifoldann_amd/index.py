@@ -65,6 +65,15 @@ def _is_torch(x):
     return type(x).__module__.startswith("torch")
 
 
+def _ready(t):
+    """The library reads device memory on its own stream: the torch work that
+    produced `t` (on torch's current stream) must be complete first."""
+    if t.is_cuda:
+        import torch
+        torch.cuda.current_stream(t.device).synchronize()
+    return t
+
+
 @dataclass
 class _Mat:
     ptr: int
@@ -88,7 +97,7 @@ def as_matrix(x, d_expected: Optional[int] = None) -> _Mat:
         if n > 1 and t.stride(0) != 1:
             t = t.t().contiguous().t()
         ld = t.stride(1) if n > 1 else d
-        return _Mat(t.data_ptr(), d, n, ld, t, t.is_cuda)
+        return _Mat(_ready(t).data_ptr(), d, n, ld, t, t.is_cuda)
     a = np.asarray(x, dtype=np.float32)
     if a.ndim == 1:
         a = a.reshape(-1, 1)
@@ -114,7 +123,7 @@ def _as_matrix_f16(x) -> _Mat:
         if n > 1 and t.stride(0) != 1:
             t = t.t().contiguous().t()
         ld = t.stride(1) if n > 1 else d
-        return _Mat(t.data_ptr(), d, n, ld, t, t.is_cuda)
+        return _Mat(_ready(t).data_ptr(), d, n, ld, t, t.is_cuda)
     a = np.asfortranarray(x if x.ndim == 2 else x.reshape(-1, 1))
     d, n = a.shape
     return _Mat(a.ctypes.data, d, n, d, a, False)
@@ -256,7 +265,7 @@ def query_arrays(index, data, Q, k, out=None, nprobe=None):
         import torch
         ids = torch.zeros((nq, k_out), dtype=torch.int64, device=qm.keep.device)
         dist = torch.full((nq, k_out), float("inf"), dtype=torch.float32, device=qm.keep.device)
-        pi, pd = ids.data_ptr(), dist.data_ptr()
+        pi, pd = _ready(ids).data_ptr(), _ready(dist).data_ptr()
     else:
         ids = np.zeros((nq, k_out), np.int64)
         dist = np.full((nq, k_out), np.inf, np.float32)

@@ -18,7 +18,7 @@ import numpy as np
 
 from ._lib import ArgumentError, DimensionMismatch, check, lib, shutting_down
 from .index import (BruteForceIndex, Dataset, Euclidean, Neighbor, _data_shape, _dataset_for,
-                    _is_torch, _to_neighbors, as_matrix, default_distance)
+                    _is_torch, _ready, _to_neighbors, as_matrix, default_distance)
 
 KMEANS_DEFAULT_MAX_ITERATIONS = 100   # utils/constants.jl:127
 KMEANS_DEFAULT_TOLERANCE = 1e-6       # utils/constants.jl:135
@@ -492,7 +492,7 @@ def query_arrays(index: MultiLevelIndex, data, Q, k, route=None, nprobe=None,
         import torch
         ids = torch.zeros((nq, k), dtype=torch.int64, device=qm.keep.device)
         dist = torch.full((nq, k), float("inf"), dtype=torch.float32, device=qm.keep.device)
-        pi, pd = ids.data_ptr(), dist.data_ptr()
+        pi, pd = _ready(ids).data_ptr(), _ready(dist).data_ptr()
     else:
         ids = np.zeros((nq, k), np.int64)
         dist = np.full((nq, k), np.inf, np.float32)
@@ -501,7 +501,8 @@ def query_arrays(index: MultiLevelIndex, data, Q, k, route=None, nprobe=None,
         return ids, dist
     rp = None
     if route is not None:
-        rp = route.data_ptr() if _is_torch(route) else np.ascontiguousarray(route, np.float32)
+        rp = _ready(route).data_ptr() if _is_torch(route) else np.ascontiguousarray(route,
+                                                                                  np.float32)
         rp = rp if isinstance(rp, int) else rp.ctypes.data
     merge = 1 if isinstance(getattr(index, "merge_strategy", None), DisjointMerge) else 0
     check(lib().mann_ivf_search_ex(index.handle, qm.ptr, qm.d, nq, qm.ld, k, npb,

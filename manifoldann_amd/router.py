@@ -14,7 +14,7 @@ import ctypes
 import numpy as np
 
 from ._lib import ArgumentError, check, lib, shutting_down
-from .index import Dataset, _is_torch, as_matrix
+from .index import Dataset, _is_torch, _ready, as_matrix
 from .ivf import KMeansAssignment, TransformResult, preserves_data  # noqa: F401
 
 
@@ -118,7 +118,7 @@ class LearnedProbeRouter:
     def train(self, X, labels, epochs=5, batch=1024, lr=2e-3, seed=1):
         m = as_matrix(X)
         if _is_torch(labels):
-            lp, L = labels.data_ptr(), labels.shape[1]
+            lp, L = _ready(labels).data_ptr(), labels.shape[1]
         else:
             lab = np.ascontiguousarray(labels, np.int64)
             lp, L = lab.ctypes.data, lab.shape[1]
@@ -193,7 +193,7 @@ def query_arrays_routed(index, router: LearnedProbeRouter, Q, k, nprobe, mass=No
         import torch
         ids = torch.zeros((m.n, k), dtype=torch.int64, device=m.keep.device)
         dist = torch.zeros((m.n, k), dtype=torch.float32, device=m.keep.device)
-        pi, pd = ids.data_ptr(), dist.data_ptr()
+        pi, pd = _ready(ids).data_ptr(), _ready(dist).data_ptr()
     else:
         ids = np.zeros((m.n, k), np.int64)
         dist = np.zeros((m.n, k), np.float32)
