@@ -134,9 +134,15 @@ def bench_config1(args, M, datasets, L, check, np, torch):
     qps = qps_of(run, 1000, args.reps, torch)
     oi, od = O.bf_knn(X, Q, 11, nthreads=host_cores())
     rep = precision_report(np, X, Q, 10, ids.cpu().numpy(), dist.cpu().numpy(), oi, od, 0)
+    # the same search captured in a HIP graph (one launch per batch)
+    g = M.SearchGraph(M.BruteForceIndex(128, 10_000), ds, Qd.t(), 10)
+    qps_g = qps_of(g.launch, 1000, args.reps, torch)
+    gi, gd = g.launch()
+    same = bool((gi == ids).all().item() and (gd == dist).all().item())
     rep.update({"workload": "exact kNN L2: base 10000x128 N(0,1), 1000 queries, k=10",
-                "qps": round(qps, 1)})
-    del ds, Xd, Qd
+                "qps": round(qps, 1), "qps_hip_graph": round(qps_g, 1),
+                "hip_graph_same_result": same})
+    del g, ds, Xd, Qd
     return rep
 
 
@@ -153,6 +159,11 @@ def bench_knn_graph(args, M, datasets, L, check, np, torch):
     def step():
         check(L.mann_knn_graph(ds.handle, k, 0, M.EUCLIDEAN, ids.data_ptr(), dist.data_ptr()))
 
+    # the autotuner picks this shape's knobs (untimed, like the harness's prepare step)
+    tuned = None
+    if not args.no_autotune:
+        tuned = M.autotune(M.BruteForceIndex(d, n), ds, None, k)
+        log(f"kNN graph: autotuned {tuned}")
     for i in range(args.warmup):
         step()
     log(f"kNN graph: {args.warmup} warmup steps done")
@@ -218,6 +229,7 @@ def bench_knn_graph(args, M, datasets, L, check, np, torch):
                "sample": f"{nsq} self-join queries (k=21) against all {n} points in {tc:.2f} s "
                          f"(oracle per-pair A16 FP32 scan + partial sort, OpenMP)"}
     res = {"ms_per_step": 1e3 * t, "value": flop / t / 1e12, "kernels_ms": kernels,
+           "autotune": tuned,
            "precision": precision if not args.skip_cpu else None,
            "stats": {"reranked": stats[0], "overflow": stats[1], "second_pass": stats[2],
                      "exact_fallback": stats[3]}}
@@ -377,6 +389,7 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
     # kernel timing of the list scan at the centroid operating point
     kern = {}
     ivf_stats = None
+    ivf_roof = None
     if bc:
         check(L.mann_profile_enable(1))
         check(L.mann_profile_reset())
@@ -387,6 +400,29 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
                                         "ivf_select", "ivf_exact_fallback"])
         ivf_stats = M.last_stats()
         check(L.mann_profile_enable(0))
+        # HBM roofline of the list scan: its algorithmic bytes are the bf16
+        # residual rows (and norms) of every probed list, read once, plus one
+        # residual query row per probed (list, query) entry
+        route = I.route_distances(index, Qd.t())
+        index.routing = I.TopKRouting(bc["nprobe"])
+        probes = I.probe_lists(index, route)
+        index.routing = I.TopKRouting(1)
+        used = np.unique(probes[probes > 0]) - 1
+        kp = (d + 31) // 32 * 32
+        rows = int(np.asarray(index.list_sizes)[used].sum())
+        entries = int((probes > 0).sum())
+        alg = rows * (kp * 2 + 4) + entries * (kp * 2 + 8)
+        scan_ms = kern.get("ivf_scan")
+        if scan_ms:
+            gbs = alg / (scan_ms * 1e-3) / 1e9
+            ivf_roof = {"bound": "hbm", "kernel": "scan_items_kernel GATHER (IVF list scan)",
+                        "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / HBM_PEAK_GBS, 4),
+                        "algorithmic_bytes_per_launch": alg,
+                        "probed_list_rows": rows, "probed_entries": entries,
+                        "avg_launch_ms": scan_ms, "nprobe": bc["nprobe"],
+                        "traffic_source": "profiles/ivf_search_nprobe16_pmc.md (FETCH_SIZE + "
+                                          "WRITE_SIZE at nprobe 16)"}
     # and at the learned router's operating point (probability mass)
     kern_l = {}
     if bl_mass:
@@ -432,6 +468,7 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
            "learned_over_centroid_fixed_qps": (round(bl["qps"] / bc["qps"], 3)
                                                if bc and bl else None),
            "kernels_ms": kern, "kernels_ms_learned_mass": kern_l, "search_stats": ivf_stats,
+           "roofline": ivf_roof,
            "cpu_baseline": cpu,
            "total_s": round(time.perf_counter() - t_all, 1)}
     return out
@@ -556,6 +593,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time the kNN graph with the default knobs")
     ap.add_argument("--skip-ivf", action="store_true")
     ap.add_argument("--ivf-search-only", type=int, default=0, metavar="NPROBE",
                     help="IVF: build the index, then only three centroid searches at NPROBE "
@@ -624,6 +663,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels_ms": res["kernels_ms"],
+        "autotune": res["autotune"],
         "search_stats": res["stats"],
         "precision": {"exact_knn_10000x128": config1, "knn_graph_60000x784": res["precision"],
                       "large_cosine_10m_256": large["precision"] if large else None},

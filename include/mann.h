@@ -47,6 +47,7 @@ extern "C" {
 typedef struct mann_dataset* mann_dataset_t;
 typedef struct mann_ivf* mann_ivf_t;
 typedef struct mann_router* mann_router_t;
+typedef struct mann_graph* mann_graph_t;
 
 /* Library status. */
 int mann_version(void);
@@ -103,6 +104,37 @@ int mann_bf_knn(mann_dataset_t base, const float* queries, int64_t d, int64_t nq
  * work that overlaps them on that GPU by other means races. */
 int mann_bf_knn_noalloc(mann_dataset_t base, const float* queries, int64_t d, int64_t nq,
                         int64_t ldq, int64_t k, int metric, int64_t* out_ids, float* out_dist);
+
+/* An exact search (mann_bf_knn) captured in a HIP graph, for serving loops
+ * that repeat one batch shape.  No counterpart in the reference.  queries,
+ * out_ids and out_dist must be device memory; the graph reads the queries and
+ * writes the outputs at those addresses on every launch, so the caller keeps
+ * them (and `base`) alive and refills the queries between launches.  Creation
+ * runs the search once eagerly (its answer is written), then captures it with
+ * all scratch in one allocation the graph owns.  mann_graph_launch replays it
+ * on the calling thread's stream and waits; flagged queries beyond the 64 the
+ * graph's exact fallback covers are finished eagerly, so every launch returns
+ * the exact answer.  Statistics of the launch are in mann_last_stats. */
+int mann_bf_knn_graph_create(mann_dataset_t base, const float* queries, int64_t d, int64_t nq,
+                             int64_t ldq, int64_t k, int metric, int64_t* out_ids,
+                             float* out_dist, mann_graph_t* out);
+int mann_graph_launch(mann_graph_t g);
+int mann_graph_destroy(mann_graph_t g);
+
+/* Autotuner of the exact search (no counterpart in the reference): times the
+ * tile configuration, the emission target and the shortlist (for an FP8
+ * dataset its tile configuration and emission target) on this batch by
+ * coordinate descent, and keeps the best for the shape (device, dataset size
+ * and dimension, metric, first pass, k, query count rounded up to a power of
+ * two; queries == NULL tunes the kNN graph's self-join).  Later searches of
+ * the same shape use it; mann_tune_set keeps setting the defaults of the
+ * others.  out_choice (optional, 6 values): scan_cfg, emit_per_k,
+ * shortlist_mult, scan8_cfg, fp8_emit_per_k, searches timed; out_ms: the best
+ * search time.  mann_autotune_clear forgets every shape. */
+int mann_autotune_bf_knn(mann_dataset_t base, const float* queries, int64_t d, int64_t nq,
+                         int64_t ldq, int64_t k, int metric, int64_t* out_choice,
+                         double* out_ms);
+int mann_autotune_clear(void);
 
 /* build_knn_graph(index, data; k, include_self) -- directed adjacency
  * (src/graphs/knn_graph.jl:42-81, _prepare_neighbors :105-130): each point is

@@ -8,7 +8,8 @@ reference's Julia API on top of it.
 """
 from ._lib import ArgumentError, DimensionMismatch, MannError, lib
 from .index import (COSINE, EUCLIDEAN, SQEUCLIDEAN, BruteForceIndex, CosineDist, Dataset,
-                    Euclidean, KNNGraph, Neighbor, SqEuclidean, build_index, build_knn_graph,
+                    Euclidean, KNNGraph, Neighbor, SearchGraph, SqEuclidean, autotune,
+                    build_index, build_knn_graph,
                     cosine_distance, default_distance, default_squared_distance,
                     euclidean_distance, graph_metadata, has_metadata, insert, knn_candidates,
                     last_emitted, last_stats,

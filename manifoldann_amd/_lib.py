@@ -28,6 +28,13 @@ SIGNATURES = [
     ("mann_dataset_destroy", ctypes.c_int, [_vp]),
     ("mann_dataset_info", ctypes.c_int, [_vp, _i64p, _i64p]),
     ("mann_dataset_set_first_pass", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("mann_bf_knn_graph_create", ctypes.c_int,
+     [_vp, _vp, _i64, _i64, _i64, _i64, ctypes.c_int, _vp, _vp, _vp]),
+    ("mann_graph_launch", ctypes.c_int, [_vp]),
+    ("mann_graph_destroy", ctypes.c_int, [_vp]),
+    ("mann_autotune_bf_knn", ctypes.c_int,
+     [_vp, _vp, _i64, _i64, _i64, _i64, ctypes.c_int, _i64p, ctypes.POINTER(ctypes.c_double)]),
+    ("mann_autotune_clear", ctypes.c_int, []),
     ("mann_bf_knn", ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, ctypes.c_int, _vp, _vp]),
     ("mann_bf_knn_noalloc", ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, ctypes.c_int, _vp,
                                            _vp]),

@@ -266,6 +266,174 @@ int mann_bf_knn_noalloc(mann_dataset_t base, const float* queries, int64_t d, in
   });
 }
 
+// ---- exact search captured in a HIP graph ---------------------------------------
+
+struct mann_graph {
+  int device = 0;
+  int nq = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void* arena_mem = nullptr;  // every scratch buffer of the captured search
+  mann::CaptureInfo ci;
+  ~mann_graph() {
+    (void)hipSetDevice(device);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (arena_mem) (void)hipFree(arena_mem);
+  }
+};
+
+int mann_bf_knn_graph_create(mann_dataset_t base, const float* queries, int64_t d, int64_t nq,
+                             int64_t ldq, int64_t k, int metric, int64_t* out_ids,
+                             float* out_dist, mann_graph_t* out) {
+  return guarded([&] {
+    MANN_REQUIRE(out != nullptr, ERR_NULL, "null output handle");
+    MANN_REQUIRE(base != nullptr, ERR_NULL, "null dataset");
+    check_metric(metric);
+    MANN_REQUIRE(d == base->d, ERR_DIMENSION,
+                 "Expected queries with " + std::to_string(base->d) + " rows, got " +
+                     std::to_string(d));
+    MANN_REQUIRE(nq > 0 && k > 0, ERR_ARGUMENT, "a captured search needs queries and k > 0");
+    MANN_REQUIRE(k <= select_exmax() / 2, ERR_ARGUMENT,
+                 "k = " + std::to_string(k) + " exceeds the GPU search's maximum of " +
+                     std::to_string(select_exmax() / 2) + " neighbours per query");
+    MANN_REQUIRE(queries && out_ids && out_dist, ERR_NULL, "null pointer");
+    MANN_REQUIRE(ldq >= d, ERR_ARGUMENT, "leading dimension smaller than d");
+    MANN_REQUIRE((metric == COSINE) == (base->normalized != 0), ERR_ARGUMENT,
+                 "dataset was prepared for another metric");
+    MANN_REQUIRE(is_device_ptr(queries) && is_device_ptr(out_ids) && is_device_ptr(out_dist),
+                 ERR_ARGUMENT, "a captured search takes device queries and outputs only");
+    ThreadCtx& c = ctx();
+    auto g = std::make_unique<mann_graph>();
+    g->device = c.device;
+    g->nq = (int)nq;
+    // the exact fallback inside the graph covers this many flagged queries
+    const int F = (int)std::min<int64_t>(nq, 64);
+    const size_t kr = (size_t)std::min<int64_t>(k, exact_kmax());
+    const size_t per = (size_t)ceil_div(base->n, exact_seg_rows()) * kr * 8;
+    const size_t need = noalloc_need(*base, nq, k) + (size_t)F * (per + 256) + ((size_t)1 << 20);
+    MANN_HIP(hipMalloc(&g->arena_mem, need));
+    Arena arena{(char*)g->arena_mem, need, 0};
+    struct Restore {
+      ThreadCtx& c;
+      ~Restore() {
+        c.arena = nullptr;
+        c.capture = nullptr;
+        c.abufs.clear();
+      }
+    } restore{c};
+    auto run = [&] {
+      c.abufs.clear();
+      arena.used = 0;
+      c.arena = &arena;
+      QueryOps Q = prepare_queries(queries, d, nq, ldq, base->normalized, (int)base->d16,
+                                   (int)base->kp, "gq", base->center);
+      bf_search(*base, Q, (int)k, metric, 0, out_ids, out_dist, nullptr);
+    };
+    // an eager run first: it sets the kernels' attributes outside the capture
+    // (and computes this batch's answer), then the same search is captured
+    run();
+    MANN_HIP(hipStreamSynchronize(c.stream));
+    g->ci.fallback_slots = F;
+    c.capture = &g->ci;
+    MANN_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    try {
+      run();
+    } catch (...) {
+      hipGraph_t tmp = nullptr;
+      (void)hipStreamEndCapture(c.stream, &tmp);
+      if (tmp) (void)hipGraphDestroy(tmp);
+      throw;
+    }
+    MANN_HIP(hipStreamEndCapture(c.stream, &g->graph));
+    MANN_REQUIRE(g->ci.filled, ERR_ASSERTION, "the captured search recorded no fallback");
+    MANN_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    *out = g.release();
+  });
+}
+
+int mann_graph_launch(mann_graph_t g) {
+  return guarded([&] {
+    MANN_REQUIRE(g != nullptr, ERR_NULL, "null graph");
+    MANN_HIP(hipSetDevice(g->device));
+    ThreadCtx& c = ctx();
+    hipStream_t st = c.stream;
+    MANN_HIP(hipGraphLaunch(g->exec, st));
+    int hf[8], hs[kStatSlots * 4];
+    MANN_HIP(hipMemcpyAsync(hf, g->ci.flags, sizeof(hf), hipMemcpyDeviceToHost, st));
+    MANN_HIP(hipMemcpyAsync(hs, g->ci.stats, sizeof(hs), hipMemcpyDeviceToHost, st));
+    MANN_HIP(hipStreamSynchronize(st));
+    // flagged queries beyond the graph's fallback slots: the rest of the
+    // exact fallback, eagerly, with the graph's own partial buffers
+    const int F = g->ci.fallback_slots;
+    if (hf[1] > F) {
+      ExactArgs e = g->ci.e;
+      for (int b = F; b < hf[1]; b += F) {
+        e.slot_base = b;
+        launch_exact_topk(e, std::min(F, hf[1] - b), st);
+      }
+      MANN_HIP(hipStreamSynchronize(st));
+    }
+    MANN_HIP(hipGetLastError());
+    SearchStats& S = last_stats();
+    S = SearchStats();
+    S.pass2 = hf[0];
+    S.exact = hf[1];
+    add_stats(hs, S);
+  });
+}
+
+int mann_graph_destroy(mann_graph_t g) {
+  return guarded([&] { delete g; });
+}
+
+// ---- autotuner ----------------------------------------------------------------------
+
+int mann_autotune_bf_knn(mann_dataset_t base, const float* queries, int64_t d, int64_t nq,
+                         int64_t ldq, int64_t k, int metric, int64_t* out_choice,
+                         double* out_ms) {
+  return guarded([&] {
+    MANN_REQUIRE(base != nullptr, ERR_NULL, "null dataset");
+    check_metric(metric);
+    MANN_REQUIRE((metric == COSINE) == (base->normalized != 0), ERR_ARGUMENT,
+                 "dataset was prepared for another metric");
+    MANN_HIP(hipSetDevice(base->device));
+    ThreadCtx& c = ctx();
+    QueryOps Q;
+    int exclude_self = 0;
+    if (queries == nullptr) {  // the kNN graph's self-join
+      Q = dataset_as_queries(*base);
+      exclude_self = 1;
+      nq = base->n;
+    } else {
+      MANN_REQUIRE(d == base->d, ERR_DIMENSION,
+                   "Expected queries with " + std::to_string(base->d) + " rows, got " +
+                       std::to_string(d));
+      MANN_REQUIRE(nq > 0 && ldq >= d, ERR_ARGUMENT, "bad query matrix");
+      Q = prepare_queries(queries, d, nq, ldq, base->normalized, (int)base->d16,
+                          (int)base->kp, "atq", base->center);
+    }
+    MANN_REQUIRE(k > 0 && k + exclude_self <= select_exmax() / 2, ERR_ARGUMENT,
+                 "k out of range for the GPU search");
+    int64_t* ids = c.buf("at_ids").as<int64_t>((size_t)nq * k);
+    float* dist = c.buf("at_dist").as<float>((size_t)nq * k);
+    AutotuneResult R = autotune_bf(*base, Q, (int)k, metric, exclude_self, ids, dist);
+    if (out_choice) {
+      out_choice[0] = R.best.scan_cfg;
+      out_choice[1] = R.best.emit_per_k;
+      out_choice[2] = R.best.shortlist_mult;
+      out_choice[3] = R.best.scan8_cfg;
+      out_choice[4] = R.best.fp8_emit_per_k;
+      out_choice[5] = R.trials;
+    }
+    if (out_ms) *out_ms = R.ms;
+  });
+}
+
+int mann_autotune_clear(void) {
+  return guarded([&] { shape_tuning_clear(); });
+}
+
 int mann_knn_graph(mann_dataset_t data, int64_t k, int include_self, int metric,
                    int64_t* out_ids, float* out_dist) {
   return guarded([&] {

@@ -1,6 +1,7 @@
 // Device-resident operands and the search pipelines shared by the C ABI.
 #pragma once
 
+#include <map>
 #include <vector>
 
 #include "runtime.h"
@@ -153,10 +154,45 @@ void add_stats(const int* slots, SearchStats& s);
 // Exact fallback for the nflag queries listed in e.qlist (count also in
 // *e.nq_dev): segmented exact top-k over their row ranges; max_segments bounds
 // the segments of one query (rows / exact_seg_rows() + ranges).
-void run_exact_fallback(ExactArgs e, int nflag, int max_segments);
+// Returns the arguments as launched (partials and segment count set).
+ExactArgs run_exact_fallback(ExactArgs e, int nflag, int max_segments);
 
 void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int exclude_self,
                int64_t* out_ids, float* out_dist, int* out_count);
+
+// Knobs chosen by the autotuner, per search shape (under the tuning mutex).
+struct ShapeKey {
+  int64_t path = 0;       // 0 exact search
+  int64_t device = 0, metric = 0, first_pass = 0, self_join = 0, k = 0;
+  int64_t n = 0, d = 0;
+  int64_t nq_bucket = 0;  // query count rounded up to a power of two
+  bool operator<(const ShapeKey& o) const;
+};
+ShapeKey bf_shape_key(const mann_dataset& ds, int64_t nq, int k_req, int metric, bool self_join);
+void shape_tuning_put(const ShapeKey& key, const Tuning& t);
+bool shape_tuning_apply(const ShapeKey& key, Tuning& t);  // false: no entry
+void shape_tuning_clear();
+bool& shape_tuning_disabled();  // this thread: ignore the shape entries
+// Replaces the calling thread's knobs for one scope.
+struct TuningScope {
+  Tuning saved;
+  bool active = false;
+  void set(const Tuning& t) {
+    if (!active) saved = tuning();
+    tuning() = t;
+    active = true;
+  }
+  ~TuningScope() {
+    if (active) tuning() = saved;
+  }
+};
+struct AutotuneResult {
+  Tuning best;
+  double ms = 0.0;
+  int trials = 0;
+};
+AutotuneResult autotune_bf(const mann_dataset& ds, const QueryOps& Q, int k, int metric,
+                           int exclude_self, int64_t* ids, float* dist);
 
 // k-means (core_kmeans.hip).  C is k x d16 device FP32; assign 0-based.
 void kmeans_estep(const mann_dataset& ds, const float* C, int k, int* assign, float* mindist,

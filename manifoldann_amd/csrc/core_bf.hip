@@ -124,6 +124,72 @@ void tuning_publish(const Tuning& t) {
   tuning_refresh();
 }
 
+// ---- per-shape knobs chosen by the autotuner -----------------------------------
+
+bool ShapeKey::operator<(const ShapeKey& o) const {
+  const int64_t a[] = {path, device, metric, first_pass, self_join, k, n, d, nq_bucket};
+  const int64_t b[] = {o.path, o.device, o.metric, o.first_pass, o.self_join, o.k, o.n, o.d,
+                       o.nq_bucket};
+  return std::lexicographical_compare(a, a + 9, b, b + 9);
+}
+
+namespace {
+// the knobs an exact search's shape entry carries
+void copy_search_knobs(const Tuning& from, Tuning& to) {
+  to.scan_cfg = from.scan_cfg;
+  to.emit_per_k = from.emit_per_k;
+  to.shortlist_mult = from.shortlist_mult;
+  to.scan8_cfg = from.scan8_cfg;
+  to.fp8_emit_per_k = from.fp8_emit_per_k;
+}
+std::map<ShapeKey, Tuning>& shape_table() {
+  static std::map<ShapeKey, Tuning> m;  // guarded by the tuning store's mutex
+  return m;
+}
+}  // namespace
+
+void shape_tuning_put(const ShapeKey& key, const Tuning& t) {
+  TuningStore& s = tuning_store();
+  std::lock_guard<std::mutex> g(s.mu);
+  shape_table()[key] = t;
+}
+
+bool shape_tuning_apply(const ShapeKey& key, Tuning& t) {
+  TuningStore& s = tuning_store();
+  std::lock_guard<std::mutex> g(s.mu);
+  auto it = shape_table().find(key);
+  if (it == shape_table().end()) return false;
+  copy_search_knobs(it->second, t);
+  return true;
+}
+
+void shape_tuning_clear() {
+  TuningStore& s = tuning_store();
+  std::lock_guard<std::mutex> g(s.mu);
+  shape_table().clear();
+}
+
+bool& shape_tuning_disabled() {
+  thread_local bool off = false;
+  return off;
+}
+
+ShapeKey bf_shape_key(const mann_dataset& ds, int64_t nq, int k_req, int metric, bool self_join) {
+  ShapeKey key;
+  key.path = 0;
+  key.device = ds.device;
+  key.metric = metric;
+  key.first_pass = ds.first_pass;
+  key.self_join = self_join ? 1 : 0;
+  key.k = k_req;
+  key.n = ds.n;
+  key.d = ds.d;
+  int64_t b = 1;
+  while (b < nq) b <<= 1;
+  key.nq_bucket = b;
+  return key;
+}
+
 SearchStats& last_stats() {
   thread_local SearchStats s;
   return s;
@@ -351,8 +417,8 @@ QueryOps dataset_as_queries(const mann_dataset& ds) {
 
 // ------------------------------------------------------------ exact search --
 
-void run_exact_fallback(ExactArgs e, int nflag, int max_segments) {
-  if (nflag <= 0) return;
+ExactArgs run_exact_fallback(ExactArgs e, int nflag, int max_segments) {
+  if (nflag <= 0) return e;
   ThreadCtx& c = ctx();
   const int kr = std::min(e.k_req, exact_kmax());
   int CH = 2048;
@@ -366,6 +432,8 @@ void run_exact_fallback(ExactArgs e, int nflag, int max_segments) {
     e.slot_base = base;
     launch_exact_topk(e, std::min(CH, nflag - base), c.stream);
   }
+  e.slot_base = 0;
+  return e;
 }
 
 // Emission threshold per query from the dataset's stride sample: the rank-th
@@ -529,6 +597,15 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   S = SearchStats();
   const int nq = (int)Q.nq;
   if (nq == 0 || k <= 0) return;
+  // knobs the autotuner chose for this shape, for this call only
+  TuningScope tscope;
+  if (!shape_tuning_disabled()) {
+    Tuning t = tuning();
+    if (shape_tuning_apply(bf_shape_key(ds, nq, k + (exclude_self ? 1 : 0), metric,
+                                        Q.qb == ds.xb),
+                           t))
+      tscope.set(t);
+  }
   // FP8 first pass (selected on the dataset; not in allocation-free calls)
   const bool fp8 = ds.first_pass == 1 && ds.x8 != nullptr && c.arena == nullptr;
   // folded-norm query operand: -2 q~ and the parts of |q~|^2 (see mann_dataset)
@@ -831,6 +908,18 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   e.out_dist = out_dist;
   e.out_count = out_count;
   MANN_REQUIRE(k_req <= exact_kmax(), ERR_UNSUPPORTED, "k too large for the exact fallback");
+  if (c.capture) {
+    // under graph capture: no read-back; the fallback covers the first
+    // fallback_slots flagged queries (the kernels skip slots past the device
+    // count), the graph's launcher runs the rest if there are more
+    CaptureInfo& ci = *c.capture;
+    ci.e = run_exact_fallback(e, std::min(nq, ci.fallback_slots),
+                              (int)ceil_div(n, exact_seg_rows()));
+    ci.flags = flags;
+    ci.stats = stats;
+    ci.filled = true;
+    return;
+  }
   int hf[8], hs[kStatSlots * 4];
   MANN_HIP(hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, st));
   MANN_HIP(hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, st));
@@ -844,6 +933,76 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   S.pass2 = hf[0];
   S.exact = hf[1];
   add_stats(hs, S);
+}
+
+// ---- autotuner -----------------------------------------------------------------
+// Coordinate descent over the exact search's knobs on the caller's own batch:
+// the tile configuration, then the emission target, then the shortlist (for
+// an FP8 dataset: its tile configuration and emission target too).  Each
+// setting runs the search twice and keeps the second time (HIP events on the
+// calling thread's stream).  The best setting is stored for the shape.
+AutotuneResult autotune_bf(const mann_dataset& ds, const QueryOps& Q, int k, int metric,
+                           int exclude_self, int64_t* ids, float* dist) {
+  ThreadCtx& c = ctx();
+  hipStream_t st = c.stream;
+  AutotuneResult R;
+  const int nq = (int)Q.nq;
+  MANN_REQUIRE(nq > 0 && k > 0, ERR_ARGUMENT, "autotuning needs queries and k > 0");
+  struct Guard {  // the thread's knobs and shape lookup come back whatever happens
+    Tuning saved = tuning();
+    ~Guard() {
+      tuning() = saved;
+      shape_tuning_disabled() = false;
+    }
+  } guard;
+  shape_tuning_disabled() = true;
+  hipEvent_t e0, e1;
+  MANN_HIP(hipEventCreate(&e0));
+  MANN_HIP(hipEventCreate(&e1));
+  struct Ev {
+    hipEvent_t a, b;
+    ~Ev() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } ev{e0, e1};
+  auto time_once = [&](const Tuning& t) {
+    tuning() = t;
+    bf_search(ds, Q, k, metric, exclude_self, ids, dist, nullptr);  // (warm: buffers sized)
+    MANN_HIP(hipEventRecord(e0, st));
+    bf_search(ds, Q, k, metric, exclude_self, ids, dist, nullptr);
+    MANN_HIP(hipEventRecord(e1, st));
+    MANN_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    MANN_HIP(hipEventElapsedTime(&ms, e0, e1));
+    ++R.trials;
+    return (double)ms;
+  };
+  Tuning best = guard.saved;
+  double best_ms = time_once(best);
+  auto sweep = [&](auto setter, std::initializer_list<int> values) {
+    for (int v : values) {
+      Tuning t = best;
+      setter(t, v);
+      const double ms = time_once(t);
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = t;
+      }
+    }
+  };
+  if (ds.first_pass == 1 && ds.x8) {
+    sweep([](Tuning& t, int v) { t.scan8_cfg = v; }, {0, 1, 4});
+    sweep([](Tuning& t, int v) { t.fp8_emit_per_k = v; }, {2, 3});
+  } else {
+    sweep([](Tuning& t, int v) { t.scan_cfg = v; }, {5, 6, 1, 3, 0});
+    sweep([](Tuning& t, int v) { t.emit_per_k = v; }, {8, 15, 24});
+  }
+  sweep([](Tuning& t, int v) { t.shortlist_mult = v; }, {1, 2});
+  R.best = best;
+  R.ms = best_ms;
+  shape_tuning_put(bf_shape_key(ds, nq, k + (exclude_self ? 1 : 0), metric, Q.qb == ds.xb), best);
+  return R;
 }
 
 }  // namespace mann

@@ -59,6 +59,17 @@ struct DevBuf {
   }
 };
 
+// Set while a search is captured into a HIP graph: the search records what its
+// uncaptured tail needs (the exact fallback beyond `fallback_slots` queries,
+// the counters) instead of reading the device counters back.
+struct CaptureInfo {
+  int fallback_slots = 0;
+  bool filled = false;
+  ExactArgs e{};          // as launched (partials for fallback_slots queries)
+  int* flags = nullptr;   // [1]: queries left for the exact fallback
+  int* stats = nullptr;   // [kStatSlots][4] selection counters
+};
+
 // Per host thread: one stream per device and named scratch buffers.  Every
 // exported call runs on the calling thread's stream, so calls from different
 // threads never share scratch memory (the re-entrancy contract of
@@ -67,6 +78,7 @@ struct ThreadCtx {
   int device = -1;
   hipStream_t stream = nullptr;
   std::map<std::string, DevBuf> bufs;
+  CaptureInfo* capture = nullptr;
   // set while an allocation-free call runs: buffers are carved from it
   Arena* arena = nullptr;
   std::map<std::string, DevBuf> abufs;

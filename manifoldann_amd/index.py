@@ -317,6 +317,72 @@ def last_stats():
     return list(out)
 
 
+def autotune(index, data, Q=None, k=None):
+    """Time the exact search's knobs on this batch and keep the best for its
+    shape (device, dataset size, metric, first pass, k, query count rounded
+    up to a power of two); Q=None tunes the kNN graph self-join.  Later
+    searches of that shape use the choice.  No reference counterpart."""
+    ds = _dataset_for(data, index.distance)
+    if k is None or int(k) <= 0:
+        raise ArgumentError("k must be positive")
+    choice = (ctypes.c_int64 * 6)()
+    ms = ctypes.c_double()
+    if Q is None:
+        check(lib().mann_autotune_bf_knn(ds.handle, None, ds.d, ds.n, ds.d, int(k),
+                                         index.distance.code, choice, ctypes.byref(ms)))
+    else:
+        qm = as_matrix(Q)
+        check(lib().mann_autotune_bf_knn(ds.handle, qm.ptr, qm.d, qm.n, qm.ld, int(k),
+                                         index.distance.code, choice, ctypes.byref(ms)))
+    keys = ("scan_cfg", "emit_per_k", "shortlist_mult", "scan8_cfg", "fp8_emit_per_k", "trials")
+    out = dict(zip(keys, list(choice)))
+    out["ms"] = ms.value
+    return out
+
+
+class SearchGraph:
+    """An exact batch query captured in a HIP graph: one shape, fixed device
+    buffers.  Q is a d x nq torch CUDA matrix whose columns are contiguous
+    (e.g. rows.t() of an nq x d tensor); refill it in place between launches.
+    launch() replays the search and returns (ids, dist) tensors, which the
+    next launch overwrites.  No reference counterpart."""
+
+    def __init__(self, index, data, Q, k):
+        import torch
+        if not _is_torch(Q) or not Q.is_cuda or Q.dtype != torch.float32 or Q.dim() != 2:
+            raise ArgumentError("SearchGraph takes a d x nq float32 CUDA tensor")
+        if Q.shape[1] > 1 and Q.stride(0) != 1:
+            raise ArgumentError("SearchGraph needs columns contiguous in memory (rows.t())")
+        self.ds = _dataset_for(data, index.distance)
+        d, nq = Q.shape
+        self.Q = Q
+        self.ids = torch.zeros((nq, int(k)), dtype=torch.int64, device=Q.device)
+        self.dist = torch.zeros((nq, int(k)), dtype=torch.float32, device=Q.device)
+        _ready(self.ids)
+        _ready(self.dist)
+        _ready(Q)
+        h = ctypes.c_void_p()
+        ld = Q.stride(1) if nq > 1 else d
+        check(lib().mann_bf_knn_graph_create(self.ds.handle, Q.data_ptr(), d, nq, ld, int(k),
+                                             index.distance.code, self.ids.data_ptr(),
+                                             self.dist.data_ptr(), ctypes.byref(h)))
+        self.handle = h
+
+    def launch(self):
+        _ready(self.Q)
+        check(lib().mann_graph_launch(self.handle))
+        return self.ids, self.dist
+
+    def __del__(self):
+        try:
+            h = getattr(self, "handle", None)
+            if h is not None and h.value and not shutting_down():
+                lib().mann_graph_destroy(h)
+                self.handle = None
+        except Exception:
+            pass
+
+
 def last_emitted():
     """First-pass candidates the selection read in the last search (all passes)."""
     out = ctypes.c_int64()
