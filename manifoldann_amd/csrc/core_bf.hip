@@ -642,7 +642,12 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
     parts = std::min(std::min(row_tiles, 32), (int)round_up(parts, tuning().xcd_band));
   int rpp = (int)round_up(ceil_div(n, parts), BM);
   parts = (int)ceil_div(n, rpp);
-  const int cap = cap_total / parts;
+  // kNN graph with folded norms: one triangle of the self-join, each block
+  // emitting both directions into twice the regions
+  const bool sym = !fp8 && Q.qb == ds.xb && ds.fold && tuning().sym_join && BM == BN &&
+                   2 * parts <= 64;
+  const int rstride = sym ? 2 * parts : parts;
+  const int cap = cap_total / rstride;
   // the refinement pass only sees the flagged queries (few query tiles): more,
   // smaller row partitions keep the chip busy (up to 64 regions per query)
   int rpp2 = (int)round_up(ceil_div(n, std::min(row_tiles, 64)), BM);
@@ -650,8 +655,10 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   const int cap2 = cap_total / parts2;
 
   unsigned* cand = c.buf("bf_cand").as<unsigned>((size_t)nq * cap_total * 2);
-  int* cnt = c.buf("bf_cnt").as<int>((size_t)nq * std::max(std::max(parts, parts2), 64));
-  float* tau = c.buf("bf_tau").as<float>(nq);
+  int* cnt = c.buf("bf_cnt").as<int>((size_t)nq * std::max(std::max(rstride, parts2), 64));
+  // (the symmetric join stages a whole tile of rows' thresholds: tiles reach
+  // up to 255 rows past the last one)
+  float* tau = c.buf("bf_tau").as<float>((size_t)round_up(nq, 256) + 256);
   float* tau2 = c.buf("bf_tau2").as<float>(nq);
   int* flags = c.buf("bf_flags").as<int>(8);   // [0] pass1 count, [1] pass2 count
   int* stats = c.buf("bf_stats").as<int>(kStatSlots * 4);
@@ -738,7 +745,21 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
     if (fp8) launch_scan8(a, grid, kind, st);
     else launch_scan(a, SCAN_FILTER, grid, kind, st);
   };
-  first_pass("scan_filter", qtiles * parts, SCAN_KIND_GRID);
+  if (sym) {
+    // items (query tile, partition) on and below the diagonal; every region
+    // starts empty (transposed emissions append, and a query tile has no
+    // direct region in the partitions above it)
+    int grid = 0;
+    for (int p = 0; p < parts; ++p)
+      grid += (int)std::min<int64_t>(qtiles, ceil_div((int64_t)(p + 1) * rpp, BN));
+    fill_i32(cnt, (int64_t)nq * rstride, 0, st);
+    a.rstride = rstride;
+    ScopedTime tm("scan_filter", st);
+    launch_scan(a, SCAN_SYM, grid, SCAN_KIND_GRID, st);
+    a.rstride = 0;
+  } else {
+    first_pass("scan_filter", qtiles * parts, SCAN_KIND_GRID);
+  }
 
   // FP8: re-score the survivors on bf16 into one region per query
   const int keep = select_cap_max();
@@ -775,7 +796,7 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   s.cand = cand_sel;
   s.cnt = cnt_sel;
   s.cap = fp8 ? keep : cap;
-  s.parts = fp8 ? 1 : parts;
+  s.parts = fp8 ? 1 : rstride;
   s.tau = tau;
   s.x32 = ds.x32;
   s.q32 = Q.q32;

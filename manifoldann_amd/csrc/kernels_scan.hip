@@ -14,6 +14,13 @@
 //   GATHER : FILTER for work items that share queries (IVF lists): emissions go
 //            to an LDS buffer per query and reach the query's global buffer
 //            with one atomic per (item, query) at the end of the item
+//   SYM    : FILTER over the tiles on and below the diagonal of a self-join
+//            (rows = queries, folded norms): S^ is symmetric, so a block below
+//            the diagonal also emits (S^, query) for each of its rows whose own
+//            threshold it passes.  Those transposed emissions are staged in LDS
+//            and appended to the row's transposed region with one global atomic
+//            each at the end of the item (an atomic's returned value inside the
+//            loop would wait for the chunks in flight)
 // Only candidates cross HBM; the BM x BN block never leaves registers.
 //
 // Layout: WR x WC waves (rows x queries); a wave owns 32*WM rows x 32*WN
@@ -56,6 +63,9 @@ __device__ __forceinline__ floatx4 lds_read_f4(const float* p) {
   floatx4 r;
   asm volatile("ds_read_b128 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(r) : "v"(lds_addr(p)) : "memory");
   return r;
+}
+__device__ __forceinline__ void lds_write_u32(void* p, unsigned v) {
+  asm volatile("ds_write_b32 %0, %1\n\ts_waitcnt lgkmcnt(0)" : : "v"(lds_addr(p)), "v"(v) : "memory");
 }
 __device__ __forceinline__ void lds_write_u2(uint2* p, uint2 v) {
   const u32x2 w = {v.x, v.y};
@@ -133,6 +143,9 @@ struct Tile {
   // GATHER mode: per query column a small LDS candidate buffer
   static constexpr int LCAP = 16;
   static constexpr size_t LDS_GATHER = LDS_BYTES + (size_t)BN * LCAP * 8;
+  // SYM mode: staged transposed emissions (row, value, query) of the item
+  static constexpr int SCAP = 4096;
+  static constexpr size_t LDS_SYM = LDS_BYTES + 16 + (size_t)SCAP * 12;
 };
 
 // One work item.  Values are in the squared-distance domain:
@@ -155,6 +168,14 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
   int* lcnt = (int*)(Bs + STAGES * BN * LROW);  // [BN] emitted count per query column
   float* nxs = (float*)(lcnt + BN);             // [NXB][BM] row norms of the tiles
   uint2* lbuf = (uint2*)(nxs + T::NXB * BM);    // [BN][LCAP] (GATHER mode)
+  // SYM mode: [0] staged count, [1] overflow flag, then rows, values, queries
+  int* scnt = (int*)(nxs + T::NXB * BM);
+  unsigned* srow = (unsigned*)(scnt + 4);
+  float* sval = (float*)(srow + T::SCAP);
+  unsigned* scol = (unsigned*)(sval + T::SCAP);
+  constexpr bool SYM = MODE == SCAN_SYM;
+  // a SYM item's query tile and whether a tile of rows lies below it
+  const int sym_ctile = qoff / BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave % WR, wc = wave / WR;
   const int kp = a.kp, nk = kp / BK;
@@ -216,10 +237,14 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
   // per-lane address is held across the loop.)
   const int nx_slot = (wv % (BM / 64)) * 64;
   const int cpol = __builtin_amdgcn_readfirstlane(a.cpol);
+  // SYM stages each row's own threshold instead of its norm (folded norms need
+  // none): its caller's threshold array holds at least 256 entries past the
+  // last row (read with the rest of the last tile, masked by the row test)
   const __amdgpu_buffer_rsrc_t nx_rsrc =
-      __builtin_amdgcn_make_buffer_rsrc((void*)a.xn, 0, 0xffffffffu, 0x00020000);
+      __builtin_amdgcn_make_buffer_rsrc(SYM ? (void*)a.tau : (void*)a.xn, 0, 0xffffffffu,
+                                        0x00020000);
   auto dma = [&](int t, int kc, int st) {
-    if (kc == 0 && !a.fold)
+    if (kc == 0 && (!a.fold || SYM))
       __builtin_amdgcn_raw_ptr_buffer_load_lds(nx_rsrc,
                                                (lds_void*)(nxs + (t % T::NXB) * BM + nx_slot), 4,
                                                lane * 4,
@@ -337,7 +362,19 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
       // of this block passes" is decided exactly
       // FILTER with folded norms: the values are S^ themselves, so one compare
       // per value serves the block test, the emission count and the store masks
-      constexpr bool CMP = FOLD && MODE == SCAN_FILTER;
+      constexpr bool CMP = FOLD && (MODE == SCAN_FILTER || MODE == SCAN_SYM);
+      // SYM below the diagonal: the thresholds of this lane's 16 rows, staged
+      // with the tile (rows at or past row_end are masked by their values)
+      float trow[SYM ? 16 : 1];
+      const bool offdiag = SYM && (r0 / BM) > sym_ctile;
+      if (SYM && offdiag) {
+#pragma unroll
+        for (int e4 = 0; e4 < 4; ++e4) {
+          const floatx4 v4 = lds_read_f4(nxs + (t % T::NXB) * BM + (rb0 - r0) + 8 * e4);
+#pragma unroll
+          for (int c = 0; c < 4; ++c) trow[SYM ? 4 * e4 + c : 0] = v4[c];
+        }
+      }
       bool pass[WN];
       if (MODE != SCAN_DENSE && !CMP) {
         __builtin_amdgcn_sched_barrier(0);
@@ -355,6 +392,40 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
       }
 #pragma unroll
       for (int j = 0; j < WN; ++j) {
+        if (SYM && offdiag) {
+          // transposed emissions: query = this lane's row, candidate = its column
+          bool tp[16];
+          uint64_t tm[16];
+          int tot = 0;
+#pragma unroll
+          for (int e = 0; e < 16; ++e) {
+            tp[e] = gqc[j] >= 0 && val(j, e) < trow[SYM ? e : 0];
+            tm[e] = __builtin_amdgcn_ballot_w64(tp[e]);
+            tot += __popcll(tm[e]);
+          }
+          if (tot) {  // wave-uniform
+            int base = 0;
+            if (ln == 0) base = lds_fetch_add(scnt, tot);
+            base = __shfl(base, 0);
+            const uint64_t lt = (1ull << ln) - 1ull;
+            int gq = gqc[j];
+            asm volatile("" : "+v"(gq));
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+              if (tp[e]) {
+                const int pos = base + __popcll(tm[e] & lt);
+                if (pos < T::SCAP) {
+                  lds_write_u32(&srow[pos], (unsigned)(rb0 + 8 * (e >> 2) + (e & 3)));
+                  lds_write_u32(&sval[pos], __float_as_uint(val(j, e)));
+                  lds_write_u32(&scol[pos], (unsigned)gq);
+                } else {
+                  lds_write_u32(&scnt[1], 1u);
+                }
+              }
+              base += __popcll(tm[e]);
+            }
+          }
+        }
         if (MODE == SCAN_DENSE) {
           if (a.dense_bmin) {
             // the minimum of each 32-row block: 16 of its values here, 16 in
@@ -416,7 +487,8 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
           asm volatile("" : "+v"(gq));
           const int qc = wc * 32 * WN + j * 32 + (ln & 31);
           int pos = lds_fetch_add(&lcnt[qc], cnt);
-          unsigned* reg = a.cand + ((int64_t)gq * a.parts + region) * (int64_t)a.cap * 2;
+          const int rs = a.rstride ? a.rstride : a.parts;
+          unsigned* reg = a.cand + ((int64_t)gq * rs + region) * (int64_t)a.cap * 2;
 #pragma unroll
           for (int e = 0; e < 16; ++e) {
             if (!p[e]) continue;
@@ -474,7 +546,7 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
   auto epilogue = [&](int t) {
     using T1 = std::integral_constant<bool, true>;
     using T0 = std::integral_constant<bool, false>;
-    if (MODE != SCAN_TOP1 && MODE != SCAN_GATHER && a.fold) {
+    if (MODE != SCAN_TOP1 && MODE != SCAN_GATHER && (a.fold || SYM)) {
       if (row_begin + (t + 1) * BM <= row_end) epilogue_t(t, T1{}, T1{});
       else epilogue_t(t, T1{}, T0{});
     } else {
@@ -484,6 +556,10 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
 
   __syncthreads();  // LDS may still be in use by a previous item of this workgroup
   for (int c = tid; c < BN; c += NT) lcnt[c] = 0;
+  if (SYM && tid == 0) {
+    scnt[0] = 0;
+    scnt[1] = 0;
+  }
   const int S = ntiles * nk;
   constexpr int D = STAGES - 1;  // prefetch distance in chunks
   // issue chunks 0 .. D-1
@@ -558,9 +634,30 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
     }
   } else if (MODE != SCAN_DENSE) {
     __syncthreads();
+    const int rs = a.rstride ? a.rstride : a.parts;
     for (int c = tid; c < nqi; c += NT) {
       int gq = qidx ? qidx[qoff + c] : qoff + c;
-      a.cnt[(int64_t)gq * a.parts + region] = lcnt[c];
+      a.cnt[(int64_t)gq * rs + region] = lcnt[c];
+    }
+    if (SYM) {
+      // the transposed emissions go to each row's region for the partition of
+      // this item's query tile (several items append to it: one atomic each)
+      const int creg = a.parts + (qoff / a.rows_per_part);
+      const int ns = min(scnt[0], T::SCAP);
+      for (int i = tid; i < ns; i += NT) {
+        const int64_t slot = (int64_t)srow[i] * rs + creg;
+        const int pos = atomicAdd(a.cnt + slot, 1);
+        if (pos < a.cap) {
+          uint2 cv;
+          cv.x = __float_as_uint(sval[i]);
+          cv.y = scol[i];
+          *(uint2*)(a.cand + (slot * a.cap + pos) * 2) = cv;
+        }
+      }
+      if (scnt[1]) {  // staging overflowed: these rows' regions must not certify
+        for (int r = row_begin + tid; r < row_end; r += NT)
+          atomicMax(a.cnt + (int64_t)r * rs + creg, a.cap + 1);
+      }
     }
   }
 }
@@ -576,8 +673,10 @@ __attribute__((amdgpu_waves_per_eu(WM * WN <= 8 ? 2 : 1))) void scan_grid_kernel
   const int G = gridDim.x;
   // XCD-aware order: consecutive logical ids share an XCD (and its L2)
   if (a.xcd_map && (G & 7) == 0) bid = (bid & 7) * (G >> 3) + (bid >> 3);
-  int qtile, part;
-  if (a.xcd_map == 2) {  // part-major: an XCD streams one row partition
+  int qtile = 0, part = 0;
+  if (MODE == SCAN_SYM) {
+    // (decoded below)
+  } else if (a.xcd_map == 2) {  // part-major: an XCD streams one row partition
     part = bid / a.qtiles;
     qtile = bid - part * a.qtiles;
   } else if (a.xcd_map == 3) {
@@ -595,11 +694,28 @@ __attribute__((amdgpu_waves_per_eu(WM * WN <= 8 ? 2 : 1))) void scan_grid_kernel
     qtile = bid / a.parts;
     part = bid - qtile * a.parts;
   }
+  if (MODE == SCAN_SYM) {
+    // partition-major items (query tile C, row partition p) with C's first
+    // row below the partition's end; the rows start at C's own tile
+    int rem = bid;
+    part = -1;
+    for (int p = 0; p < a.parts; ++p) {
+      const int cnt_p = min(a.qtiles, ((p + 1) * a.rows_per_part + T::BN - 1) / T::BN);
+      if (rem < cnt_p) {
+        part = p;
+        qtile = rem;
+        break;
+      }
+      rem -= cnt_p;
+    }
+    if (part < 0) return;
+  }
   int qoff = qtile * T::BN;
   if (qoff >= nq || part >= a.parts) return;
   int nqi = min(T::BN, nq - qoff);
   int rb = part * a.rows_per_part;
-  int re = min(a.n_rows, rb + a.rows_per_part);
+  if (MODE == SCAN_SYM) rb = max(rb, qoff);
+  int re = min(a.n_rows, part * a.rows_per_part + a.rows_per_part);
   scan_item<WM, WN, WR, WC, BK, ST, MODE>(a, qoff, nqi, rb, re, part, smem);
 }
 
@@ -629,7 +745,8 @@ void set_lds(K kern, size_t lds) {
 template <int WM, int WN, int WR, int WC, int BK, int ST, int MODE>
 void launch_t(const ScanArgs& a, int grid, int kind, hipStream_t st) {
   using T = Tile<WM, WN, WR, WC, BK, ST>;
-  const size_t lds = MODE == SCAN_GATHER ? T::LDS_GATHER : T::LDS_BYTES;
+  const size_t lds =
+      MODE == SCAN_GATHER ? T::LDS_GATHER : (MODE == SCAN_SYM ? T::LDS_SYM : T::LDS_BYTES);
   static bool attr_set = false;  // dynamic LDS above 64 KiB must be opted into
   if (!attr_set) {
     set_lds(scan_items_kernel<WM, WN, WR, WC, BK, ST, MODE>, lds);
@@ -680,6 +797,12 @@ static void launch_cfg(const ScanArgs& a, int mode, int grid, int kind, hipStrea
     case SCAN_DENSE: launch_t<WM, WN, WR, WC, 32, ST, SCAN_DENSE>(a, grid, kind, st); break;
     case SCAN_TOP1: launch_t<WM, WN, WR, WC, 32, ST, SCAN_TOP1>(a, grid, kind, st); break;
     case SCAN_GATHER: launch_t<WM, WN, WR, WC, 32, ST, SCAN_GATHER>(a, grid, kind, st); break;
+    case SCAN_SYM:
+      if constexpr (WM * WR == WN * WC)
+        launch_t<WM, WN, WR, WC, 32, ST, SCAN_SYM>(a, grid, kind, st);
+      else
+        throw Error{ERR_ASSERTION, "the symmetric join needs square tiles"};
+      break;
     default: break;
   }
 }

@@ -5,7 +5,13 @@
 
 namespace mann {
 
-enum ScanMode : int { SCAN_FILTER = 0, SCAN_DENSE = 1, SCAN_TOP1 = 2, SCAN_GATHER = 3 };
+enum ScanMode : int {
+  SCAN_FILTER = 0,
+  SCAN_DENSE = 1,
+  SCAN_TOP1 = 2,
+  SCAN_GATHER = 3,
+  SCAN_SYM = 4  // FILTER over one triangle of a self-join, emitting both directions
+};
 enum ScanKind : int { SCAN_KIND_GRID = 0, SCAN_KIND_REFINE = 1, SCAN_KIND_ITEMS = 2 };
 
 struct ScanItem {
@@ -75,6 +81,9 @@ struct ScanArgs {
   const unsigned char* x8;
   const unsigned char* q8;
   const float* qmul;
+  // candidate regions per query when not `parts` (SCAN_SYM: parts regions
+  // for its direct emissions, then parts for the transposed ones)
+  int rstride;
 };
 
 // Run-time tuning knobs (set through mann_tune_set; chosen by the autotuner).
@@ -100,6 +109,8 @@ struct Tuning {
   int scan8_cfg = 1;         // FP8 first-pass tile config (table of kernels_fp8.hip)
   int fp8_emit_per_k = 2;    // FP8 first pass: bf16 sample rank aimed at fp8_emit_per_k * k_req
                              // rows, then widened by the FP8 band
+  int sym_join = 0;          // kNN graph with folded norms: scan one triangle of the self-join
+                             // (switched off: see mann_tune_set)
 };
 // the calling thread's snapshot of the knobs (refreshed by tuning_refresh at
 // every API entry)
