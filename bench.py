@@ -412,17 +412,34 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
         rows = int(np.asarray(index.list_sizes)[used].sum())
         entries = int((probes > 0).sum())
         alg = rows * (kp * 2 + 4) + entries * (kp * 2 + 8)
+        # and its MFMA roofline: 2 d FLOP per (query, row of a probed list)
+        sizes = np.asarray(index.list_sizes)
+        scanned = int(sizes[probes[probes > 0] - 1].sum())
+        flop = 2.0 * d * scanned
         scan_ms = kern.get("ivf_scan")
         if scan_ms:
-            gbs = alg / (scan_ms * 1e-3) / 1e9
-            ivf_roof = {"bound": "hbm", "kernel": "scan_items_kernel GATHER (IVF list scan)",
-                        "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(gbs / HBM_PEAK_GBS, 4),
-                        "algorithmic_bytes_per_launch": alg,
-                        "probed_list_rows": rows, "probed_entries": entries,
-                        "avg_launch_ms": scan_ms, "nprobe": bc["nprobe"],
-                        "traffic_source": "profiles/ivf_search_nprobe16_pmc.md (FETCH_SIZE + "
-                                          "WRITE_SIZE at nprobe 16)"}
+            secs = scan_ms * 1e-3
+            gbs = alg / secs / 1e9
+            tfs = flop / secs / 1e12
+            f_hbm, f_mfma = gbs / HBM_PEAK_GBS, tfs / MFMA_BF16_PEAK_TFLOPS
+            # the bound is the roof nearer to the measured time:
+            # max(bytes / BW, FLOP / peak) (the search roofline of the IVF workload)
+            if f_mfma >= f_hbm:
+                ivf_roof = {"bound": "mfma", "achieved": round(tfs, 1), "peak": MFMA_BF16_PEAK_TFLOPS,
+                            "unit": "TFLOP/s", "frac": round(f_mfma, 4)}
+            else:
+                ivf_roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": round(f_hbm, 4)}
+            ivf_roof.update({"kernel": "scan_items_kernel GATHER (IVF list scan)",
+                             "hbm_gbs": round(gbs, 1), "hbm_frac": round(f_hbm, 4),
+                             "mfma_tflops": round(tfs, 1), "mfma_frac": round(f_mfma, 4),
+                             "algorithmic_bytes_per_launch": alg,
+                             "algorithmic_flop_per_launch": flop,
+                             "scanned_pairs": scanned,
+                             "probed_list_rows": rows, "probed_entries": entries,
+                             "avg_launch_ms": scan_ms, "nprobe": bc["nprobe"],
+                             "traffic_source": "profiles/ivf_search_nprobe16_pmc.md "
+                                               "(FETCH_SIZE + WRITE_SIZE at nprobe 16)"})
     # and at the learned router's operating point (probability mass)
     kern_l = {}
     if bl_mass:
