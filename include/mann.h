@@ -121,6 +121,24 @@ int mann_bf_knn_graph_create(mann_dataset_t base, const float* queries, int64_t 
 int mann_graph_launch(mann_graph_t g);
 int mann_graph_destroy(mann_graph_t g);
 
+/* The batched MultiLevelIndex search (mann_ivf_search: routing, TopKRouting
+ * probes, the persistent list scan, the certified selection in the batched
+ * SimpleMerge order of query.jl:279-399, merge.jl:79-124) captured in a HIP
+ * graph for one batch shape.  No counterpart in the reference.  Every step
+ * runs device-gated: the refinement pass and its selection cover whatever the
+ * device flags, the exact fallback its first 64 queries; mann_graph_launch
+ * finishes any further fallback queries eagerly, so every launch returns the
+ * exact answer.  queries (d x nq, stride ldq), route_dist (optional nq x
+ * nlist routing distances, read at every launch, e.g. a router's scores) and
+ * the outputs must be device memory the caller keeps alive.  Creation runs the
+ * device-gated search once to size the graph's own scratch buffers, then
+ * captures it; the outputs are defined after the first mann_graph_launch.
+ * IVF-Flat handles (heap merge) are rejected with ArgumentError. */
+int mann_ivf_search_graph_create(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq,
+                                 int64_t ldq, int64_t k, int64_t nprobe, int metric,
+                                 const float* route_dist, int64_t* out_ids, float* out_dist,
+                                 mann_graph_t* out);
+
 /* Autotuner of the exact search (no counterpart in the reference): times the
  * tile configuration, the emission target and the shortlist (for an FP8
  * dataset its tile configuration and emission target) on this batch by
@@ -134,6 +152,18 @@ int mann_graph_destroy(mann_graph_t g);
 int mann_autotune_bf_knn(mann_dataset_t base, const float* queries, int64_t d, int64_t nq,
                          int64_t ldq, int64_t k, int metric, int64_t* out_choice,
                          double* out_ms);
+/* Autotuner of the batched MultiLevelIndex search (no counterpart in the
+ * reference): times the list-scan tile configuration, the rows of a work item
+ * (the persistent scan's work partition), the first-list threshold tile and
+ * the shortlist on the caller's batch, and keeps the fastest for this shape
+ * (index, k, nprobe, metric, query-count bucket); later searches of the shape
+ * use it.  route_dist as in mann_ivf_search_ex.  out_choice (5 entries):
+ * ivf_cfg, ivf_rows_item, ivf_tau_cfg, shortlist_mult, trials; out_ms: the
+ * best search time.  Results are exact whatever the knobs; mann_autotune_clear
+ * forgets these shapes too. */
+int mann_autotune_ivf(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq, int64_t ldq,
+                      int64_t k, int64_t nprobe, int metric, const float* route_dist,
+                      int64_t* out_choice, double* out_ms);
 int mann_autotune_clear(void);
 
 /* build_knn_graph(index, data; k, include_self) -- directed adjacency

@@ -32,8 +32,13 @@ SIGNATURES = [
      [_vp, _vp, _i64, _i64, _i64, _i64, ctypes.c_int, _vp, _vp, _vp]),
     ("mann_graph_launch", ctypes.c_int, [_vp]),
     ("mann_graph_destroy", ctypes.c_int, [_vp]),
+    ("mann_ivf_search_graph_create", ctypes.c_int,
+     [_vp, _vp, _i64, _i64, _i64, _i64, _i64, ctypes.c_int, _vp, _vp, _vp, _vp]),
     ("mann_autotune_bf_knn", ctypes.c_int,
      [_vp, _vp, _i64, _i64, _i64, _i64, ctypes.c_int, _i64p, ctypes.POINTER(ctypes.c_double)]),
+    ("mann_autotune_ivf", ctypes.c_int,
+     [_vp, _vp, _i64, _i64, _i64, _i64, _i64, ctypes.c_int, _vp, _i64p,
+      ctypes.POINTER(ctypes.c_double)]),
     ("mann_autotune_clear", ctypes.c_int, []),
     ("mann_bf_knn", ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, ctypes.c_int, _vp, _vp]),
     ("mann_bf_knn_noalloc", ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, ctypes.c_int, _vp,

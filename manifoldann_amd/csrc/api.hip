@@ -275,12 +275,15 @@ struct mann_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   void* arena_mem = nullptr;  // every scratch buffer of the captured search
+  // (or, for a captured IVF search, its named scratch buffers)
+  std::map<std::string, mann::DevBuf> owned;
   mann::CaptureInfo ci;
   ~mann_graph() {
     (void)hipSetDevice(device);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (arena_mem) (void)hipFree(arena_mem);
+    for (auto& kv : owned) kv.second.release();
   }
 };
 
@@ -388,6 +391,70 @@ int mann_graph_destroy(mann_graph_t g) {
   return guarded([&] { delete g; });
 }
 
+int mann_ivf_search_graph_create(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq,
+                                 int64_t ldq, int64_t k, int64_t nprobe, int metric,
+                                 const float* route_dist, int64_t* out_ids, float* out_dist,
+                                 mann_graph_t* out) {
+  return guarded([&] {
+    MANN_REQUIRE(out != nullptr, ERR_NULL, "null output handle");
+    MANN_REQUIRE(ivf != nullptr, ERR_NULL, "null index");
+    check_metric(metric);
+    MANN_REQUIRE((metric == COSINE) == (ivf->normalized != 0), ERR_ARGUMENT,
+                 "index was built for another metric");
+    MANN_REQUIRE(!ivf->flat, ERR_ARGUMENT,
+                 "a captured IVF search merges in the batched SimpleMerge order only (not the "
+                 "IVF-Flat heap)");
+    MANN_REQUIRE(d == ivf->d, ERR_DIMENSION,
+                 "Expected queries with " + std::to_string(ivf->d) + " rows, got " +
+                     std::to_string(d));
+    MANN_REQUIRE(nq > 0 && k > 0, ERR_ARGUMENT, "a captured search needs queries and k > 0");
+    MANN_REQUIRE(nprobe > 0, ERR_ARGUMENT, "nprobe must be positive (TopKRouting: k must be positive)");
+    MANN_REQUIRE(queries && out_ids && out_dist, ERR_NULL, "null pointer");
+    MANN_REQUIRE(ldq >= d, ERR_ARGUMENT, "leading dimension smaller than d");
+    MANN_REQUIRE(is_device_ptr(queries) && is_device_ptr(out_ids) && is_device_ptr(out_dist) &&
+                     (route_dist == nullptr || is_device_ptr(route_dist)),
+                 ERR_ARGUMENT, "a captured search takes device queries, routes and outputs only");
+    ThreadCtx& c = ctx();
+    auto g = std::make_unique<mann_graph>();
+    g->device = c.device;
+    g->nq = (int)nq;
+    g->ci.fallback_slots = (int)std::min<int64_t>(nq, 64);
+    struct Restore {
+      ThreadCtx& c;
+      ~Restore() {
+        c.owned = nullptr;
+        c.capture = nullptr;
+      }
+    } restore{c};
+    c.owned = &g->owned;
+    c.capture = &g->ci;
+    auto run = [&] {
+      QueryOps Q = prepare_queries(queries, d, nq, ldq, ivf->normalized, (int)ivf->d16,
+                                   (int)ivf->kp, "ivq");
+      ivf_search(*ivf, Q, (int)k, (int)nprobe, metric, route_dist, out_ids, out_dist, 0.0f, 0,
+                 0);
+    };
+    // an eager run of the device-gated search sizes the graph's buffers and
+    // sets the kernels' attributes; the capture then allocates nothing
+    run();
+    MANN_HIP(hipStreamSynchronize(c.stream));
+    g->ci.filled = false;
+    MANN_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    try {
+      run();
+    } catch (...) {
+      hipGraph_t tmp = nullptr;
+      (void)hipStreamEndCapture(c.stream, &tmp);
+      if (tmp) (void)hipGraphDestroy(tmp);
+      throw;
+    }
+    MANN_HIP(hipStreamEndCapture(c.stream, &g->graph));
+    MANN_REQUIRE(g->ci.filled, ERR_ASSERTION, "the captured search recorded no fallback");
+    MANN_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    *out = g.release();
+  });
+}
+
 // ---- autotuner ----------------------------------------------------------------------
 
 int mann_autotune_bf_knn(mann_dataset_t base, const float* queries, int64_t d, int64_t nq,
@@ -428,6 +495,46 @@ int mann_autotune_bf_knn(mann_dataset_t base, const float* queries, int64_t d, i
       out_choice[5] = R.trials;
     }
     if (out_ms) *out_ms = R.ms;
+  });
+}
+
+static QueryOps ivf_queries(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq,
+                            int64_t ldq);
+
+int mann_autotune_ivf(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq, int64_t ldq,
+                      int64_t k, int64_t nprobe, int metric, const float* route_dist,
+                      int64_t* out_choice, double* out_ms) {
+  return guarded([&] {
+    MANN_REQUIRE(ivf != nullptr, ERR_NULL, "null index");
+    check_metric(metric);
+    MANN_REQUIRE((metric == COSINE) == (ivf->normalized != 0), ERR_ARGUMENT,
+                 "index was built for another metric");
+    MANN_REQUIRE(!ivf->flat, ERR_ARGUMENT, "the IVF autotuner covers MultiLevelIndex searches");
+    MANN_REQUIRE(nprobe > 0, ERR_ARGUMENT, "nprobe must be positive (TopKRouting: k must be positive)");
+    MANN_REQUIRE(k > 0 && k <= select_exmax() / 2, ERR_ARGUMENT,
+                 "k must lie in [1, " + std::to_string(select_exmax() / 2) + "]");
+    QueryOps Q = ivf_queries(ivf, queries, d, nq, ldq);
+    MANN_REQUIRE(nq > 0, ERR_ARGUMENT, "autotuning needs queries");
+    ThreadCtx& c = ctx();
+    const float* R = route_dist;
+    if (R && !is_device_ptr(R)) {
+      float* rb = c.buf("at_R").as<float>((size_t)nq * ivf->nlist);
+      MANN_HIP(hipMemcpyAsync(rb, route_dist, sizeof(float) * nq * ivf->nlist,
+                              hipMemcpyHostToDevice, c.stream));
+      R = rb;
+    }
+    int64_t* ids = c.buf("at_ids").as<int64_t>((size_t)nq * k);
+    float* dist = c.buf("at_dist").as<float>((size_t)nq * k);
+    AutotuneResult A = autotune_ivf(*ivf, Q, (int)k, (int)nprobe, metric, R, ids, dist);
+    if (out_choice) {
+      out_choice[0] = A.best.ivf_cfg;
+      out_choice[1] = A.best.ivf_rows_item;
+      out_choice[2] = A.best.ivf_tau_cfg;
+      out_choice[3] = A.best.shortlist_mult;
+      out_choice[4] = A.trials;
+    }
+    if (out_ms) *out_ms = A.ms;
+    timer_flush();
   });
 }
 

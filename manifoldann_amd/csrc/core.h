@@ -162,10 +162,12 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
 
 // Knobs chosen by the autotuner, per search shape (under the tuning mutex).
 struct ShapeKey {
-  int64_t path = 0;       // 0 exact search
+  int64_t path = 0;       // 0 exact search, 1 IVF search
   int64_t device = 0, metric = 0, first_pass = 0, self_join = 0, k = 0;
   int64_t n = 0, d = 0;
   int64_t nq_bucket = 0;  // query count rounded up to a power of two
+  int64_t nprobe = 0;     // IVF: lists probed per query
+  int64_t lists = 0;      // IVF: lists of the index
   bool operator<(const ShapeKey& o) const;
 };
 ShapeKey bf_shape_key(const mann_dataset& ds, int64_t nq, int k_req, int metric, bool self_join);
@@ -193,6 +195,11 @@ struct AutotuneResult {
 };
 AutotuneResult autotune_bf(const mann_dataset& ds, const QueryOps& Q, int k, int metric,
                            int exclude_self, int64_t* ids, float* dist);
+// IVF search (core_ivf.hip): the list-scan tile configuration, the rows of a
+// work item, the first-list threshold tile and the shortlist, per shape
+ShapeKey ivf_shape_key(const mann_ivf& iv, int64_t nq, int k, int nprobe, int metric);
+AutotuneResult autotune_ivf(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int metric,
+                            const float* route, int64_t* ids, float* dist);
 
 // k-means (core_kmeans.hip).  C is k x d16 device FP32; assign 0-based.
 void kmeans_estep(const mann_dataset& ds, const float* C, int k, int* assign, float* mindist,

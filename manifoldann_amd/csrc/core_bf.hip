@@ -127,15 +127,23 @@ void tuning_publish(const Tuning& t) {
 // ---- per-shape knobs chosen by the autotuner -----------------------------------
 
 bool ShapeKey::operator<(const ShapeKey& o) const {
-  const int64_t a[] = {path, device, metric, first_pass, self_join, k, n, d, nq_bucket};
+  const int64_t a[] = {path, device, metric, first_pass, self_join, k, n, d, nq_bucket, nprobe,
+                       lists};
   const int64_t b[] = {o.path, o.device, o.metric, o.first_pass, o.self_join, o.k, o.n, o.d,
-                       o.nq_bucket};
-  return std::lexicographical_compare(a, a + 9, b, b + 9);
+                       o.nq_bucket, o.nprobe, o.lists};
+  return std::lexicographical_compare(a, a + 11, b, b + 11);
 }
 
 namespace {
-// the knobs an exact search's shape entry carries
-void copy_search_knobs(const Tuning& from, Tuning& to) {
+// the knobs a shape entry carries: the IVF search's, or the exact search's
+void copy_search_knobs(const ShapeKey& key, const Tuning& from, Tuning& to) {
+  if (key.path == 1) {
+    to.ivf_cfg = from.ivf_cfg;
+    to.ivf_rows_item = from.ivf_rows_item;
+    to.ivf_tau_cfg = from.ivf_tau_cfg;
+    to.shortlist_mult = from.shortlist_mult;
+    return;
+  }
   to.scan_cfg = from.scan_cfg;
   to.emit_per_k = from.emit_per_k;
   to.shortlist_mult = from.shortlist_mult;
@@ -159,7 +167,7 @@ bool shape_tuning_apply(const ShapeKey& key, Tuning& t) {
   std::lock_guard<std::mutex> g(s.mu);
   auto it = shape_table().find(key);
   if (it == shape_table().end()) return false;
-  copy_search_knobs(it->second, t);
+  copy_search_knobs(key, it->second, t);
   return true;
 }
 

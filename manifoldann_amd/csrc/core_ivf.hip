@@ -33,12 +33,34 @@ static void ivf_search_sel(const mann_ivf& iv, const QueryOps& Q, int k, int npr
                            const float* route_in, int64_t* out_ids, float* out_dist, float mass,
                            ProbeState* ps = nullptr);
 
+ShapeKey ivf_shape_key(const mann_ivf& iv, int64_t nq, int k, int nprobe, int metric) {
+  ShapeKey key;
+  key.path = 1;
+  key.device = iv.device;
+  key.metric = metric;
+  key.k = k;
+  key.n = iv.n;
+  key.d = iv.d;
+  key.nprobe = nprobe;
+  key.lists = iv.nlist;
+  int64_t b = 1;
+  while (b < nq) b <<= 1;
+  key.nq_bucket = b;
+  return key;
+}
+
 void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int metric,
                 const float* route_in, int64_t* out_ids, float* out_dist, float mass, int merge,
                 int probe_order) {
   // TopKRouting takes min(nprobe, available buckets) lists (routing.jl:108-112):
   // a larger nprobe changes nothing but the buffers sized by it
   if (mass <= 0.0f) nprobe = std::max(1, std::min(nprobe, std::min(iv.nb, iv.nlist)));
+  // knobs the autotuner chose for this shape, for this call only
+  TuningScope tscope;
+  if (!shape_tuning_disabled() && mass <= 0.0f) {
+    Tuning t = tuning();
+    if (shape_tuning_apply(ivf_shape_key(iv, Q.nq, k, nprobe, metric), t)) tscope.set(t);
+  }
   const int kind = iv.flat ? MERGE_HEAP_ROWS
                            : (merge == 1 ? MERGE_HEAP_LISTS
                                          : (probe_order ? MERGE_PROBE : MERGE_BATCHED));
@@ -46,6 +68,8 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
     ivf_search_sel(iv, Q, k, nprobe, metric, route_in, out_ids, out_dist, mass);
     return;
   }
+  MANN_REQUIRE(!ctx().capture, ERR_ARGUMENT,
+               "a captured IVF search merges in the batched SimpleMerge order only");
   // the (dist, bucket, id) top-(k+1), then the kind's merge: untied queries
   // re-sorted, queries tied at the k-th place replayed (kernels_merge.hip)
   ThreadCtx& c = ctx();
@@ -324,12 +348,12 @@ static void ivf_search_sel(const mann_ivf& iv, const QueryOps& Q, int k, int npr
   int* first_cnt = dense_tau ? c.buf("iv_first_cnt").as<int>(nb) : nullptr;
   // [0] n_items, [1] item counter, [2] flagged after pass 1, [3] flagged after
   // pass 2, [4] entries, [5] threshold-scan / refinement items, [6] their item
-  // counter
-  int* ctr = c.buf("iv_ctr").as<int>(8);
+  // counter (16 entries: a captured search's launcher reads 8 from [2])
+  int* ctr = c.buf("iv_ctr").as<int>(16);
   int* stats = c.buf("iv_stats").as<int>(kStatSlots * 4);
   fill_i32(qcount, nb, 0, st);
   fill_i32(qfill, nb, 0, st);
-  fill_i32(ctr, 8, 0, st);
+  fill_i32(ctr, 16, 0, st);
   fill_i32(stats, kStatSlots * 4, 0, st);
   {
     ScopedTime tm("ivf_plan", st);
@@ -515,6 +539,64 @@ static void ivf_search_sel(const mann_ivf& iv, const QueryOps& Q, int k, int npr
     ScopedTime tm("ivf_select", st);
     launch_select_rerank(s, nq, st);
   }
+  if (c.capture) {
+    // a captured search reads nothing back: the refinement pass and the
+    // second selection run for every query the device flags (launch shapes
+    // sized for all of them), the exact fallback for the first
+    // fallback_slots; the graph's launcher runs the rest of the fallback
+    const int64_t rmax = (int64_t)nq * (ceil_div(iv.n, ROWS_ITEM) + nprobe);
+    ScanItem* ritems = c.buf("iv_ritems").as<ScanItem>((size_t)rmax);
+    int* flist2 = c.buf("iv_flist2").as<int>(nq);
+    {
+      ScopedTime tm("ivf_refine_scan", st);
+      fill_i32(ctr + 5, 2, 0, st);
+      launch_refine_items(flist, ctr + 2, nq, nprobe, probes, pcount, slot_of, iv.list_off,
+                          ROWS_ITEM, ritems, ctr + 5, st);
+      reset_listed(cnt, flist, ctr + 2, nq, st);
+      a.items = ritems;
+      a.n_items_dev = ctr + 5;
+      a.item_counter = ctr + 6;
+      a.tau = tau2;
+      launch_scan(a, SCAN_GATHER, 2 * dev_cus, SCAN_KIND_ITEMS, st);
+    }
+    s.qlist = flist;
+    s.nq_dev = ctr + 2;
+    s.tau = tau2;
+    s.flag_count = ctr + 3;
+    s.flag_list = flist2;
+    s.tau_new = tau;
+    {
+      ScopedTime tm("ivf_select_pass2", st);
+      launch_select_rerank(s, nq, st);
+    }
+    int2* ranges = c.buf("iv_ranges").as<int2>((size_t)nq * nprobe);
+    launch_probe_ranges(probes, pcount, nq, nprobe, iv.list_off, ranges, st);
+    ExactArgs e{};
+    e.qlist = flist2;
+    e.nq_dev = ctr + 3;
+    e.x32 = iv.x32;
+    e.q32 = Q.q32;
+    e.xn2 = iv.xn2;
+    e.qn2 = Q.qn2;
+    e.d = (int)iv.d;
+    e.d16 = (int)iv.d16;
+    e.metric = metric;
+    e.perm = iv.perm;
+    e.range_cnt = pcount;
+    e.ranges = ranges;
+    e.range_stride = nprobe;
+    e.k = k;
+    e.k_req = k;
+    e.out_ids = out_ids;
+    e.out_dist = out_dist;
+    CaptureInfo& ci = *c.capture;
+    ci.e = run_exact_fallback(e, std::min(nq, ci.fallback_slots),
+                              (int)ceil_div(iv.n, exact_seg_rows()) + nprobe);
+    ci.flags = ctr + 2;  // [0] refined, [1] left for the exact fallback
+    ci.stats = stats;
+    ci.filled = true;
+    return;
+  }
   int h[8], hs[kStatSlots * 4];
   MANN_HIP(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, st));
   MANN_HIP(hipStreamSynchronize(st));
@@ -584,6 +666,73 @@ static void ivf_search_sel(const mann_ivf& iv, const QueryOps& Q, int k, int npr
   S.pass2 = nflag;
   S.exact = nflag2;
   add_stats(hs, S);
+}
+
+// ---- autotuner -----------------------------------------------------------------
+// Coordinate descent over the IVF search's knobs on the caller's own batch: the
+// list-scan tile configuration, the rows of a work item (the work partition of
+// the persistent scan), the first-list threshold tile and the shortlist.  Each
+// setting runs the search twice and keeps the second time (HIP events on the
+// calling thread's stream); the best is stored for the shape.
+AutotuneResult autotune_ivf(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int metric,
+                            const float* route, int64_t* ids, float* dist) {
+  ThreadCtx& c = ctx();
+  hipStream_t st = c.stream;
+  AutotuneResult R;
+  MANN_REQUIRE(Q.nq > 0 && k > 0, ERR_ARGUMENT, "autotuning needs queries and k > 0");
+  nprobe = std::max(1, std::min(nprobe, std::min(iv.nb, iv.nlist)));
+  struct Guard {  // the thread's knobs and shape lookup come back whatever happens
+    Tuning saved = tuning();
+    ~Guard() {
+      tuning() = saved;
+      shape_tuning_disabled() = false;
+    }
+  } guard;
+  shape_tuning_disabled() = true;
+  hipEvent_t e0, e1;
+  MANN_HIP(hipEventCreate(&e0));
+  MANN_HIP(hipEventCreate(&e1));
+  struct Ev {
+    hipEvent_t a, b;
+    ~Ev() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } ev{e0, e1};
+  auto time_once = [&](const Tuning& t) {
+    tuning() = t;
+    ivf_search(iv, Q, k, nprobe, metric, route, ids, dist, 0.0f, 0, 0);  // (warm)
+    MANN_HIP(hipEventRecord(e0, st));
+    ivf_search(iv, Q, k, nprobe, metric, route, ids, dist, 0.0f, 0, 0);
+    MANN_HIP(hipEventRecord(e1, st));
+    MANN_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    MANN_HIP(hipEventElapsedTime(&ms, e0, e1));
+    ++R.trials;
+    return (double)ms;
+  };
+  Tuning best = guard.saved;
+  double best_ms = time_once(best);
+  auto sweep = [&](auto setter, std::initializer_list<int> values) {
+    for (int v : values) {
+      Tuning t = best;
+      setter(t, v);
+      const double ms = time_once(t);
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = t;
+      }
+    }
+  };
+  // (the 4-stage configurations exceed the LDS with the gather buffers)
+  sweep([](Tuning& t, int v) { t.ivf_cfg = v; }, {5, 0, 2, 1});
+  sweep([](Tuning& t, int v) { t.ivf_rows_item = v; }, {1024, 2048, 4096});
+  sweep([](Tuning& t, int v) { t.ivf_tau_cfg = v; }, {9, 8, 0});
+  sweep([](Tuning& t, int v) { t.shortlist_mult = v; }, {1, 2});
+  R.best = best;
+  R.ms = best_ms;
+  shape_tuning_put(ivf_shape_key(iv, Q.nq, k, nprobe, metric), best);
+  return R;
 }
 
 }  // namespace mann

@@ -82,7 +82,11 @@ struct ThreadCtx {
   // set while an allocation-free call runs: buffers are carved from it
   Arena* arena = nullptr;
   std::map<std::string, DevBuf> abufs;
+  // set while a captured search is built: its scratch buffers belong to the
+  // graph (sized by an eager run, then captured without growing)
+  std::map<std::string, DevBuf>* owned = nullptr;
   DevBuf& buf(const std::string& name) {
+    if (owned) return (*owned)[name];
     if (arena) {
       DevBuf& b = abufs[name];
       b.arena = arena;

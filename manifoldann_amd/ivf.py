@@ -519,3 +519,85 @@ def query(index, data, q, k, nprobe=None):
     ids, dist = query_arrays(index, data, q, k, nprobe=nprobe, probe_order=single)
     res = _to_neighbors(ids, dist)
     return res[0] if single else res
+
+
+def autotune(index: MultiLevelIndex, Q, k, nprobe=None, route=None):
+    """Time the batched search's knobs on this batch (list-scan tile, work-item
+    rows, first-list threshold tile, shortlist) and keep the fastest for the
+    shape; later searches of the shape use it.  Returns the choice.  No
+    reference counterpart."""
+    if isinstance(index, IVFFlatIndex):
+        raise ArgumentError("the IVF autotuner covers MultiLevelIndex searches")
+    qm = as_matrix(Q)
+    npb = index.routing.k if nprobe is None else int(nprobe)
+    rp = None
+    if route is not None:
+        rp = _ready(route).data_ptr() if _is_torch(route) else \
+            np.ascontiguousarray(route, np.float32)
+        rp = rp if isinstance(rp, int) else rp.ctypes.data
+    choice = (ctypes.c_int64 * 5)()
+    ms = ctypes.c_double()
+    check(lib().mann_autotune_ivf(index.handle, qm.ptr, qm.d, qm.n, qm.ld, int(k), npb,
+                                  index.distance.code, rp, choice, ctypes.byref(ms)))
+    out = dict(zip(("ivf_cfg", "ivf_rows_item", "ivf_tau_cfg", "shortlist_mult", "trials"),
+                   list(choice)))
+    out["ms"] = ms.value
+    return out
+
+
+class IVFSearchGraph:
+    """The batched MultiLevelIndex query (query_arrays with the batched
+    SimpleMerge order) captured in a HIP graph for one batch shape.  Q is a
+    d x nq float32 CUDA matrix with contiguous columns (rows.t() of an nq x d
+    tensor), refilled in place between launches; `route`, if given, is an
+    nq x nlist CUDA tensor of routing distances read at every launch.
+    launch() replays the search and returns (ids, dist), overwritten by the
+    next launch.  No reference counterpart."""
+
+    def __init__(self, index: MultiLevelIndex, Q, k, nprobe=None, route=None):
+        import torch
+        if isinstance(index, IVFFlatIndex):
+            raise ArgumentError("IVF-Flat searches (heap merge) are not captured")
+        if not _is_torch(Q) or not Q.is_cuda or Q.dtype != torch.float32 or Q.dim() != 2:
+            raise ArgumentError("IVFSearchGraph takes a d x nq float32 CUDA tensor")
+        if Q.shape[1] > 1 and Q.stride(0) != 1:
+            raise ArgumentError("IVFSearchGraph needs columns contiguous in memory (rows.t())")
+        if isinstance(getattr(index, "merge_strategy", None), DisjointMerge):
+            raise ArgumentError("a captured IVF search merges in the SimpleMerge order only")
+        d, nq = Q.shape
+        self.index, self.Q, self.route = index, Q, route
+        self.ids = torch.zeros((nq, int(k)), dtype=torch.int64, device=Q.device)
+        self.dist = torch.zeros((nq, int(k)), dtype=torch.float32, device=Q.device)
+        npb = index.routing.k if nprobe is None else int(nprobe)
+        rp = None
+        if route is not None:
+            if not _is_torch(route) or not route.is_cuda or tuple(route.shape) != (nq, index.nlist):
+                raise ArgumentError("route must be an nq x nlist CUDA tensor")
+            rp = _ready(route.contiguous()).data_ptr()
+        _ready(self.ids)
+        _ready(self.dist)
+        _ready(Q)
+        h = ctypes.c_void_p()
+        ld = Q.stride(1) if nq > 1 else d
+        check(lib().mann_ivf_search_graph_create(index.handle, Q.data_ptr(), d, nq, ld, int(k),
+                                                 npb, index.distance.code, rp,
+                                                 self.ids.data_ptr(), self.dist.data_ptr(),
+                                                 ctypes.byref(h)))
+        self.handle = h
+
+    def launch(self):
+        _ready(self.Q)
+        if self.route is not None:
+            _ready(self.route)
+        check(lib().mann_graph_launch(self.handle))
+        return self.ids, self.dist
+
+    def __del__(self):
+        try:
+            h = getattr(self, "handle", None)
+            if h is not None and h.value and not shutting_down():
+                lib().mann_graph_destroy(h)
+                self.handle = None
+        except Exception:
+            pass
+
