@@ -452,6 +452,30 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
                                           "ivf_plan", "ivf_residual", "ivf_tau_scan", "ivf_tau",
                                           "ivf_scan", "ivf_select", "ivf_exact_fallback"])
         check(L.mann_profile_enable(0))
+    # the centroid operating point again, with the autotuner's knobs for its
+    # shape and as a HIP-graph replay (after every sweep, so that the sweeps
+    # above all ran on the same default knobs)
+    tuned = None
+    if bc and not args.no_autotune:
+        npb = bc["nprobe"]
+        choice = I.autotune(index, Qd.t(), 10, nprobe=npb)
+
+        def run_t():
+            check(L.mann_ivf_search(index.handle, Qd.data_ptr(), d, nq, d, 10, npb, 0, None,
+                                    ids.data_ptr(), dist.data_ptr()))
+        qps_t = qps_of(run_t, nq, args.reps, torch)
+        ref_i, ref_d = ids.clone(), dist.clone()
+        g = I.IVFSearchGraph(index, Qd.t(), 10, nprobe=npb)
+        qps_g = qps_of(g.launch, nq, args.reps, torch)
+        gi, gd = g.launch()
+        same = bool((gi == ref_i).all().item() and (gd == ref_d).all().item())
+        tuned = {"nprobe": npb, "recall": round(recall(ref_i), 4), "autotune": choice,
+                 "qps": round(qps_t, 1), "qps_hip_graph": round(qps_g, 1),
+                 "hip_graph_same_result": same}
+        log(f"IVF centroid nprobe={npb} tuned: qps {qps_t:.0f}, HIP graph {qps_g:.0f} "
+            f"(same result {same}), choice {choice}")
+        del g
+        check(L.mann_autotune_clear())
     cpu = None
     if not args.skip_cpu and bc:
         from oracle import oracle as O
@@ -476,6 +500,7 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
            "centroid": sweep_c, "ivf_flat": sweep_f, "learned": sweep_l, "learned_mass": sweep_m,
            "centroid_mass": sweep_p,
            "centroid_at_0.95": bc, "centroid_mass_at_0.95": bcm, "centroid_best_at_0.95": bc_best,
+           "centroid_at_0.95_tuned": tuned,
            "learned_at_0.95": bl,
            "learned_fixed_at_0.95": bl_fixed, "learned_mass_at_0.95": bl_mass,
            # learned router against the best centroid point (fixed nprobe on the

@@ -364,7 +364,10 @@ int mann_last_emitted(int64_t* total);
  * neighbour requested), "sample_rank" (multiplier of that target),
  * "shortlist_mult" (first exact re-rank batch = that multiple of k),
  * "ivf_tau_mode" (IVF emission threshold: 0 = exact FP32 sample of the probed
- * rows, 1 = bf16 scan of each query's first list, the default), "tau_rows"
+ * rows, 1 = bf16 scan of each query's first list, the default), "ivf_tau_grp"
+ * (that scan stores the minimum of every 4 consecutive rows of a first list
+ * of at least max(64, 4 k) rows instead of every value: 1, the default, or
+ * 0), "tau_rows"
  * (rows of that exact sample, 16..1024), "fold_norms" (exact kNN: 1 = when a
  * dataset has 6 spare padded columns, carry the norms there so that the MFMA
  * produces the first-pass distances itself; read when a dataset is created),

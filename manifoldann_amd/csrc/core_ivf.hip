@@ -391,7 +391,13 @@ static void ivf_search_sel(const mann_ivf& iv, const QueryOps& Q, int k, int npr
     ScanItem* fitems = c.buf("iv_fitems").as<ScanItem>((size_t)max_fitems);
     int* fioff = c.buf("iv_fioff").as<int>(nb);
     // rows of max_list values, padded to a multiple of 4 for 16-byte stores
-    const int64_t ldd = round_up(iv.max_list, 4);
+    // (4-row minima take 8 slots per 32-row block: fewer)
+    const int64_t ldd = round_up(std::max<int64_t>(iv.max_list, 8 * ceil_div(iv.max_list, 32)), 4);
+    // lists of at least grp_min rows store 4-row minima: at least k of them
+    // (work items hold at least min(list, rows_item) rows, so grp_min <= rows_item)
+    const int grp_min = tuning().ivf_tau_grp && std::max(64, 4 * k) <= ROWS_ITEM
+                            ? std::max(64, 4 * k)
+                            : 0;
     float* dense = c.buf("iv_dense").as<float>((size_t)nq * ldd);
     {
       ScopedTime tm("ivf_tau_scan", st);
@@ -416,6 +422,7 @@ static void ivf_search_sel(const mann_ivf& iv, const QueryOps& Q, int k, int npr
       a.dense_rel = 1;
       a.dvec = 1;  // items start at their list's first row plus multiples of 256
       a.parts = 1;
+      a.dense_grp = grp_min;
       int cus = 256;
       (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device);
       launch_scan(a, SCAN_DENSE, 2 * cus, SCAN_KIND_ITEMS, st);
@@ -440,6 +447,7 @@ static void ivf_search_sel(const mann_ivf& iv, const QueryOps& Q, int k, int npr
     td.tau = tau;
     td.k_eff_q = keff;
     td.max_rows = tuning().ivf_first_rows;
+    td.grp_min = grp_min;
     ScopedTime tm("ivf_tau", st);
     launch_ivf_tau_dense(td, st);
   } else {

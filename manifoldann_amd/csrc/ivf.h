@@ -69,6 +69,7 @@ struct IvfDenseTauArgs {
   float* tau;
   int* k_eff_q;
   int max_rows;  // rows of the first list the DENSE scan covered (0: all)
+  int grp_min;   // lists of at least this many covered rows hold 4-row minima (0: never)
 };
 void launch_ivf_tau_dense(const IvfDenseTauArgs& a, hipStream_t st);
 // single-query items over the probed lists of the flagged queries

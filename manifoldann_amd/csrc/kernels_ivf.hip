@@ -618,8 +618,13 @@ __global__ __launch_bounds__(256) void ivf_tau_dense_kernel(IvfDenseTauArgs a) {
     if (lane == 0) a.tau[q] = INFINITY;
     return;
   }
+  // 4-row minima (8 per 32-row block, +inf past the list): each is the value
+  // of a distinct row, so their k-th smallest bounds the k-th smallest value
+  // from above, as the k-th value itself does (grp_min >= 4 k leaves at least
+  // k finite minima)
+  const int mv = (a.grp_min > 0 && m >= a.grp_min) ? 8 * ((m + 31) / 32) : m;
   const uint32_t key =
-      wave_rank_key(a.dense + (int64_t)q * a.ldd, m, k_eff, buf_all[w], hist_all[w], lane);
+      wave_rank_key(a.dense + (int64_t)q * a.ldd, mv, k_eff, buf_all[w], hist_all[w], lane);
   if (lane == 0) {
     const double dub = metric_upper_bound((double)ord2f(key), a.metric, a.qe[q], a.qr[q], a.ex,
                                           a.rx, a.d);

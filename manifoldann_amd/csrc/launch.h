@@ -68,6 +68,12 @@ struct ScanArgs {
   // dense_bmin: dense[q * ldd + row / 32] = the minimum over each 32-row block
   // (row partitions start at multiples of 32)
   int dense_bmin;
+  // dense_grp > 0 (with dense_rel): work items reaching at least dense_grp rows
+  // past their list's first row store the minimum of every run of 4
+  // consecutive rows instead of each value: 8 per 32-row block of the list,
+  // dense[q * ldd + 8 b + slot] (the slots of a block in lane order); shorter
+  // lists store every value
+  int dense_grp;
   // TOP1 certification inputs
   const float* t1_qe;
   const float* t1_qr;
@@ -105,6 +111,8 @@ struct Tuning {
   int shortlist_mult = 1;    // first re-rank batch = shortlist_mult * k_req (>= 32)
   int tau_rows = 1024;       // IVF: exact rows sampled per query for the emission threshold
   int ivf_tau_mode = 1;      // IVF threshold: 0 exact FP32 row sample, 1 bf16 scan of the first list
+  int ivf_tau_grp = 1;       // IVF first-list scan: store 4-row minima of lists of at least
+                             // max(64, 4 k) rows (0: every value)
   int sample_rows = 2048;    // exact kNN: stride-sample rows (fixed when a dataset is created)
   int scan8_cfg = 1;         // FP8 first-pass tile config (table of kernels_fp8.hip)
   int fp8_emit_per_k = 2;    // FP8 first pass: bf16 sample rank aimed at fp8_emit_per_k * k_req

@@ -430,6 +430,23 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
           }
         }
         if (MODE == SCAN_DENSE) {
+          if (a.dense_grp > 0 && row_end - region >= a.dense_grp) {
+            // minima of the lane's four runs of 4 consecutive rows (rows past
+            // row_end are +inf), one 16-byte store per lane and 32-row block
+            const int b0 = rb0 - 4 * (ln >> 5);  // the block's first row
+            floatx4 v4;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              float mn = fminf(fminf(val(j, 4 * u), val(j, 4 * u + 1)),
+                               fminf(val(j, 4 * u + 2), val(j, 4 * u + 3)));
+              v4[u] = FOLD ? mn : mn + qo[j];
+            }
+            if (gqc[j] >= 0 && b0 < row_end) {
+              const int blk = (b0 - region) >> 5;
+              *(floatx4*)(a.dense + (int64_t)gqc[j] * a.ldd + blk * 8 + (ln >> 5) * 4) = v4;
+            }
+            continue;
+          }
           if (a.dense_bmin) {
             // the minimum of each 32-row block: 16 of its values here, 16 in
             // lane ^ 32 (fl(min a + qo) = min fl(a + qo): monotone rounding)

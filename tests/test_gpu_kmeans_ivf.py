@@ -201,6 +201,8 @@ def test_ivf_external_route_scores():
     {"ivf_cfg": 8, "ivf_tau_cfg": 8},
     {"ivf_first_rows": 64, "ivf_rows_item": 256},
     {"ivf_tau_mode": 0},
+    {"ivf_tau_grp": 0},
+    {"ivf_first_rows": 300, "ivf_rows_item": 256},
 ])
 def test_ivf_tuning_keeps_results_exact(settings):
     # tile configs, first-list row caps, item sizes and threshold modes change
