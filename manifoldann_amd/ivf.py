@@ -571,9 +571,12 @@ class IVFSearchGraph:
         npb = index.routing.k if nprobe is None else int(nprobe)
         rp = None
         if route is not None:
-            if not _is_torch(route) or not route.is_cuda or tuple(route.shape) != (nq, index.nlist):
-                raise ArgumentError("route must be an nq x nlist CUDA tensor")
-            rp = _ready(route.contiguous()).data_ptr()
+            # the graph reads this buffer at every launch: it must be the
+            # caller's own contiguous tensor (no temporary copy)
+            if (not _is_torch(route) or not route.is_cuda or route.dtype != torch.float32
+                    or tuple(route.shape) != (nq, index.nlist) or not route.is_contiguous()):
+                raise ArgumentError("route must be a contiguous nq x nlist float32 CUDA tensor")
+            rp = _ready(route).data_ptr()
         _ready(self.ids)
         _ready(self.dist)
         _ready(Q)

@@ -141,6 +141,9 @@ def test_ivf_graph_argument_checks():
     flat = I.ivf_flat_from_assignment(X.T, index.transform, a, nprobe=3)
     with pytest.raises(M.ArgumentError):
         I.IVFSearchGraph(flat, rows.t(), 5)
+    R = torch.zeros((16, 50), dtype=torch.float32, device="cuda").t()  # 50 x 16, not contiguous
+    with pytest.raises(M.ArgumentError):
+        I.IVFSearchGraph(index, rows.t(), 5, route=R)
     index.merge_strategy = I.DisjointMerge()
     with pytest.raises(M.ArgumentError):
         I.IVFSearchGraph(index, rows.t(), 5)
