@@ -1234,6 +1234,9 @@ int mann_tune_set(const char* key, int64_t value) {
     } else if (k == "ivf_tau_mode") {
       MANN_REQUIRE(value == 0 || value == 1, ERR_ARGUMENT, "ivf_tau_mode must be 0 or 1");
       t.ivf_tau_mode = (int)value;
+    } else if (k == "sample_grp") {
+      MANN_REQUIRE(value == 0 || value == 1, ERR_ARGUMENT, "sample_grp must be 0 or 1");
+      t.sample_grp = (int)value;
     } else if (k == "ivf_tau_grp") {
       MANN_REQUIRE(value == 0 || value == 1, ERR_ARGUMENT, "ivf_tau_grp must be 0 or 1");
       t.ivf_tau_grp = (int)value;
@@ -1288,6 +1291,7 @@ int mann_tune_get(const char* key, int64_t* value) {
     else if (k == "tau_rows") *value = t.tau_rows;
     else if (k == "ivf_tau_mode") *value = t.ivf_tau_mode;
     else if (k == "ivf_tau_grp") *value = t.ivf_tau_grp;
+    else if (k == "sample_grp") *value = t.sample_grp;
     else if (k == "sample_rows") *value = t.sample_rows;
     else if (k == "emit_per_k") *value = t.emit_per_k;
     else if (k == "fold_norms") *value = t.fold_norms;

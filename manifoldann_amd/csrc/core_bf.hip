@@ -465,7 +465,12 @@ static void sample_tau(const mann_dataset& ds, const QueryOps& Q, const bf16_t* 
   // sample's rank-th smallest unless two of its rank smallest values share
   // a block, and is never below it (a threshold estimate either way)
   const bool bmin = ds.m / 32 >= 64 * std::max(rank, rank2);
-  const int mcols = bmin ? (int)ceil_div(ds.m, 32) : ds.m;
+  // otherwise the minimum of every 4 consecutive sample rows (8 per 32-row
+  // block): the rank-th smallest of those minima is never below the sample's
+  // rank-th smallest value and equals it unless two of its rank smallest share
+  // a run of 4 rows; a quarter of the values are written and ranked
+  const bool grp = !bmin && tuning().sample_grp && 8 * ceil_div(ds.m, 32) >= std::max(rank, rank2);
+  const int mcols = bmin ? (int)ceil_div(ds.m, 32) : (grp ? (int)(8 * ceil_div(ds.m, 32)) : ds.m);
   // the dense sample block is processed in query chunks of at most 2^28 values
   int64_t qc_max = std::max<int64_t>(BN, ((int64_t)1 << 28) / mcols / BN * BN);
   if (c.arena)  // the sample block in query chunks the scratch holds
@@ -477,7 +482,7 @@ static void sample_tau(const mann_dataset& ds, const QueryOps& Q, const bf16_t* 
   // self-join: the rows are the queries and the sample rows the query
   // columns, so each store instruction writes whole 128-byte row segments of
   // the [query][sample] block (S^ is the same quantity either way round)
-  const bool swap = Q.qb == ds.xb && !bmin;
+  const bool swap = Q.qb == ds.xb && !bmin && !grp;
   for (int q0 = 0; q0 < nq; q0 += qc) {
     const int nqc = std::min(qc, nq - q0);
     if (swap) {
@@ -536,6 +541,7 @@ static void sample_tau(const mann_dataset& ds, const QueryOps& Q, const bf16_t* 
     a.dense = dense;
     a.ldd = mcols;
     a.dense_bmin = bmin;
+    a.dense_grp = grp ? 1 : 0;
     a.dvec = mcols % 4 == 0;  // row partitions start at multiples of BM
     {
       ScopedTime tm("scan_dense", st);

@@ -68,11 +68,12 @@ struct ScanArgs {
   // dense_bmin: dense[q * ldd + row / 32] = the minimum over each 32-row block
   // (row partitions start at multiples of 32)
   int dense_bmin;
-  // dense_grp > 0 (with dense_rel): work items reaching at least dense_grp rows
-  // past their list's first row store the minimum of every run of 4
-  // consecutive rows instead of each value: 8 per 32-row block of the list,
-  // dense[q * ldd + 8 b + slot] (the slots of a block in lane order); shorter
-  // lists store every value
+  // dense_grp > 0: the minimum of every run of 4 consecutive rows is stored
+  // instead of each value, 8 per 32-row block, dense[q * ldd + 8 b + slot]
+  // (the slots of a block in lane order).  With dense_rel, blocks count from
+  // the list's first row and only work items reaching at least dense_grp rows
+  // past it do so (shorter lists store every value); otherwise blocks count
+  // from row 0 and every item does (dense_grp = 1)
   int dense_grp;
   // TOP1 certification inputs
   const float* t1_qe;
@@ -106,6 +107,7 @@ struct Tuning {
   int scan_cpol = 0;         // operand loads non-temporal: bit 0 rows, bit 1 queries
   int target_groups = 2048;  // workgroups the row partitioning aims for
   int sample_rank = 1;       // multiplier of the emission target below
+  int sample_grp = 1;        // exact kNN threshold sample: store 4-row minima (0: every value)
   int emit_per_k = 15;       // exact kNN: aim at emit_per_k * k_req first-pass emissions per query
   int fold_norms = 1;        // exact kNN: fold the norms into spare K columns when there are 6
   int shortlist_mult = 1;    // first re-rank batch = shortlist_mult * k_req (>= 32)

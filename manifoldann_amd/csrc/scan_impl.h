@@ -430,7 +430,10 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
           }
         }
         if (MODE == SCAN_DENSE) {
-          if (a.dense_grp > 0 && row_end - region >= a.dense_grp) {
+          // (blocks are counted from the list's first row with dense_rel, else
+          // from row 0 of the scanned rows)
+          const int gbase = a.dense_rel ? region : 0;
+          if (a.dense_grp > 0 && row_end - gbase >= a.dense_grp) {
             // minima of the lane's four runs of 4 consecutive rows (rows past
             // row_end are +inf), one 16-byte store per lane and 32-row block
             const int b0 = rb0 - 4 * (ln >> 5);  // the block's first row
@@ -442,7 +445,7 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
               v4[u] = FOLD ? mn : mn + qo[j];
             }
             if (gqc[j] >= 0 && b0 < row_end) {
-              const int blk = (b0 - region) >> 5;
+              const int blk = (b0 - gbase) >> 5;
               *(floatx4*)(a.dense + (int64_t)gqc[j] * a.ldd + blk * 8 + (ln >> 5) * 4) = v4;
             }
             continue;
