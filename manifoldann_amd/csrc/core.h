@@ -152,10 +152,10 @@ void add_stats(const int* slots, SearchStats& s);
 // FP32 re-rank, refinement pass, exact fallback).  out_* are device pointers,
 // nq x k.  exclude_self drops the row whose id equals the query index.
 // Exact fallback for the nflag queries listed in e.qlist (count also in
-// *e.nq_dev): segmented exact top-k over their row ranges; max_segments bounds
-// the segments of one query (rows / exact_seg_rows() + ranges).
+// *e.nq_dev): segmented exact top-k over their row ranges (at most total_rows
+// rows in at most nranges + 1 ranges per query).
 // Returns the arguments as launched (partials and segment count set).
-ExactArgs run_exact_fallback(ExactArgs e, int nflag, int max_segments);
+ExactArgs run_exact_fallback(ExactArgs e, int nflag, int64_t total_rows, int nranges);
 
 void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int exclude_self,
                int64_t* out_ids, float* out_dist, int* out_count);

@@ -425,17 +425,31 @@ QueryOps dataset_as_queries(const mann_dataset& ds) {
 
 // ------------------------------------------------------------ exact search --
 
-ExactArgs run_exact_fallback(ExactArgs e, int nflag, int max_segments) {
+ExactArgs run_exact_fallback(ExactArgs e, int nflag, int64_t total_rows, int nranges) {
   if (nflag <= 0) return e;
   ThreadCtx& c = ctx();
   const int kr = std::min(e.k_req, exact_kmax());
+  // segment rows: few queries get short segments, so that the launch still
+  // spreads over about 1,024 workgroups (allocation-free calls keep the
+  // longest, which their scratch sizing assumes)
+  int seg = exact_seg_rows();
+  if (!c.arena) {
+    const int64_t want = ceil_div(total_rows, std::max<int64_t>(1, 1024 / nflag));
+    seg = (int)std::min<int64_t>(seg, std::max<int64_t>(256, round_up(want, 256)));
+  }
+  const int max_segments = (int)ceil_div(total_rows, seg) + nranges;
   int CH = 2048;
   if (c.arena) {  // as many queries per launch as the scratch holds partials for
-    const size_t per = (size_t)max_segments * kr * sizeof(uint64_t);
-    CH = (int)std::max<size_t>(1, std::min<size_t>(2048, c.arena->left() / (per + 256)));
+    // (a carve takes an eighth more than asked, plus alignment, per buffer)
+    const size_t per = (size_t)max_segments * kr * sizeof(uint64_t) + 8;
+    const size_t left = c.arena->left();
+    CH = (int)std::max<size_t>(
+        1, std::min<size_t>(2048, (left > 1024 ? left - 1024 : 0) / (per + per / 8 + 16)));
   }
   e.nseg = max_segments;
+  e.seg_rows = seg;
   e.partial = c.buf("ex_partial").as<uint64_t>((size_t)std::min(nflag, CH) * max_segments * kr);
+  e.bound = c.buf("ex_bound").as<unsigned long long>((size_t)std::min(nflag, CH));
   for (int base = 0; base < nflag; base += CH) {
     e.slot_base = base;
     launch_exact_topk(e, std::min(CH, nflag - base), c.stream);
@@ -948,8 +962,7 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
     // fallback_slots flagged queries (the kernels skip slots past the device
     // count), the graph's launcher runs the rest if there are more
     CaptureInfo& ci = *c.capture;
-    ci.e = run_exact_fallback(e, std::min(nq, ci.fallback_slots),
-                              (int)ceil_div(n, exact_seg_rows()));
+    ci.e = run_exact_fallback(e, std::min(nq, ci.fallback_slots), n, 0);
     ci.flags = flags;
     ci.stats = stats;
     ci.filled = true;
@@ -961,7 +974,7 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   MANN_HIP(hipStreamSynchronize(st));
   {
     ScopedTime tm("exact_fallback", st);
-    run_exact_fallback(e, hf[1], (int)ceil_div(n, exact_seg_rows()));
+    run_exact_fallback(e, hf[1], n, 0);
   }
   MANN_HIP(hipStreamSynchronize(st));
   MANN_HIP(hipGetLastError());

@@ -598,8 +598,7 @@ static void ivf_search_sel(const mann_ivf& iv, const QueryOps& Q, int k, int npr
     e.out_ids = out_ids;
     e.out_dist = out_dist;
     CaptureInfo& ci = *c.capture;
-    ci.e = run_exact_fallback(e, std::min(nq, ci.fallback_slots),
-                              (int)ceil_div(iv.n, exact_seg_rows()) + nprobe);
+    ci.e = run_exact_fallback(e, std::min(nq, ci.fallback_slots), iv.n, nprobe);
     ci.flags = ctr + 2;  // [0] refined, [1] left for the exact fallback
     ci.stats = stats;
     ci.filled = true;
@@ -665,7 +664,7 @@ static void ivf_search_sel(const mann_ivf& iv, const QueryOps& Q, int k, int npr
       e.out_ids = out_ids;
       e.out_dist = out_dist;
       ScopedTime tm("ivf_exact_fallback", st);
-      run_exact_fallback(e, nflag2, (int)ceil_div(iv.n, exact_seg_rows()) + nprobe);
+      run_exact_fallback(e, nflag2, iv.n, nprobe);
     }
   }
   MANN_HIP(hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, st));

@@ -17,7 +17,13 @@ namespace {
 constexpr int SEL_CAP = 1024;   // max candidates kept per query (LDS sort size)
 constexpr int EX_BATCH = 256;   // rows per batch in the exact fallback
 constexpr int EX_KMAX = 256;    // max k_req of the exact fallback
-constexpr int EX_SEG = 4096;    // rows per block of the exact fallback
+constexpr int EX_SEG = 4096;    // rows per block of the exact fallback (at most)
+constexpr int EX_MCAP = 2048;   // keys the parallel merge collects per query
+
+__global__ void fill_u64_kernel(unsigned long long* p, int n, unsigned long long v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
 
 __device__ __forceinline__ void bitonic_sort(uint64_t* s, int P) {
   for (int k = 2; k <= P; k <<= 1) {
@@ -477,10 +483,10 @@ __global__ __launch_bounds__(256) void exact_partial_kernel(ExactArgs a) {
         lo = rg.x;
         hi = rg.y;
       }
-      const int ns = (hi - lo + EX_SEG - 1) / EX_SEG;
+      const int ns = (hi - lo + a.seg_rows - 1) / a.seg_rows;
       if (seg < ns) {
-        rb = lo + seg * EX_SEG;
-        re = min(hi, rb + EX_SEG);
+        rb = lo + seg * a.seg_rows;
+        re = min(hi, rb + a.seg_rows);
         break;
       }
       seg -= ns;
@@ -492,8 +498,12 @@ __global__ __launch_bounds__(256) void exact_partial_kernel(ExactArgs a) {
   __syncthreads();
   const int rb = s_rb, re = s_re;
   uint64_t thr = ~0ull;
+  unsigned long long* bound = a.bound + blockIdx.x;
   for (int base = rb; base < re; base += EX_BATCH) {
+    // keys at or above the query's bound (another segment's kr-th key) cannot
+    // be among its kr smallest
     if (tid == 0) s_any = 0;
+    thr = min(thr, (uint64_t)__hip_atomic_load(bound, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     __syncthreads();
     for (int i = g; i < EX_BATCH; i += 16) {
       const int r = base + i;
@@ -518,6 +528,7 @@ __global__ __launch_bounds__(256) void exact_partial_kernel(ExactArgs a) {
       if (tid < kr) m[EX_BATCH + tid] = keep;
       __syncthreads();
       thr = m[EX_BATCH + kr - 1];
+      if (tid == 0 && thr != ~0ull) atomicMin(bound, (unsigned long long)thr);
     }
   }
   if (tid < kr) out[tid] = m[EX_BATCH + tid];
@@ -531,22 +542,51 @@ __global__ __launch_bounds__(256) void exact_merge_kernel(ExactArgs a) {
   const int q = a.qlist[slot];
   const int tid = threadIdx.x;
   const int kr = a.k_req < EX_KMAX ? a.k_req : EX_KMAX;
-  __shared__ uint64_t m[EX_BATCH + EX_KMAX];
-  for (int i = tid; i < EX_BATCH + EX_KMAX; i += 256) m[i] = ~0ull;
-  __syncthreads();
+  __shared__ uint64_t m[EX_MCAP];
+  __shared__ int s_n;
   const uint64_t* part = a.partial + (int64_t)blockIdx.x * a.nseg * kr;
-  for (int sg = 0; sg < a.nseg; ++sg) {
-    const uint64_t* src = part + (int64_t)sg * kr;
-    if (src[0] == ~0ull || src[0] >= m[EX_BATCH + kr - 1]) continue;  // uniform per block
-    if (tid < kr) m[tid] = src[tid];
+  // every segment key at or below the bound, collected in parallel: the kr
+  // smallest keys are among them
+  const uint64_t bnd = (uint64_t)a.bound[blockIdx.x];
+  if (tid == 0) s_n = 0;
+  for (int i = tid; i < EX_MCAP; i += 256) m[i] = ~0ull;
+  __syncthreads();
+  for (int64_t i = tid; i < (int64_t)a.nseg * kr; i += 256) {
+    const uint64_t key = part[i];
+    if (key != ~0ull && key <= bnd) {
+      const int p = atomicAdd(&s_n, 1);
+      if (p < EX_MCAP) m[p] = key;
+    }
+  }
+  __syncthreads();
+  const int nc = s_n;
+  if (nc <= EX_MCAP) {
+    int P = 256;
+    while (P < nc) P <<= 1;
+    bitonic_sort(m, P);
     __syncthreads();
-    bitonic_sort(m, EX_BATCH + EX_KMAX);
+    // the kr smallest now lead m: move them where the write-out reads them
     const uint64_t keep = tid < kr ? m[tid] : ~0ull;
-    __syncthreads();
-    for (int i = tid; i < EX_BATCH + EX_KMAX; i += 256) m[i] = ~0ull;
     __syncthreads();
     if (tid < kr) m[EX_BATCH + tid] = keep;
     __syncthreads();
+  } else {
+    // more keys below the bound than the buffer holds: the sequential merge
+    for (int i = tid; i < EX_BATCH + EX_KMAX; i += 256) m[i] = ~0ull;
+    __syncthreads();
+    for (int sg = 0; sg < a.nseg; ++sg) {
+      const uint64_t* src = part + (int64_t)sg * kr;
+      if (src[0] == ~0ull || src[0] >= m[EX_BATCH + kr - 1]) continue;  // uniform per block
+      if (tid < kr) m[tid] = src[tid];
+      __syncthreads();
+      bitonic_sort(m, EX_BATCH + EX_KMAX);
+      const uint64_t keep = tid < kr ? m[tid] : ~0ull;
+      __syncthreads();
+      for (int i = tid; i < EX_BATCH + EX_KMAX; i += 256) m[i] = ~0ull;
+      __syncthreads();
+      if (tid < kr) m[EX_BATCH + tid] = keep;
+      __syncthreads();
+    }
   }
   if (tid == 0) {
     int o = 0;
@@ -595,6 +635,8 @@ int exact_seg_rows() { return EX_SEG; }
 
 void launch_exact_topk(const ExactArgs& a, int nslots, hipStream_t st) {
   if (nslots <= 0 || a.nseg <= 0) return;
+  hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)ceil_div(nslots, 256)), dim3(256), 0, st,
+                     a.bound, nslots, ~0ull);
   hipLaunchKernelGGL(exact_partial_kernel, dim3(nslots, a.nseg), dim3(256), 0, st, a);
   hipLaunchKernelGGL(exact_merge_kernel, dim3(nslots), dim3(256), 0, st, a);
 }

@@ -248,10 +248,14 @@ struct ExactArgs {
   float* out_dist;
   int* out_count;
   // segment partials: slots [slot_base, slot_base + nslots) of qlist, nseg
-  // segments each (EX_SEG rows inside one range), kr keys per segment
+  // segments each (seg_rows rows inside one range), kr keys per segment
   int slot_base;
   int nseg;
+  int seg_rows;       // a multiple of 256, at most exact_seg_rows()
   uint64_t* partial;  // [nslots][nseg][min(k_req, exact_kmax())]
+  // per slot of the launch: the smallest kr-th key any segment has found (an
+  // upper bound of the query's kr-th key; reset by launch_exact_topk)
+  unsigned long long* bound;
 };
 // Two launches: per-segment exact top-k, then the per-query merge.
 void launch_exact_topk(const ExactArgs& a, int nslots, hipStream_t st);
