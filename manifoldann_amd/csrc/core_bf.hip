@@ -31,7 +31,8 @@ Timer& timer() {
 
 int timer_begin(const char* name, hipStream_t st) {
   Timer& t = timer();
-  if (!t.enabled || ctx().arena) return -1;  // (event creation allocates)
+  // (event creation allocates; a search being captured records no events)
+  if (!t.enabled || ctx().arena || ctx().capture) return -1;
   std::lock_guard<std::mutex> g(t.mu);
   Timer::Pending p;
   p.name = name;
