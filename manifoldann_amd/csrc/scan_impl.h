@@ -23,7 +23,8 @@
 //            threshold it passes.  Those transposed emissions are staged in LDS
 //            and appended to the row's transposed region with one global atomic
 //            each at the end of the item (an atomic's returned value inside the
-//            loop would wait for the chunks in flight)
+//            loop would wait for the chunks in flight); past the staging
+//            capacity they are appended at once
 // Only candidates cross HBM; the BM x BN block never leaves registers.
 //
 // Layout: WR x WC waves (rows x queries); a wave owns 32*WM rows x 32*WN
@@ -417,12 +418,25 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
             for (int e = 0; e < 16; ++e) {
               if (tp[e]) {
                 const int pos = base + __popcll(tm[e] & lt);
+                const unsigned row = (unsigned)(rb0 + 8 * (e >> 2) + (e & 3));
                 if (pos < T::SCAP) {
-                  lds_write_u32(&srow[pos], (unsigned)(rb0 + 8 * (e >> 2) + (e & 3)));
+                  lds_write_u32(&srow[pos], row);
                   lds_write_u32(&sval[pos], __float_as_uint(val(j, e)));
                   lds_write_u32(&scol[pos], (unsigned)gq);
                 } else {
-                  lds_write_u32(&scnt[1], 1u);
+                  // staging full: straight to the row's region, so that a
+                  // region's count never exceeds the entries written to it
+                  // (before the capacity): the selection reads that many
+                  const int rs = a.rstride ? a.rstride : a.parts;
+                  const int64_t slot =
+                      (int64_t)row * rs + a.parts + (qoff / a.rows_per_part);
+                  const int p2 = atomicAdd(a.cnt + slot, 1);
+                  if (p2 < a.cap) {
+                    uint2 cv;
+                    cv.x = __float_as_uint(val(j, e));
+                    cv.y = (unsigned)gq;
+                    *(uint2*)(a.cand + (slot * a.cap + p2) * 2) = cv;
+                  }
                 }
               }
               base += __popcll(tm[e]);
@@ -676,10 +690,6 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
           cv.y = scol[i];
           *(uint2*)(a.cand + (slot * a.cap + pos) * 2) = cv;
         }
-      }
-      if (scnt[1]) {  // staging overflowed: these rows' regions must not certify
-        for (int r = row_begin + tid; r < row_end; r += NT)
-          atomicMax(a.cnt + (int64_t)r * rs + creg, a.cap + 1);
       }
     }
   }
