@@ -431,12 +431,15 @@ ExactArgs run_exact_fallback(ExactArgs e, int nflag, int64_t total_rows, int nra
   ThreadCtx& c = ctx();
   const int kr = std::min(e.k_req, exact_kmax());
   // segment rows: few queries get short segments, so that the launch still
-  // spreads over about 1,024 workgroups (allocation-free calls keep the
-  // longest, which their scratch sizing assumes)
+  // spreads over about 1,024 workgroups, but not so short that the keys below
+  // the segments' common bound (about total_rows * kr / seg of them) overflow
+  // half the merge's sort; allocation-free calls keep the longest, which their
+  // scratch sizing assumes
   int seg = exact_seg_rows();
   if (!c.arena) {
-    const int64_t want = ceil_div(total_rows, std::max<int64_t>(1, 1024 / nflag));
-    seg = (int)std::min<int64_t>(seg, std::max<int64_t>(256, round_up(want, 256)));
+    const int64_t par = ceil_div(total_rows, std::max<int64_t>(1, 1024 / nflag));
+    const int64_t mrg = ceil_div(2 * total_rows * kr, (int64_t)exact_merge_cap());
+    seg = (int)std::min<int64_t>(seg, std::max<int64_t>(256, round_up(std::max(par, mrg), 256)));
   }
   const int max_segments = (int)ceil_div(total_rows, seg) + nranges;
   int CH = 2048;

@@ -18,7 +18,7 @@ constexpr int SEL_CAP = 1024;   // max candidates kept per query (LDS sort size)
 constexpr int EX_BATCH = 256;   // rows per batch in the exact fallback
 constexpr int EX_KMAX = 256;    // max k_req of the exact fallback
 constexpr int EX_SEG = 4096;    // rows per block of the exact fallback (at most)
-constexpr int EX_MCAP = 2048;   // keys the parallel merge collects per query
+constexpr int EX_MCAP = 4096;   // keys the parallel merge collects per query
 
 __global__ void fill_u64_kernel(unsigned long long* p, int n, unsigned long long v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -632,6 +632,7 @@ void launch_select_rerank(const SelectArgs& a, int nq_max, hipStream_t st) {
 }
 
 int exact_seg_rows() { return EX_SEG; }
+int exact_merge_cap() { return EX_MCAP; }
 
 void launch_exact_topk(const ExactArgs& a, int nslots, hipStream_t st) {
   if (nslots <= 0 || a.nseg <= 0) return;

@@ -260,5 +260,6 @@ struct ExactArgs {
 // Two launches: per-segment exact top-k, then the per-query merge.
 void launch_exact_topk(const ExactArgs& a, int nslots, hipStream_t st);
 int exact_seg_rows();
+int exact_merge_cap();  // keys the merge sorts at once per query
 
 }  // namespace mann
