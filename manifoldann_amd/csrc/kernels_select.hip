@@ -18,7 +18,8 @@ constexpr int SEL_CAP = 1024;   // max candidates kept per query (LDS sort size)
 constexpr int EX_BATCH = 256;   // rows per batch in the exact fallback
 constexpr int EX_KMAX = 256;    // max k_req of the exact fallback
 constexpr int EX_SEG = 4096;    // rows per block of the exact fallback (at most)
-constexpr int EX_MCAP = 4096;   // keys the parallel merge collects per query
+constexpr int EX_MCAP = 4096;   // keys the merge holds per query
+constexpr int EX_MPER = 8;      // keys each merge thread reads per round
 
 __global__ void fill_u64_kernel(unsigned long long* p, int n, unsigned long long v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -544,49 +545,55 @@ __global__ __launch_bounds__(256) void exact_merge_kernel(ExactArgs a) {
   const int kr = a.k_req < EX_KMAX ? a.k_req : EX_KMAX;
   __shared__ uint64_t m[EX_MCAP];
   __shared__ int s_n;
+  __shared__ uint64_t s_cut;
   const uint64_t* part = a.partial + (int64_t)blockIdx.x * a.nseg * kr;
-  // every segment key at or below the bound, collected in parallel: the kr
-  // smallest keys are among them
-  const uint64_t bnd = (uint64_t)a.bound[blockIdx.x];
-  if (tid == 0) s_n = 0;
+  const int64_t total = (int64_t)a.nseg * kr;
+  // Every segment key at or below the cut is collected in parallel, EX_MPER
+  // keys per thread per round; the kr smallest keys are among them.  The cut
+  // starts at the bound (another segment's kr-th key) and, whenever the
+  // buffer has no room for another round, the buffer is sorted, cut back to
+  // its kr smallest and the cut lowered to the kr-th of them.
+  if (tid == 0) {
+    s_n = 0;
+    s_cut = (uint64_t)a.bound[blockIdx.x];
+  }
   for (int i = tid; i < EX_MCAP; i += 256) m[i] = ~0ull;
   __syncthreads();
-  for (int64_t i = tid; i < (int64_t)a.nseg * kr; i += 256) {
-    const uint64_t key = part[i];
-    if (key != ~0ull && key <= bnd) {
-      const int p = atomicAdd(&s_n, 1);
-      if (p < EX_MCAP) m[p] = key;
+  for (int64_t b0 = 0; b0 < total; b0 += 256 * EX_MPER) {
+    const uint64_t cut = s_cut;
+    uint64_t key[EX_MPER];
+#pragma unroll
+    for (int u = 0; u < EX_MPER; ++u) {
+      const int64_t i = b0 + u * 256 + tid;
+      key[u] = i < total ? part[i] : ~0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < EX_MPER; ++u)
+      if (key[u] != ~0ull && key[u] <= cut) m[atomicAdd(&s_n, 1)] = key[u];
+    __syncthreads();
+    const bool full = s_n > EX_MCAP - 256 * EX_MPER;
+    __syncthreads();  // every thread has read s_n before the next round adds to it
+    if (full) {
+      bitonic_sort(m, EX_MCAP);
+      for (int i = kr + tid; i < EX_MCAP; i += 256) m[i] = ~0ull;
+      const uint64_t kth = m[kr - 1];
+      __syncthreads();
+      if (tid == 0) {
+        s_n = kr;
+        if (kth < s_cut) s_cut = kth;
+      }
+      __syncthreads();
     }
   }
-  __syncthreads();
-  const int nc = s_n;
-  if (nc <= EX_MCAP) {
+  {
     int P = 256;
-    while (P < nc) P <<= 1;
+    while (P < s_n) P <<= 1;
     bitonic_sort(m, P);
-    __syncthreads();
     // the kr smallest now lead m: move them where the write-out reads them
     const uint64_t keep = tid < kr ? m[tid] : ~0ull;
     __syncthreads();
     if (tid < kr) m[EX_BATCH + tid] = keep;
     __syncthreads();
-  } else {
-    // more keys below the bound than the buffer holds: the sequential merge
-    for (int i = tid; i < EX_BATCH + EX_KMAX; i += 256) m[i] = ~0ull;
-    __syncthreads();
-    for (int sg = 0; sg < a.nseg; ++sg) {
-      const uint64_t* src = part + (int64_t)sg * kr;
-      if (src[0] == ~0ull || src[0] >= m[EX_BATCH + kr - 1]) continue;  // uniform per block
-      if (tid < kr) m[tid] = src[tid];
-      __syncthreads();
-      bitonic_sort(m, EX_BATCH + EX_KMAX);
-      const uint64_t keep = tid < kr ? m[tid] : ~0ull;
-      __syncthreads();
-      for (int i = tid; i < EX_BATCH + EX_KMAX; i += 256) m[i] = ~0ull;
-      __syncthreads();
-      if (tid < kr) m[EX_BATCH + tid] = keep;
-      __syncthreads();
-    }
   }
   if (tid == 0) {
     int o = 0;

@@ -98,6 +98,28 @@ def test_ties_and_fallback_paths():
     assert st[3] == 4, st  # all four went to the exact fallback
 
 
+def test_fallback_merge_compacts_many_keys():
+    # 2,500 copies of one point spread over 300,000 rows, k = 200: the tie at
+    # the k-th place sends every query to the exact fallback, and every copy is
+    # at or below the segments' shared bound, so the merge holds more keys than
+    # its buffer and must compact it on the way; the lowest ids must come back
+    _banner("fallback merge 300000x16 k=200, 2500 copies")
+    rng = np.random.default_rng(11)
+    X = rng.standard_normal((300_000, 16)).astype(np.float32)
+    p = rng.standard_normal(16).astype(np.float32)
+    X[rng.choice(300_000, 2500, replace=False)] = p
+    Q = np.stack([p, p + 0.01 * rng.standard_normal(16), p + 0.02 * rng.standard_normal(16)])
+    Q = Q.astype(np.float32)
+    index = M.build_index(M.BruteForceIndex, X.T)
+    ids, dist = M.query_arrays(index, X.T, Q.T, 200)
+    st = M.last_stats()
+    print(f"stats {st}", flush=True)
+    oi, od = O.bf_knn(X, Q, 200)
+    assert (ids == oi).all(), np.argwhere(ids != oi)[:5]
+    assert (dist.view(np.uint32) == od.view(np.uint32)).all()
+    assert st[3] == 3, st
+
+
 def test_device_queries_torch():
     torch = pytest.importorskip("torch")
     _banner("torch device tensors")
