@@ -395,7 +395,8 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
         check(L.mann_profile_reset())
         check(L.mann_ivf_search(index.handle, Qd.data_ptr(), d, nq, d, 10, bc["nprobe"], 0, None,
                                 ids.data_ptr(), dist.data_ptr()))
-        kern = kernel_avg_ms(L, check, ["prep_queries", "ivf_route", "ivf_probe", "ivf_plan",
+        kern = kernel_avg_ms(L, check, ["prep_queries", "ivf_route", "ivf_probe",
+                                        "ivf_route_scan", "ivf_route_select", "ivf_plan",
                                         "ivf_residual", "ivf_tau_scan", "ivf_tau", "ivf_scan",
                                         "ivf_select", "ivf_exact_fallback"])
         ivf_stats = M.last_stats()

@@ -367,7 +367,11 @@ int mann_last_emitted(int64_t* total);
  * rows, 1 = bf16 scan of each query's first list, the default), "ivf_tau_grp"
  * (that scan stores the minimum of every 4 consecutive rows of a first list
  * of at least max(64, 4 k) rows instead of every value: 1, the default, or
- * 0), "tau_rows"
+ * 0), "ivf_route_cert" (centroid TopKRouting of the IVF searches from a bf16
+ * first pass over the centroids, with FP32 norms-form re-rank of the
+ * centroids its bound cannot exclude: 1, the default, or 0 = every routing
+ * distance in FP32; the probes are the same), "ivf_route_cfg" (tile
+ * configuration of that first pass, default 0), "tau_rows"
  * (rows of that exact sample, 16..1024), "fold_norms" (exact kNN: 1 = when a
  * dataset has 6 spare padded columns, carry the norms there so that the MFMA
  * produces the first-pass distances itself; read when a dataset is created),

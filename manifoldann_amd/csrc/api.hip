@@ -1240,6 +1240,13 @@ int mann_tune_set(const char* key, int64_t value) {
     } else if (k == "ivf_tau_grp") {
       MANN_REQUIRE(value == 0 || value == 1, ERR_ARGUMENT, "ivf_tau_grp must be 0 or 1");
       t.ivf_tau_grp = (int)value;
+    } else if (k == "ivf_route_cert") {
+      MANN_REQUIRE(value == 0 || value == 1, ERR_ARGUMENT, "ivf_route_cert must be 0 or 1");
+      t.ivf_route_cert = (int)value;
+    } else if (k == "ivf_route_cfg") {
+      MANN_REQUIRE(value >= 0 && value < scan_num_cfg(), ERR_ARGUMENT,
+                   "ivf_route_cfg must be a tile configuration index");
+      t.ivf_route_cfg = (int)value;
     } else if (k == "tau_rows") {
       MANN_REQUIRE(value >= 16 && value <= 1024, ERR_ARGUMENT, "tau_rows must be in [16, 1024]");
       t.tau_rows = (int)value;
@@ -1291,6 +1298,8 @@ int mann_tune_get(const char* key, int64_t* value) {
     else if (k == "tau_rows") *value = t.tau_rows;
     else if (k == "ivf_tau_mode") *value = t.ivf_tau_mode;
     else if (k == "ivf_tau_grp") *value = t.ivf_tau_grp;
+    else if (k == "ivf_route_cert") *value = t.ivf_route_cert;
+    else if (k == "ivf_route_cfg") *value = t.ivf_route_cfg;
     else if (k == "sample_grp") *value = t.sample_grp;
     else if (k == "sample_rows") *value = t.sample_rows;
     else if (k == "emit_per_k") *value = t.emit_per_k;

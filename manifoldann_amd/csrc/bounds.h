@@ -30,11 +30,37 @@ __device__ inline double exclusion_threshold(double dk, int metric, double qe, d
     double delta = (2.0 * d + 16.0) * u;
     L = sqrt(2.0 * (dk + delta)) + qe + ex;
   } else {
+    // (R32 bounds the uncentred |q| + |x| only when the bf16 operands are
+    // not centred; norms_form_threshold below takes the uncentred norms)
     double R32 = R + qe + ex;
     double H = (d + 8.0) * u * R32 * R32;
     double f = dk / (1.0 - 8.0 * u);
     L = sqrt(f * f + H) + qe + ex;
   }
+  return (L * L + G) * (1.0 + 1e-9) + 1e-30;
+}
+
+// Norms form (kmeans/distance.jl:99-101): given a first-pass value v, every row
+// whose first-pass value is at least the result has an FP32 norms-form
+// distance strictly above that of any row whose first-pass value is at most v.
+// Ingredients: |S^ - |q~ - x~|^2| <= G = (2d + 8) u (|q~| + max|x~|)^2, the
+// bf16 rounding |q - q~| <= qe and |x - x~| <= ex, and the FP32 norms-form
+// error |F^2 - |q - x|^2| <= H = (d + 8) u (|q| + |x|)^2.  H takes the
+// uncentred norms (the bf16 operands may be centred, the FP32 form is not):
+// |q| from its A16 square qn2, and ra >= max |x|.
+__device__ inline double norms_form_threshold(double v, double qe, double qr, double qn2,
+                                              double ex, double rx, double ra, int d) {
+  const double u = 1.0 / 16777216.0;
+  const double R = qr + rx;
+  const double G = (2.0 * d + 8.0) * u * R * R;
+  const double qa = sqrt((qn2 > 0.0 ? qn2 : 0.0) * (1.0 + (d + 8.0) * u)) * (1.0 + 4.0 * u);
+  const double R32 = qa + ra;
+  const double H = (d + 8.0) * u * R32 * R32;
+  const double s = v + G;
+  const double dub = sqrt(s > 0.0 ? s : 0.0) + qe + ex;      // >= |q - x| of such a row
+  const double fub = sqrt(dub * dub + H) * (1.0 + 8.0 * u);  // >= its FP32 norms form
+  const double f2 = fub / (1.0 - 8.0 * u);
+  const double L = sqrt(f2 * f2 + H) + qe + ex;
   return (L * L + G) * (1.0 + 1e-9) + 1e-30;
 }
 

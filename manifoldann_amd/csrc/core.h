@@ -60,6 +60,15 @@ struct mann_ivf {
   int route_metric = 0;
   float* c32 = nullptr;
   float* cn2 = nullptr;
+  // bf16 first-pass operands of the first min(nb, nlist) centroids for the
+  // certified TopKRouting (null when it does not apply: cosine rows or
+  // centroids, or more centroids than one wavefront selects from): rows
+  // padded to r_pad with |c~|^2 = +inf, the maxima of |c - c~| and |c~|, and
+  // an upper bound of the centroids' norms
+  mann::bf16_t* rb = nullptr;
+  float* rn = nullptr;
+  int64_t r_pad = 0;
+  float rex = 0.0f, rrx = 0.0f, rra = 0.0f;
   // residual centres of the bf16 rows: the centroids, or for cosine (unit
   // rows) the unit centroids c / |c|
   float* cres = nullptr;
@@ -76,7 +85,7 @@ struct mann_ivf {
   ~mann_ivf() {
     (void)hipSetDevice(device);
     void* ps[] = {c32, cn2, cres != c32 ? cres : nullptr, list_off, perm, bucket_of, lorder,
-                  x32, xb, xn, xn2};
+                  x32, xb, xn, xn2, rb, rn};
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }

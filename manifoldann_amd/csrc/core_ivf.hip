@@ -230,6 +230,26 @@ mann_ivf* ivf_build(const mann_dataset& ds, const float* C, int nlist, const int
     MANN_HIP(hipMemcpyAsync(iv->c32, C, sizeof(float) * (size_t)nlist * iv->d16,
                             hipMemcpyDeviceToDevice, st));
     launch_a16_norms(iv->c32, nlist, (int)iv->d16, iv->cn2, st);
+    // bf16 operands of the certified TopKRouting (norms form, rows not unit)
+    const int avail = std::min(nb, nlist);
+    if (!iv->normalized && (route_metric == EUCLIDEAN || route_metric == SQEUCLIDEAN) &&
+        avail <= route_select_cap()) {
+      iv->r_pad = round_up(avail, 256);
+      MANN_HIP(hipMalloc(&iv->rb, sizeof(bf16_t) * (size_t)iv->r_pad * iv->kp));
+      MANN_HIP(hipMalloc(&iv->rn, sizeof(float) * (size_t)iv->r_pad));
+      float* t32 = c.buf("ivb_rt32").as<float>((size_t)iv->r_pad * iv->d16);
+      unsigned* rm = c.buf("ivb_rmax").as<unsigned>(3);
+      MANN_HIP(hipMemsetAsync(rm, 0, sizeof(unsigned) * 3, st));
+      launch_prep_rows(iv->c32, iv->d16, avail, iv->r_pad, (int)iv->d, (int)iv->d16, (int)iv->kp,
+                       0, t32, iv->rb, iv->rn, nullptr, nullptr, rm, rm + 1, st);
+      launch_norm_bound_max(iv->cn2, avail, (int)iv->d, rm + 2, st);
+      unsigned hr[3];
+      MANN_HIP(hipMemcpyAsync(hr, rm, sizeof(hr), hipMemcpyDeviceToHost, st));
+      MANN_HIP(hipStreamSynchronize(st));
+      std::memcpy(&iv->rex, &hr[0], 4);
+      std::memcpy(&iv->rrx, &hr[1], 4);
+      std::memcpy(&iv->rra, &hr[2], 4);
+    }
     // residual centres: unit rows residualise against the unit centroids
     // (any centre serves, distances being translation invariant; the nearer
     // one keeps the residuals, and so the certification band, small)
@@ -291,6 +311,81 @@ void ivf_probe(const mann_ivf& iv, const float* R, int nq, int nprobe, int* prob
                       st);
 }
 
+// TopKRouting (routing.jl:94-118) from a bf16 first pass over the centroids:
+// a DENSE scan of S^ for every (query, centroid), then route_select_kernel
+// re-ranks in FP32 only the centroids the norms-form bound cannot exclude.
+// The probes equal ivf_route + ivf_probe's (same keys, same tie order).
+static bool route_certified_applies(const mann_ivf& iv, int nprobe, float mass) {
+  return iv.rb != nullptr && mass <= 0.0f && nprobe <= 64 && tuning().ivf_route_cert;
+}
+
+static void ivf_route_certified(const mann_ivf& iv, const QueryOps& Q, int nprobe, int* probes,
+                                int* pcount, int* prows) {
+  ThreadCtx& c = ctx();
+  hipStream_t st = c.stream;
+  const int nq = (int)Q.nq;
+  const int avail = std::min(iv.nb, iv.nlist);
+  const int cfg = std::min(std::max(tuning().ivf_route_cfg, 0), scan_num_cfg() - 1);
+  const int BM = scan_tile_rows(cfg), BN = scan_tile_queries(cfg);
+  const int64_t ldd = round_up(avail, 4);
+  // query chunks of at most 2^26 values
+  const int qc = (int)std::min<int64_t>(
+      nq, std::max<int64_t>(BN, ((int64_t)1 << 26) / ldd / BN * BN));
+  float* dense = c.buf("iv_rdense").as<float>((size_t)qc * ldd);
+  for (int q0 = 0; q0 < nq; q0 += qc) {
+    const int nqc = std::min(qc, nq - q0);
+    ScanArgs a{};
+    a.cfg = cfg;
+    a.xb = iv.rb;
+    a.xn = iv.rn;
+    a.qb = Q.qb + (int64_t)q0 * iv.kp;
+    a.qnorm = Q.qn + q0;
+    a.kp = (int)iv.kp;
+    a.d = (int)iv.d;
+    a.nq = nqc;
+    // row partitions when the chunk has too few query tiles to fill the chip
+    const int qt = (int)ceil_div(nqc, BN);
+    const int pd = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(avail, BM),
+                                                                ceil_div(512, qt)));
+    a.rows_per_part = (int)round_up(ceil_div(avail, pd), BM);
+    a.parts = (int)ceil_div(avail, a.rows_per_part);
+    a.qtiles = qt;
+    a.n_rows = avail;
+    a.dense = dense;
+    a.ldd = ldd;
+    a.dvec = 1;  // ldd and the partitions' first rows are multiples of 4
+    {
+      ScopedTime tm("ivf_route_scan", st);
+      launch_scan(a, SCAN_DENSE, qt * a.parts, SCAN_KIND_GRID, st);
+    }
+    RouteSelArgs r{};
+    r.dense = dense;
+    r.ldd = ldd;
+    r.avail = avail;
+    r.nprobe = nprobe;
+    r.q0 = q0;
+    r.nq = nqc;
+    r.c32 = iv.c32;
+    r.cn2 = iv.cn2;
+    r.q32 = Q.q32;
+    r.qn2 = Q.qn2;
+    r.d16 = (int)iv.d16;
+    r.d = (int)iv.d;
+    r.sq = iv.route_metric == SQEUCLIDEAN ? 1 : 0;
+    r.qe = Q.qe;
+    r.qr = Q.qr;
+    r.ex = iv.rex;
+    r.rx = iv.rrx;
+    r.ra = iv.rra;
+    r.list_off = iv.list_off;
+    r.probes = probes;
+    r.pcount = pcount;
+    r.prows = prows;
+    ScopedTime tm("ivf_route_select", st);
+    launch_route_select(r, st);
+  }
+}
+
 static void ivf_search_sel(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int metric,
                            const float* route_in, int64_t* out_ids, float* out_dist, float mass,
                            ProbeState* ps) {
@@ -307,16 +402,20 @@ static void ivf_search_sel(const mann_ivf& iv, const QueryOps& Q, int k, int npr
   const int cfg = tuning().ivf_cfg;
   const int BN = scan_tile_queries(cfg);
   const int ROWS_ITEM = tuning().ivf_rows_item;  // rows per work item
-  const float* R = route_in;
-  if (!R) {
-    float* Rb = c.buf("iv_R").as<float>((size_t)nq * iv.nlist);
-    ivf_route(iv, Q, Rb);
-    R = Rb;
-  }
   int* probes = c.buf("iv_probes").as<int>((size_t)nq * nprobe);
   int* pcount = c.buf("iv_pcount").as<int>(nq);
   int* prows = c.buf("iv_prows").as<int>(nq);
-  ivf_probe(iv, R, nq, nprobe, probes, pcount, prows, mass);
+  if (!route_in && route_certified_applies(iv, nprobe, mass)) {
+    ivf_route_certified(iv, Q, nprobe, probes, pcount, prows);
+  } else {
+    const float* R = route_in;
+    if (!R) {
+      float* Rb = c.buf("iv_R").as<float>((size_t)nq * iv.nlist);
+      ivf_route(iv, Q, Rb);
+      R = Rb;
+    }
+    ivf_probe(iv, R, nq, nprobe, probes, pcount, prows, mass);
+  }
   if (ps) {
     ps->probes = probes;
     ps->pcount = pcount;

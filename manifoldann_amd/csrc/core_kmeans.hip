@@ -29,6 +29,7 @@ struct CentroidOps {
   float* cn = nullptr;      // |c~|^2, +inf on padding
   float* cn2 = nullptr;     // A16 |c|^2
   float ex = 0, rx = 0;
+  float ra = 0;  // upper bound of the largest uncentred centroid norm
 };
 
 CentroidOps prepare_centroids(const float* C, int k, const mann_dataset& ds, const char* tag) {
@@ -44,16 +45,18 @@ CentroidOps prepare_centroids(const float* C, int k, const mann_dataset& ds, con
   P.cn2 = c.buf(t + "_cn2").as<float>(P.k_pad);
   float* ce = c.buf(t + "_ce").as<float>(P.k_pad);
   float* cr = c.buf(t + "_cr").as<float>(P.k_pad);
-  unsigned* mx = c.buf(t + "_mx").as<unsigned>(2);
-  fill_i32((int*)mx, 2, 0, st);
+  unsigned* mx = c.buf(t + "_mx").as<unsigned>(3);
+  fill_i32((int*)mx, 3, 0, st);
   launch_prep_rows(C, ds.d16, k, P.k_pad, (int)ds.d, (int)ds.d16, (int)ds.kp, 0, P.c32, P.cb, P.cn,
                    ce, cr, mx, mx + 1, st, ds.center);
   launch_a16_norms(P.c32, k, (int)ds.d16, P.cn2, st);
-  unsigned h[2];
+  launch_norm_bound_max(P.cn2, k, (int)ds.d, mx + 2, st);
+  unsigned h[3];
   MANN_HIP(hipMemcpyAsync(h, mx, sizeof(h), hipMemcpyDeviceToHost, st));
   MANN_HIP(hipStreamSynchronize(st));
   memcpy(&P.ex, &h[0], 4);
   memcpy(&P.rx, &h[1], 4);
+  memcpy(&P.ra, &h[2], 4);
   return P;
 }
 
@@ -88,7 +91,7 @@ void kmeans_estep(const mann_dataset& ds, const float* C, int k, int* assign, fl
   t.overflow = ovf;
   if (primed) {
     ScopedTime tm("kmeans_prime", st);
-    launch_prime_threshold(t, ds.err, ds.nrm, P.ex, P.rx, (int)ds.d, tau, st);
+    launch_prime_threshold(t, ds.err, ds.nrm, P.ex, P.rx, P.ra, (int)ds.d, tau, st);
   } else {
     fill_f32(tau, n, INFINITY, st);
   }
@@ -110,8 +113,10 @@ void kmeans_estep(const mann_dataset& ds, const float* C, int k, int* assign, fl
   a.qnorm = ds.xn;
   a.t1_qe = ds.err;
   a.t1_qr = ds.nrm;
+  a.t1_qn2 = ds.xn2;
   a.t1_ex = P.ex;
   a.t1_rx = P.rx;
+  a.t1_ra = P.ra;
   {
     ScopedTime tm("kmeans_assign_scan", st);
     launch_scan(a, SCAN_TOP1, (int)ceil_div(n, scan_tile_queries(a.cfg)), SCAN_KIND_GRID, st);

@@ -42,6 +42,30 @@ void launch_route_dist(const float* q32, const float* qn2, int nq, const float* 
 void launch_probe_select(const float* R, int nq, int nlist, int avail, int nprobe, float mass,
                          const int* list_off, int* probes, int* pcount, int* prows,
                          hipStream_t st);
+// TopKRouting over the dense bf16 first pass of the first `avail` centroids
+// (certified: the probes equal those of the exact routing distances), queries
+// [q0, q0 + nq) of the batch; dense row qi holds query q0 + qi.
+struct RouteSelArgs {
+  const float* dense;
+  int64_t ldd;
+  int avail, nprobe, q0, nq;
+  const float* c32;  // centroids, d16 floats each, and their A16 |c|^2
+  const float* cn2;
+  const float* q32;  // queries, d16 floats each, and their A16 |q|^2
+  const float* qn2;
+  int d16, d;
+  int sq;            // 1: SQEUCLIDEAN (no square root)
+  const float* qe;   // query bf16 rounding error and rounded norm
+  const float* qr;
+  float ex, rx;      // centroid maxima of the same
+  float ra;          // upper bound of the centroids' norms
+  const int* list_off;
+  int* probes;
+  int* pcount;
+  int* prows;
+};
+int route_select_cap();  // largest avail
+void launch_route_select(const RouteSelArgs& a, hipStream_t st);
 void launch_ivf_plan(const int* probes, const int* pcount, int nq, int nprobe, int nb,
                      const int* list_off, const int* lorder, int bn, int rows_item, int* qcount,
                      int* qfill, int* qoff, int* ioff, int* qlist, int* qlist_list,

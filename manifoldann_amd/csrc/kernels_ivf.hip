@@ -258,6 +258,117 @@ __global__ __launch_bounds__(256) void probe_select_wave_kernel(
   }
 }
 
+// TopKRouting from a bf16 first pass (certified; the result equals the exact
+// route followed by probe_select_wave_kernel).  One wavefront per query over
+// the dense first-pass values S^ of the first `avail` centroids:
+// 1. the kk-th smallest S^ (kk = min(nprobe, avail)) and its norms-form
+//    exclusion threshold T: every centroid with S^ > T has an FP32 norms-form
+//    distance above that of each of the kk centroids at or below the kk-th
+//    S^, so it cannot be among the kk nearest;
+// 2. the exact FP32 norms form (route_dist_kernel's arithmetic: A16 dot, then
+//    (|c|^2 + |q|^2) - 2 c.q, clamped, square root unless SQEUCLIDEAN) of every
+//    centroid with S^ <= T;
+// 3. the kk smallest (distance, centroid) keys in order, empty buckets
+//    skipped, as probe_select_wave_kernel writes them.
+constexpr int RS_CAP = 64 * PS_KPL;  // centroids per query (avail <= RS_CAP)
+__global__ __launch_bounds__(256) void route_select_kernel(RouteSelArgs a) {
+  __shared__ uint32_t buf_all[4][256];
+  __shared__ unsigned hist_all[4][256];
+  __shared__ int cidx_all[4][RS_CAP];
+  __shared__ float cval_all[4][RS_CAP];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int qi = blockIdx.x * 4 + w;
+  if (qi >= a.nq) return;
+  const int q = a.q0 + qi;
+  int* cidx = cidx_all[w];
+  float* cval = cval_all[w];
+  const float* row = a.dense + (int64_t)qi * a.ldd;
+  const int avail = a.avail;
+  const int kk = a.nprobe < avail ? a.nprobe : avail;
+  float sv[PS_KPL];
+  int lsz[PS_KPL];
+#pragma unroll
+  for (int u = 0; u < PS_KPL; ++u) {
+    const int i = lane + 64 * u;
+    const int ic = min(i, avail - 1);
+    sv[u] = i < avail ? row[i] : INFINITY;
+    lsz[u] = a.list_off[ic + 1] - a.list_off[ic];
+  }
+  const uint32_t kkey = wave_rank_key(row, avail, kk, buf_all[w], hist_all[w], lane);
+  const double T = norms_form_threshold((double)ord2f(kkey), a.qe[q], a.qr[q], a.qn2[q], a.ex,
+                                        a.rx, a.ra, a.d);
+  const float Tf = T < 3.0e38 ? nextafterf((float)T * (1.0f + 1e-6f), INFINITY) : INFINITY;
+  // candidates, compacted in lane order
+  int pos[PS_KPL];
+  int cnt = 0;
+#pragma unroll
+  for (int u = 0; u < PS_KPL; ++u) {
+    const int i = lane + 64 * u;
+    const bool c = i < avail && sv[u] <= Tf;
+    const uint64_t m = __ballot(c);
+    const int p = cnt + __popcll(m & ((1ull << lane) - 1ull));
+    pos[u] = c ? p : -1;
+    if (c) cidx[p] = i;
+    cnt += __popcll(m);
+  }
+  wave_fence_sync();
+  // exact norms form of each candidate, one per 16-lane group
+  const int g = lane >> 4, j = lane & 15;
+  const float* y = a.q32 + (int64_t)q * a.d16;
+  for (int i0 = g; i0 < cnt; i0 += 4) {
+    const int c = cidx[i0];
+    const float* x = a.c32 + (int64_t)c * a.d16;
+    float acc = 0.0f;
+    for (int t = j; t < a.d16; t += 16) acc = __builtin_fmaf(x[t], y[t], acc);
+    const float dot = group16_tree(acc);
+    float tt = a.cn2[c] + a.qn2[q];
+    float sq = tt - 2.0f * dot;
+    sq = sq > 0.0f ? sq : 0.0f;
+    if (j == 0) cval[i0] = a.sq ? sq : __builtin_sqrtf(sq);
+  }
+  wave_fence_sync();
+  uint64_t key[PS_KPL];
+  uint64_t lmin = ~0ull;
+#pragma unroll
+  for (int u = 0; u < PS_KPL; ++u) {
+    key[u] = pos[u] >= 0 ? pack_key(cval[pos[u]], (uint32_t)(lane + 64 * u)) : ~0ull;
+    lmin = key[u] < lmin ? key[u] : lmin;
+  }
+  int m = 0, rows = 0;
+  for (int r = 0; r < kk; ++r) {
+    uint64_t gk = lmin;
+    for (int o = 32; o >= 1; o >>= 1) {
+      const uint64_t t = __shfl_xor(gk, o);
+      gk = t < gk ? t : gk;
+    }
+    const bool own = lmin == gk;
+    int osz = 0;
+    if (own) {  // the owner removes it (keys are unique: centroid in the low bits)
+      lmin = ~0ull;
+#pragma unroll
+      for (int u = 0; u < PS_KPL; ++u) {
+        if (key[u] == gk) {
+          key[u] = ~0ull;
+          osz = lsz[u];
+        }
+        lmin = key[u] < lmin ? key[u] : lmin;
+      }
+    }
+    const int b = (int)(gk & 0xffffffffu);
+    const int sz = __shfl(osz, __ffsll((long long)__ballot(own)) - 1);
+    if (sz > 0) {
+      if (lane == 0) a.probes[(int64_t)q * a.nprobe + m] = b;
+      ++m;
+      rows += sz;
+    }
+  }
+  if (lane == 0) {
+    for (int r = m; r < a.nprobe; ++r) a.probes[(int64_t)q * a.nprobe + r] = -1;
+    a.pcount[q] = m;
+    a.prows[q] = rows;
+  }
+}
+
 // per-list count of probing queries
 __global__ void list_count_kernel(const int* __restrict__ probes, const int* __restrict__ pcount,
                                   int nq, int nprobe, int* __restrict__ qcount) {
@@ -800,6 +911,13 @@ void launch_probe_select(const float* R, int nq, int nlist, int avail, int nprob
   else
     hipLaunchKernelGGL(probe_select_kernel, dim3(nq), dim3(256), 0, st, R, nq, nlist, avail,
                        nprobe, list_off, probes, pcount, prows);
+}
+
+int route_select_cap() { return RS_CAP; }
+
+void launch_route_select(const RouteSelArgs& a, hipStream_t st) {
+  if (a.nq <= 0) return;
+  hipLaunchKernelGGL(route_select_kernel, dim3((unsigned)ceil_div(a.nq, 4)), dim3(256), 0, st, a);
 }
 
 void launch_ivf_plan(const int* probes, const int* pcount, int nq, int nprobe, int nb,

@@ -131,7 +131,7 @@ __global__ __launch_bounds__(256) void top1_select_long_kernel(Top1Args a) {
 // minimum), so the scan emits only centroids that can still win.
 __global__ __launch_bounds__(256) void prime_threshold_kernel(Top1Args a, const float* qe,
                                                               const float* qr,
-                                                              float ex, float rx, int d,
+                                                              float ex, float rx, float ra, int d,
                                                               float* tau) {
   const int j = threadIdx.x & 15;
   const int64_t p = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
@@ -143,7 +143,9 @@ __global__ __launch_bounds__(256) void prime_threshold_kernel(Top1Args a, const 
     const double u = 1.0 / 16777216.0;
     double R = (double)qr[p] + rx;
     double G = (2.0 * d + 8.0) * u * R * R;
-    double R32 = R + qe[p] + ex;
+    // FP32 norms-form error: uncentred norms (the bf16 operands may be centred)
+    double qa = sqrt((double)a.xn2[p] * (1.0 + (d + 8.0) * u)) * (1.0 + 4.0 * u);
+    double R32 = qa + ra;
     double H = (d + 8.0) * u * R32 * R32;
     double f = (double)v / (1.0 - 8.0 * u);
     double L = sqrt(f * f + H) + qe[p] + ex;
@@ -514,10 +516,10 @@ void launch_top1_select(const Top1Args& a, hipStream_t st) {
 }
 
 void launch_prime_threshold(const Top1Args& a, const float* qe, const float* qr, float ex, float rx,
-                            int d, float* tau, hipStream_t st) {
+                            float ra, int d, float* tau, hipStream_t st) {
   if (a.n <= 0) return;
   hipLaunchKernelGGL(prime_threshold_kernel, dim3((unsigned)ceil_div(a.n, 16)), dim3(256), 0, st,
-                     a, qe, qr, ex, rx, d, tau);
+                     a, qe, qr, ex, rx, ra, d, tau);
 }
 
 int sort_block() { return SORT_BLOCK; }

@@ -75,6 +75,19 @@ __global__ __launch_bounds__(256) void prep_rows_kernel(const float* __restrict_
   }
 }
 
+// Upper bound of the largest norm sqrt(n2[i]) of n vectors given their A16
+// FP32 squared norms (relative error below (d + 8) u), as float bits for
+// atomicMax (non-negative floats order like their bits).
+__global__ void norm_bound_max_kernel(const float* __restrict__ n2, int64_t n, int d,
+                                      unsigned* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double u = 1.0 / 16777216.0;
+  const double v = n2[i] > 0.0f ? (double)n2[i] : 0.0;
+  const float b = (float)(sqrt(v * (1.0 + (d + 8.0) * u)) * (1.0 + 4.0 * u)) + 1e-30f;
+  atomicMax(out, __float_as_uint(b));
+}
+
 // A16-order FP32 squared norm of each row of x32 (16 lanes per row).
 __global__ __launch_bounds__(256) void a16_norms_kernel(const float* __restrict__ x32, int64_t n,
                                                         int d16, float* __restrict__ out) {
@@ -171,6 +184,12 @@ __global__ void mean_from_sum_kernel(const double* __restrict__ sum, int64_t n, 
                                      float* __restrict__ mean) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < d) mean[c] = (float)(sum[c] / (double)n);
+}
+
+void launch_norm_bound_max(const float* n2, int64_t n, int d, unsigned* out, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(norm_bound_max_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, n2,
+                     n, d, out);
 }
 
 void launch_mean_rows(const float* x, int64_t ld, int64_t n, int d, double* tmp, float* mean,

@@ -23,8 +23,9 @@ struct Top1Args {
 };
 
 void launch_top1_select(const Top1Args& a, hipStream_t st);
+// ra: an upper bound of the centroids' uncentred norms
 void launch_prime_threshold(const Top1Args& a, const float* qe, const float* qr, float ex, float rx,
-                            int d, float* tau, hipStream_t st);
+                            float ra, int d, float* tau, hipStream_t st);
 int sort_block();
 // ints of the `table` scratch of launch_counting_sort (block histograms and
 // the scan's tile sums)

@@ -75,10 +75,14 @@ struct ScanArgs {
   // past it do so (shorter lists store every value); otherwise blocks count
   // from row 0 and every item does (dense_grp = 1)
   int dense_grp;
-  // TOP1 certification inputs
+  // TOP1 certification inputs: the queries' bf16 rounding errors and rounded
+  // norms, their A16 |x|^2 (the FP32 norms form's error scales with the
+  // uncentred norms), the rows' maxima of the first two and an upper bound of
+  // the rows' uncentred norms
   const float* t1_qe;
   const float* t1_qr;
-  float t1_ex, t1_rx;
+  const float* t1_qn2;
+  float t1_ex, t1_rx, t1_ra;
   // Folded norms (FILTER / DENSE): the operands carry |x~|^2 and |q~|^2 in
   // spare K columns and the query side is -2 q~, so the MFMA accumulates S^
   // itself; xn and qnorm are not read
@@ -113,6 +117,10 @@ struct Tuning {
   int shortlist_mult = 1;    // first re-rank batch = shortlist_mult * k_req (>= 32)
   int tau_rows = 1024;       // IVF: exact rows sampled per query for the emission threshold
   int ivf_tau_mode = 1;      // IVF threshold: 0 exact FP32 row sample, 1 bf16 scan of the first list
+  int ivf_route_cert = 1;    // IVF: TopKRouting from a bf16 first pass over the centroids
+                             // with FP32 re-rank of the uncertain ones (0: every routing
+                             // distance in FP32)
+  int ivf_route_cfg = 0;     // tile config of that first pass
   int ivf_tau_grp = 1;       // IVF first-list scan: store 4-row minima of lists of at least
                              // max(64, 4 k) rows (0: every value)
   int sample_rows = 2048;    // exact kNN: stride-sample rows (fixed when a dataset is created)
@@ -139,6 +147,9 @@ void launch_prep_rows(const float* src, int64_t ld, int64_t n, int64_t n_pad, in
                       float* nrm, unsigned* max_err, unsigned* max_nrm, hipStream_t st,
                       const float* center = nullptr, const int* center_row = nullptr,
                       int64_t center_ld = 0);
+// atomicMax into *out (float bits, start at 0) of an upper bound of the largest
+// norm sqrt(n2[i]), n2 holding A16 FP32 squared norms of d-vectors
+void launch_norm_bound_max(const float* n2, int64_t n, int d, unsigned* out, hipStream_t st);
 // FP32 mean of n rows (FP64 sums); tmp holds d doubles
 void launch_mean_rows(const float* x, int64_t ld, int64_t n, int d, double* tmp, float* mean,
                       hipStream_t st);

@@ -30,6 +30,7 @@
 // Layout: WR x WC waves (rows x queries); a wave owns 32*WM rows x 32*WN
 // queries, i.e. WM x WN MFMA blocks.  MFMA 32x32 output: lane l holds query
 // column l%32 and rows 8*(e/4) + 4*(l/32) + e%4 of register e.
+#include "bounds.h"
 #include "launch.h"
 
 namespace mann {
@@ -100,21 +101,10 @@ __device__ __forceinline__ int swz(int r) {
 // Exclusion threshold (squared-distance domain) for the smallest norms-form
 // distance, given the first-pass value v of an emitted candidate: any centroid
 // whose first-pass value is at least the result has an FP32 norms-form distance
-// strictly above the candidate's.  Derivation: |S^ - |q~ - x~|^2| <= G, triangle
-// inequality for the bf16 rounding (qe + ex), FP32 norms-form error H.
-__device__ float top1_threshold(float v, float qe, float qr, float ex, float rx, int d) {
-  const double u = 1.0 / 16777216.0;
-  double R = (double)qr + rx;
-  double G = (2.0 * d + 8.0) * u * R * R;
-  double R32 = R + qe + ex;
-  double H = (d + 8.0) * u * R32 * R32;
-  double s = (double)v + G;
-  double dub = sqrt(s > 0.0 ? s : 0.0) + qe + ex;       // upper bound of |q - x|
-  double fub = sqrt(dub * dub + H) * (1.0 + 8.0 * u);   // upper bound of FP32 F(v)
-  double f2 = fub / (1.0 - 8.0 * u);
-  double L = sqrt(f2 * f2 + H) + qe + ex;
-  double t = (L * L + G) * (1.0 + 1e-9) + 1e-30;
-  return (float)t * (1.0f + 1e-6f);
+// strictly above the candidate's (norms_form_threshold).
+__device__ float top1_threshold(float v, float qe, float qr, float qn2, float ex, float rx,
+                                float ra, int d) {
+  return (float)norms_form_threshold(v, qe, qr, qn2, ex, rx, ra, d) * (1.0f + 1e-6f);
 }
 
 // 16-byte global->LDS load per lane with cache-policy bits AUX (gfx950: 2 =
@@ -214,6 +204,7 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
   // TOP1: the query's rounding norms for the certification band, loaded here
   // (a global load in the epilogue would wait for every store before it)
   float t1e[MODE == SCAN_TOP1 ? WN : 1], t1r[MODE == SCAN_TOP1 ? WN : 1];
+  float t1n[MODE == SCAN_TOP1 ? WN : 1];
 #pragma unroll
   for (int bn = 0; bn < WN; ++bn) {
     int c = wc * 32 * WN + bn * 32 + (lane & 31);
@@ -226,6 +217,7 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
     if (MODE == SCAN_TOP1) {
       t1e[bn] = gq >= 0 ? a.t1_qe[gq] : 0.0f;
       t1r[bn] = gq >= 0 ? a.t1_qr[gq] : 0.0f;
+      t1n[bn] = gq >= 0 ? a.t1_qn2[gq] : 0.0f;
     }
   }
 
@@ -573,7 +565,9 @@ __device__ __forceinline__ void scan_item(const ScanArgs& a, int qoff, int nqi, 
           ++pos;
           if (MODE == SCAN_TOP1) {
             float th = top1_threshold(w[e], t1e[MODE == SCAN_TOP1 ? j : 0],
-                                      t1r[MODE == SCAN_TOP1 ? j : 0], a.t1_ex, a.t1_rx, a.d);
+                                      t1r[MODE == SCAN_TOP1 ? j : 0],
+                                      t1n[MODE == SCAN_TOP1 ? j : 0], a.t1_ex, a.t1_rx,
+                                      a.t1_ra, a.d);
             tau_l[j] = fminf(tau_l[j], th);
           }
         }
