@@ -295,6 +295,41 @@ int mann_ivf_search_ex(mann_ivf_t ivf, const float* queries, int64_t d, int64_t 
                        int64_t k, int64_t nprobe, int metric, const float* route_dist, int merge,
                        int probe_order, int64_t* out_ids, float* out_dist);
 
+/* build_ivf_hnsw_index's terminals (src/indices/multilevel/ivf_hnsw.jl:46-88:
+ * TerminalConfig(HNSWIndex, (M, ef_construction, ef_search, neighbor_policy)),
+ * built per child by build_index(HNSWIndex, child data) src/indices/hnsw/
+ * index.jl:10-67): one HNSW graph per non-empty list of an IVF handle
+ * (mann_ivf_build), over the list's points in ascending id, by the serial
+ * insertion of src/indices/hnsw/query.jl:264-325 (threaded = false; the
+ * threaded build is not deterministic).  neighbor_policy: 0 :heuristic, 1
+ * :diversified (neighbor_policy.jl:22-112).  Levels: the reference's children
+ * draw from Random.default_rng(); here level = floor(-log(max(u, eps)) /
+ * log(max(M, 2))) (planner.jl:19-30) over the splitmix64 stream of `seed`, one
+ * draw per point in list order (bucket order, ascending id).  Euclidean
+ * distance (the HNSWIndex default).  M <= 63, ef_construction <= 512.  A
+ * second call replaces the graphs. */
+int mann_ivf_hnsw_build(mann_ivf_t ivf, int64_t M, int64_t ef_construction, int neighbor_policy,
+                        uint64_t seed);
+/* query(::MultiLevelIndex{KMeans, TopK, HNSW}, data, Q, k; ef_search)
+ * (src/indices/multilevel/query.jl:279-512, single query :25-106): TopK routing
+ * (route_dist as in mann_ivf_search), empty buckets dropped, each probed list's
+ * HNSW query (src/indices/hnsw/query.jl:95-123: entry point marked, greedy
+ * descent, base-layer search with ef = max(ef_search, k), first min(k, found)),
+ * then the merge (MANN_MERGE_SIMPLE / MANN_MERGE_DISJOINT, merge.jl:79-169) of the
+ * per-list results in bucket order, or in probe order when probe_order = 1.
+ * Output k x nq as mann_ivf_search, 0 / +inf padded.  max(ef_search, k) <= 512. */
+int mann_ivf_hnsw_search(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq, int64_t ldq,
+                         int64_t k, int64_t nprobe, int64_t ef_search, const float* route_dist,
+                         int merge, int probe_order, int64_t* out_ids, float* out_dist);
+/* The graphs, as the reference's HNSWLayer slabs (hnsw/types.jl:18-49) over the
+ * list-contiguous rows: layers and M, then (each optional) out_nbr
+ * [layers][n][M + 1] local 0-based ids (the live prefix is out_deg's count; rows
+ * absent from a layer have degree 0), out_deg [layers][n], entry point (local
+ * id) and top layer per bucket (-1 when empty), level per row. */
+int mann_ivf_hnsw_graph(mann_ivf_t ivf, int64_t* out_layers, int64_t* out_M, int32_t* out_nbr,
+                        int32_t* out_deg, int32_t* out_entry, int32_t* out_max_layer,
+                        int32_t* out_level);
+
 /* Learned probe router -- no counterpart in the reference: an MLP
  * d -> hidden (ReLU) -> nlist whose scores rank the IVF lists of a query
  * (higher = probe first).  It plugs into the reference's AbstractTransform seam

@@ -15,6 +15,6 @@ from .index import (COSINE, EUCLIDEAN, SQEUCLIDEAN, BruteForceIndex, CosineDist,
                     last_emitted, last_stats,
                     neighbor_ids, node_metadata, query, query_arrays, recall_at_k,
                     symmetrize_adjacency)
-from .ivf import IVFFlatIndex, MultiLevelIndex
+from .ivf import HNSWIndex, IVFFlatIndex, MultiLevelIndex, build_ivf_hnsw_index
 
 __all__ = [n for n in dir() if not n.startswith("_")]

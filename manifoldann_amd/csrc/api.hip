@@ -970,6 +970,101 @@ static void search_ex(mann_ivf_t ivf, const float* queries, int64_t d, int64_t n
   }
 }
 
+int mann_ivf_hnsw_build(mann_ivf_t ivf, int64_t M, int64_t ef_construction, int neighbor_policy,
+                        uint64_t seed) {
+  return guarded([&] {
+    MANN_REQUIRE(ivf != nullptr, ERR_NULL, "null index");
+    MANN_REQUIRE(M > 0 && M < (1 << 30), ERR_ARGUMENT, "M must be positive");
+    MANN_REQUIRE(ef_construction >= 1 && ef_construction < (1 << 30), ERR_ARGUMENT,
+                 "ef_construction must be at least 1");
+    ivf_hnsw_build(*ivf, (int)M, (int)ef_construction, neighbor_policy, seed);
+  });
+}
+
+int mann_ivf_hnsw_search(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq, int64_t ldq,
+                         int64_t k, int64_t nprobe, int64_t ef_search, const float* route_dist,
+                         int merge, int probe_order, int64_t* out_ids, float* out_dist) {
+  return guarded([&] {
+    MANN_REQUIRE(ivf != nullptr, ERR_NULL, "null index");
+    MANN_REQUIRE(merge == MANN_MERGE_SIMPLE || merge == MANN_MERGE_DISJOINT, ERR_ARGUMENT,
+                 "unknown merge strategy");
+    MANN_REQUIRE(nprobe > 0, ERR_ARGUMENT,
+                 "nprobe must be positive (TopKRouting: k must be positive)");
+    MANN_REQUIRE(ef_search >= 1 && ef_search < (1 << 30), ERR_ARGUMENT,
+                 "ef_search must be at least 1");
+    QueryOps Q = ivf_queries(ivf, queries, d, nq, ldq);
+    if (nq == 0 || k <= 0) return;
+    MANN_REQUIRE(out_ids && out_dist, ERR_NULL, "null pointer");
+    MANN_REQUIRE(k < (1 << 30), ERR_ARGUMENT, "k too large");
+    ThreadCtx& c = ctx();
+    const float* R = route_dist;
+    if (R && !is_device_ptr(R)) {
+      float* rb = c.buf("hn_Rin").as<float>((size_t)nq * ivf->nlist);
+      MANN_HIP(hipMemcpyAsync(rb, route_dist, sizeof(float) * nq * ivf->nlist,
+                              hipMemcpyHostToDevice, c.stream));
+      R = rb;
+    }
+    OutBuf<int64_t> oi(out_ids, (size_t)(nq * k), "out_ids");
+    OutBuf<float> od(out_dist, (size_t)(nq * k), "out_dist");
+    ivf_hnsw_search(*ivf, Q, (int)k, (int)std::min<int64_t>(nprobe, 1 << 30),
+                    (int)ef_search, R, merge, probe_order != 0, oi.dev, od.dev);
+    oi.finish();
+    od.finish();
+    MANN_HIP(hipStreamSynchronize(c.stream));
+    timer_flush();
+  });
+}
+
+int mann_ivf_hnsw_graph(mann_ivf_t ivf, int64_t* out_layers, int64_t* out_M, int32_t* out_nbr,
+                        int32_t* out_deg, int32_t* out_entry, int32_t* out_max_layer,
+                        int32_t* out_level) {
+  return guarded([&] {
+    MANN_REQUIRE(ivf != nullptr, ERR_NULL, "null index");
+    MANN_REQUIRE(ivf->hn_nbr0 != nullptr, ERR_ARGUMENT, "the index has no HNSW terminals");
+    const int64_t n = ivf->n, L = ivf->hn_layers, cap = ivf->hn_M + 1, n1 = ivf->hn_n1;
+    if (out_layers) *out_layers = L;
+    if (out_M) *out_M = ivf->hn_M;
+    MANN_HIP(hipStreamSynchronize(ctx().stream));
+    auto get = [&](const int* src, int64_t count) {
+      std::vector<int> h((size_t)std::max<int64_t>(count, 1));
+      if (count > 0) MANN_HIP(hipMemcpy(h.data(), src, sizeof(int) * count, hipMemcpyDeviceToHost));
+      return h;
+    };
+    if (out_entry || out_max_layer) {
+      std::vector<int> e = get(ivf->hn_entry, ivf->nb), m = get(ivf->hn_maxl, ivf->nb);
+      for (int b = 0; b < ivf->nb; ++b) {
+        if (out_entry) out_entry[b] = e[b];
+        if (out_max_layer) out_max_layer[b] = m[b];
+      }
+    }
+    if (out_level) {
+      std::vector<int> lv = get(ivf->hn_level, n);
+      for (int64_t i = 0; i < n; ++i) out_level[i] = lv[i];
+    }
+    if (!out_nbr && !out_deg) return;
+    // dense [layer][row][M + 1] / [layer][row], rows absent from a layer zero
+    std::vector<int> nb0 = get(ivf->hn_nbr0, n * cap), dg0 = get(ivf->hn_deg0, n);
+    std::vector<int> nbu = get(ivf->hn_nbru, (L - 1) * n1 * cap), dgu = get(ivf->hn_degu, (L - 1) * n1);
+    std::vector<int> up = get(ivf->hn_upslot, n);
+    for (int64_t l = 0; l < L; ++l)
+      for (int64_t r = 0; r < n; ++r) {
+        int dg = 0;
+        const int* src = nullptr;
+        if (l == 0) {
+          dg = dg0[r];
+          src = nb0.data() + r * cap;
+        } else if (up[r] >= 0) {
+          const int64_t s = (l - 1) * n1 + up[r];
+          dg = dgu[s];
+          src = nbu.data() + s * cap;
+        }
+        if (out_deg) out_deg[l * n + r] = dg;
+        if (out_nbr)
+          for (int64_t j = 0; j < cap; ++j) out_nbr[(l * n + r) * cap + j] = src ? src[j] : 0;
+      }
+  });
+}
+
 int mann_route_distances(const float* centroids, int64_t nlist, int64_t d, const float* queries,
                          int64_t nq, int64_t ldq, float* out_dist) {
   return guarded([&] {

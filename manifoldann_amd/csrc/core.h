@@ -82,8 +82,32 @@ struct mann_ivf {
   float* xn = nullptr;
   float* xn2 = nullptr;
   float ex = 0.0f, rx = 0.0f;
+  // HNSW terminals (mann_ivf_hnsw_build): one graph per list over the
+  // list-contiguous rows; base-layer lists hn_nbr0[row][M + 1] and hn_deg0[row],
+  // upper layers in compact slots of the hn_n1 rows of level >= 1
+  // (hn_nbru[(layer - 1) n1 + hn_upslot[row]][M + 1], hn_degu likewise); ids in
+  // the lists are local to the list
+  int hn_M = 0, hn_layers = 0, hn_policy = 1, hn_ef_c = 0;
+  int64_t hn_n1 = 0;
+  int* hn_nbr0 = nullptr;
+  int* hn_deg0 = nullptr;
+  int* hn_upslot = nullptr;
+  int* hn_nbru = nullptr;
+  int* hn_degu = nullptr;
+  int* hn_level = nullptr;
+  int* hn_entry = nullptr;   // per bucket, local id (-1: empty)
+  int* hn_maxl = nullptr;
+  void free_hnsw() {
+    void* hs[] = {hn_nbr0, hn_deg0, hn_upslot, hn_nbru, hn_degu, hn_level, hn_entry, hn_maxl};
+    for (void* p : hs)
+      if (p) (void)hipFree(p);
+    hn_nbr0 = hn_deg0 = hn_upslot = hn_nbru = hn_degu = hn_level = hn_entry = hn_maxl = nullptr;
+    hn_M = hn_layers = hn_ef_c = 0;
+    hn_n1 = 0;
+  }
   ~mann_ivf() {
     (void)hipSetDevice(device);
+    free_hnsw();
     void* ps[] = {c32, cn2, cres != c32 ? cres : nullptr, list_off, perm, bucket_of, lorder,
                   x32, xb, xn, xn2, rb, rn};
     for (void* p : ps)
@@ -235,6 +259,14 @@ void ivf_probe(const mann_ivf& iv, const float* R, int nq, int nprobe, int* prob
 void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int metric,
                 const float* route_in, int64_t* out_ids, float* out_dist, float mass = 0.0f,
                 int merge = 0, int probe_order = 0);
+// IVF-HNSW (core_hnsw.hip): build the HNSW terminals of every list (policy 0
+// heuristic, 1 diversified; levels from the splitmix64 stream of seed over the
+// list-contiguous rows), and search them (route_in: optional routing
+// distances; merge 0 SimpleMerge, 1 DisjointMerge; probe_order as ivf_search)
+void ivf_hnsw_build(mann_ivf& iv, int M, int ef_construction, int policy, uint64_t seed);
+void ivf_hnsw_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int ef_search,
+                     const float* route_in, int merge, int probe_order, int64_t* out_ids,
+                     float* out_dist);
 
 // learned router (core_router.hip)
 mann_router* router_create(int d, int h, int c, uint64_t seed);

@@ -242,7 +242,7 @@ def _validate(index, data, d_q):
         raise ArgumentError(f"Data contains {n} points but index tracks {index.n_points}")
 
 
-def query_arrays(index, data, Q, k, out=None, nprobe=None):
+def query_arrays(index, data, Q, k, out=None, nprobe=None, ef_search=None):
     """Batch exact kNN; returns (ids (nq, k) int64 1-based, dist (nq, k) f32).
 
     Device (torch) queries give device results.  Equivalent to the generic
@@ -250,7 +250,9 @@ def query_arrays(index, data, Q, k, out=None, nprobe=None):
     """
     if hasattr(index, "root"):
         from .ivf import query_arrays as ivf_query_arrays
-        return ivf_query_arrays(index, data, Q, k, nprobe=nprobe)
+        return ivf_query_arrays(index, data, Q, k, nprobe=nprobe, ef_search=ef_search)
+    if ef_search is not None:
+        raise ArgumentError("ef_search is a keyword of HNSW queries")
     if nprobe is not None:
         raise ArgumentError("nprobe is a keyword of IVF index queries")
     qm = as_matrix(Q)
@@ -287,7 +289,7 @@ def _to_neighbors(ids, dist):
     return out
 
 
-def query(index, data, q, k, nprobe=None):
+def query(index, data, q, k, nprobe=None, ef_search=None):
     """query(index, data, q|Q, k[; nprobe]) -> Vector{Neighbor} or
     Vector{Vector{Neighbor}}; nprobe (IVF indexes only) as in ivf_flat.jl:138-144."""
     if isinstance(q, (list, tuple)) and q and not np.isscalar(q[0]):
@@ -298,7 +300,8 @@ def query(index, data, q, k, nprobe=None):
         # IVF: one vector takes the single-query path, whose per-list results
         # merge in probe order (query.jl:25-106); a matrix the batched one
         from .ivf import query_arrays as ivf_query_arrays
-        ids, dist = ivf_query_arrays(index, data, q, k, nprobe=nprobe, probe_order=single)
+        ids, dist = ivf_query_arrays(index, data, q, k, nprobe=nprobe, probe_order=single,
+                                     ef_search=ef_search)
         res = _to_neighbors(ids, dist)
         return (res[0] if k > 0 else []) if single else res
     if single:

@@ -252,8 +252,57 @@ class TransformedConfig:
     child_config: Any
 
 
+class HNSWIndex:
+    """The HNSWIndex terminal type (hnsw/types.jl:170-216) for
+    TerminalConfig(HNSWIndex, params): the GPU builds and searches the HNSW
+    graph of every list of a MultiLevelIndex (the IVF-HNSW of
+    multilevel/ivf_hnsw.jl:46-88).  params: M, ef_construction, ef_search,
+    neighbor_policy (:heuristic / :diversified) as in hnsw/index.jl:10-67."""
+
+
+HNSW_DEFAULT_M = 16                  # constants.jl:21
+HNSW_DEFAULT_EF_CONSTRUCTION = 200   # constants.jl:30
+HNSW_DEFAULT_EF_SEARCH = 64          # constants.jl:38
+IVF_DEFAULT_NPROBE = 5               # constants.jl:154
+
+
+@dataclass
+class HnswTerminal:
+    M: int = HNSW_DEFAULT_M
+    ef_construction: int = HNSW_DEFAULT_EF_CONSTRUCTION
+    ef_search: int = HNSW_DEFAULT_EF_SEARCH
+    neighbor_policy: str = "diversified"
+
+    @classmethod
+    def from_params(cls, params):
+        kw = dict(params) if params else {}
+        unknown = set(kw) - {"M", "ef_construction", "ef_search", "neighbor_policy"}
+        if unknown:
+            raise ArgumentError(f"unsupported HNSWIndex parameters {sorted(unknown)}")
+        t = cls(**kw)
+        # build_index(HNSWIndex) argument checks (hnsw/index.jl:25-31, :69-88)
+        if t.M <= 0:
+            raise ArgumentError("M must be positive")
+        if t.ef_construction < 1:
+            raise ArgumentError("ef_construction must be at least 1")
+        if t.ef_search < 1:
+            raise ArgumentError("ef_search must be at least 1")
+        if t.neighbor_policy not in ("heuristic", "diversified"):
+            raise ArgumentError(f"Unknown neighbor_policy symbol: {t.neighbor_policy}")
+        return t
+
+    @property
+    def policy_code(self):
+        return 0 if self.neighbor_policy == "heuristic" else 1
+
+
+def _attach_hnsw(handle, terminal: HnswTerminal, seed: int):
+    check(lib().mann_ivf_hnsw_build(handle, terminal.M, terminal.ef_construction,
+                                    terminal.policy_code, seed & 0xFFFFFFFFFFFFFFFF))
+
+
 class MultiLevelIndex:
-    """One KMeans level over BruteForce leaves, held on the GPU."""
+    """One KMeans level over BruteForce (or HNSW) leaves, held on the GPU."""
 
     def __init__(self, transform, routing, handle, d, n, nlist, merge_strategy, distance):
         self.transform = transform
@@ -276,6 +325,7 @@ class MultiLevelIndex:
                 pos += 1
                 self.bucket_lookup[b] = pos
         self.root = self
+        self.terminal = None    # HnswTerminal for HNSW leaves
 
     @property
     def id_mappings(self):
@@ -311,19 +361,81 @@ def build_index(index_type, X, config=None, merge_strategy=None, distance=defaul
         raise ArgumentError(f"unsupported index type {index_type}")
     if not isinstance(config, TransformedConfig) or not isinstance(config.transform, KMeansTransform):
         raise ArgumentError("the GPU MultiLevelIndex needs a KMeansTransform root")
-    if not (isinstance(config.child_config, TerminalConfig)
-            and config.child_config.index_type is BruteForceIndex):
-        raise ArgumentError("the GPU MultiLevelIndex needs BruteForce terminals")
+    child = config.child_config
+    if not (isinstance(child, TerminalConfig)
+            and child.index_type in (BruteForceIndex, HNSWIndex)):
+        raise ArgumentError("the GPU MultiLevelIndex needs BruteForce or HNSW terminals")
+    hnsw = HnswTerminal.from_params(child.params) if child.index_type is HNSWIndex else None
     import copy
     t = copy.deepcopy(config.transform)     # builder.jl:103
     fit(t, X, rng=rng)
     assign = take_pending_assignments(t)
-    ds = _point_dataset(X, distance)
+    # HNSW leaves measure the Euclidean distance (the HNSWIndex default)
+    ds = _point_dataset(X, Euclidean() if hnsw else distance)
     C = _centroids_rows(t.centroids)
     h = ctypes.c_void_p()
     check(lib().mann_ivf_build(ds.handle, C.ctypes.data, t.k, assign.ctypes.data, ctypes.byref(h)))
-    return MultiLevelIndex(t, config.routing, h, ds.d, ds.n, t.k,
-                           merge_strategy or SimpleMerge(), distance)
+    idx = MultiLevelIndex(t, config.routing, h, ds.d, ds.n, t.k,
+                          merge_strategy or SimpleMerge(), distance)
+    if hnsw is not None:
+        # levels: an independent splitmix64 stream next to the k-means draws
+        _attach_hnsw(h, hnsw, _seed_of(rng) * 0x9E3779B97F4A7C15 + 2)
+        idx.terminal = hnsw
+    return idx
+
+
+def build_ivf_hnsw_index(X, nlist, routing_k=IVF_DEFAULT_NPROBE, kmeans_distance=None,
+                         kmeans_init="kmeans_plus_plus", kmeans_max_iters=25, kmeans_tol=1e-4,
+                         kmeans_subsample_size="default", hnsw_M=HNSW_DEFAULT_M,
+                         hnsw_ef_construction=HNSW_DEFAULT_EF_CONSTRUCTION,
+                         hnsw_ef_search=HNSW_DEFAULT_EF_SEARCH,
+                         hnsw_neighbor_policy="diversified", merge_strategy=None,
+                         distance=default_distance, rng=None):
+    """build_ivf_hnsw_index (multilevel/ivf_hnsw.jl:46-88): KMeans(nlist) routing
+    to per-list HNSW graphs, DisjointMerge by default."""
+    if nlist <= 0:
+        raise ArgumentError(f"nlist must be positive, got {nlist}")
+    if kmeans_subsample_size == "default":
+        kmeans_subsample_size = max(256 * nlist, 32_768)
+    routing = TopKRouting(min(max(routing_k, 1), nlist))
+    transform = KMeansTransform(k=nlist, distance=kmeans_distance or Euclidean(),
+                                init=kmeans_init, max_iters=kmeans_max_iters, tol=kmeans_tol,
+                                subsample_size=kmeans_subsample_size)
+    params = dict(M=hnsw_M, ef_construction=hnsw_ef_construction, ef_search=hnsw_ef_search,
+                  neighbor_policy=hnsw_neighbor_policy)
+    config = TransformedConfig(transform, routing, TerminalConfig(HNSWIndex, params))
+    return build_index(MultiLevelIndex, X, config,
+                       merge_strategy=merge_strategy or DisjointMerge(), distance=distance,
+                       rng=rng)
+
+
+def ivf_hnsw_from_assignment(X, centroids_dk, assign, routing, hnsw: HnswTerminal, seed=0,
+                             merge_strategy=None):
+    """An IVF-HNSW index from an existing k-means fit (centroids d x k,
+    1-based assignment) with level seed `seed`."""
+    idx = build_index_from_assignment(X, centroids_dk, assign, routing,
+                                      merge_strategy=merge_strategy or DisjointMerge())
+    _attach_hnsw(idx.handle, hnsw, seed)
+    idx.terminal = hnsw
+    return idx
+
+
+def hnsw_graph(index: MultiLevelIndex):
+    """The HNSW graphs of an IVF-HNSW index: (nbr (layers, n, M + 1) local 0-based
+    ids, deg (layers, n), entry and max_layer per bucket, level per row), rows in
+    list-contiguous order."""
+    L, M = ctypes.c_int64(), ctypes.c_int64()
+    check(lib().mann_ivf_hnsw_graph(index.handle, ctypes.byref(L), ctypes.byref(M), None, None,
+                                    None, None, None))
+    L, M, n, nb = L.value, M.value, index.n, index.n_buckets
+    nbr = np.zeros((L, n, M + 1), np.int32)
+    deg = np.zeros((L, n), np.int32)
+    entry = np.zeros(nb, np.int32)
+    maxl = np.zeros(nb, np.int32)
+    level = np.zeros(n, np.int32)
+    check(lib().mann_ivf_hnsw_graph(index.handle, None, None, nbr.ctypes.data, deg.ctypes.data,
+                                    entry.ctypes.data, maxl.ctypes.data, level.ctypes.data))
+    return nbr, deg, entry, maxl, level
 
 
 def build_index_from_assignment(X, centroids_dk, assign, routing, merge_strategy=None,
@@ -468,7 +580,7 @@ def probe_lists(index: MultiLevelIndex, route):
 
 
 def query_arrays(index: MultiLevelIndex, data, Q, k, route=None, nprobe=None,
-                 probe_order=False):
+                 probe_order=False, ef_search=None):
     """Batched query (query.jl:279-300) -> (ids (nq, k), dist (nq, k)).
     probe_order=True merges each query's per-list results in probe order, as
     the single-query path does (query.jl:25-106); the index's merge strategy
@@ -505,18 +617,29 @@ def query_arrays(index: MultiLevelIndex, data, Q, k, route=None, nprobe=None,
                                                                                   np.float32)
         rp = rp if isinstance(rp, int) else rp.ctypes.data
     merge = 1 if isinstance(getattr(index, "merge_strategy", None), DisjointMerge) else 0
+    term = getattr(index, "terminal", None)
+    if term is not None:
+        # HNSW leaves take ef_search (hnsw/query.jl:63-75, passed down by the
+        # multi-level query's keyword arguments)
+        ef = term.ef_search if ef_search is None else int(ef_search)
+        check(lib().mann_ivf_hnsw_search(index.handle, qm.ptr, qm.d, nq, qm.ld, k, npb, ef, rp,
+                                         merge, int(bool(probe_order)), pi, pd))
+        return ids, dist
+    if ef_search is not None:
+        raise ArgumentError("ef_search is a keyword of HNSW leaves")
     check(lib().mann_ivf_search_ex(index.handle, qm.ptr, qm.d, nq, qm.ld, k, npb,
                                    index.distance.code, rp, merge, int(bool(probe_order)), pi,
                                    pd))
     return ids, dist
 
 
-def query(index, data, q, k, nprobe=None):
+def query(index, data, q, k, nprobe=None, ef_search=None):
     """query(index, data, q | Q, k): one vector goes through the single-query
     path (per-list results in probe order, query.jl:25-106), a matrix through
     the batched one (child order, query.jl:279-399)."""
     single = (np.ndim(q) == 1) if not _is_torch(q) else q.dim() == 1
-    ids, dist = query_arrays(index, data, q, k, nprobe=nprobe, probe_order=single)
+    ids, dist = query_arrays(index, data, q, k, nprobe=nprobe, probe_order=single,
+                             ef_search=ef_search)
     res = _to_neighbors(ids, dist)
     return res[0] if single else res
 

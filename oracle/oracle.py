@@ -85,6 +85,15 @@ def lib():
         L.orc_ivf_flat_query.argtypes = [_f32p, _i64, _i64, _i64p, _i64p, _i64, _f32p, _i64,
                                          _i64, _f32p, _i64, _i64, ctypes.c_int, _i64p, _f32p,
                                          _i64p]
+        _i32p = ctypes.POINTER(ctypes.c_int32)
+        L.orc_hnsw_levels.argtypes = [_i64, _i64, ctypes.c_uint64, _i32p]
+        L.orc_ivf_hnsw_build.argtypes = [_f32p, _i64, _i64, _i64p, _i64p, _i64, _i64, _i64,
+                                         ctypes.c_int, _i32p, _i64, _i32p, _i32p, _i32p, _i32p,
+                                         ctypes.c_int]
+        L.orc_ivf_hnsw_search.argtypes = [_f32p, _i64, _i64, _i64p, _i64p, _i64, _i64, _i32p,
+                                          _i32p, _i32p, _i32p, _f32p, _i64, _i64, _f32p, _i64,
+                                          _i64, _i64, _i64, ctypes.c_int, ctypes.c_int, _i64p,
+                                          _f32p, ctypes.c_int]
         L.orc_recall_at_k.restype = ctypes.c_double
         L.orc_recall_at_k.argtypes = [_i64p, _i64, _i64p, _i64, _i64]
         _lib = L
@@ -299,6 +308,84 @@ def ivf_flat_query(X, offsets, ids, Q, route, k, nprobe, metric=EUCLIDEAN):
     if rc != 0:
         raise ValueError("nlist must be positive")
     return oi, od, cnt[:nq].copy()
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _p32(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+HNSW_HEURISTIC, HNSW_DIVERSIFIED = 0, 1
+
+
+def hnsw_levels(n, M, seed):
+    """HNSW levels of n points from one splitmix64 stream (planner.jl:19-30)."""
+    out = np.zeros(max(n, 1), np.int32)
+    rc = lib().orc_hnsw_levels(n, M, seed, _p32(out))
+    if rc != 0:
+        raise ValueError("M must be positive")
+    return out[:n].copy()
+
+
+class HnswGraphs:
+    """The per-list HNSW graphs of an IVF-HNSW index, list-contiguous rows:
+    nbr (layers, n, M + 1) local 0-based ids, deg (layers, n), entry and
+    max_layer per bucket (-1 when empty), level per row."""
+
+    def __init__(self, nbr, deg, entry, max_layer, level, M):
+        self.nbr, self.deg, self.entry, self.max_layer = nbr, deg, entry, max_layer
+        self.level, self.M = level, M
+
+
+def ivf_hnsw_build(X, offsets, ids, nb, M, ef_construction, policy=HNSW_DIVERSIFIED, seed=0,
+                   level=None, nthreads=0):
+    """Serial HNSW build of every non-empty list (the terminals of
+    build_ivf_hnsw_index).  level: per list-contiguous row (default: drawn from
+    the seed's stream)."""
+    X = _f32(X)
+    offsets, ids = _i64a(offsets), _i64a(ids)
+    n, d = X.shape
+    n_all = int(offsets[nb])
+    lev = hnsw_levels(n_all, M, seed) if level is None else _i32(level)
+    layers = int(lev.max()) + 1 if n_all else 1
+    nbr = np.zeros((layers, max(n_all, 1), M + 1), np.int32)
+    deg = np.zeros((layers, max(n_all, 1)), np.int32)
+    entry = np.zeros(max(nb, 1), np.int32)
+    maxl = np.zeros(max(nb, 1), np.int32)
+    rc = lib().orc_ivf_hnsw_build(_p(X), d, d, _p(offsets), _p(ids), nb, M, ef_construction,
+                                  policy, _p32(lev), layers, _p32(nbr), _p32(deg), _p32(entry),
+                                  _p32(maxl), nthreads)
+    if rc != 0:
+        raise ValueError(f"HNSW build failed ({rc})")
+    return HnswGraphs(nbr[:, :n_all], deg[:, :n_all], entry[:nb], maxl[:nb], lev, M)
+
+
+def ivf_hnsw_search(X, offsets, ids, nb, graphs, Q, route, k, nprobe, ef_search,
+                    merge="disjoint", probe_order=False, nthreads=0):
+    """query(::MultiLevelIndex{KMeans, TopK, HNSW}): ids (nq, k) 1-based padded
+    with 0, dist (nq, k) padded with inf."""
+    X, Q, route = _f32(X), _f32(Q), _f32(route)
+    offsets, ids = _i64a(offsets), _i64a(ids)
+    n, d = X.shape
+    nq = Q.shape[0]
+    g = graphs
+    nbr = np.ascontiguousarray(g.nbr, np.int32)
+    deg = np.ascontiguousarray(g.deg, np.int32)
+    oi = np.zeros((nq, max(k, 0)), np.int64)
+    od = np.full((nq, max(k, 0)), np.inf, np.float32)
+    if nq == 0 or k <= 0:
+        return oi, od
+    rc = lib().orc_ivf_hnsw_search(_p(X), d, d, _p(offsets), _p(ids), nb, g.M, _p32(nbr),
+                                   _p32(deg), _p32(_i32(g.entry)), _p32(_i32(g.max_layer)),
+                                   _p(Q), nq, d, _p(route), route.shape[1], k, nprobe,
+                                   ef_search, 0 if merge == "simple" else 1, int(probe_order),
+                                   _p(oi), _p(od), nthreads)
+    if rc != 0:
+        raise ValueError(f"IVF-HNSW search failed ({rc})")
+    return oi, od
 
 
 def recall_at_k(found, truth):
