@@ -494,6 +494,8 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
         cpu = {"value": round(m / tc, 1), "unit": "QPS", "cores": cores, "kind": "port",
                "sample": f"{m} queries at nprobe={bc['nprobe']} (oracle routing + per-list "
                          f"scans + merge, OpenMP) in {tc:.2f} s"}
+    hn = None if args.skip_hnsw else bench_ivf_hnsw(args, M, I, L, check, np, torch, index, X,
+                                                     Q, Qd, ids, dist, recall)
     out = {"workload": f"IVF n={n} d={d} nlist={nlist} iters={t.n_iters} nq={nq} k=10",
            "kmeans_fit_s": round(fit_s, 3), "router_train_s": round(train_s, 3),
            "router_final_loss": round(loss, 4), "train_queries": args.train_q,
@@ -513,8 +515,80 @@ def bench_ivf(args, M, datasets, L, check, np, torch):
            "kernels_ms": kern, "kernels_ms_learned_mass": kern_l, "search_stats": ivf_stats,
            "roofline": ivf_roof,
            "cpu_baseline": cpu,
+           "ivf_hnsw": hn,
            "total_s": round(time.perf_counter() - t_all, 1)}
     return out
+
+
+def bench_ivf_hnsw(args, M, I, L, check, np, torch, index, X, Q, Qd, ids, dist, recall):
+    """IVF-HNSW (build_ivf_hnsw_index's defaults: M 16, ef_construction 200,
+    ef_search 64, diversified neighbours, DisjointMerge) on the IVF workload's
+    k-means lists: build time, recall@10 and QPS over (nprobe, ef_search), the
+    kernel times, 16 queries against the oracle and the oracle's CPU search over
+    the same graphs."""
+    n, d = X.shape
+    nq = Q.shape[0]
+    term = I.HnswTerminal()
+    t0 = time.perf_counter()
+    I._attach_hnsw(index.handle, term, 5)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    log(f"IVF-HNSW: graphs built in {build_s:.2f} s")
+
+    def search(npb, ef, q=Qd, m=nq, oi=ids, od=dist):
+        check(L.mann_ivf_hnsw_search(index.handle, q.data_ptr(), d, m, d, 10, npb, ef, None, 1, 0,
+                                     oi.data_ptr(), od.data_ptr()))
+
+    sweep = []
+    for npb, ef in ((5, 64), (8, 64), (16, 64), (16, 128), (24, 128), (32, 128), (32, 256)):
+        qps = qps_of(lambda npb=npb, ef=ef: search(npb, ef), nq, args.reps, torch)
+        rc = recall(ids)
+        sweep.append({"nprobe": npb, "ef_search": ef, "recall": round(rc, 4),
+                      "qps": round(qps, 1)})
+        log(f"IVF-HNSW nprobe={npb} ef={ef}: recall {rc:.4f} qps {qps:.0f}")
+    ok = [s_ for s_ in sweep if s_["recall"] >= 0.95]
+    best = max(ok, key=lambda s_: s_["qps"]) if ok else None
+    check(L.mann_profile_enable(1))
+    check(L.mann_profile_reset())
+    search(5, 64)
+    kern = kernel_avg_ms(L, check, ["prep_queries", "ivf_route", "ivf_probe", "hnsw_search",
+                                    "hnsw_merge"])
+    check(L.mann_profile_enable(0))
+    # the oracle over the GPU's graphs: 16 queries checked, a sample timed
+    from oracle import oracle as O
+    nbr, deg, entry, maxl, level = I.hnsw_graph(index)
+    g = O.HnswGraphs(nbr, deg, entry, maxl, level, term.M)
+    off = np.concatenate([[0], np.cumsum(index.list_sizes)]).astype(np.int64)
+    lids = np.zeros(n, np.int64)
+    check(L.mann_ivf_list_ids(index.handle, lids.ctypes.data))
+    Cc = np.ascontiguousarray(index.transform.centroids.T)
+    R16 = O.route_distances(Q[:16], Cc)
+    oi, od = O.ivf_hnsw_search(X, off, lids, index.n_buckets, g, Q[:16], R16, 10, 5, 64,
+                               merge="disjoint")
+    search(5, 64)
+    gi, gd = ids[:16].cpu().numpy(), dist[:16].cpu().numpy()
+    parity = {"queries": 16, "ids_equal_oracle": bool(np.array_equal(gi, oi)),
+              "dist_bits_equal_oracle": bool(np.array_equal(gd.view(np.uint32),
+                                                            od.view(np.uint32)))}
+    cpu = None
+    if not args.skip_cpu:
+        cores = host_cores()
+        m = min(2000, nq)
+        t2 = time.perf_counter()
+        R = O.route_distances(Q[:m], Cc, nthreads=cores)
+        O.ivf_hnsw_search(X, off, lids, index.n_buckets, g, Q[:m], R, 10, 5, 64,
+                          merge="disjoint", nthreads=cores)
+        tc = time.perf_counter() - t2
+        cpu = {"value": round(m / tc, 1), "unit": "QPS", "cores": cores, "kind": "port",
+               "sample": f"{m} queries at nprobe=5, ef_search=64 (oracle routing + per-list "
+                         f"HNSW searches over the same graphs + DisjointMerge, OpenMP) in "
+                         f"{tc:.2f} s"}
+    del nbr, deg, g
+    return {"workload": f"IVF-HNSW n={n} d={d} nlist={index.nlist} M={term.M} "
+                        f"ef_construction={term.ef_construction} nq={nq} k=10",
+            "build_s": round(build_s, 2), "max_list": int(np.max(index.list_sizes)),
+            "sweep": sweep, "best_at_0.95": best, "kernels_ms": kern, "precision": parity,
+            "cpu_baseline": cpu}
 
 
 def bench_large_cosine(args, M, L, check, np, torch):
@@ -639,6 +713,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the kNN graph with the default knobs")
     ap.add_argument("--skip-ivf", action="store_true")
+    ap.add_argument("--skip-hnsw", action="store_true",
+                    help="skip the IVF-HNSW key of the IVF workload")
     ap.add_argument("--ivf-search-only", type=int, default=0, metavar="NPROBE",
                     help="IVF: build the index, then only three centroid searches at NPROBE "
                          "(for counter collection)")

@@ -147,16 +147,23 @@ void ivf_hnsw_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, i
   int* orow = c.buf("hn_orow").as<int>((size_t)np * k);
   float* od = c.buf("hn_od").as<float>((size_t)np * k);
   int* ocnt = c.buf("hn_ocnt").as<int>((size_t)np);
-  // pending heap in LDS up to 4 ef entries (at least 256), the visited bitmap
+  // pending heap in LDS up to 4 ef entries (at least 512), the visited bitmap
   // in LDS when the largest list fits 1,024 words; the global spill regions
-  // hold |largest list| + 1 entries and its words per workgroup
+  // hold |largest list| + 1 entries and its words per workgroup.  The searches
+  // wait on memory latency: as many one-wavefront workgroups as the LDS of
+  // 256 CUs holds, within 1 GiB of spill regions
   const int64_t max_list = std::max<int64_t>(iv.max_list, 1);
-  const int pend_lds = (int)std::min<int64_t>(std::max(256, 4 * ef), max_list + 1);
+  const int pend_lds = (int)std::min<int64_t>(std::max(512, 4 * ef), max_list + 1);
   const int64_t words = (max_list + 31) >> 5;
   const int vis_words = words <= 1024 ? (int)words : 0;
-  const int grid = (int)std::min<int64_t>(np, 2048);
   const int64_t pstride = max_list + 1 > pend_lds ? max_list + 1 : 0;
   const int64_t vstride = vis_words ? 0 : words;
+  const size_t lds = hnsw_search_lds(ef, pend_lds, vis_words);
+  const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(32, (160 * 1024) / (int64_t)lds));
+  const int64_t spill = pstride * (int64_t)sizeof(HnCand) + vstride * 4;
+  int64_t g = std::min<int64_t>(np, 256 * per_cu);
+  if (spill > 0) g = std::min<int64_t>(g, std::max<int64_t>(256, ((int64_t)1 << 30) / spill));
+  const int grid = (int)std::max<int64_t>(g, 1);
   HnCand* pg = c.buf("hn_spill").as<HnCand>((size_t)std::max<int64_t>(pstride * grid, 1));
   unsigned* vg = c.buf("hn_svis").as<unsigned>((size_t)std::max<int64_t>(vstride * grid, 1));
   HnswSearchArgs a{};

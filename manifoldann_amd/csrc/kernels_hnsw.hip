@@ -152,7 +152,30 @@ __device__ void dist_batch(const float* __restrict__ x32, int64_t row0, int d16,
       xr[u] = x32 + (row0 + ids[i < cnt ? i : b]) * (int64_t)d16;
     }
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int t = j; t < d16; t += 16) {
+    // the loads of 8 chunks are issued before their multiply-adds (the rows
+    // are scattered: each wait is a full memory latency); the adds keep the
+    // chunk order of the A16 sum
+    const int nch = d16 >> 4;
+    int c0 = 0;
+    for (; c0 + 8 <= nch; c0 += 8) {
+      float xv[8][4], qq[8];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int t = ((c0 + c) << 4) + j;
+        qq[c] = qv[t];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xv[c][u] = xr[u][t];
+      }
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const float df = xv[c][u] - qq[c];
+          acc[u] = __builtin_fmaf(df, df, acc[u]);
+        }
+    }
+    for (; c0 < nch; ++c0) {
+      const int t = (c0 << 4) + j;
       const float qq = qv[t];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -177,9 +200,11 @@ __device__ HnCand greedy(const HnGraph& G, int layer, int64_t row0, const float*
   float bd = cur.d;
   for (;;) {
     const int64_t s = list_slot(G, layer, row0 + c);
-    const int deg = *deg_of(G, layer, s);
+    // the whole slot (cap ids) is read together with the degree
     const int* lst = list_of(G, layer, s);
-    if (lane < deg) w.id[lane] = lst[lane];
+    const int v = lane < G.cap ? lst[lane] : 0;
+    const int deg = *deg_of(G, layer, s);
+    if (lane < deg) w.id[lane] = v;
     wg_sync();
     dist_batch(x32, row0, d16, qv, w.id, deg, w.d, lane);
     wg_sync();
@@ -231,12 +256,12 @@ __device__ int search_layer(const HnGraph& G, int layer, int64_t row0, const flo
     cid = bcast(cid);
     // the unvisited neighbours of cid, in list order, marked visited
     const int64_t s = list_slot(G, layer, row0 + cid);
-    const int deg = *deg_of(G, layer, s);
+    // the whole slot (cap ids) is read together with the degree
     const int* lst = list_of(G, layer, s);
-    int v = 0;
+    const int v = lane < G.cap ? lst[lane] : 0;
+    const int deg = *deg_of(G, layer, s);
     bool fresh = false;
     if (lane < deg) {
-      v = lst[lane];
       const unsigned bit = 1u << (v & 31);
       fresh = (atomicOr(vis + (v >> 5), bit) & bit) == 0u;
     }
