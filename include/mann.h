@@ -321,6 +321,15 @@ int mann_ivf_hnsw_build(mann_ivf_t ivf, int64_t M, int64_t ef_construction, int 
 int mann_ivf_hnsw_search(mann_ivf_t ivf, const float* queries, int64_t d, int64_t nq, int64_t ldq,
                          int64_t k, int64_t nprobe, int64_t ef_search, const float* route_dist,
                          int merge, int probe_order, int64_t* out_ids, float* out_dist);
+/* Install HNSW graphs built elsewhere -- the reference's own HNSWIndex children,
+ * whose HNSWLayer slabs (hnsw/types.jl:18-49) the shim passes -- as the
+ * handle's terminals, in the layout mann_ivf_hnsw_graph returns (host arrays;
+ * level may be null: then a row's level is the highest layer where it has
+ * neighbours, the entry point's its bucket's top layer).  Every id, degree
+ * (<= M <= 63) and layer membership is checked before anything is installed. */
+int mann_ivf_hnsw_set_graph(mann_ivf_t ivf, int64_t layers, int64_t M, const int32_t* nbr,
+                            const int32_t* deg, const int32_t* entry, const int32_t* max_layer,
+                            const int32_t* level);
 /* The graphs, as the reference's HNSWLayer slabs (hnsw/types.jl:18-49) over the
  * list-contiguous rows: layers and M, then (each optional) out_nbr
  * [layers][n][M + 1] local 0-based ids (the live prefix is out_deg's count; rows

@@ -16,6 +16,10 @@ keep calling them unchanged:
 - `query(::MultiLevelIndex{<:KMeansTransform,<:BruteForceIndex}, data, Q | q, k)`
   (multilevel/query.jl:25-39, :279-300) with the index's merge strategy
 - `query(::IVFFlatIndex{Float32}, data, Q | q, k; nprobe)`  (ivf_flat.jl:138-168)
+- `query(::MultiLevelIndex{<:KMeansTransform,<:HNSWIndex}, data, Q | q, k; ef_search)`
+  (build_ivf_hnsw_index, multilevel/ivf_hnsw.jl:46-88): the children's own
+  graphs (their HNSWLayer slabs) are installed on the device on first query and
+  searched there
 
 `D` is Euclidean, SqEuclidean or CosineDist: the metric comes from the index.
 Other element types, metrics and index configurations keep the reference
@@ -38,7 +42,7 @@ module ManifoldANNHIP
 
 using ManifoldANN
 using ManifoldANN: Neighbor, BruteForceIndex, KMeansTransform, MultiLevelIndex, IVFFlatIndex,
-                   KNNGraph, SimpleMerge, DisjointMerge, TopKRouting
+                   KNNGraph, SimpleMerge, DisjointMerge, TopKRouting, HNSWIndex, max_degree
 using Distances: Euclidean, SqEuclidean, CosineDist
 using Random: AbstractRNG, default_rng
 import ManifoldANN: query, build_knn_graph, fit!
@@ -320,6 +324,7 @@ mutable struct IVFHandle
     data_ptr::Ptr{Float32}
     dims::Tuple{Int,Int}
     cosine::Bool
+    graphs::Bool    # HNSW terminals installed
 end
 
 function release!(h::IVFHandle)
@@ -386,7 +391,7 @@ function ivf_handle(index, data::StridedMatrix{Float32}, metric::HIPMetric)
                             dd.h, C, size(C, 2), assign, out))
             end
         end
-        h = IVFHandle(out[], pointer(data), size(data), cosine)
+        h = IVFHandle(out[], pointer(data), size(data), cosine, false)
         finalizer(release!, h)
         IVF[key] = h
         return h
@@ -462,6 +467,93 @@ function query(index::IVFFlatIndex{Float32,Dm,Dc}, data::StridedMatrix{Float32},
     np = clamp(nprobe, 1, length(index.lists))
     return ivf_search(index, data, reshape(dense_vec(q), :, 1), Int(k), np, index.distance,
                       Cint(0), true)[1]
+end
+
+# ---------------------------------------------------------------------------
+# MultiLevelIndex{KMeansTransform, HNSWIndex} (build_ivf_hnsw_index)
+# ---------------------------------------------------------------------------
+
+const HIPMultiLevelHNSW = MultiLevelIndex{<:KMeansTransform{Euclidean},<:HNSWIndex}
+
+# The children's HNSWLayer slabs (hnsw/types.jl:18-49) as one [layer][row][slot]
+# array over the handle's list-contiguous rows (buckets ascending, each child's
+# local ids in order), 0-based local ids, installed with mann_ivf_hnsw_set_graph
+# (which checks every id and degree before installing).
+function install_graphs!(h::IVFHandle, index::MultiLevelIndex, n::Int)
+    root = index.root
+    kids = root.indices
+    L = maximum(c.max_layer for c in kids) + 1
+    M = maximum(max_degree(c.neighbor_policy) for c in kids)
+    nbr = zeros(Int32, M + 1, n, L)
+    deg = zeros(Int32, n, L)
+    nb = length(root.bucket_lookup)
+    entry = fill(Int32(-1), nb)
+    maxl = fill(Int32(-1), nb)
+    row = 0
+    for b in 1:nb
+        c = root.bucket_lookup[b]
+        c == 0 && continue
+        child = kids[c]
+        m = length(root.id_mappings[c])
+        for l in 0:child.max_layer
+            layer = child.layers[l + 1]
+            for i in 1:m
+                dg = layer.degree[i]
+                deg[row + i, l + 1] = dg
+                for j in 1:dg
+                    nbr[j, row + i, l + 1] = layer.neighbors[j, i] - 1
+                end
+            end
+        end
+        entry[b] = child.entry_point - 1
+        maxl[b] = child.max_layer
+        row += m
+    end
+    GC.@preserve nbr deg entry maxl check(ccall((:mann_ivf_hnsw_set_graph, LIB), Cint,
+        (Ptr{Cvoid}, Int64, Int64, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}),
+        h.h, L, M, nbr, deg, entry, maxl, C_NULL))
+    h.graphs = true
+    return h
+end
+
+# ef of the children's traversal policy (hnsw/query.jl:176-188)
+hnsw_default_ef(index::MultiLevelIndex) = index.root.indices[1].traversal_policy.ef_search
+
+function hnsw_search(index::MultiLevelIndex, data::StridedMatrix{Float32},
+                     Q::StridedMatrix{Float32}, k::Int, ef::Int, probe_order::Bool)
+    nq = size(Q, 2)
+    ids = zeros(Int64, k, nq)
+    dist = fill(Inf32, k, nq)
+    Qc = dense_cols(Q)
+    h = ivf_handle(index, data, Euclidean())
+    lock(LOCK) do
+        h.graphs || install_graphs!(h, index, size(data, 2))
+    end
+    GC.@preserve Qc ids dist check(ccall((:mann_ivf_hnsw_search, LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Float32}, Int64, Int64, Int64, Int64, Int64, Int64, Ptr{Float32}, Cint,
+         Cint, Ptr{Int64}, Ptr{Float32}),
+        h.h, Qc, size(Qc, 1), nq, stride(Qc, 2), k, routing_k(index).k, ef, C_NULL,
+        merge_code(index.merge_strategy), Cint(probe_order), ids, dist))
+    return to_neighbors(ids, dist)
+end
+
+function query(index::HIPMultiLevelHNSW, data::StridedMatrix{Float32},
+               Q::StridedMatrix{Float32}, k::Integer; ef_search::Union{Nothing,Int} = nothing)
+    size(Q, 1) == size(data, 1) || throw(DimensionMismatch(
+        "Expected queries with $(size(data, 1)) rows, got $(size(Q, 1))"))
+    size(Q, 2) == 0 && return Vector{Vector{Neighbor{Float32}}}()
+    k <= 0 && return [Neighbor{Float32}[] for _ in 1:size(Q, 2)]
+    ef = ef_search === nothing ? hnsw_default_ef(index) : ef_search
+    return hnsw_search(index, data, Q, Int(k), ef, false)
+end
+
+function query(index::HIPMultiLevelHNSW, data::StridedMatrix{Float32},
+               q::StridedVector{Float32}, k::Integer; ef_search::Union{Nothing,Int} = nothing)
+    length(q) == size(data, 1) || throw(DimensionMismatch(
+        "Expected query with $(size(data, 1)) rows, got $(length(q))"))
+    k <= 0 && return Neighbor{Float32}[]
+    ef = ef_search === nothing ? hnsw_default_ef(index) : ef_search
+    return hnsw_search(index, data, reshape(dense_vec(q), :, 1), Int(k), ef, true)[1]
 end
 
 end # module

@@ -65,6 +65,7 @@ SIGNATURES = [
     ("mann_ivf_hnsw_build", ctypes.c_int, [_vp, _i64, _i64, ctypes.c_int, ctypes.c_uint64]),
     ("mann_ivf_hnsw_search", ctypes.c_int, [_vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _vp,
                                             ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    ("mann_ivf_hnsw_set_graph", ctypes.c_int, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
     ("mann_ivf_hnsw_graph", ctypes.c_int, [_vp, _i64p, _i64p, _vp, _vp, _vp, _vp, _vp]),
     ("mann_route_distances", ctypes.c_int, [_vp, _i64, _i64, _vp, _i64, _i64, _vp]),
     ("mann_router_create", ctypes.c_int, [_i64, _i64, _i64, ctypes.c_uint64, ctypes.POINTER(_vp)]),

@@ -9,6 +9,7 @@
 
 #include "../../include/mann.h"
 #include "core.h"
+#include "hnsw.h"
 #include "fp8.h"
 #include "ivf.h"
 #include "mlp.h"
@@ -1012,6 +1013,21 @@ int mann_ivf_hnsw_search(mann_ivf_t ivf, const float* queries, int64_t d, int64_
     od.finish();
     MANN_HIP(hipStreamSynchronize(c.stream));
     timer_flush();
+  });
+}
+
+int mann_ivf_hnsw_set_graph(mann_ivf_t ivf, int64_t layers, int64_t M, const int32_t* nbr,
+                            const int32_t* deg, const int32_t* entry, const int32_t* max_layer,
+                            const int32_t* level) {
+  return guarded([&] {
+    MANN_REQUIRE(ivf != nullptr, ERR_NULL, "null index");
+    MANN_REQUIRE(nbr && deg && entry && max_layer, ERR_NULL, "null pointer");
+    MANN_REQUIRE(layers >= 1 && layers <= 64 && M >= 1 && M <= HN_M_MAX, ERR_ARGUMENT,
+                 "layers must lie in [1, 64] and M in [1, " + std::to_string(HN_M_MAX) + "]");
+    for (const void* p : {(const void*)nbr, (const void*)deg, (const void*)entry,
+                          (const void*)max_layer, (const void*)level})
+      MANN_REQUIRE(!p || !is_device_ptr(p), ERR_ARGUMENT, "the graph arrays are host arrays");
+    ivf_hnsw_set_graph(*ivf, (int)layers, (int)M, nbr, deg, entry, max_layer, level);
   });
 }
 

@@ -264,6 +264,11 @@ void ivf_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int me
 // list-contiguous rows), and search them (route_in: optional routing
 // distances; merge 0 SimpleMerge, 1 DisjointMerge; probe_order as ivf_search)
 void ivf_hnsw_build(mann_ivf& iv, int M, int ef_construction, int policy, uint64_t seed);
+// install graphs built elsewhere (dense [layers][n][M + 1] / [layers][n] host
+// arrays over the list-contiguous rows), validated so that every access the
+// kernels make stays inside the handle; level null: inferred
+void ivf_hnsw_set_graph(mann_ivf& iv, int layers, int M, const int* nbr, const int* deg,
+                        const int* entry, const int* maxl, const int* level);
 void ivf_hnsw_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int ef_search,
                      const float* route_in, int merge, int probe_order, int64_t* out_ids,
                      float* out_dist);

@@ -117,6 +117,98 @@ void ivf_hnsw_build(mann_ivf& iv, int M, int ef_c, int policy, uint64_t seed) {
   timer_flush();
 }
 
+void ivf_hnsw_set_graph(mann_ivf& iv, int layers, int M, const int* nbr, const int* deg,
+                        const int* entry, const int* maxl, const int* level_in) {
+  MANN_REQUIRE(!iv.flat, ERR_ARGUMENT,
+               "HNSW terminals belong to a MultiLevelIndex, not an IVFFlatIndex handle");
+  MANN_REQUIRE(!iv.normalized, ERR_ARGUMENT,
+               "HNSW terminals search with the Euclidean distance (the HNSWIndex default)");
+  MANN_REQUIRE(M > 0 && M <= HN_M_MAX, ERR_ARGUMENT,
+               "M must lie in [1, " + std::to_string(HN_M_MAX) + "]");
+  MANN_REQUIRE(layers >= 1 && layers <= 64, ERR_ARGUMENT, "layers must lie in [1, 64]");
+  const int64_t n = iv.n, cap = M + 1;
+  // levels: given, or the highest layer where a row has neighbours (the entry
+  // point: its bucket's top layer)
+  std::vector<int> lev((size_t)std::max<int64_t>(n, 1), 0);
+  for (int64_t r = 0; r < n; ++r) {
+    if (level_in) {
+      lev[r] = level_in[r];
+      MANN_REQUIRE(lev[r] >= 0 && lev[r] < layers, ERR_ARGUMENT, "level out of range");
+    } else {
+      for (int l = layers - 1; l >= 1; --l)
+        if (deg[(int64_t)l * n + r] > 0) {
+          lev[r] = l;
+          break;
+        }
+    }
+  }
+  for (int b = 0; b < iv.nb; ++b) {
+    const int64_t lo = iv.h_off[b], sz = iv.h_off[b + 1] - lo;
+    if (sz == 0) {
+      MANN_REQUIRE(entry[b] == -1 || entry[b] == 0, ERR_ARGUMENT, "entry point of an empty list");
+      continue;
+    }
+    MANN_REQUIRE(entry[b] >= 0 && entry[b] < sz, ERR_ARGUMENT, "entry point out of range");
+    MANN_REQUIRE(maxl[b] >= 0 && maxl[b] < layers, ERR_ARGUMENT, "top layer out of range");
+    int& le = lev[lo + entry[b]];
+    if (!level_in) le = std::max(le, maxl[b]);
+    MANN_REQUIRE(le >= maxl[b], ERR_ARGUMENT, "the entry point is not on its top layer");
+    // every list within its bucket, its degree at most M, every listed node on
+    // the layer (the kernels index the upper layers by that)
+    for (int64_t i = 0; i < sz; ++i)
+      for (int l = 0; l < layers; ++l) {
+        const int dg = deg[(int64_t)l * n + lo + i];
+        MANN_REQUIRE(dg >= 0 && dg <= M, ERR_ARGUMENT, "degree out of range");
+        MANN_REQUIRE(dg == 0 || lev[lo + i] >= l, ERR_ARGUMENT,
+                     "a row has neighbours above its level");
+        const int* lst = nbr + ((int64_t)l * n + lo + i) * cap;
+        for (int j = 0; j < dg; ++j) {
+          MANN_REQUIRE(lst[j] >= 0 && lst[j] < sz, ERR_ARGUMENT, "neighbour id out of range");
+          MANN_REQUIRE(lev[lo + lst[j]] >= l, ERR_ARGUMENT, "a neighbour is above its level");
+        }
+      }
+  }
+  std::vector<int> up((size_t)std::max<int64_t>(n, 1), -1);
+  int64_t n1 = 0;
+  for (int64_t r = 0; r < n; ++r)
+    if (lev[r] >= 1) up[r] = (int)n1++;
+  std::vector<int> nbu((size_t)std::max<int64_t>((layers - 1) * n1 * cap, 1), 0),
+      dgu((size_t)std::max<int64_t>((layers - 1) * n1, 1), 0);
+  for (int l = 1; l < layers; ++l)
+    for (int64_t r = 0; r < n; ++r) {
+      if (up[r] < 0) continue;
+      const int64_t s = (int64_t)(l - 1) * n1 + up[r];
+      dgu[s] = deg[(int64_t)l * n + r];
+      for (int64_t j = 0; j < cap; ++j) nbu[s * cap + j] = nbr[((int64_t)l * n + r) * cap + j];
+    }
+  std::vector<int> ent(iv.nb), mx(iv.nb);
+  for (int b = 0; b < iv.nb; ++b) {
+    const bool empty = iv.h_off[b + 1] == iv.h_off[b];
+    ent[b] = empty ? -1 : entry[b];
+    mx[b] = empty ? -1 : maxl[b];
+  }
+  ThreadCtx& c = ctx();
+  hipStream_t st = c.stream;
+  iv.free_hnsw();
+  iv.hn_M = M;
+  iv.hn_layers = layers;
+  iv.hn_n1 = n1;
+  auto upload = [&](int*& dst, const int* src, int64_t count) {
+    MANN_HIP(hipMalloc(&dst, sizeof(int) * (size_t)std::max<int64_t>(count, 1)));
+    if (count > 0)
+      MANN_HIP(hipMemcpyAsync(dst, src, sizeof(int) * count, hipMemcpyHostToDevice, st));
+  };
+  upload(iv.hn_nbr0, nbr, n * cap);
+  upload(iv.hn_deg0, deg, n);
+  upload(iv.hn_nbru, nbu.data(), (layers - 1) * n1 * cap);
+  upload(iv.hn_degu, dgu.data(), (layers - 1) * n1);
+  upload(iv.hn_upslot, up.data(), n);
+  upload(iv.hn_level, lev.data(), n);
+  upload(iv.hn_entry, ent.data(), iv.nb);
+  upload(iv.hn_maxl, mx.data(), iv.nb);
+  MANN_HIP(hipStreamSynchronize(st));
+}
+
 void ivf_hnsw_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, int ef_search,
                      const float* route_in, int merge, int probe_order, int64_t* out_ids,
                      float* out_dist) {

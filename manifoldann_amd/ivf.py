@@ -420,6 +420,24 @@ def ivf_hnsw_from_assignment(X, centroids_dk, assign, routing, hnsw: HnswTermina
     return idx
 
 
+def set_hnsw_graph(index: MultiLevelIndex, nbr, deg, entry, max_layer, level=None,
+                   ef_search=HNSW_DEFAULT_EF_SEARCH):
+    """Install graphs built elsewhere (the layout of hnsw_graph) as the index's
+    HNSW terminals."""
+    nbr = np.ascontiguousarray(nbr, np.int32)
+    deg = np.ascontiguousarray(deg, np.int32)
+    layers, n, cap = nbr.shape
+    if n != index.n or deg.shape != (layers, n):
+        raise DimensionMismatch("graph arrays do not match the index")
+    lv = None if level is None else np.ascontiguousarray(level, np.int32)
+    check(lib().mann_ivf_hnsw_set_graph(index.handle, layers, cap - 1, nbr.ctypes.data,
+                                        deg.ctypes.data,
+                                        np.ascontiguousarray(entry, np.int32).ctypes.data,
+                                        np.ascontiguousarray(max_layer, np.int32).ctypes.data,
+                                        None if lv is None else lv.ctypes.data))
+    index.terminal = HnswTerminal(M=cap - 1, ef_search=ef_search)
+
+
 def hnsw_graph(index: MultiLevelIndex):
     """The HNSW graphs of an IVF-HNSW index: (nbr (layers, n, M + 1) local 0-based
     ids, deg (layers, n), entry and max_layer per bucket, level per row), rows in

@@ -178,3 +178,55 @@ def test_argument_errors():
         I.query_arrays(idx, data, np.zeros((7, 3), np.float32), 10)
     ids, dist = I.query_arrays(idx, data, np.asfortranarray(Q.T[:, :0]), 10)
     assert ids.shape == (0, 10)
+
+
+def test_installed_graphs_search_like_the_oracle():
+    # graphs built elsewhere (here: the oracle's CPU build, the role the
+    # reference's own HNSWIndex children play behind the Julia shim) installed
+    # with mann_ivf_hnsw_set_graph: the GPU search over them equals the
+    # oracle's, with the levels given and inferred; the export returns them
+    _banner("IVF-HNSW over installed oracle graphs, 4000 x 20, 10 lists")
+    X, C, assign, Q = _data(17, 4000, 20, 10)
+    off, ids, nb = O.partition(assign)
+    g = O.ivf_hnsw_build(X, off, ids, nb, 10, 40, seed=123)
+    route = O.route_distances(Q, C)
+    data = np.asfortranarray(X.T)
+    for with_level in (True, False):
+        idx = I.build_index_from_assignment(data, np.asfortranarray(C.T), assign,
+                                            I.TopKRouting(3), merge_strategy=I.DisjointMerge())
+        I.set_hnsw_graph(idx, g.nbr, g.deg, g.entry, g.max_layer,
+                         level=g.level if with_level else None, ef_search=20)
+        nbr, deg, entry, maxl, level = I.hnsw_graph(idx)
+        assert np.array_equal(deg, g.deg) and np.array_equal(entry, g.entry)
+        for layer in range(deg.shape[0]):
+            live = np.arange(g.nbr.shape[2])[None, :] < deg[layer][:, None]
+            assert np.array_equal(np.where(live, nbr[layer], 0), np.where(live, g.nbr[layer], 0))
+        for po in (False, True):
+            ids_g, dist_g = I.query_arrays(idx, data, np.asfortranarray(Q.T), 10, probe_order=po)
+            oi, od = O.ivf_hnsw_search(X, off, ids, nb, g, Q, route, 10, 3, 20, merge="disjoint",
+                                       probe_order=po)
+            _same(ids_g, dist_g, oi, od, ("installed", with_level, po))
+
+
+def test_installed_graphs_are_validated():
+    _banner("IVF-HNSW graph validation")
+    X, C, assign, Q = _data(2, 600, 8, 3)
+    off, ids, nb = O.partition(assign)
+    g = O.ivf_hnsw_build(X, off, ids, nb, 4, 16, seed=1)
+    data = np.asfortranarray(X.T)
+    idx = I.build_index_from_assignment(data, np.asfortranarray(C.T), assign, I.TopKRouting(2))
+    bad = g.nbr.copy()
+    bad[0, 0, 0] = off[1] - off[0] + 5          # an id past its list
+    with pytest.raises(MA.ArgumentError):
+        I.set_hnsw_graph(idx, bad, g.deg, g.entry, g.max_layer, level=g.level)
+    badd = g.deg.copy()
+    badd[0, 3] = 9                                # a degree above M
+    with pytest.raises(MA.ArgumentError):
+        I.set_hnsw_graph(idx, g.nbr, badd, g.entry, g.max_layer, level=g.level)
+    if g.nbr.shape[0] > 1:
+        lv = g.level.copy()
+        r = int(np.nonzero(g.deg[1])[0][0])
+        lv[r] = 0                                 # a row with neighbours above its level
+        with pytest.raises(MA.ArgumentError):
+            I.set_hnsw_graph(idx, g.nbr, g.deg, g.entry, g.max_layer, level=lv)
+    I.set_hnsw_graph(idx, g.nbr, g.deg, g.entry, g.max_layer, level=g.level)
