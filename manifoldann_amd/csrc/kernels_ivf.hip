@@ -312,19 +312,49 @@ __global__ __launch_bounds__(256) void route_select_kernel(RouteSelArgs a) {
     cnt += __popcll(m);
   }
   wave_fence_sync();
-  // exact norms form of each candidate, one per 16-lane group
+  // exact norms form of the candidates, four per 16-lane group at a time, the
+  // loads of 8 chunks issued before their multiply-adds (each candidate's A16
+  // sum keeps its chunk order)
   const int g = lane >> 4, j = lane & 15;
   const float* y = a.q32 + (int64_t)q * a.d16;
-  for (int i0 = g; i0 < cnt; i0 += 4) {
-    const int c = cidx[i0];
-    const float* x = a.c32 + (int64_t)c * a.d16;
-    float acc = 0.0f;
-    for (int t = j; t < a.d16; t += 16) acc = __builtin_fmaf(x[t], y[t], acc);
-    const float dot = group16_tree(acc);
-    float tt = a.cn2[c] + a.qn2[q];
-    float sq = tt - 2.0f * dot;
-    sq = sq > 0.0f ? sq : 0.0f;
-    if (j == 0) cval[i0] = a.sq ? sq : __builtin_sqrtf(sq);
+  const int nch = a.d16 >> 4;
+  for (int i0 = 4 * g; i0 < cnt; i0 += 16) {
+    int cc[4];
+    const float* x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      cc[u] = cidx[i0 + u < cnt ? i0 + u : i0];
+      x[u] = a.c32 + (int64_t)cc[u] * a.d16;
+    }
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    int c0 = 0;
+    for (; c0 + 8 <= nch; c0 += 8) {
+      float yv[8], xv[8][4];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int e = ((c0 + t) << 4) + j;
+        yv[t] = y[e];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xv[t][u] = x[u][e];
+      }
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] = __builtin_fmaf(xv[t][u], yv[t], acc[u]);
+    }
+    for (; c0 < nch; ++c0) {
+      const int e = (c0 << 4) + j;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[u] = __builtin_fmaf(x[u][e], y[e], acc[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float dot = group16_tree(acc[u]);
+      const float tt = a.cn2[cc[u]] + a.qn2[q];
+      float sq = tt - 2.0f * dot;
+      sq = sq > 0.0f ? sq : 0.0f;
+      if (j == 0 && i0 + u < cnt) cval[i0 + u] = a.sq ? sq : __builtin_sqrtf(sq);
+    }
   }
   wave_fence_sync();
   uint64_t key[PS_KPL];
