@@ -211,6 +211,63 @@ class ManifoldANN_IVFFlat(ManifoldANNHIPWrapper):
         return "ManifoldANN-IVF-Flat"
 
 
+class ManifoldANN_IVFHNSW(ManifoldANNHIPWrapper):
+    """build_ivf_hnsw_index (manifoldann.py:649-737): KMeans(nlist) routing to
+    per-list HNSW graphs built and searched on the GPU (M, ef_construction,
+    ef_search, neighbour policy), routing_k clamped to [1, nlist],
+    DisjointMerge.  The HNSW leaves measure the Euclidean distance for both
+    metrics, as the reference's HNSWIndex children do (they receive no
+    distance); the GPU k-means is Euclidean, where the reference clusters
+    'angular' data with CosineDist."""
+
+    def __init__(self, metric, nlist=64, routing_k=8, kmeans_init="kmeans_plus_plus",
+                 kmeans_max_iters=50, kmeans_tol=1e-4, M=16, ef_construction=200,
+                 ef_search=64, neighbor_policy="diversified"):
+        super().__init__(metric)
+        self._nlist = int(nlist)
+        self._routing_k = int(routing_k)
+        self._kmeans_init = kmeans_init
+        self._kmeans_max_iters = int(kmeans_max_iters)
+        self._kmeans_tol = float(kmeans_tol)
+        self._M = int(M)
+        self._ef_construction = int(ef_construction)
+        self._ef_search = int(ef_search)
+        self._neighbor_policy = neighbor_policy
+
+    def fit(self, X) -> None:
+        self._data = X
+        routing_k = max(1, min(self._routing_k, self._nlist))
+        self._index = _ivf.build_ivf_hnsw_index(
+            X, self._nlist, routing_k=routing_k, kmeans_init=self._kmeans_init,
+            kmeans_max_iters=self._kmeans_max_iters, kmeans_tol=self._kmeans_tol,
+            hnsw_M=self._M, hnsw_ef_construction=self._ef_construction,
+            hnsw_ef_search=self._ef_search, hnsw_neighbor_policy=self._neighbor_policy,
+            distance=self._get_distance_function())
+
+    def set_query_params(self, **params) -> None:
+        if "ef_search" in params:
+            self._ef_search = int(params["ef_search"])
+
+    def _query_arrays(self, Q, n):
+        return _ivf.query_arrays(self._index, self._data, Q, n, ef_search=self._ef_search)
+
+    def query(self, v: np.ndarray, n: int) -> List[int]:
+        q = np.asarray(v, dtype=np.float32).reshape(-1)
+        res = _ivf.query(self._index, self._data, q, n, ef_search=self._ef_search)
+        return [nb.id - 1 for nb in res]
+
+    def __str__(self):
+        return ("ManifoldANN-IVF-HNSW("
+                f"nlist={self._nlist}, routing_k={self._routing_k}, "
+                f"M={self._M}, ef_construction={self._ef_construction}, "
+                f"ef_search={self._ef_search}, neighbor_policy={self._neighbor_policy}, "
+                f"kmeans_init={self._kmeans_init})")
+
+    @staticmethod
+    def get_name():
+        return "ManifoldANN-IVF-HNSW"
+
+
 def compute_recall(predicted: List[int], ground_truth, k: int = None) -> float:
     """|set(predicted[:k]) ∩ set(truth[:k])| / |set(truth[:k])|, 0 for an
     empty truth; k = len(predicted) when None (benchmarking/utils/

@@ -76,3 +76,26 @@ def test_ivf_flat_wrapper_matches_heap_oracle(metric):
         assert preds == truth, nprobe
         # the single-query path (probe order) gives the same ids here
         assert algo.query(Q[5], 10) == truth[5]
+
+
+def test_ivf_hnsw_wrapper_matches_the_oracle():
+    # manifoldann.py:649-737 through the harness contract: the ids equal the
+    # oracle's IVF-HNSW search over the index's own lists, levels and routing
+    _banner("harness IVF-HNSW euclidean")
+    rng = np.random.default_rng(6)
+    X = rng.standard_normal((4000, 32)).astype(np.float32)
+    Q = rng.standard_normal((150, 32)).astype(np.float32)
+    algo = H.ManifoldANN_IVFHNSW("euclidean", nlist=12, routing_k=3, M=8, ef_construction=40,
+                                 ef_search=20)
+    preds = _run_contract(algo, X, Q, 10)
+    idx = algo._index
+    nbr, deg, entry, maxl, level = I.hnsw_graph(idx)
+    ids_all = np.concatenate(idx.id_mappings)
+    off = np.concatenate([[0], np.cumsum(idx.list_sizes)]).astype(np.int64)
+    g = O.ivf_hnsw_build(X, off, ids_all, idx.n_buckets, 8, 40, level=level)
+    C = np.ascontiguousarray(idx.transform.centroids.T)
+    oi, _ = O.ivf_hnsw_search(X, off, ids_all, idx.n_buckets, g, Q, O.route_distances(Q, C), 10,
+                              3, 20, merge="disjoint")
+    assert preds == [[int(i) - 1 for i in row if i > 0] for row in oi]
+    # the single-query path agrees with the batch on untied rows
+    assert algo.query(Q[0], 10) == preds[0]

@@ -33,3 +33,13 @@ def test_recall_matches_the_harness_definition():
     assert H.compute_recall([1], np.array([]), k=1) == 0.0
     assert H.compute_recall_batch([], truth, 4) == 0.0
     assert H.compute_recall_batch([[0, 1, 2, 3], [7, 7, 7, 7]], truth, 4) == pytest.approx(0.5)
+
+
+def test_ivf_hnsw_wrapper_names_and_params():
+    h = H.ManifoldANN_IVFHNSW("euclidean", nlist=32, routing_k=4, M=12, ef_construction=80,
+                              ef_search=40)
+    assert H.ManifoldANN_IVFHNSW.get_name() == "ManifoldANN-IVF-HNSW"
+    assert str(h) == ("ManifoldANN-IVF-HNSW(nlist=32, routing_k=4, M=12, ef_construction=80, "
+                      "ef_search=40, neighbor_policy=diversified, kmeans_init=kmeans_plus_plus)")
+    h.set_query_params(ef_search=90)
+    assert h._ef_search == 90
