@@ -554,6 +554,21 @@ def bench_ivf_hnsw(args, M, I, L, check, np, torch, index, X, Q, Qd, ids, dist, 
     kern = kernel_avg_ms(L, check, ["prep_queries", "ivf_route", "ivf_probe", "hnsw_search",
                                     "hnsw_merge"])
     check(L.mann_profile_enable(0))
+    # the walk's algorithmic bytes: one FP32 row (d floats) per exact distance,
+    # divided by the search kernel's HIP-event time; it is bound by the latency
+    # of its dependent steps, which this fraction of the HBM roof shows
+    ndist = ctypes.c_int64()
+    check(L.mann_tune_get(b"last_hnsw_distances", ctypes.byref(ndist)))
+    roof = None
+    if kern.get("hnsw_search"):
+        gbs = ndist.value * d * 4 / (kern["hnsw_search"] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4),
+                "kernel": "hnsw_search_kernel (per-(query, list) graph walks, nprobe 5, ef 64)",
+                "distances_per_query": round(ndist.value / nq, 1),
+                "algorithmic_bytes_per_launch": ndist.value * d * 4,
+                "avg_launch_ms": kern["hnsw_search"],
+                "note": "latency bound: each step waits for the neighbour list, then the rows"}
     # the oracle over the GPU's graphs: 16 queries checked, a sample timed
     from oracle import oracle as O
     nbr, deg, entry, maxl, level = I.hnsw_graph(index)
@@ -587,7 +602,8 @@ def bench_ivf_hnsw(args, M, I, L, check, np, torch, index, X, Q, Qd, ids, dist, 
     return {"workload": f"IVF-HNSW n={n} d={d} nlist={index.nlist} M={term.M} "
                         f"ef_construction={term.ef_construction} nq={nq} k=10",
             "build_s": round(build_s, 2), "max_list": int(np.max(index.list_sizes)),
-            "sweep": sweep, "best_at_0.95": best, "kernels_ms": kern, "precision": parity,
+            "sweep": sweep, "best_at_0.95": best, "kernels_ms": kern, "roofline": roof,
+            "precision": parity,
             "cpu_baseline": cpu}
 
 

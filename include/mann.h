@@ -427,7 +427,9 @@ int mann_last_emitted(int64_t* total);
  * it).  mann_tune_get also reads
  * "last_heap_replays": the queries of the last IVF-Flat search on this thread
  * whose k-th place had a distance tie and went through the bounded-heap merge
- * (read only).  No reference counterpart: the reference has no tunable
+ * (read only), and "last_hnsw_distances": the exact distances the graph walks
+ * of the last IVF-HNSW search on this thread computed (read only).  No
+ * reference counterpart: the reference has no tunable
  * kernels. */
 int mann_tune_set(const char* key, int64_t value);
 int mann_tune_get(const char* key, int64_t* value);

@@ -1419,6 +1419,7 @@ int mann_tune_get(const char* key, int64_t* value) {
     else if (k == "fp8_emit_per_k") *value = t.fp8_emit_per_k;
     else if (k == "sym_join") *value = t.sym_join;
     else if (k == "last_heap_replays") *value = last_stats().heap_replays;
+    else if (k == "last_hnsw_distances") *value = last_stats().hnsw_distances;
     else throw Error{ERR_ARGUMENT, "unknown tuning key " + k};
   });
 }

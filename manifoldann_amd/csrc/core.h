@@ -176,6 +176,7 @@ struct SearchStats {
   long long probes = 0;  // IVF: (list, query) pairs scanned
   long long emitted = 0; // first-pass candidates read by the selection (all passes)
   long long heap_replays = 0;  // IVF-Flat queries merged through the bounded heap
+  long long hnsw_distances = 0;  // IVF-HNSW: exact distances the graph walks computed
 };
 SearchStats& last_stats();
 // adds the selection counters of a kStatSlots x 4 host copy to s

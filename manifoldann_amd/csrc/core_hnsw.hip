@@ -7,6 +7,7 @@
 
 #include "core.h"
 #include "hnsw.h"
+#include "util.h"
 
 namespace mann {
 
@@ -281,6 +282,10 @@ void ivf_hnsw_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, i
   a.out_row = orow;
   a.out_d = od;
   a.out_cnt = ocnt;
+  // the walks' distance count (two ints cleared by a fill kernel)
+  int* nd = c.buf("hn_ndist").as<int>(2);
+  fill_i32(nd, 2, 0, st);
+  a.ndist = (unsigned long long*)nd;
   {
     ScopedTime tm("hnsw_search", st);
     launch_hnsw_search(a, grid, st);
@@ -306,6 +311,13 @@ void ivf_hnsw_search(const mann_ivf& iv, const QueryOps& Q, int k, int nprobe, i
     launch_hnsw_merge(m, st);
   }
   MANN_HIP(hipGetLastError());
+  unsigned long long ndh = 0;
+  MANN_HIP(hipMemcpyAsync(&ndh, nd, sizeof(ndh), hipMemcpyDeviceToHost, st));
+  MANN_HIP(hipStreamSynchronize(st));
+  SearchStats s{};
+  s.probes = np;
+  s.hnsw_distances = (long long)ndh;
+  last_stats() = s;
 }
 
 }  // namespace mann

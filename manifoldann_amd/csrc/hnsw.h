@@ -66,6 +66,7 @@ struct HnswSearchArgs {
   int* out_row;         // [nq][nprobe][k] reordered rows
   float* out_d;
   int* out_cnt;         // [nq][nprobe]
+  unsigned long long* ndist;  // distances computed, added per pair (cleared by the caller)
 };
 
 struct HnswMergeArgs {

@@ -195,7 +195,7 @@ __device__ void dist_batch(const float* __restrict__ x32, int64_t row0, int d16,
 // _greedy_descent (query.jl:327-345): each pass scores the neighbour list of
 // the node current at its start and moves on every strict improvement
 __device__ HnCand greedy(const HnGraph& G, int layer, int64_t row0, const float* x32, int d16,
-                         const float* qv, HnCand cur, Work& w, int lane) {
+                         const float* qv, HnCand cur, Work& w, int lane, int* ndist = nullptr) {
   int c = cur.id;
   float bd = cur.d;
   for (;;) {
@@ -207,6 +207,7 @@ __device__ HnCand greedy(const HnGraph& G, int layer, int64_t row0, const float*
     if (lane < deg) w.id[lane] = v;
     wg_sync();
     dist_batch(x32, row0, d16, qv, w.id, deg, w.d, lane);
+    if (ndist) *ndist += deg;
     wg_sync();
     int nc = c, imp = 0;
     float nd = bd;
@@ -231,7 +232,8 @@ __device__ HnCand greedy(const HnGraph& G, int layer, int64_t row0, const float*
 // the best heap's size; its entries are best[0 .. size).
 __device__ int search_layer(const HnGraph& G, int layer, int64_t row0, const float* x32, int d16,
                             const float* qv, HnCand entry, int ef, HnCand* best, HnCand* pl,
-                            int plcap, HnCand* pg, unsigned* vis, Work& w, int lane) {
+                            int plcap, HnCand* pg, unsigned* vis, Work& w, int lane,
+                            int* ndist = nullptr) {
   int blen = 0, plen = 0;
   HnCand* pend = pl;
   if (lane == 0) {
@@ -270,6 +272,7 @@ __device__ int search_layer(const HnGraph& G, int layer, int64_t row0, const flo
     if (fresh) w.id[__popcll(m & ((1ull << lane) - 1ull))] = v;
     wg_sync();
     dist_batch(x32, row0, d16, qv, w.id, cnt, w.d, lane);
+    if (ndist) *ndist += cnt;
     wg_sync();
     if (lane == 0) {
       for (int t = 0; t < cnt; ++t) {
@@ -523,10 +526,12 @@ __global__ void __launch_bounds__(64) hnsw_search_kernel(HnswSearchArgs a) {
     wg_sync();
     HnCand cur{w.d[0], ep};
     wg_sync();
+    int nd = 1;
     for (int layer = a.maxl[b]; layer >= 1; --layer)
-      cur = greedy(G, layer, row0, a.x32, a.d16, qv, cur, w, lane);
+      cur = greedy(G, layer, row0, a.x32, a.d16, qv, cur, w, lane, &nd);
     const int nr = search_layer(G, 0, row0, a.x32, a.d16, qv, cur, a.ef, best, pl, a.pend_lds,
-                                pg, vis, w, lane);
+                                pg, vis, w, lane, &nd);
+    if (lane == 0 && a.ndist) atomicAdd(a.ndist, (unsigned long long)nd);
     const int lim = a.k < nr ? a.k : nr;
     stable_sort_into(best, nr, res, lim, lane);
     wg_sync();
