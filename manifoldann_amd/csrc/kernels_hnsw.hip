@@ -490,7 +490,8 @@ __global__ void __launch_bounds__(64) hnsw_build_kernel(HnswBuildArgs a) {
 // pair: the entry point is marked visited, the upper layers are descended
 // greedily, the base layer is searched from the node reached (ef), and the
 // first min(k, results) in distance order are written.
-__global__ void __launch_bounds__(64) hnsw_search_kernel(HnswSearchArgs a) {
+// (at most 96 VGPRs: 5 wavefronts per SIMD, no spills)
+__global__ void __launch_bounds__(64, 5) hnsw_search_kernel(HnswSearchArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   HnCand* best = (HnCand*)smem;      // ef
   HnCand* res = best + a.ef;          // ef
