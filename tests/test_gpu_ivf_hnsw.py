@@ -230,3 +230,18 @@ def test_installed_graphs_are_validated():
         with pytest.raises(MA.ArgumentError):
             I.set_hnsw_graph(idx, g.nbr, g.deg, g.entry, g.max_layer, level=lv)
     I.set_hnsw_graph(idx, g.nbr, g.deg, g.entry, g.max_layer, level=g.level)
+
+
+def test_graphs_at_a_realistic_list_size():
+    # 64 lists of about 1,000 rows in 64 dimensions with the builder's defaults
+    # (M 16, ef_construction 200): every graph equal to the oracle's, and the
+    # searches at ef 64
+    _banner("IVF-HNSW 64,000 x 64, 64 lists, M 16, ef_c 200")
+    X, C, assign, Q = _data(29, 64_000, 64, 64, nq=128)
+    term = I.HnswTerminal(M=16, ef_construction=200, ef_search=64)
+    idx = _index(X, C, assign, 5, term, 77, I.DisjointMerge())
+    off, ids, nb, g = _check_graph(idx, X, assign, term, 77)
+    route = O.route_distances(Q, C)
+    ids_g, dist_g = I.query_arrays(idx, np.asfortranarray(X.T), np.asfortranarray(Q.T), 10)
+    oi, od = O.ivf_hnsw_search(X, off, ids, nb, g, Q, route, 10, 5, 64, merge="disjoint")
+    _same(ids_g, dist_g, oi, od, "realistic")
