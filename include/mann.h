@@ -307,7 +307,9 @@ int mann_ivf_search_ex(mann_ivf_t ivf, const float* queries, int64_t d, int64_t 
  * log(max(M, 2))) (planner.jl:19-30) over the splitmix64 stream of `seed`, one
  * draw per point in list order (bucket order, ascending id).  Euclidean
  * distance (the HNSWIndex default).  M <= 63, ef_construction <= 512.  A
- * second call replaces the graphs. */
+ * second call replaces the graphs; like the reference's insert! (hnsw/query.jl:
+ * 161-163), it must not run while searches of the same handle are in flight
+ * (searches from several threads may share a handle). */
 int mann_ivf_hnsw_build(mann_ivf_t ivf, int64_t M, int64_t ef_construction, int neighbor_policy,
                         uint64_t seed);
 /* query(::MultiLevelIndex{KMeans, TopK, HNSW}, data, Q, k; ef_search)
