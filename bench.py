@@ -586,8 +586,24 @@ def bench_ivf_hnsw(args, M, I, L, check, np, torch, index, X, Q, Qd, ids, dist, 
               "dist_bits_equal_oracle": bool(np.array_equal(gd.view(np.uint32),
                                                             od.view(np.uint32)))}
     cpu = None
+    build_cpu = None
     if not args.skip_cpu:
         cores = host_cores()
+        # the oracle's serial insertion over the same lists and levels, timed on
+        # the host's cores, and every graph compared with the GPU's
+        t3 = time.perf_counter()
+        go = O.ivf_hnsw_build(X, off, lids, index.n_buckets, term.M, term.ef_construction,
+                              term.policy_code, level=level, nthreads=cores)
+        tb = time.perf_counter() - t3
+        live = np.arange(term.M + 1)[None, None, :] < deg[:, :, None]
+        same = bool(np.array_equal(deg, go.deg) and np.array_equal(entry, go.entry) and
+                    np.array_equal(np.where(live, nbr, -1), np.where(live, go.nbr, -1)))
+        del go, live
+        build_cpu = {"value_s": round(tb, 2), "cores": cores, "kind": "port",
+                     "graphs_equal_gpu": same,
+                     "sample": f"all {index.n_buckets} lists (oracle serial insertion per list, "
+                               f"lists in parallel, OpenMP)"}
+        log(f"IVF-HNSW: oracle build {tb:.1f} s on {cores} cores, graphs equal {same}")
         m = min(2000, nq)
         t2 = time.perf_counter()
         R = O.route_distances(Q[:m], Cc, nthreads=cores)
@@ -604,7 +620,7 @@ def bench_ivf_hnsw(args, M, I, L, check, np, torch, index, X, Q, Qd, ids, dist, 
             "build_s": round(build_s, 2), "max_list": int(np.max(index.list_sizes)),
             "sweep": sweep, "best_at_0.95": best, "kernels_ms": kern, "roofline": roof,
             "precision": parity,
-            "cpu_baseline": cpu}
+            "cpu_baseline": cpu, "build_cpu_baseline": build_cpu}
 
 
 def bench_large_cosine(args, M, L, check, np, torch):
