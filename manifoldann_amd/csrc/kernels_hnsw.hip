@@ -139,7 +139,11 @@ struct Work {
 
 // Euclidean distances (the A16 order of the oracle) of rows row0 + ids[i],
 // i < cnt <= 64, to the vector qv (d16 floats, zero padded): out[i].  Every
-// lane takes part (the group tree reads all lanes of its 16-lane row).
+// lane takes part (the group tree reads all lanes of its 16-lane row).  CH:
+// chunks whose loads are issued together (the build, one wavefront per SIMD,
+// affords 16: one memory round trip per pass at d <= 256; the search, which
+// needs occupancy, 8).
+template <int CH>
 __device__ void dist_batch(const float* __restrict__ x32, int64_t row0, int d16,
                            const float* __restrict__ qv, const int* ids, int cnt, float* out,
                            int lane) {
@@ -152,28 +156,46 @@ __device__ void dist_batch(const float* __restrict__ x32, int64_t row0, int d16,
       xr[u] = x32 + (row0 + ids[i < cnt ? i : b]) * (int64_t)d16;
     }
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    // the loads of 8 chunks are issued before their multiply-adds (the rows
+    // the loads of CH chunks are issued before their multiply-adds (the rows
     // are scattered: each wait is a full memory latency); the adds keep the
     // chunk order of the A16 sum
     const int nch = d16 >> 4;
     int c0 = 0;
-    for (; c0 + 8 <= nch; c0 += 8) {
-      float xv[8][4], qq[8];
+    for (; c0 + CH <= nch; c0 += CH) {
+      float xv[CH][4], qq[CH];
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
+      for (int c = 0; c < CH; ++c) {
         const int t = ((c0 + c) << 4) + j;
         qq[c] = qv[t];
 #pragma unroll
         for (int u = 0; u < 4; ++u) xv[c][u] = xr[u][t];
       }
 #pragma unroll
-      for (int c = 0; c < 8; ++c)
+      for (int c = 0; c < CH; ++c)
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const float df = xv[c][u] - qq[c];
           acc[u] = __builtin_fmaf(df, df, acc[u]);
         }
     }
+    if (CH > 8)
+      for (; c0 + 8 <= nch; c0 += 8) {
+        float xv[8][4], qq[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const int t = ((c0 + c) << 4) + j;
+          qq[c] = qv[t];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) xv[c][u] = xr[u][t];
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const float df = xv[c][u] - qq[c];
+            acc[u] = __builtin_fmaf(df, df, acc[u]);
+          }
+      }
     for (; c0 < nch; ++c0) {
       const int t = (c0 << 4) + j;
       const float qq = qv[t];
@@ -194,6 +216,7 @@ __device__ void dist_batch(const float* __restrict__ x32, int64_t row0, int d16,
 
 // _greedy_descent (query.jl:327-345): each pass scores the neighbour list of
 // the node current at its start and moves on every strict improvement
+template <int CH>
 __device__ HnCand greedy(const HnGraph& G, int layer, int64_t row0, const float* x32, int d16,
                          const float* qv, HnCand cur, Work& w, int lane, int* ndist = nullptr) {
   int c = cur.id;
@@ -206,7 +229,7 @@ __device__ HnCand greedy(const HnGraph& G, int layer, int64_t row0, const float*
     const int deg = *deg_of(G, layer, s);
     if (lane < deg) w.id[lane] = v;
     wg_sync();
-    dist_batch(x32, row0, d16, qv, w.id, deg, w.d, lane);
+    dist_batch<CH>(x32, row0, d16, qv, w.id, deg, w.d, lane);
     if (ndist) *ndist += deg;
     wg_sync();
     int nc = c, imp = 0;
@@ -230,6 +253,7 @@ __device__ HnCand greedy(const HnGraph& G, int layer, int64_t row0, const float*
 // cleared the bitmap or marked what it must): best-first expansion until the
 // popped candidate is farther than the worst of a full best heap.  Returns
 // the best heap's size; its entries are best[0 .. size).
+template <int CH>
 __device__ int search_layer(const HnGraph& G, int layer, int64_t row0, const float* x32, int d16,
                             const float* qv, HnCand entry, int ef, HnCand* best, HnCand* pl,
                             int plcap, HnCand* pg, unsigned* vis, Work& w, int lane,
@@ -271,7 +295,7 @@ __device__ int search_layer(const HnGraph& G, int layer, int64_t row0, const flo
     const int cnt = __popcll(m);
     if (fresh) w.id[__popcll(m & ((1ull << lane) - 1ull))] = v;
     wg_sync();
-    dist_batch(x32, row0, d16, qv, w.id, cnt, w.d, lane);
+    dist_batch<CH>(x32, row0, d16, qv, w.id, cnt, w.d, lane);
     if (ndist) *ndist += cnt;
     wg_sync();
     if (lane == 0) {
@@ -314,11 +338,12 @@ __device__ void stable_sort_into(const HnCand* src, int n, HnCand* dst, int lim,
 // candidate unless a kept one is strictly closer to it than the centre, stop at
 // cap, then refill from the rejected in order.  kept/keptd receive the result
 // (keptd may be null); returns its size.
+template <int CH>
 __device__ int diversify(const float* x32, int64_t row0, int d16, const int* cid, const float* cd,
                          int n, int cap, int* kept, float* keptd, Work& w, int lane) {
   int nk = 0;
   for (int ci = 0; ci < n && nk < cap; ++ci) {
-    dist_batch(x32, row0, d16, x32 + (row0 + cid[ci]) * (int64_t)d16, kept, nk, w.d, lane);
+    dist_batch<CH>(x32, row0, d16, x32 + (row0 + cid[ci]) * (int64_t)d16, kept, nk, w.d, lane);
     wg_sync();
     const bool dom = lane < nk && w.d[lane] < cd[ci];
     if (__ballot(dom) == 0) {
@@ -386,17 +411,17 @@ __global__ void __launch_bounds__(64) hnsw_build_kernel(HnswBuildArgs a) {
     const float* qv = x32 + ri * d16;
     if (lane == 0) w.id[0] = ep;
     wg_sync();
-    dist_batch(x32, row0, d16, qv, w.id, 1, w.d, lane);
+    dist_batch<16>(x32, row0, d16, qv, w.id, 1, w.d, lane);
     wg_sync();
     HnCand cur{w.d[0], ep};
     wg_sync();
     for (int layer = maxL; layer >= (L + 1 > 1 ? L + 1 : 1); --layer)
-      cur = greedy(G, layer, row0, x32, d16, qv, cur, w, lane);
+      cur = greedy<16>(G, layer, row0, x32, d16, qv, cur, w, lane);
     for (int layer = L < maxL ? L : maxL; layer >= 0; --layer) {
       for (int t = lane; t < nw; t += 64) vis[t] = 0u;
       wg_sync();
-      const int nr = search_layer(G, layer, row0, x32, d16, qv, cur, a.ef, best, pl, B_PEND, pg,
-                                  vis, w, lane);
+      const int nr = search_layer<16>(G, layer, row0, x32, d16, qv, cur, a.ef, best, pl, B_PEND,
+                                      pg, vis, w, lane);
       for (int t = lane; t < nr; t += 64) {
         const HnCand c = best[t];
         int r = 0;
@@ -419,7 +444,7 @@ __global__ void __launch_bounds__(64) hnsw_build_kernel(HnswBuildArgs a) {
         }
         wg_sync();
       } else {
-        ns = diversify(x32, row0, d16, rid, rd, nr, M, sel, seld, w, lane);
+        ns = diversify<16>(x32, row0, d16, rid, rd, nr, M, sel, seld, w, lane);
       }
       // _connect_new_node! (query.jl:444-469): the node's own list, then the
       // reverse edges in selection order, each pruned when over M
@@ -446,7 +471,8 @@ __global__ void __launch_bounds__(64) hnsw_build_kernel(HnswBuildArgs a) {
         // sort, selection written over the list
         if (lane < dg) pid[lane] = clst[lane];
         wg_sync();
-        dist_batch(x32, row0, d16, x32 + (row0 + center) * (int64_t)d16, pid, dg, pdist, lane);
+        dist_batch<16>(x32, row0, d16, x32 + (row0 + center) * (int64_t)d16, pid, dg, pdist,
+                       lane);
         wg_sync();
         if (lane < dg) {
           const float dl = pdist[lane];
@@ -466,7 +492,7 @@ __global__ void __launch_bounds__(64) hnsw_build_kernel(HnswBuildArgs a) {
           if (lane < nk) kept[lane] = sid[lane];
           wg_sync();
         } else {
-          nk = diversify(x32, row0, d16, sid, sdist, dg, cap, kept, nullptr, w, lane);
+          nk = diversify<16>(x32, row0, d16, sid, sdist, dg, cap, kept, nullptr, w, lane);
         }
         if (lane < nk) clst[lane] = kept[lane];
         if (lane == 0) *cdeg = nk;
@@ -523,15 +549,15 @@ __global__ void __launch_bounds__(64, 5) hnsw_search_kernel(HnswSearchArgs a) {
       vis[ep >> 5] |= 1u << (ep & 31);
     }
     wg_sync();
-    dist_batch(a.x32, row0, a.d16, qv, w.id, 1, w.d, lane);
+    dist_batch<8>(a.x32, row0, a.d16, qv, w.id, 1, w.d, lane);
     wg_sync();
     HnCand cur{w.d[0], ep};
     wg_sync();
     int nd = 1;
     for (int layer = a.maxl[b]; layer >= 1; --layer)
-      cur = greedy(G, layer, row0, a.x32, a.d16, qv, cur, w, lane, &nd);
-    const int nr = search_layer(G, 0, row0, a.x32, a.d16, qv, cur, a.ef, best, pl, a.pend_lds,
-                                pg, vis, w, lane, &nd);
+      cur = greedy<8>(G, layer, row0, a.x32, a.d16, qv, cur, w, lane, &nd);
+    const int nr = search_layer<8>(G, 0, row0, a.x32, a.d16, qv, cur, a.ef, best, pl, a.pend_lds,
+                                   pg, vis, w, lane, &nd);
     if (lane == 0 && a.ndist) atomicAdd(a.ndist, (unsigned long long)nd);
     const int lim = a.k < nr ? a.k : nr;
     stable_sort_into(best, nr, res, lim, lane);
