@@ -86,7 +86,10 @@ int mann_dataset_set_first_pass(mann_dataset_t ds, int kind);
 
 /* query(::BruteForceIndex, data, Q, k) -- batch form (src/ann_index.jl:71-101
  * over src/indices/bruteforce.jl:46-71, validation src/utils/validation.jl:8-34).
- * k_eff = min(k, n); slots past k_eff are empty.  nq = 0 is valid. */
+ * k_eff = min(k, n); slots past k_eff are empty.  nq = 0 is valid.  Any k:
+ * up to 256 (255 for the self-joins of mann_knn_graph / mann_knn_candidates)
+ * the certified selection runs; above, every exact distance is sorted per
+ * query (same results, one full pass and sort per query). */
 int mann_bf_knn(mann_dataset_t base, const float* queries, int64_t d, int64_t nq, int64_t ldq,
                 int64_t k, int metric, int64_t* out_ids, float* out_dist);
 
@@ -97,7 +100,8 @@ int mann_bf_knn(mann_dataset_t base, const float* queries, int64_t d, int64_t nq
  * queries, out_ids and out_dist must be device memory, and every scratch
  * buffer of the search comes from one 64 MiB array that the library keeps per
  * GPU for the life of the process: a batch whose scratch would exceed it is
- * an ArgumentError (the message names the bytes needed; split the batch).
+ * an ArgumentError (the message names the bytes needed; split the batch), and
+ * so is k > 256 (the all-distances sort's scratch grows with n).
  * The first call on a host thread and device does one-time set-up (the
  * thread's stream, the array's device address); no call allocates after that.
  * Calls on one GPU share the array: the library runs them one at a time, and
@@ -180,7 +184,7 @@ int mann_knn_graph(mann_dataset_t data, int64_t k, int include_self, int metric,
  * n - 1) candidates (fewer only if it is not among its own first
  * candidate_k + 1), and empty slots hold id 0.  out_ids (and out_dist when
  * given) are candidate_k x n; out_count (optional) the candidates per point.
- * candidate_k <= 255. */
+ * Any candidate_k (above 255 through the all-distances sort of mann_bf_knn). */
 int mann_knn_candidates(mann_dataset_t data, int64_t candidate_k, int metric, int64_t* out_ids,
                         float* out_dist, int64_t* out_count);
 

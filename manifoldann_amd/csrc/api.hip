@@ -244,6 +244,10 @@ int mann_bf_knn_noalloc(mann_dataset_t base, const float* queries, int64_t d, in
                  ERR_ARGUMENT,
                  "the allocation-free search takes device queries and outputs only "
                  "(host buffers would need staging allocations)");
+    // larger k takes the all-distances sort, whose scratch grows with n
+    MANN_REQUIRE(k <= select_exmax() / 2, ERR_ARGUMENT,
+                 "k = " + std::to_string(k) + " exceeds the allocation-free search's maximum of " +
+                     std::to_string(select_exmax() / 2) + " neighbours per query");
     const size_t need = noalloc_need(*base, nq, k);
     MANN_REQUIRE(need <= kNoAllocScratch, ERR_ARGUMENT,
                  std::to_string(nq) + " queries need " + std::to_string(need) +
@@ -581,10 +585,6 @@ int mann_knn_candidates(mann_dataset_t data, int64_t candidate_k, int metric, in
     MANN_REQUIRE(data != nullptr, ERR_NULL, "null dataset");
     check_metric(metric);
     MANN_REQUIRE(candidate_k > 0, ERR_ARGUMENT, "candidate_k must be positive");
-    MANN_REQUIRE(candidate_k + 1 <= select_exmax() / 2, ERR_ARGUMENT,
-                 "candidate_k = " + std::to_string(candidate_k) +
-                     " exceeds the GPU search's maximum of " +
-                     std::to_string(select_exmax() / 2 - 1));
     MANN_REQUIRE(out_ids != nullptr, ERR_NULL, "null pointer");
     MANN_REQUIRE((metric == COSINE) == (data->normalized != 0), ERR_ARGUMENT,
                  "dataset was prepared for another metric");

@@ -191,6 +191,10 @@ void add_stats(const int* slots, SearchStats& s);
 // Returns the arguments as launched (partials and segment count set).
 ExactArgs run_exact_fallback(ExactArgs e, int nflag, int64_t total_rows, int nranges);
 
+// exact search for k + exclude_self above the selection's capacity: every
+// distance, a segmented sort per query (core_bf_all.hip)
+void bf_search_all(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int exclude_self,
+                   int64_t* out_ids, float* out_dist, int* out_count);
 void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int exclude_self,
                int64_t* out_ids, float* out_dist, int* out_count);
 

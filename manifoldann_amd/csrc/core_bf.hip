@@ -629,6 +629,11 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   S = SearchStats();
   const int nq = (int)Q.nq;
   if (nq == 0 || k <= 0) return;
+  // k beyond the certified selection's capacity: the exact all-distances path
+  if (k + (exclude_self ? 1 : 0) > select_exmax() / 2) {
+    bf_search_all(ds, Q, k, metric, exclude_self, out_ids, out_dist, out_count);
+    return;
+  }
   // knobs the autotuner chose for this shape, for this call only
   TuningScope tscope;
   if (!shape_tuning_disabled()) {
@@ -653,10 +658,6 @@ void bf_search(const mann_dataset& ds, const QueryOps& Q, int k, int metric, int
   const int n = (int)ds.n;
   const int k_req = k + (exclude_self ? 1 : 0);
   const int k_eff = std::min(k_req, n);
-  MANN_REQUIRE(k_req <= select_exmax() / 2, ERR_ARGUMENT,
-               "k = " + std::to_string(k) + " exceeds the GPU search's maximum of " +
-                   std::to_string(select_exmax() / 2 - (exclude_self ? 1 : 0)) +
-                   " neighbours per query");
   const int cfg = tuning().scan_cfg;
   // first-pass tiles (the bf16 sample pass keeps the bf16 configuration)
   const int fcfg = fp8 ? std::min(std::max(tuning().scan8_cfg, 0), scan8_num_cfg() - 1) : cfg;

@@ -249,3 +249,65 @@ def test_knn_candidates_match_oracle(metric):
     lists = M.knn_candidates(idx2, small.T, 100)
     assert all(len(l) <= 39 for l in lists)
     assert sum(len(l) == 39 for l in lists) >= 20
+
+
+@pytest.mark.parametrize("metric", [M.Euclidean(), M.SqEuclidean(), M.CosineDist()])
+def test_large_k_exact_all_distances(metric):
+    # bruteforce.jl:46-71 takes any k: k above the certified selection's
+    # capacity goes through every exact distance and a sort per query; ids,
+    # tie order (duplicated rows) and distance bits against the oracle, k > n
+    # padded with id 0 and Inf
+    _banner(f"large k, metric {metric.code}")
+    rng = np.random.default_rng(90 + metric.code)
+    base = rng.standard_normal((1900, 32)).astype(np.float32)
+    X = np.concatenate([base, base[:100]])                    # 2,000 rows with twins
+    Q = np.concatenate([rng.standard_normal((60, 32)).astype(np.float32), X[:10]])
+    index = M.build_index(M.BruteForceIndex, X.T, distance=metric)
+    for k in (257, 300, 1000, 2000, 2500):
+        ids, dist = M.query_arrays(index, X.T, Q.T, k)
+        oi, od = O.bf_knn(X, Q, k, metric.code)
+        assert (ids == oi).all(), (k, np.argwhere(ids != oi)[:5])
+        assert (dist.view(np.uint32) == od.view(np.uint32)).all(), k
+        if k > X.shape[0]:
+            assert (ids[:, X.shape[0]:] == 0).all() and np.isinf(dist[:, X.shape[0]:]).all()
+
+
+def test_large_k_exact_query_chunks():
+    # 1,048,576 rows leave room for 64 queries per sort: 150 queries run in
+    # three chunks, each writing its own rows of the output
+    _banner("large k, 1M rows, 3 query chunks")
+    rng = np.random.default_rng(93)
+    X = rng.standard_normal((1 << 20, 16)).astype(np.float32)
+    Q = rng.standard_normal((150, 16)).astype(np.float32)
+    index = M.build_index(M.BruteForceIndex, X.T)
+    ids, dist = M.query_arrays(index, X.T, Q.T, 300)
+    oi, od = O.bf_knn(X, Q, 300)
+    assert (ids == oi).all(), np.argwhere(ids != oi)[:5]
+    assert (dist.view(np.uint32) == od.view(np.uint32)).all()
+
+
+def test_large_k_knn_graph_and_candidates():
+    # knn_graph.jl:105-130 and weighted_knn_graph.jl:497-503 with k above the
+    # certified selection's capacity: the self id is dropped from the sorted
+    # row, twins keep the lower-id-first order
+    _banner("large k graph and candidates")
+    rng = np.random.default_rng(95)
+    base = rng.standard_normal((700, 12)).astype(np.float32)
+    X = np.concatenate([base, base[:50]])                     # 750 rows
+    index = M.build_index(M.BruteForceIndex, X.T)
+    for k, incl in ((300, False), (300, True), (749, False)):
+        g = M.build_knn_graph(index, X.T, k=k, include_self=incl)
+        og = O.knn_graph(X, k, include_self=incl)
+        assert np.array_equal(np.array(g.neighbors), og), (k, incl)
+    with pytest.raises(M.ArgumentError):
+        M.build_knn_graph(index, X.T, k=750)
+    for ck in (400, 749, 900):
+        ids, dist, cnt = M.knn_candidates(index, X.T, ck, return_arrays=True)
+        k1 = min(ck + 1, X.shape[0])
+        oi, od = O.bf_knn(X, X, k1)
+        for i in range(X.shape[0]):
+            keep = [(a, b) for a, b in zip(oi[i], od[i]) if a != i + 1][:ck]
+            assert cnt[i] == len(keep), (ck, i)
+            assert ids[i, :cnt[i]].tolist() == [a for a, _ in keep], (ck, i)
+            assert dist[i, :cnt[i]].tolist() == [float(b) for _, b in keep]
+            assert (ids[i, cnt[i]:] == 0).all()

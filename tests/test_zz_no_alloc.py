@@ -60,6 +60,12 @@ def test_noalloc_rejects_oversized_batches_and_host_buffers():
     rc = L.mann_bf_knn_noalloc(ds.handle, Qh.ctypes.data, 64, 10, 64, 3, M.EUCLIDEAN,
                                ids.data_ptr(), dist.data_ptr())
     assert rc == -2
+    # k above the selection's capacity takes the all-distances sort, which
+    # needs scratch that grows with the dataset
+    Qd = torch.from_numpy(datasets.gaussian(10, 64, seed=9)).cuda()
+    rc, _, _ = _run(L.mann_bf_knn_noalloc, ds, Qd, 300, M.EUCLIDEAN)
+    assert rc == -2
+    assert b"allocation-free search's maximum" in L.mann_last_error()
     # the search still works afterwards
     Qd = torch.from_numpy(datasets.gaussian(100, 64, seed=6)).cuda()
     rc0, i0, d0 = _run(L.mann_bf_knn, ds, Qd, 10, M.EUCLIDEAN)
