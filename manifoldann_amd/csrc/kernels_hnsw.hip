@@ -142,56 +142,58 @@ struct Work {
 // lane takes part (the group tree reads all lanes of its 16-lane row).  CH:
 // chunks whose loads are issued together (the build, one wavefront per SIMD,
 // affords 16: one memory round trip per pass at d <= 256; the search, which
-// needs occupancy, 8).
-template <int CH>
-__device__ void dist_batch(const float* __restrict__ x32, int64_t row0, int d16,
-                           const float* __restrict__ qv, const int* ids, int cnt, float* out,
-                           int lane) {
+// needs occupancy, 8).  One pass: 4 U rows from ids[b], U per 16-lane group.
+template <int CH, int U>
+__device__ __forceinline__ void dist_rows(const float* __restrict__ x32, int64_t row0, int d16,
+                                          const float* __restrict__ qv, const int* ids, int cnt,
+                                          int b, float* out, int lane) {
   const int g = lane >> 4, j = lane & 15;
-  for (int b = 0; b < cnt; b += 16) {
-    const float* xr[4];
+  {
+    const float* xr[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < U; ++u) {
       const int i = b + g + 4 * u;
       xr[u] = x32 + (row0 + ids[i < cnt ? i : b]) * (int64_t)d16;
     }
-    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float acc[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc[u] = 0.0f;
     // the loads of CH chunks are issued before their multiply-adds (the rows
     // are scattered: each wait is a full memory latency); the adds keep the
     // chunk order of the A16 sum
     const int nch = d16 >> 4;
     int c0 = 0;
     for (; c0 + CH <= nch; c0 += CH) {
-      float xv[CH][4], qq[CH];
+      float xv[CH][U], qq[CH];
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
         const int t = ((c0 + c) << 4) + j;
         qq[c] = qv[t];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) xv[c][u] = xr[u][t];
+        for (int u = 0; u < U; ++u) xv[c][u] = xr[u][t];
       }
 #pragma unroll
       for (int c = 0; c < CH; ++c)
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < U; ++u) {
           const float df = xv[c][u] - qq[c];
           acc[u] = __builtin_fmaf(df, df, acc[u]);
         }
     }
     if (CH > 8)
       for (; c0 + 8 <= nch; c0 += 8) {
-        float xv[8][4], qq[8];
+        float xv[8][U], qq[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
           const int t = ((c0 + c) << 4) + j;
           qq[c] = qv[t];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) xv[c][u] = xr[u][t];
+          for (int u = 0; u < U; ++u) xv[c][u] = xr[u][t];
         }
 #pragma unroll
         for (int c = 0; c < 8; ++c)
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
+          for (int u = 0; u < U; ++u) {
             const float df = xv[c][u] - qq[c];
             acc[u] = __builtin_fmaf(df, df, acc[u]);
           }
@@ -200,16 +202,43 @@ __device__ void dist_batch(const float* __restrict__ x32, int64_t row0, int d16,
       const int t = (c0 << 4) + j;
       const float qq = qv[t];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < U; ++u) {
         const float df = xr[u][t] - qq;
         acc[u] = __builtin_fmaf(df, df, acc[u]);
       }
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < U; ++u) {
       const float s = group16_tree(acc[u]);
       const int i = b + g + 4 * u;
       if (j == 0 && i < cnt) out[i] = __builtin_sqrtf(s);
+    }
+  }
+}
+
+template <int CH>
+__device__ void dist_batch(const float* __restrict__ x32, int64_t row0, int d16,
+                           const float* __restrict__ qv, const int* ids, int cnt, float* out,
+                           int lane) {
+  // U rows per 16-lane group and pass, from what is left: a pass costs one
+  // memory latency whatever its width, and a wider one than the rows left
+  // only issues loads of a repeated row (the diversified rule's tests against
+  // a kept set of 1-16 rows, the late steps of a walk with few unvisited
+  // neighbours).  Passes of 8 rows only with whole-row load batches (the
+  // build); the search, held at 96 registers, takes 16 or 4.  (Passes of 32
+  // rows, 128 loads per lane, slowed the 1M x 256 build from 10.4 to 12.3 s:
+  // a wavefront tracks at most 63 loads in flight.)
+  for (int b = 0; b < cnt;) {
+    const int rem = cnt - b;
+    if (rem > 8) {
+      dist_rows<CH, 4>(x32, row0, d16, qv, ids, cnt, b, out, lane);
+      b += 16;
+    } else if (CH >= 16 && rem > 4) {
+      dist_rows<CH, 2>(x32, row0, d16, qv, ids, cnt, b, out, lane);
+      b += 8;
+    } else {
+      dist_rows<CH, 1>(x32, row0, d16, qv, ids, cnt, b, out, lane);
+      b += 4;
     }
   }
 }
