@@ -139,10 +139,28 @@ def bench_config1(args, M, datasets, L, check, np, torch):
     qps_g = qps_of(g.launch, 1000, args.reps, torch)
     gi, gd = g.launch()
     same = bool((gi == ids).all().item() and (gd == dist).all().item())
+    # k beyond the certified selection's capacity (any k, as bruteforce.jl:46-71):
+    # every exact distance sorted per query; all 1,000 queries against the oracle
+    kl = 1000
+    idl = torch.zeros((1000, kl), dtype=torch.int64, device="cuda")
+    dil = torch.zeros((1000, kl), dtype=torch.float32, device="cuda")
+
+    def run_large():
+        check(L.mann_bf_knn(ds.handle, Qd.data_ptr(), 128, 1000, 128, kl, M.EUCLIDEAN,
+                            idl.data_ptr(), dil.data_ptr()))
+
+    qps_l = qps_of(run_large, 1000, args.reps, torch)
+    li, ld = O.bf_knn(X, Q, kl, nthreads=host_cores())
+    large_k = {"k": kl, "qps": round(qps_l, 1),
+               "path": "every exact FP32 distance + per-query segmented radix sort",
+               "ids_equal_oracle": bool((idl.cpu().numpy() == li).all()),
+               "dist_bits_equal_oracle": bool((dil.cpu().numpy().view(np.uint32)
+                                               == ld.view(np.uint32)).all())}
+    log(f"exact k={kl}: {qps_l:.0f} QPS, oracle equal {large_k['ids_equal_oracle']}")
     rep.update({"workload": "exact kNN L2: base 10000x128 N(0,1), 1000 queries, k=10",
                 "qps": round(qps, 1), "qps_hip_graph": round(qps_g, 1),
-                "hip_graph_same_result": same})
-    del g, ds, Xd, Qd
+                "hip_graph_same_result": same, "large_k": large_k})
+    del g, ds, Xd, Qd, idl, dil
     return rep
 
 
